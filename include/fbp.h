@@ -1,0 +1,148 @@
+/*
+ * fbp.h -- C-ABI of libfbp.so, the MI355X (gfx950) Full Binary Pursuit engine.
+ *
+ * The library replaces the OpenCL program and the device-side half of the
+ * reference's host driver:
+ *   spikesorting_fullpursuit/parallel/binary_pursuit_parallel.py:128-845
+ *     (binary_pursuit(): device pick, buffers, pass loop, output copies)
+ *   spikesorting_fullpursuit/kernels/binary_pursuit.cl:1-1130
+ *     (compute_residual, compute_full_likelihood,
+ *      compute_template_maximum_likelihood, overlap_recheck_indices,
+ *      parse_overlap_recheck_indices, check_overlap_reassignments,
+ *      binary_pursuit, get_adjusted_clips)
+ * The reference has no FFI of its own: its "plugin" is the Python function
+ * binary_pursuit(), which drives pyopencl.  The Python package
+ * spikesorting_fullpursuit_amd binds these entry points with ctypes.
+ *
+ * Conventions follow the reference (binary_pursuit.cl:11-29):
+ *   voltage   float32 [C][K]  channel-major, one chunk of K samples
+ *   templates float32 [T][C][N] (row t = template label t, 0-based)
+ *   bias      float32 [T] = float32(-0.5 * template_SS)  (bpp.py:199)
+ *   thr       float32 [T] = neuron_lower_thresholds     (bpp.py:201)
+ *   spike indices are template ONSETS within the chunk (uint32)
+ *
+ * Every call returns 0 on success or a negative FBP_E* code; the message of
+ * the last failure on the calling thread is returned by fbp_last_error().
+ * No device pointer is returned to the caller except through the *_device
+ * entry points, which take caller-owned device memory.
+ */
+#ifndef FBP_H
+#define FBP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FBP_OK = 0,
+    FBP_EINVAL = -1,   /* bad argument (bpp.py raises ValueError/KeyError)      */
+    FBP_ENOMEM = -2,   /* device memory exhausted                              */
+    FBP_ENODEV = -3,   /* no usable GPU (bpp.py:229-230 RuntimeError)          */
+    FBP_EHIP = -4,     /* HIP runtime error                                    */
+    FBP_EPASSES = -5,  /* pass cap reached before convergence                   */
+    FBP_ESTATE = -6    /* call out of order (e.g. results before run)           */
+};
+
+/* Arithmetic of the likelihood correlation (binary_pursuit.cl:216-300). */
+enum {
+    FBP_PREC_FP32 = 0,  /* float32 fused multiply-add chain, reference order     */
+    FBP_PREC_BF16 = 1,  /* bf16 MFMA screen, float32 re-score of every decision  */
+    FBP_PREC_FP16 = 2,  /* fp16 MFMA screen, float32 re-score                    */
+    FBP_PREC_FP8 = 3    /* fp8 e4m3 MFMA screen (per-template scale), re-score   */
+};
+
+/* Arithmetic of the overlap recheck (binary_pursuit.cl:444-654). */
+enum {
+    FBP_OVERLAP_DIRECT = 0,   /* every (t, f, s) scored as the reference does      */
+    FBP_OVERLAP_SCREENED = 1  /* decomposed float64 screen, direct re-score of the
+                                 candidates inside the rounding bound            */
+};
+
+typedef struct fbp_config {
+    uint32_t n_chans;        /* C (num_neighbor_channels)                       */
+    uint32_t n_samples;      /* K: samples per channel of the chunk (> N)       */
+    uint32_t n_templates;    /* T (num_templates)                               */
+    uint32_t template_len;   /* N (template_length = n_samples_per_chan)        */
+    uint32_t max_shift;      /* M = n_max_shift_inds (bpp.py:294-298)          */
+    uint32_t do_overlap;     /* sort_info['do_overlap_recheck'] (bpp.py:570)    */
+    uint32_t reduce_wg;      /* width of the overlap work-group reduction that
+                                breaks exact score ties (power of two; the
+                                reference uses the device's work-group size,
+                                bpp.py:259; 256 by default)                    */
+    uint32_t precision;      /* FBP_PREC_*                                      */
+    uint32_t overlap_mode;   /* FBP_OVERLAP_*                                   */
+    uint32_t max_passes;     /* 0 = unbounded (the reference has no bound)      */
+} fbp_config;
+
+typedef struct fbp_stats {
+    uint32_t n_passes;
+    uint32_t reserved;
+    uint64_t n_spikes;
+    uint64_t windows_evaluated;   /* sum over passes of W_p                     */
+    uint64_t overlap_windows;     /* sum over passes of O_p                     */
+    uint64_t rescored_positions;  /* low-precision paths: float32 re-scores     */
+    uint64_t rescored_combos;     /* screened overlap: direct re-scores         */
+    double   ms_total;            /* device time of the last run (HIP events)   */
+    double   ms_correlation;
+    double   ms_overlap;
+} fbp_stats;
+
+typedef struct fbp_segment fbp_segment;
+
+/* Library version (major*10000 + minor*100 + patch). No reference counterpart. */
+int fbp_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *fbp_last_error(void);
+
+/* Number of visible GPUs; FBP_ENODEV when there is none.
+ * Replaces the device search of bpp.py:219-230. */
+int fbp_device_count(int *count);
+
+/* Total and free bytes of HBM on a device.
+ * Replaces GLOBAL_MEM_SIZE, bpp.py:278 (input of the chunk planner). */
+int fbp_device_memory(int device, uint64_t *total_bytes, uint64_t *free_bytes);
+
+/* Device-resident state of one chunk: voltage/residual, likelihood buffer,
+ * per-window pass state and spike lists (bpp.py:355-423). */
+int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out);
+void fbp_segment_destroy(fbp_segment *seg);
+
+/* Upload templates [T][C][N], bias [T] and thresholds [T] from host memory
+ * (bpp.py:355-357, :410).  The library copies them. */
+int fbp_segment_set_templates(fbp_segment *seg, const float *templates, const float *bias,
+                              const float *thresholds);
+
+/* Upload the chunk voltage from host memory; row_stride is the distance in
+ * floats between channel rows (>= K), so a slice of a longer recording needs
+ * no host copy (bpp.py:374-377, :386). */
+int fbp_segment_set_voltage(fbp_segment *seg, const float *voltage, uint64_t row_stride);
+
+/* Same from caller-owned device memory (device-to-device). */
+int fbp_segment_set_voltage_device(fbp_segment *seg, const float *d_voltage, uint64_t row_stride);
+
+/* Run the pass loop to convergence (bpp.py:532-736).  The voltage buffer is
+ * turned into the residual in place.  `stream` is a hipStream_t or NULL for
+ * the segment's own stream.  Blocks until done. */
+int fbp_segment_run(fbp_segment *seg, void *stream);
+
+/* Results of the last run. */
+int fbp_segment_num_spikes(const fbp_segment *seg, uint64_t *n);
+/* Onsets and labels in record order (pass, then ascending window). */
+int fbp_segment_get_spikes(fbp_segment *seg, uint32_t *onsets, uint32_t *labels);
+/* Adjusted clips [n][C*N] = residual + template (binary_pursuit.cl:1088-1130,
+ * bpp.py:771-809). */
+int fbp_segment_get_clips(fbp_segment *seg, float *clips);
+/* Residual voltage [C][K] (not returned by the reference, bpp.py:822). */
+int fbp_segment_get_residual(fbp_segment *seg, float *residual);
+/* Likelihood buffer [T][K] as it stands after the run (parity checks). */
+int fbp_segment_get_likelihood(fbp_segment *seg, float *likelihood);
+int fbp_segment_get_stats(const fbp_segment *seg, fbp_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FBP_H */

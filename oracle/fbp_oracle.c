@@ -1,0 +1,507 @@
+/*
+ * fbp_oracle.c -- CPU oracle for the Full Binary Pursuit hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is compiled into oracle/_build/
+ * libfbp_oracle.so and is loaded by tests/, by __graft_entry__.smoke() and by
+ * the cpu_baseline leg of bench.py, always as the checker.  The product path
+ * (spikesorting_fullpursuit_amd) never imports, links or executes it.
+ *
+ * It restates, in plain C, the algorithm of the reference's OpenCL kernels
+ * (spikesorting_fullpursuit/kernels/binary_pursuit.cl, "bp.cl" below) and of
+ * the per-chunk pass loop of the reference host driver
+ * (spikesorting_fullpursuit/parallel/binary_pursuit_parallel.py, "bpp.py").
+ * Each function names the lines it follows.
+ *
+ * Floating point follows the reference's arithmetic exactly as the OpenCL
+ * compiler would contract it under the default FP_CONTRACT ON: every
+ * `acc = acc + a * b` is one fused multiply-add (fmaf), evaluated in the
+ * reference's loop order (channel-major, sample-minor).  The `0.5` literal in
+ * the overlap score (bp.cl:629) is a double, so that subtraction is done in
+ * double and rounded once to float.  Build with -ffp-contract=off so that no
+ * other expression is contracted.
+ *
+ * Where the reference races (documented at bp.cl:35-43, :852, :1019-1022 and
+ * the cross-work-group atomic append at :1047-1061) this oracle fixes one
+ * deterministic outcome, the one a single lock-stepped wavefront produces:
+ *   - check_overlap_reassignments reads overlap flags of window id-2 as they
+ *     stood after parse_overlap_recheck_indices (snapshot);
+ *   - in binary_pursuit every add (and its flag writes) happens before any
+ *     pruning, and pruning reads neighbour flags as they stood after the adds;
+ *   - spikes are recorded in ascending window order within a pass;
+ *   - compute_residual subtracts, at every sample, the templates of the pass's
+ *     spikes in ascending (onset, label, record position) order, with no lost
+ *     update.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+    uint32_t n_chans;      /* C: channels packed in voltage and templates        */
+    uint32_t n_samples;    /* K: samples per channel of this chunk              */
+    uint32_t n_templates;  /* T                                                  */
+    uint32_t template_len; /* N: samples per channel of a template              */
+    uint32_t max_shift;    /* M: n_max_shift_inds                                */
+    uint32_t do_overlap;   /* sort_info['do_overlap_recheck']                    */
+    uint32_t reduce_wg;    /* work-group width of the overlap reduction (2^k)    */
+    uint32_t max_passes;   /* safety cap on passes (0 = no cap)                  */
+} fbo_params;
+
+enum { FBO_OK = 0, FBO_EINVAL = -1, FBO_ENOMEM = -2, FBO_ECAP = -3, FBO_EPASSES = -4 };
+
+int fbo_version(void) { return 1; }
+
+int fbo_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ---------------------------------------------------------------------------
+ * Delta likelihood of template t with onset i.
+ * bp.cl:216-252 (compute_maximum_likelihood): 0 when i + N >= K; otherwise the
+ * accumulator starts at bias[t] (= float32(-0.5*template_SS[t]), bpp.py:199)
+ * and adds Tm[t,c,j]*V[c,i+j] channel-major, sample-minor.
+ * ------------------------------------------------------------------------- */
+static void likelihood_window(const fbo_params *p, const float *V, const float *Tm,
+                              const float *bias, uint32_t w, float *L)
+{
+    const uint32_t C = p->n_chans, K = p->n_samples, T = p->n_templates, N = p->template_len;
+    /* bp.cl:286-288: window w covers [wN, min((w+1)N, K)). */
+    const uint64_t start = (uint64_t)w * N;
+    uint64_t stop = start + N;
+    if (stop > K) stop = K;
+    if (start >= stop) return;
+    const uint32_t cnt = (uint32_t)(stop - start);
+    float acc[512];
+    for (uint32_t t = 0; t < T; t++) {
+        const float *tt = Tm + (uint64_t)t * C * N;
+        for (uint32_t base = 0; base < cnt; base += 512) {
+            uint32_t n = cnt - base < 512 ? cnt - base : 512;
+            /* positions with i + N >= K are 0 (bp.cl:231-234) */
+            uint32_t nvalid = 0;
+            for (uint32_t r = 0; r < n; r++) {
+                uint64_t i = start + base + r;
+                if (i + N < K) nvalid = r + 1;
+            }
+            for (uint32_t r = 0; r < nvalid; r++) acc[r] = bias[t];
+            for (uint32_t c = 0; c < C; c++) {
+                const float *vc = V + (uint64_t)c * K + start + base;
+                const float *tc = tt + (uint64_t)c * N;
+                for (uint32_t j = 0; j < N; j++) {
+                    const float tv = tc[j];
+                    const float *vj = vc + j;
+                    for (uint32_t r = 0; r < nvalid; r++) acc[r] = fmaf(tv, vj[r], acc[r]);
+                }
+            }
+            for (uint32_t r = 0; r < n; r++)
+                L[(uint64_t)t * K + start + base + r] = r < nvalid ? acc[r] : 0.0f;
+        }
+    }
+}
+
+/* bp.cl:255-300 (compute_full_likelihood) over a list of windows. */
+int fbo_likelihood(const fbo_params *p, const float *V, const float *Tm, const float *bias,
+                   const uint32_t *windows, uint64_t n_windows, float *L)
+{
+    if (!p || !V || !Tm || !bias || !L || p->template_len == 0 || p->n_chans == 0) return FBO_EINVAL;
+    int64_t nw = (int64_t)n_windows;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < nw; k++) likelihood_window(p, V, Tm, bias, windows[k], L);
+    return FBO_OK;
+}
+
+/* The same sums in double precision (no reference counterpart): used to state
+ * the error of the float32 likelihood against exact arithmetic. */
+int fbo_likelihood_f64(const fbo_params *p, const float *V, const float *Tm, const float *bias_sumsq,
+                       const uint32_t *windows, uint64_t n_windows, double *L)
+{
+    const uint32_t C = p->n_chans, K = p->n_samples, T = p->n_templates, N = p->template_len;
+    int64_t nw = (int64_t)n_windows;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < nw; k++) {
+        uint64_t start = (uint64_t)windows[k] * N, stop = start + N;
+        if (stop > K) stop = K;
+        for (uint32_t t = 0; t < T; t++)
+            for (uint64_t i = start; i < stop; i++) {
+                double acc = 0.0;
+                if (i + N < K) {
+                    acc = -0.5 * (double)bias_sumsq[t];
+                    for (uint32_t c = 0; c < C; c++)
+                        for (uint32_t j = 0; j < N; j++)
+                            acc += (double)Tm[((uint64_t)t * C + c) * N + j] * (double)V[(uint64_t)c * K + i + j];
+                }
+                L[(uint64_t)t * K + i] = acc;
+            }
+    }
+    return FBO_OK;
+}
+
+/* ---------------------------------------------------------------------------
+ * Per-chunk state, mirroring the device buffers of bpp.py:386-423.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    float *L;            /* [T][K]   full_likelihood_function, zero at start (bpp.py:387) */
+    float *best_L;       /* [W]      best_spike_likelihoods */
+    uint32_t *best_lab;  /* [W]      best_spike_labels       */
+    uint32_t *best_idx;  /* [W]      best_spike_indices      */
+    uint32_t *ov_idx;    /* [W]      overlap_best_spike_indices */
+    uint32_t *ov_lab;    /* [W]      overlap_best_spike_labels  */
+    uint8_t *next_check; /* [W]      check_window_on_next_pass  */
+    uint8_t *ov;         /* [W]      overlap_recheck            */
+    uint8_t *snap;       /* [W]      scratch snapshot of flags  */
+    uint32_t *windows;   /* [W]      window_indices             */
+    uint32_t *ov_list;   /* [W]      overlap_window_indices     */
+    uint8_t *has_spike;  /* [W]      per window of the pass list */
+} fbo_state;
+
+/* bp.cl:350-441 (compute_template_maximum_likelihood) for one window, with the
+ * host's T sequential launches (bpp.py:551-562) folded into one loop over t. */
+static void argmax_window(const fbo_params *p, fbo_state *s, uint32_t id)
+{
+    const uint32_t K = p->n_samples, N = p->template_len, T = p->n_templates;
+    const uint64_t start_w = (uint64_t)id * N, end_w = (uint64_t)(id + 1) * N;
+    /* bp.cl:380-383: windows ending at or after K-N are never processed. */
+    if (end_w >= (uint64_t)(K - N)) return;
+    const uint64_t start = N > start_w ? 0 : start_w - N;
+    const uint64_t stop = end_w + N > K ? K : end_w + N;
+    float best = 0.0f;                 /* reset at template 0 (bp.cl:389-393) */
+    uint32_t lab = s->best_lab[id];    /* label and index stay stale until beaten */
+    uint32_t idx = s->best_idx[id];
+    int check = 0;
+    for (uint32_t t = 0; t < T; t++) {
+        const float *Lt = s->L + (uint64_t)t * K;
+        for (uint64_t i = start; i < stop; i++) {
+            const float v = Lt[i];
+            if (v > best) { best = v; lab = t; idx = (uint32_t)i; }          /* bp.cl:402-408 */
+            if (v > 0.0f && i >= start_w && i < end_w) check = 1;            /* bp.cl:409-417 */
+        }
+    }
+    if (check) s->next_check[id] = 1;                                         /* bp.cl:419-422 */
+    s->ov[id] = (best > 0.0f && idx >= start_w && idx < end_w) ? 1 : 0;       /* bp.cl:423-433 */
+    s->best_L[id] = best;
+    s->best_lab[id] = lab;
+    s->best_idx[id] = idx;
+    s->ov_idx[id] = idx;                                                      /* bp.cl:435-439 */
+    s->ov_lab[id] = lab;
+}
+
+/* Score of one (template t, fixed shift f, template shift s) combination for a
+ * window whose best spike is (lab, bidx); bp.cl:578-632.  Returns 0 when the
+ * work item is skipped (bp.cl:516-577). */
+static float overlap_score(const fbo_params *p, const float *V, const float *Tm, const float *thr,
+                           const float *L, uint32_t lab, uint32_t bidx, uint32_t t, uint32_t f,
+                           uint32_t sh)
+{
+    const uint32_t C = p->n_chans, K = p->n_samples, N = p->template_len, M = p->max_shift;
+    const float Lb = L[(uint64_t)lab * K + bidx];
+    if (Lb < thr[lab]) return 0.0f;                                           /* bp.cl:518-522 */
+    if (bidx >= K - N) return 0.0f;                                           /* bp.cl:524-527 */
+    if ((uint64_t)f + bidx < M || (uint64_t)f + bidx >= (uint64_t)K - N + M) return 0.0f;   /* :534 */
+    if ((uint64_t)sh + bidx < M || (uint64_t)sh + bidx >= (uint64_t)K - N + M) return 0.0f; /* :538 */
+    if (f > sh && f - sh > N) return 0.0f;                                    /* bp.cl:543-556 */
+    if (sh > f && sh - f > N) return 0.0f;
+    const uint32_t abs_f = bidx + f - M, abs_s = bidx + sh - M;
+    const float Lf = L[(uint64_t)lab * K + abs_f];
+    const float Ls = L[(uint64_t)t * K + abs_s];
+    if (Lf <= 0.0f && Ls <= 0.0f) return 0.0f;                               /* bp.cl:572-576 */
+
+    const int32_t fa = (int32_t)bidx - (int32_t)abs_f;  /* = M - f */
+    const int32_t sa = (int32_t)bidx - (int32_t)abs_s;  /* = M - s */
+    const uint32_t ffj = fa < 0 ? (uint32_t)(-fa) : 0;
+    const uint32_t sfj = sa < 0 ? (uint32_t)(-sa) : 0;
+    float acc = 0.0f;
+    for (uint32_t c = 0; c < C; c++) {
+        const float *tf = Tm + ((uint64_t)lab * C + c) * N;
+        const float *ts = Tm + ((uint64_t)t * C + c) * N;
+        const float *vc = V + (uint64_t)c * K + bidx;
+        float ss = 0.0f;
+        for (uint32_t j = 0; j < N; j++) {
+            float fv = 0.0f, sv = 0.0f;
+            if (j >= ffj && (int64_t)fa + j < (int64_t)N) fv = tf[fa + (int32_t)j];
+            if (j >= sfj && (int64_t)sa + j < (int64_t)N) sv = ts[sa + (int32_t)j];
+            const float sum = fv + sv;
+            ss = fmaf(sum, sum, ss);                                          /* bp.cl:621 */
+            acc = fmaf(sum, vc[j], acc);                                      /* bp.cl:622 */
+        }
+        if (ss > 0.0f) acc = (float)((double)acc - 0.5 * (double)ss);         /* bp.cl:627-630 */
+    }
+    return acc;
+}
+
+/* bp.cl:444-654 (overlap_recheck_indices, one window's n_local_ID groups with
+ * the max_local_reduction tree of bp.cl:167-186) followed by bp.cl:657-815
+ * (parse_overlap_recheck_indices) for that window. */
+static void overlap_window(const fbo_params *p, fbo_state *s, const float *V, const float *Tm,
+                           const float *thr, uint32_t id, float *vals, uint32_t *ids)
+{
+    const uint32_t T = p->n_templates, M = p->max_shift, WG = p->reduce_wg, K = p->n_samples;
+    const uint64_t S = 2ull * M + 1;
+    const uint64_t items = S * S * T;
+    const uint64_t nl = (items - 1) / WG + 1;
+    const uint32_t lab = s->best_lab[id], bidx = s->best_idx[id];
+
+    /* parse: running maximum over the groups with strict > (bp.cl:690-706) */
+    const float best_L = s->best_L[id];
+    float bg = best_L;
+    uint64_t bsid = 0;
+    for (uint64_t g = 0; g < nl; g++) {
+        for (uint32_t l = 0; l < WG; l++) {
+            const uint64_t q = (g * WG + l) % items;   /* offset_global_id modulo items */
+            const uint32_t t = (uint32_t)(q % T);
+            const uint32_t f = (uint32_t)((q / T) % S);
+            const uint32_t sh = (uint32_t)((q / (S * T)) % S);
+            vals[l] = overlap_score(p, V, Tm, thr, s->L, lab, bidx, t, f, sh);
+            ids[l] = l;
+        }
+        for (uint32_t na = WG >> 1; na > 0; na >>= 1)                        /* bp.cl:172-185 */
+            for (uint32_t l = 0; l < na; l++)
+                if (vals[l + na] > vals[l]) { vals[l] = vals[l + na]; ids[l] = ids[l + na]; }
+        const uint64_t gid = ids[0] + (uint64_t)WG * g;                      /* bp.cl:650-651 */
+        if (vals[0] > bg) { bg = vals[0]; bsid = gid; }
+    }
+    if (bg <= best_L || bg <= 0.0f) { s->ov[id] = 0; return; }              /* bp.cl:707-711 */
+    if (bsid % (nl * WG) >= items) { s->ov[id] = 0; return; }                /* bp.cl:714-718 */
+    const uint32_t t = (uint32_t)(bsid % T);
+    const uint32_t f = (uint32_t)((bsid / T) % S);
+    const uint32_t sh = (uint32_t)((bsid / (S * T)) % S);
+    const uint32_t abs_f = bidx + f - M, abs_s = bidx + sh - M;
+    const float Lm = s->L[(uint64_t)lab * K + abs_f];
+    const float Lo = s->L[(uint64_t)t * K + abs_s];
+    if (bg < Lm || bg < Lo) { s->ov[id] = 0; return; }                       /* bp.cl:733-738 */
+    if (Lm <= 0.0f || Lo <= 0.0f) { s->ov[id] = 0; return; }                 /* bp.cl:739-744 */
+    if (Lm >= Lo && Lm > thr[lab]) {                                          /* bp.cl:752-759 */
+        s->best_L[id] = Lm; s->ov_lab[id] = lab; s->ov_idx[id] = abs_f;
+    } else if (Lo > Lm && Lo > thr[t]) {                                      /* bp.cl:760-767 */
+        s->best_L[id] = Lo; s->ov_lab[id] = t; s->ov_idx[id] = abs_s;
+    } else if (Lm > thr[lab]) {                                               /* bp.cl:768-775 */
+        s->best_L[id] = Lm; s->ov_lab[id] = lab; s->ov_idx[id] = abs_f;
+    } else {                                                                  /* bp.cl:806-814 */
+        s->ov_idx[id] = s->best_idx[id]; s->ov_lab[id] = s->best_lab[id]; s->ov[id] = 0;
+    }
+}
+
+/* bp.cl:822-875 (check_overlap_reassignments) over the overlap list, reading
+ * the flags of window id-2 from the snapshot taken after parsing. */
+static void check_reassignments(const fbo_params *p, fbo_state *s, uint64_t n_ov)
+{
+    const uint32_t K = p->n_samples, N = p->template_len;
+    const uint32_t n_floor = K / N;
+    memcpy(s->snap, s->ov, ((uint64_t)K + N - 1) / N);
+    for (uint64_t k = 0; k < n_ov; k++) {
+        const uint32_t id = s->ov_list[k];
+        const uint64_t start_w = (uint64_t)id * N, end_w = (uint64_t)(id + 1) * N;
+        if (s->snap[id] == 0) continue;
+        s->best_idx[id] = s->ov_idx[id];
+        s->best_lab[id] = s->ov_lab[id];
+        if (s->ov_idx[id] < start_w && id > 1) {
+            if (s->snap[id - 2] == 1) {
+                s->ov[id] = 0;
+                s->best_idx[id] = 0;
+                s->next_check[id - 1] = 2;
+                s->next_check[id] = 2;
+                s->next_check[id + 1] = 2;
+            } else {
+                s->next_check[id - 2] = 2;
+            }
+        }
+        if (s->ov_idx[id] >= end_w && id + 2 < n_floor) s->next_check[id + 2] = 2;
+    }
+}
+
+/* bp.cl:918-1063 (binary_pursuit): the add decision, the flag writes of an add,
+ * then (after every add) the pruning of flags 1/2.  Returns number added. */
+static uint64_t pursuit_windows(const fbo_params *p, fbo_state *s, const float *thr, uint64_t nw,
+                                uint32_t *out_idx, uint32_t *out_lab)
+{
+    const uint32_t K = p->n_samples, N = p->template_len;
+    const uint32_t n_floor = K / N;
+    uint64_t added = 0;
+    for (uint64_t k = 0; k < nw; k++) {
+        const uint32_t id = s->windows[k];
+        const uint64_t start_w = (uint64_t)id * N, end_w = (uint64_t)(id + 1) * N;
+        float ml = 0.0f; uint32_t lab = 0, idx = 0;
+        if (end_w < (uint64_t)(K - N) && p->n_chans > 0) {                    /* bp.cl:971-976 */
+            ml = s->best_L[id]; lab = s->best_lab[id]; idx = s->best_idx[id];
+        }
+        uint8_t hs = 0;
+        if (ml > thr[lab] || s->ov[id] == 1) {                                /* bp.cl:982-999 */
+            if ((idx >= start_w && idx < end_w) || s->ov[id] == 1) {
+                hs = 1;
+                s->next_check[id] = 3;
+                if (id > 0) s->next_check[id - 1] = 3;
+                if (id + 1 < n_floor) s->next_check[id + 1] = 3;
+            }
+        }
+        s->has_spike[k] = hs;
+        if (hs) { out_idx[added] = idx; out_lab[added] = lab; added++; }
+    }
+    /* pruning (bp.cl:1009-1045), neighbours read from the post-add snapshot */
+    memcpy(s->snap, s->next_check, ((uint64_t)K + N - 1) / N);
+    for (uint64_t k = 0; k < nw; k++) {
+        const uint32_t id = s->windows[k];
+        const uint64_t start_w = (uint64_t)id * N, end_w = (uint64_t)(id + 1) * N;
+        if (!(end_w < (uint64_t)(K - N) && p->n_chans > 0) || s->has_spike[k]) continue;
+        const float ml = s->best_L[id];
+        const uint32_t lab = s->best_lab[id], idx = s->best_idx[id];
+        uint8_t own = s->snap[id];
+        if (idx < start_w && own == 1 && id > 0 && s->snap[id - 1] == 0) own = 0;
+        if (idx >= end_w && own == 1 && id + 1 < n_floor && s->snap[id + 1] == 0) own = 0;
+        if (ml <= thr[lab] && own < 3) own = 0;
+        s->next_check[id] = own;
+    }
+    return added;
+}
+
+typedef struct { uint32_t idx, lab, pos; } spike_key;
+
+static int cmp_spike(const void *a, const void *b)
+{
+    const spike_key *x = (const spike_key *)a, *y = (const spike_key *)b;
+    if (x->idx != y->idx) return x->idx < y->idx ? -1 : 1;
+    if (x->lab != y->lab) return x->lab < y->lab ? -1 : 1;
+    return x->pos < y->pos ? -1 : (x->pos > y->pos);
+}
+
+/* bp.cl:57-102 (compute_residual), applied in ascending (onset, label) order. */
+int fbo_residual(const fbo_params *p, float *V, const float *Tm, const uint32_t *idx,
+                 const uint32_t *lab, uint64_t n)
+{
+    const uint32_t C = p->n_chans, K = p->n_samples, T = p->n_templates, N = p->template_len;
+    spike_key *keys = (spike_key *)malloc((n ? n : 1) * sizeof(spike_key));
+    if (!keys) return FBO_ENOMEM;
+    for (uint64_t k = 0; k < n; k++) { keys[k].idx = idx[k]; keys[k].lab = lab[k]; keys[k].pos = (uint32_t)k; }
+    qsort(keys, n, sizeof(spike_key), cmp_spike);
+    for (uint64_t k = 0; k < n; k++) {
+        const uint32_t si = keys[k].idx, sl = keys[k].lab;
+        if (sl >= T) continue;                                                /* bp.cl:78-81 */
+        for (uint32_t c = 0; c < C; c++) {
+            float *vc = V + (uint64_t)c * K;
+            const float *tc = Tm + ((uint64_t)sl * C + c) * N;
+            for (uint32_t j = 0; j < N; j++) {
+                if ((uint64_t)si + j >= K) break;                              /* bp.cl:93-96 */
+                vc[si + j] -= tc[j];
+            }
+        }
+    }
+    free(keys);
+    return FBO_OK;
+}
+
+/* bp.cl:1088-1130 (get_adjusted_clips): clip = residual + template, 0 past K. */
+int fbo_adjusted_clips(const fbo_params *p, const float *V, const float *Tm, const uint32_t *idx,
+                       const uint32_t *lab, uint64_t n, float *clips)
+{
+    const uint32_t C = p->n_chans, K = p->n_samples, T = p->n_templates, N = p->template_len;
+    memset(clips, 0, n * (uint64_t)C * N * sizeof(float));
+    for (uint64_t k = 0; k < n; k++) {
+        if (lab[k] >= T) continue;
+        for (uint32_t c = 0; c < C; c++)
+            for (uint32_t j = 0; j < N; j++) {
+                if ((uint64_t)idx[k] + j >= K) break;
+                clips[(k * C + c) * N + j] = V[(uint64_t)c * K + idx[k] + j] + Tm[((uint64_t)lab[k] * C + c) * N + j];
+            }
+    }
+    return FBO_OK;
+}
+
+/* ---------------------------------------------------------------------------
+ * One chunk: the pass loop of bpp.py:524-736.
+ * voltage [C][K] is overwritten by the residual.  Spikes are written in record
+ * order (pass, then ascending window) to out_idx/out_lab (capacity cap).
+ * If pass_counts is given it receives the spikes added in each pass.
+ * If L_out is given it receives the final likelihood buffer [T][K].
+ * ------------------------------------------------------------------------- */
+int fbo_pursuit_chunk(const fbo_params *p, float *V, const float *Tm, const float *bias,
+                      const float *thr, uint32_t *out_idx, uint32_t *out_lab, uint64_t cap,
+                      uint64_t *n_out, uint32_t *n_passes, uint32_t *pass_counts,
+                      uint32_t pass_counts_cap, float *L_out)
+{
+    if (!p || !V || !Tm || !bias || !thr || !n_out) return FBO_EINVAL;
+    const uint32_t C = p->n_chans, K = p->n_samples, T = p->n_templates, N = p->template_len;
+    if (C == 0 || T == 0 || N == 0 || K <= N) return FBO_EINVAL;
+    if (p->reduce_wg == 0 || (p->reduce_wg & (p->reduce_wg - 1))) return FBO_EINVAL;
+    const uint64_t W = ((uint64_t)K + N - 1) / N;                             /* bpp.py:402 */
+    fbo_state s;
+    memset(&s, 0, sizeof(s));
+    s.L = (float *)calloc((uint64_t)T * K, sizeof(float));
+    s.best_L = (float *)calloc(W, sizeof(float));
+    s.best_lab = (uint32_t *)calloc(W, 4);
+    s.best_idx = (uint32_t *)calloc(W, 4);
+    s.ov_idx = (uint32_t *)calloc(W, 4);
+    s.ov_lab = (uint32_t *)calloc(W, 4);
+    s.next_check = (uint8_t *)calloc(W + 2, 1);
+    s.ov = (uint8_t *)calloc(W + 2, 1);
+    s.snap = (uint8_t *)calloc(W + 2, 1);
+    s.windows = (uint32_t *)malloc(W * 4);
+    s.ov_list = (uint32_t *)malloc(W * 4);
+    s.has_spike = (uint8_t *)calloc(W, 1);
+    uint32_t *pidx = (uint32_t *)malloc(W * 4), *plab = (uint32_t *)malloc(W * 4);
+    const uint32_t WG = p->reduce_wg;
+    float *vals = NULL; uint32_t *ids = NULL;
+    int rc = FBO_OK;
+    if (!s.L || !s.best_L || !s.best_lab || !s.best_idx || !s.ov_idx || !s.ov_lab || !s.next_check ||
+        !s.ov || !s.snap || !s.windows || !s.ov_list || !s.has_spike || !pidx || !plab) {
+        rc = FBO_ENOMEM;
+        goto done;
+    }
+    for (uint64_t w = 0; w < W; w++) s.windows[w] = (uint32_t)w;              /* bpp.py:418 */
+    uint64_t nw = W, total = 0;
+    uint32_t passes = 0;
+    for (;;) {
+        if (p->max_passes && passes >= p->max_passes) { rc = FBO_EPASSES; break; }
+        passes++;
+        fbo_likelihood(p, V, Tm, bias, s.windows, nw, s.L);                    /* bpp.py:538-549 */
+        for (uint64_t k = 0; k < nw; k++) argmax_window(p, &s, s.windows[k]);  /* bpp.py:551-562 */
+        uint64_t n_ov = 0;                                                     /* bpp.py:566-569 */
+        for (uint64_t w = 0; w < W; w++) if (s.ov[w]) s.ov_list[n_ov++] = (uint32_t)w;
+        if (n_ov > 0 && p->do_overlap) {                                       /* bpp.py:570-661 */
+            int nt = fbo_num_threads();
+            vals = (float *)malloc((uint64_t)WG * nt * sizeof(float));
+            ids = (uint32_t *)malloc((uint64_t)WG * nt * sizeof(uint32_t));
+            if (!vals || !ids) { rc = FBO_ENOMEM; break; }
+            int64_t no = (int64_t)n_ov;
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int64_t k = 0; k < no; k++) {
+                int th = 0;
+#ifdef _OPENMP
+                th = omp_get_thread_num();
+#endif
+                overlap_window(p, &s, V, Tm, thr, s.ov_list[k], vals + (uint64_t)th * WG, ids + (uint64_t)th * WG);
+            }
+            free(vals); free(ids); vals = NULL; ids = NULL;
+            check_reassignments(p, &s, n_ov);
+        }
+        const uint64_t added = pursuit_windows(p, &s, thr, nw, pidx, plab);    /* bpp.py:663-671 */
+        if (pass_counts && passes <= pass_counts_cap) pass_counts[passes - 1] = (uint32_t)added;
+        if (added == 0) break;                                                 /* bpp.py:675-676 */
+        memset(s.ov, 0, W);                                                    /* bpp.py:679-680 */
+        nw = 0;                                                                /* bpp.py:683-689 */
+        for (uint64_t w = 0; w < W; w++) if (s.next_check[w]) s.windows[nw++] = (uint32_t)w;
+        memset(s.next_check, 0, W);                                            /* bpp.py:699-700 */
+        if (out_idx && out_lab) {
+            if (total + added > cap) { rc = FBO_ECAP; break; }
+            memcpy(out_idx + total, pidx, added * 4);
+            memcpy(out_lab + total, plab, added * 4);
+        }
+        total += added;
+        rc = fbo_residual(p, V, Tm, pidx, plab, added);                        /* bpp.py:716-731 */
+        if (rc) break;
+    }
+    *n_out = total;
+    if (n_passes) *n_passes = passes;
+    if (L_out) memcpy(L_out, s.L, (uint64_t)T * K * sizeof(float));
+done:
+    free(s.L); free(s.best_L); free(s.best_lab); free(s.best_idx); free(s.ov_idx); free(s.ov_lab);
+    free(s.next_check); free(s.ov); free(s.snap); free(s.windows); free(s.ov_list); free(s.has_spike);
+    free(pidx); free(plab); free(vals); free(ids);
+    return rc;
+}

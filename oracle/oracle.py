@@ -1,0 +1,228 @@
+"""CPU oracle for the Full Binary Pursuit hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, and only as the checker.  The product package never imports it.
+
+`pursuit_chunk` drives the C restatement in fbp_oracle.c (one chunk, the pass
+loop of spikesorting_fullpursuit/parallel/binary_pursuit_parallel.py:524-736
+and the kernels of spikesorting_fullpursuit/kernels/binary_pursuit.cl).
+`binary_pursuit` restates the host driver around it (chunk onsets
+binary_pursuit_parallel.py:359-368, per-chunk slicing :371-377, output
+assembly :814-845) for a caller-fixed chunk length.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libfbp_oracle.so")
+_lib = None
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("n_chans", ctypes.c_uint32),
+        ("n_samples", ctypes.c_uint32),
+        ("n_templates", ctypes.c_uint32),
+        ("template_len", ctypes.c_uint32),
+        ("max_shift", ctypes.c_uint32),
+        ("do_overlap", ctypes.c_uint32),
+        ("reduce_wg", ctypes.c_uint32),
+        ("max_passes", ctypes.c_uint32),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        lib = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.POINTER
+        f32p, u32p, u64p, f64p = P(ctypes.c_float), P(ctypes.c_uint32), P(ctypes.c_uint64), P(ctypes.c_double)
+        lib.fbo_pursuit_chunk.argtypes = [P(Params), f32p, f32p, f32p, f32p, u32p, u32p, ctypes.c_uint64,
+                                          u64p, u32p, u32p, ctypes.c_uint32, f32p]
+        lib.fbo_pursuit_chunk.restype = ctypes.c_int
+        lib.fbo_likelihood.argtypes = [P(Params), f32p, f32p, f32p, u32p, ctypes.c_uint64, f32p]
+        lib.fbo_likelihood.restype = ctypes.c_int
+        lib.fbo_likelihood_f64.argtypes = [P(Params), f32p, f32p, f32p, u32p, ctypes.c_uint64, f64p]
+        lib.fbo_likelihood_f64.restype = ctypes.c_int
+        lib.fbo_residual.argtypes = [P(Params), f32p, f32p, u32p, u32p, ctypes.c_uint64]
+        lib.fbo_residual.restype = ctypes.c_int
+        lib.fbo_adjusted_clips.argtypes = [P(Params), f32p, f32p, u32p, u32p, ctypes.c_uint64, f32p]
+        lib.fbo_adjusted_clips.restype = ctypes.c_int
+        lib.fbo_num_threads.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def num_threads():
+    return int(load().fbo_num_threads())
+
+
+def make_params(C, K, T, N, M=0, do_overlap=True, reduce_wg=256, max_passes=0):
+    return Params(C, K, T, N, M, 1 if do_overlap else 0, reduce_wg, max_passes)
+
+
+def likelihood(V, templates, bias, N, windows=None):
+    """Likelihood buffer [T][K] for the listed windows (all windows by default)."""
+    V = np.ascontiguousarray(V, dtype=np.float32)
+    C, K = V.shape
+    tm = np.ascontiguousarray(templates, dtype=np.float32).reshape(-1)
+    T = tm.size // (C * N)
+    bias = np.ascontiguousarray(bias, dtype=np.float32)
+    if windows is None:
+        windows = np.arange((K + N - 1) // N, dtype=np.uint32)
+    windows = np.ascontiguousarray(windows, dtype=np.uint32)
+    L = np.zeros((T, K), dtype=np.float32)
+    p = make_params(C, K, T, N)
+    rc = load().fbo_likelihood(ctypes.byref(p), _ptr(V, ctypes.c_float), _ptr(tm, ctypes.c_float),
+                               _ptr(bias, ctypes.c_float), _ptr(windows, ctypes.c_uint32),
+                               windows.size, _ptr(L, ctypes.c_float))
+    assert rc == 0, rc
+    return L
+
+
+def likelihood_f64(V, templates, template_ss, N, windows=None):
+    """Same sums as `likelihood`, accumulated in float64 from -0.5*template_SS."""
+    V = np.ascontiguousarray(V, dtype=np.float32)
+    C, K = V.shape
+    tm = np.ascontiguousarray(templates, dtype=np.float32).reshape(-1)
+    T = tm.size // (C * N)
+    ss = np.ascontiguousarray(template_ss, dtype=np.float32)
+    if windows is None:
+        windows = np.arange((K + N - 1) // N, dtype=np.uint32)
+    windows = np.ascontiguousarray(windows, dtype=np.uint32)
+    L = np.zeros((T, K), dtype=np.float64)
+    p = make_params(C, K, T, N)
+    rc = load().fbo_likelihood_f64(ctypes.byref(p), _ptr(V, ctypes.c_float), _ptr(tm, ctypes.c_float),
+                                   _ptr(ss, ctypes.c_float), _ptr(windows, ctypes.c_uint32),
+                                   windows.size, _ptr(L, ctypes.c_double))
+    assert rc == 0, rc
+    return L
+
+
+def adjusted_clips(residual, templates, N, idx, lab):
+    residual = np.ascontiguousarray(residual, dtype=np.float32)
+    C, K = residual.shape
+    tm = np.ascontiguousarray(templates, dtype=np.float32).reshape(-1)
+    T = tm.size // (C * N)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    lab = np.ascontiguousarray(lab, dtype=np.uint32)
+    clips = np.zeros((idx.size, C * N), dtype=np.float32)
+    p = make_params(C, K, T, N)
+    rc = load().fbo_adjusted_clips(ctypes.byref(p), _ptr(residual, ctypes.c_float), _ptr(tm, ctypes.c_float),
+                                   _ptr(idx, ctypes.c_uint32), _ptr(lab, ctypes.c_uint32), idx.size,
+                                   _ptr(clips, ctypes.c_float))
+    assert rc == 0, rc
+    return clips
+
+
+def pursuit_chunk(V, templates, bias, thr, N, M, do_overlap=True, reduce_wg=256, max_passes=0,
+                  want_L=False):
+    """Run the pass loop on one chunk.
+
+    Returns dict(idx, lab (uint32, record order), residual [C][K], n_passes,
+    pass_counts, L (final likelihood buffer, if want_L)).
+    """
+    V = np.array(V, dtype=np.float32, order="C", copy=True)
+    C, K = V.shape
+    tm = np.ascontiguousarray(templates, dtype=np.float32).reshape(-1)
+    T = tm.size // (C * N)
+    bias = np.ascontiguousarray(bias, dtype=np.float32)
+    thr = np.ascontiguousarray(thr, dtype=np.float32)
+    W = (K + N - 1) // N
+    cap = max(16, 4 * W)
+    pc_cap = 4096
+    while True:
+        out_idx = np.zeros(cap, dtype=np.uint32)
+        out_lab = np.zeros(cap, dtype=np.uint32)
+        n_out = ctypes.c_uint64(0)
+        n_passes = ctypes.c_uint32(0)
+        pass_counts = np.zeros(pc_cap, dtype=np.uint32)
+        L = np.zeros((T, K), dtype=np.float32) if want_L else None
+        Vw = V.copy()
+        p = make_params(C, K, T, N, M, do_overlap, reduce_wg, max_passes)
+        rc = load().fbo_pursuit_chunk(
+            ctypes.byref(p), _ptr(Vw, ctypes.c_float), _ptr(tm, ctypes.c_float), _ptr(bias, ctypes.c_float),
+            _ptr(thr, ctypes.c_float), _ptr(out_idx, ctypes.c_uint32), _ptr(out_lab, ctypes.c_uint32), cap,
+            ctypes.byref(n_out), ctypes.byref(n_passes), _ptr(pass_counts, ctypes.c_uint32), pc_cap,
+            _ptr(L, ctypes.c_float) if want_L else None)
+        if rc == -3:
+            cap *= 4
+            continue
+        if rc not in (0, -4):
+            raise RuntimeError("oracle failed with code %d" % rc)
+        break
+    n = int(n_out.value)
+    npass = int(n_passes.value)
+    return dict(idx=out_idx[:n].copy(), lab=out_lab[:n].copy(), residual=Vw, n_passes=npass,
+                pass_counts=pass_counts[:min(npass, pc_cap)].copy(), L=L, capped=(rc == -4))
+
+
+def time_window_to_samples(time_window, sampling_rate):
+    """parallel/segment_parallel.py:82-98 (restated)."""
+    tw = list(time_window)
+    if tw[0] > 0:
+        tw[0] *= -1
+    first = min(int(round(tw[0] * sampling_rate)), 0)
+    second = max(int(round(tw[1] * sampling_rate)), 1) + 1
+    return [first, second], tw
+
+
+def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shift_inds,
+                   chunk_len, reduce_wg=256):
+    """Reference host-driver semantics around `pursuit_chunk` for a fixed chunk
+    length (the reference derives it from device memory, bpp.py:276-344).
+    Returns (event_indices, neuron_labels, bp_bool, adjusted_clips, residual)."""
+    n_samples = voltage.shape[1]
+    chan_win, _ = time_window_to_samples(sort_info['clip_width'], sort_info['sampling_rate'])
+    N = int(sort_info['n_samples_per_chan'])
+    templates = separability_metrics['templates']
+    if templates.ndim == 1:
+        templates = templates.reshape(1, -1)
+    templates = np.float32(templates)
+    clip_init = int(abs(chan_win[0]))
+    bias = np.float32(-0.5 * separability_metrics['template_SS'])
+    thr = np.float32(separability_metrics['neuron_lower_thresholds'])
+    M = N // 2 if (n_max_shift_inds is None or n_max_shift_inds < 0) else int(n_max_shift_inds)
+    onsets = []
+    cur = 0
+    while cur < n_samples - N:
+        onsets.append(cur)
+        cur += chunk_len - 3 * N
+    idx_all, lab_all, clips_all, bool_all = [], [], [], []
+    residual = np.array(voltage, dtype=np.float32, copy=True)
+    for k, start in enumerate(onsets):
+        stop = min(n_samples, start + chunk_len)
+        V = np.float32(voltage[:, start:stop])
+        r = pursuit_chunk(V, templates, bias, thr, N, M, sort_info['do_overlap_recheck'], reduce_wg)
+        nxt = onsets[k + 1] if k + 1 < len(onsets) else n_samples
+        residual[:, start:nxt] = r['residual'][:, :nxt - start]
+        if r['idx'].size > 0:
+            if sort_info['get_adjusted_clips']:
+                clips = adjusted_clips(r['residual'], templates, N, r['idx'], r['lab'])
+            else:
+                clips = []
+            idx_all.append(r['idx'].astype(np.int64) + start)
+            lab_all.append(r['lab'])
+            clips_all.append(clips)
+            bool_all.append(np.ones(r['idx'].size, dtype=bool))
+    if idx_all:
+        ev = np.int64(np.hstack(idx_all)) + clip_init
+        lab = np.int64(np.hstack(lab_all))
+        clips = np.vstack(clips_all).astype(v_dtype)
+        bpb = np.hstack(bool_all)
+    else:
+        ev, lab, bpb, clips = [], [], [], []
+    return ev, lab, bpb, clips, residual

@@ -1,0 +1,254 @@
+"""ctypes binding of libfbp.so (include/fbp.h).
+
+The library is loaded lazily, on first use, so that importing the package
+never initialises HIP (the reference imports pyopencl inside binary_pursuit()
+for the same reason: a parent process that forks workers must not hold a GPU
+context, spikesorting_fullpursuit/parallel/binary_pursuit_parallel.py:166-169).
+There is no fallback: if the library is missing or the GPU is absent the
+calls raise.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfbp.so")
+
+FBP_OK = 0
+FBP_EINVAL = -1
+FBP_ENOMEM = -2
+FBP_ENODEV = -3
+FBP_EHIP = -4
+FBP_EPASSES = -5
+FBP_ESTATE = -6
+
+PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2, "fp8": 3}
+OVERLAP_MODES = {"direct": 0, "screened": 1}
+
+# Names every build of libfbp.so must export (checked by the CPU test suite).
+EXPORTS = (
+    "fbp_version", "fbp_last_error", "fbp_device_count", "fbp_device_memory",
+    "fbp_segment_create", "fbp_segment_destroy", "fbp_segment_set_templates",
+    "fbp_segment_set_voltage", "fbp_segment_set_voltage_device", "fbp_segment_run",
+    "fbp_segment_num_spikes", "fbp_segment_get_spikes", "fbp_segment_get_clips",
+    "fbp_segment_get_residual", "fbp_segment_get_likelihood", "fbp_segment_get_stats",
+)
+
+
+class FbpConfig(ctypes.Structure):
+    _fields_ = [
+        ("n_chans", ctypes.c_uint32),
+        ("n_samples", ctypes.c_uint32),
+        ("n_templates", ctypes.c_uint32),
+        ("template_len", ctypes.c_uint32),
+        ("max_shift", ctypes.c_uint32),
+        ("do_overlap", ctypes.c_uint32),
+        ("reduce_wg", ctypes.c_uint32),
+        ("precision", ctypes.c_uint32),
+        ("overlap_mode", ctypes.c_uint32),
+        ("max_passes", ctypes.c_uint32),
+    ]
+
+
+class FbpStats(ctypes.Structure):
+    _fields_ = [
+        ("n_passes", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("n_spikes", ctypes.c_uint64),
+        ("windows_evaluated", ctypes.c_uint64),
+        ("overlap_windows", ctypes.c_uint64),
+        ("rescored_positions", ctypes.c_uint64),
+        ("rescored_combos", ctypes.c_uint64),
+        ("ms_total", ctypes.c_double),
+        ("ms_correlation", ctypes.c_double),
+        ("ms_overlap", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class FbpError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def _declare(lib):
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    f32p = P(ctypes.c_float)
+    u32p = P(ctypes.c_uint32)
+    u64p = P(ctypes.c_uint64)
+    lib.fbp_version.restype = ctypes.c_int
+    lib.fbp_last_error.restype = ctypes.c_char_p
+    lib.fbp_device_count.argtypes = [P(ctypes.c_int)]
+    lib.fbp_device_memory.argtypes = [ctypes.c_int, u64p, u64p]
+    lib.fbp_segment_create.argtypes = [ctypes.c_int, P(FbpConfig), P(vp)]
+    lib.fbp_segment_destroy.argtypes = [vp]
+    lib.fbp_segment_destroy.restype = None
+    lib.fbp_segment_set_templates.argtypes = [vp, f32p, f32p, f32p]
+    lib.fbp_segment_set_voltage.argtypes = [vp, f32p, ctypes.c_uint64]
+    lib.fbp_segment_set_voltage_device.argtypes = [vp, vp, ctypes.c_uint64]
+    lib.fbp_segment_run.argtypes = [vp, vp]
+    lib.fbp_segment_num_spikes.argtypes = [vp, u64p]
+    lib.fbp_segment_get_spikes.argtypes = [vp, u32p, u32p]
+    lib.fbp_segment_get_clips.argtypes = [vp, f32p]
+    lib.fbp_segment_get_residual.argtypes = [vp, f32p]
+    lib.fbp_segment_get_likelihood.argtypes = [vp, f32p]
+    lib.fbp_segment_get_stats.argtypes = [vp, P(FbpStats)]
+    for name in EXPORTS:
+        if name not in ("fbp_last_error", "fbp_segment_destroy"):
+            getattr(lib, name).restype = ctypes.c_int
+
+
+def load():
+    """Load libfbp.so (raises if it is not built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError("libfbp.so is not built (%s); run "
+                                   "`python -m spikesorting_fullpursuit_amd.build`" % LIB_PATH)
+            lib = ctypes.CDLL(LIB_PATH)
+            _declare(lib)
+            _lib = lib
+    return _lib
+
+
+def check(rc, what=""):
+    if rc == FBP_OK:
+        return
+    msg = load().fbp_last_error().decode(errors="replace")
+    if rc == FBP_ENODEV:
+        raise RuntimeError("No GPU available" + (": " + msg if msg else ""))
+    if rc == FBP_EINVAL:
+        raise ValueError(msg or what)
+    if rc == FBP_ENOMEM:
+        raise MemoryError(msg or what)
+    raise FbpError(rc, "%s failed (%d): %s" % (what, rc, msg))
+
+
+def _f32p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _u32p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def device_count():
+    lib = load()
+    n = ctypes.c_int(0)
+    rc = lib.fbp_device_count(ctypes.byref(n))
+    check(rc, "fbp_device_count")
+    return n.value
+
+
+def device_memory(device=0):
+    lib = load()
+    tot, fr = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    check(lib.fbp_device_memory(device, ctypes.byref(tot), ctypes.byref(fr)), "fbp_device_memory")
+    return tot.value, fr.value
+
+
+class Segment:
+    """Device-resident state of one chunk (owner of an fbp_segment*)."""
+
+    def __init__(self, n_chans, n_samples, n_templates, template_len, max_shift, do_overlap=True,
+                 reduce_wg=256, precision="fp32", overlap_mode="direct", max_passes=0, device=0):
+        self._lib = load()
+        self.cfg = FbpConfig(n_chans, n_samples, n_templates, template_len, max_shift,
+                             1 if do_overlap else 0, reduce_wg, PRECISIONS[precision],
+                             OVERLAP_MODES[overlap_mode], max_passes)
+        self._h = ctypes.c_void_p(None)
+        check(self._lib.fbp_segment_create(device, ctypes.byref(self.cfg), ctypes.byref(self._h)),
+              "fbp_segment_create")
+        self.C, self.K, self.T, self.N = n_chans, n_samples, n_templates, template_len
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            self._lib.fbp_segment_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_templates(self, templates, bias, thresholds):
+        tm = np.ascontiguousarray(templates, dtype=np.float32)
+        b = np.ascontiguousarray(bias, dtype=np.float32)
+        th = np.ascontiguousarray(thresholds, dtype=np.float32)
+        assert tm.size == self.T * self.C * self.N and b.size == self.T and th.size == self.T
+        check(self._lib.fbp_segment_set_templates(self._h, _f32p(tm), _f32p(b), _f32p(th)),
+              "fbp_segment_set_templates")
+
+    def set_voltage(self, voltage):
+        """voltage: float32 [C][>=K] host array whose rows are contiguous."""
+        v = voltage
+        if v.dtype != np.float32 or v.strides[1] != 4 or v.shape[0] != self.C or v.shape[1] < self.K:
+            v = np.ascontiguousarray(v[:, :self.K], dtype=np.float32)
+        stride = v.strides[0] // 4
+        check(self._lib.fbp_segment_set_voltage(self._h, _f32p(v), stride), "fbp_segment_set_voltage")
+
+    def set_voltage_device(self, ptr, row_stride):
+        check(self._lib.fbp_segment_set_voltage_device(self._h, ctypes.c_void_p(ptr), row_stride),
+              "fbp_segment_set_voltage_device")
+
+    def run(self, stream=None, allow_cap=False):
+        """Run the pass loop.  With allow_cap, reaching max_passes returns
+        False (results of the passes run stay readable) instead of raising."""
+        rc = self._lib.fbp_segment_run(self._h, ctypes.c_void_p(stream) if stream else None)
+        if allow_cap and rc == FBP_EPASSES:
+            return False
+        check(rc, "fbp_segment_run")
+        return True
+
+    def num_spikes(self):
+        n = ctypes.c_uint64(0)
+        check(self._lib.fbp_segment_num_spikes(self._h, ctypes.byref(n)), "fbp_segment_num_spikes")
+        return n.value
+
+    def spikes(self):
+        n = self.num_spikes()
+        idx = np.zeros(n, dtype=np.uint32)
+        lab = np.zeros(n, dtype=np.uint32)
+        if n:
+            check(self._lib.fbp_segment_get_spikes(self._h, _u32p(idx), _u32p(lab)), "fbp_segment_get_spikes")
+        return idx, lab
+
+    def clips(self):
+        n = self.num_spikes()
+        out = np.zeros((n, self.C * self.N), dtype=np.float32)
+        if n:
+            check(self._lib.fbp_segment_get_clips(self._h, _f32p(out)), "fbp_segment_get_clips")
+        return out
+
+    def residual(self):
+        out = np.zeros((self.C, self.K), dtype=np.float32)
+        check(self._lib.fbp_segment_get_residual(self._h, _f32p(out)), "fbp_segment_get_residual")
+        return out
+
+    def likelihood(self):
+        out = np.zeros((self.T, self.K), dtype=np.float32)
+        check(self._lib.fbp_segment_get_likelihood(self._h, _f32p(out)), "fbp_segment_get_likelihood")
+        return out
+
+    def stats(self):
+        st = FbpStats()
+        check(self._lib.fbp_segment_get_stats(self._h, ctypes.byref(st)), "fbp_segment_get_stats")
+        return st.as_dict()

@@ -1,0 +1,93 @@
+// Internal interface between the kernel translation unit (kernels.hip) and
+// the host runtime (runtime.hip) of libfbp.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fbp {
+
+// Shape of one chunk, passed by value to every kernel.
+struct Dims {
+    uint32_t C;        // channels
+    uint32_t K;        // samples per channel
+    uint32_t T;        // templates
+    uint32_t N;        // samples per template and channel
+    uint32_t M;        // n_max_shift_inds
+    uint32_t S;        // 2M+1 shifts
+    uint32_t W;        // ceil(K/N) windows
+    uint32_t n_floor;  // floor(K/N): bound used by the flag writes of the reference
+    uint32_t Tp;       // T padded to the correlation kernel's template block
+    uint32_t reduce_wg;// work-group width of the reference's overlap reduction
+    uint64_t items;    // S*S*T overlap combinations per window
+    uint64_t nl;       // ceil(items / reduce_wg) reduction groups per window
+};
+
+// Device counters, read back by the host once per pass.
+struct Counters {
+    uint32_t n_ov;       // overlap windows of the pass
+    uint32_t n_new;      // spikes added by the pass
+    uint32_t n_next;     // windows of the next pass
+    uint32_t n_rescore;  // low-precision path: positions re-scored in float32
+    uint32_t n_combo;    // screened overlap: combinations re-scored directly
+    uint32_t n_rescore_overflow;
+    uint32_t pad[2];
+};
+
+// Device buffers of one chunk.
+struct DeviceState {
+    float *V;            // [C][K] voltage, residual in place
+    float *V0;           // [C][K] pristine voltage (restored at the start of a run)
+    float *Tm;           // [T][C][N] templates
+    float *TmT;          // [C][N][Tp] templates, template-fastest, zero padded
+    float *bias;         // [Tp]
+    float *thr;          // [T]
+    float *L;            // [T][K] likelihood
+    float *best_L;       // [W]
+    uint32_t *best_lab;  // [W]
+    uint32_t *best_idx;  // [W]
+    uint32_t *ov_idx;    // [W]
+    uint32_t *ov_lab;    // [W]
+    uint8_t *nc;         // [W+2] next-pass flags: bits 0-1 value, bit 2 pruned
+    uint8_t *ov;         // [W+2] overlap flags: bit 0 flag, bit 1 gave up
+    uint32_t *win;       // [W] window list of the pass
+    uint32_t *win_next;  // [W] window list of the next pass
+    uint32_t *ov_list;   // [W] overlap windows of the pass
+    uint8_t *has;        // [W] add decision per entry of the pass list
+    uint32_t *cand_idx;  // [W] spike onset per entry of the pass list
+    uint32_t *cand_lab;  // [W]
+    uint32_t *win_spike; // [W] 1 + position in the pass's spike list, 0 = none
+    uint32_t *blk;       // [n_blocks+1] compaction block counts / offsets
+    Counters *ctr;       // device counters
+    uint32_t *sp_idx;    // [cap] spike onsets, record order
+    uint32_t *sp_lab;    // [cap]
+    float *clips;        // [n][C*N] adjusted clips (allocated on demand)
+};
+
+constexpr int kCompactItems = 1024;   // items per block of the compaction kernels
+
+// Launch wrappers (kernels.hip).  All enqueue on `st` and return hipError_t.
+hipError_t launch_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_iota_u32(uint32_t *p, uint64_t n, hipStream_t st);
+hipError_t launch_copy_rows(float *dst, const float *src, uint32_t rows, uint64_t cols,
+                            uint64_t src_stride, hipStream_t st);
+hipError_t launch_transpose_templates(const Dims &d, const float *Tm, const float *bias_in,
+                                      float *TmT, float *bias_out, hipStream_t st);
+
+hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
+hipError_t launch_argmax(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
+hipError_t launch_compact_ov(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
+hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);
+hipError_t launch_check_overlap(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);
+hipError_t launch_pursuit_add(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
+hipError_t launch_compact_spikes(const Dims &d, const DeviceState &s, uint32_t n_win,
+                                 uint64_t spike_base, hipStream_t st);
+hipError_t launch_pursuit_prune(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
+hipError_t launch_compact_next(const Dims &d, const DeviceState &s, hipStream_t st);
+hipError_t launch_residual(const Dims &d, const DeviceState &s, uint64_t spike_base,
+                           uint32_t n_new, hipStream_t st);
+hipError_t launch_clear_win_spike(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
+hipError_t launch_clips(const Dims &d, const DeviceState &s, uint64_t n_spikes, hipStream_t st);
+
+}  // namespace fbp

@@ -1,0 +1,747 @@
+// kernels.hip -- gfx950 kernels of the Full Binary Pursuit engine.
+//
+// Each kernel names the reference kernel of
+// spikesorting_fullpursuit/kernels/binary_pursuit.cl ("bp.cl") whose results
+// it reproduces.  Where the reference races, the kernels implement the outcome
+// of one lock-stepped wavefront, applied to the whole chunk:
+//   * flags written while deciding adds happen before any pruning read, and
+//     pruning reads neighbour flags as they stood after the adds (bp.cl:1009);
+//   * check_overlap_reassignments reads the id-2 flag as parsed (bp.cl:852);
+//   * spikes of a pass are recorded in ascending window order (bp.cl:1047);
+//   * every sample's template subtractions run in ascending (onset, label,
+//     record position) order, none lost (bp.cl:35-43, :89-97).
+// All float arithmetic that decides an event is IEEE float32 with explicit
+// fused multiply-adds in the reference's order; the file is compiled with
+// -ffp-contract=off so nothing else is fused.
+#include "fbp_internal.h"
+
+namespace fbp {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_of() { return threadIdx.x & (kWave - 1); }
+
+// ---------------------------------------------------------------------------
+// Fills and layout helpers
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u8(uint8_t *p, uint64_t n, uint8_t v)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_iota_u32(uint32_t *p, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+
+__global__ void k_copy_rows(float *dst, const float *src, uint32_t rows, uint64_t cols, uint64_t stride)
+{
+    const uint64_t total = (uint64_t)rows * cols;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = e / cols, c = e - r * cols;
+        dst[e] = src[r * stride + c];
+    }
+}
+
+// TmT[(c*N + j)*Tp + t] = Tm[t][c][j]; zero for t >= T (padding).
+__global__ void k_transpose_templates(Dims d, const float *__restrict__ Tm, const float *__restrict__ bias_in,
+                                      float *__restrict__ TmT, float *__restrict__ bias_out)
+{
+    const uint64_t total = (uint64_t)d.C * d.N * d.Tp;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t t = (uint32_t)(e % d.Tp);
+        const uint64_t cj = e / d.Tp;
+        const uint32_t c = (uint32_t)(cj / d.N), j = (uint32_t)(cj % d.N);
+        TmT[e] = t < d.T ? Tm[((uint64_t)t * d.C + c) * d.N + j] : 0.0f;
+        if (cj == 0) bias_out[t] = t < d.T ? bias_in[t] : 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Correlation, float32 (bp.cl:216-300, compute_full_likelihood).
+// One 64-lane block per listed window; lane = position in the window.  The
+// window's voltage span [wN, wN + 64 + N - 1) is staged in LDS channel block by
+// channel block; template values are wave-uniform (scalar loads from the
+// transposed [C][N][Tp] copy) and feed CORR_TB accumulators per lane, so one
+// LDS read serves CORR_TB fused multiply-adds.  Every accumulator follows the
+// reference chain exactly: bias, then c = 0..C-1, j = 0..N-1.
+// ---------------------------------------------------------------------------
+constexpr int kCorrTB = 8;
+constexpr int kCorrLdsFloats = 5120;   // 20 KB per block
+
+__global__ __launch_bounds__(64) void k_corr_fp32(Dims d, const float *__restrict__ V,
+                                                  const float *__restrict__ TmT,
+                                                  const float *__restrict__ bias,
+                                                  const uint32_t *__restrict__ win, float *__restrict__ L,
+                                                  uint32_t cb_max)
+{
+    __shared__ float lds[kCorrLdsFloats];
+    const int lane = threadIdx.x;
+    const uint32_t w = win[blockIdx.x];
+    const uint64_t start = (uint64_t)w * d.N;
+    const uint64_t stop = min(start + d.N, (uint64_t)d.K);
+    const uint32_t span = kWave + d.N - 1;
+    const bool staged_once = cb_max >= d.C;
+    for (uint64_t base = start; base < stop; base += kWave) {
+        const uint64_t i = base + lane;
+        const bool active = i < stop;
+        const bool valid = active && (i + d.N < d.K);   // bp.cl:231-234
+        bool have = false;
+        for (uint32_t tb = 0; tb < d.Tp; tb += kCorrTB) {
+            float acc[kCorrTB];
+#pragma unroll
+            for (int r = 0; r < kCorrTB; r++) acc[r] = bias[tb + r];
+            for (uint32_t c0 = 0; c0 < d.C; c0 += cb_max) {
+                const uint32_t cb = min(cb_max, d.C - c0);
+                if (!(staged_once && have)) {
+                    __syncthreads();
+                    for (uint32_t c = 0; c < cb; c++) {
+                        const float *vr = V + (uint64_t)(c0 + c) * d.K;
+                        for (uint32_t x = lane; x < span; x += kWave) {
+                            const uint64_t g = base + x;
+                            lds[c * span + x] = g < d.K ? vr[g] : 0.0f;
+                        }
+                    }
+                    __syncthreads();
+                    have = true;
+                }
+                for (uint32_t c = 0; c < cb; c++) {
+                    const float *vrow = lds + c * span + lane;
+                    const float *trow = TmT + (uint64_t)(c0 + c) * d.N * d.Tp + tb;
+                    for (uint32_t j = 0; j < d.N; j++) {
+                        const float v = vrow[j];
+                        const float *tj = trow + (uint64_t)j * d.Tp;
+#pragma unroll
+                        for (int r = 0; r < kCorrTB; r++) acc[r] = __builtin_fmaf(tj[r], v, acc[r]);
+                    }
+                }
+            }
+            if (active) {
+#pragma unroll
+                for (int r = 0; r < kCorrTB; r++)
+                    if (tb + r < d.T) L[(uint64_t)(tb + r) * d.K + i] = valid ? acc[r] : 0.0f;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Windowed argmax over all templates (bp.cl:350-441, with the host's T
+// sequential launches, bpp.py:551-562, fused).  One wave per listed window.
+// The reference scans t ascending, then position ascending, with strict >,
+// from a best of 0: the winner is the smallest (t, position) holding the
+// largest positive value; with no positive value the label and index stay.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_argmax(Dims d, const float *__restrict__ L,
+                                                const uint32_t *__restrict__ win, uint32_t n_win,
+                                                float *best_L, uint32_t *best_lab, uint32_t *best_idx,
+                                                uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *nc, uint8_t *ov)
+{
+    const uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = lane_of();
+    if (k >= n_win) return;
+    const uint32_t w = win[k];
+    const uint64_t start_w = (uint64_t)w * d.N, end_w = start_w + d.N;
+    if (end_w >= (uint64_t)(d.K - d.N)) return;   // bp.cl:380-383
+    const uint64_t start = start_w >= d.N ? start_w - d.N : 0;
+    const uint64_t stop = min(end_w + d.N, (uint64_t)d.K);
+    float bv = 0.0f;
+    uint32_t bt = 0xffffffffu, bp = 0xffffffffu;
+    bool chk = false;
+    for (uint32_t t = 0; t < d.T; t++) {
+        const float *Lt = L + (uint64_t)t * d.K;
+        for (uint64_t p = start + lane; p < stop; p += kWave) {
+            const float v = Lt[p];
+            if (v > bv) { bv = v; bt = t; bp = (uint32_t)p; }
+            if (v > 0.0f && p >= start_w && p < end_w) chk = true;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ov_ = __shfl_xor(bv, off);
+        const uint32_t ot = __shfl_xor(bt, off), op = __shfl_xor(bp, off);
+        if (ov_ > bv || (ov_ == bv && (ot < bt || (ot == bt && op < bp)))) { bv = ov_; bt = ot; bp = op; }
+    }
+    const bool any_chk = __any(chk);
+    if (lane == 0) {
+        uint32_t lab, idx;
+        if (bv > 0.0f) { lab = bt; idx = bp; }
+        else { lab = best_lab[w]; idx = best_idx[w]; }
+        if (any_chk) nc[w] = 1;
+        ov[w] = (bv > 0.0f && idx >= start_w && idx < end_w) ? 1 : 0;
+        best_L[w] = bv;
+        best_lab[w] = lab;
+        best_idx[w] = idx;
+        ov_idx[w] = idx;
+        ov_lab[w] = lab;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ordered stream compaction: count per block, scan the block counts, scatter.
+// Output order equals input order, so the reference's np.nonzero lists
+// (bpp.py:569, :686) and a deterministic spike record order come out.
+// ---------------------------------------------------------------------------
+template <class Pred>
+__global__ __launch_bounds__(256) void k_compact_count(Pred pred, uint64_t n, uint32_t *blk)
+{
+    __shared__ uint32_t part[4];
+    const uint64_t base = (uint64_t)blockIdx.x * kCompactItems;
+    uint32_t cnt = 0;
+    for (int r = 0; r < kCompactItems / 256; r++) {
+        const uint64_t i = base + r * 256 + threadIdx.x;
+        if (i < n && pred(i)) cnt++;
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if (lane_of() == 0) part[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) blk[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Exclusive scan of nb block counts by one 1024-thread block; total to *out.
+__global__ __launch_bounds__(1024) void k_compact_scan(uint32_t *blk, uint32_t nb, uint32_t *total)
+{
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < nb ? blk[i] : 0;
+        uint32_t x = v;
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane_of() >= off) x += y;
+        }
+        if (lane_of() == kWave - 1) wsum[threadIdx.x >> 6] = x;
+        __syncthreads();
+        uint32_t wpre = 0;
+        for (int q = 0; q < (int)(threadIdx.x >> 6); q++) wpre += wsum[q];
+        const uint32_t c = carry;
+        if (i < nb) blk[i] = c + wpre + x - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = c + wpre + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+template <class Pred, class Emit>
+__global__ __launch_bounds__(256) void k_compact_scatter(Pred pred, Emit emit, uint64_t n, const uint32_t *blk)
+{
+    __shared__ uint32_t wsum[4];
+    constexpr int per = kCompactItems / 256;
+    const uint64_t base = (uint64_t)blockIdx.x * kCompactItems + (uint64_t)threadIdx.x * per;
+    bool f[per];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int r = 0; r < per; r++) {
+        f[r] = (base + r < n) && pred(base + r);
+        cnt += f[r] ? 1 : 0;
+    }
+    uint32_t x = cnt;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane_of() >= off) x += y;
+    }
+    if (lane_of() == kWave - 1) wsum[threadIdx.x >> 6] = x;
+    __syncthreads();
+    uint32_t pos = blk[blockIdx.x] + x - cnt;
+    for (int q = 0; q < (int)(threadIdx.x >> 6); q++) pos += wsum[q];
+#pragma unroll
+    for (int r = 0; r < per; r++)
+        if (f[r]) emit(base + r, pos++);
+}
+
+template <class Pred, class Emit>
+hipError_t compact(Pred pred, Emit emit, uint64_t n, uint32_t *blk, uint32_t *total, hipStream_t st)
+{
+    const uint32_t nb = (uint32_t)((n + kCompactItems - 1) / kCompactItems);
+    if (nb > 0) hipLaunchKernelGGL(k_compact_count<Pred>, dim3(nb), dim3(256), 0, st, pred, n, blk);
+    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, st, blk, nb, total);
+    if (nb > 0) hipLaunchKernelGGL((k_compact_scatter<Pred, Emit>), dim3(nb), dim3(256), 0, st, pred, emit, n, blk);
+    return hipGetLastError();
+}
+
+struct OvPred {
+    const uint32_t *win; const uint8_t *ov;
+    __device__ bool operator()(uint64_t i) const { return (ov[win[i]] & 1) != 0; }
+};
+struct OvEmit {
+    const uint32_t *win; uint32_t *out;
+    __device__ void operator()(uint64_t i, uint32_t pos) const { out[pos] = win[i]; }
+};
+struct SpikePred {
+    const uint8_t *has;
+    __device__ bool operator()(uint64_t i) const { return has[i] != 0; }
+};
+struct SpikeEmit {
+    const uint32_t *win; const uint32_t *cand_idx; const uint32_t *cand_lab;
+    uint32_t *sp_idx; uint32_t *sp_lab; uint32_t *win_spike;
+    __device__ void operator()(uint64_t i, uint32_t pos) const {
+        sp_idx[pos] = cand_idx[i];
+        sp_lab[pos] = cand_lab[i];
+        win_spike[win[i]] = pos + 1;
+    }
+};
+struct NextPred {
+    const uint8_t *nc;
+    __device__ bool operator()(uint64_t w) const { const uint8_t v = nc[w]; return (v & 3) != 0 && !(v & 4); }
+};
+struct NextEmit {
+    uint32_t *out;
+    __device__ void operator()(uint64_t w, uint32_t pos) const { out[pos] = (uint32_t)w; }
+};
+
+// ---------------------------------------------------------------------------
+// Overlap recheck, direct form (bp.cl:444-654 + parse bp.cl:657-815).
+// One 256-thread block per overlap window (grid-stride over the device-side
+// count).  Threads take combinations in (t, f, s) order with s fastest so a
+// wave's template reads are contiguous; each combination keeps the reference
+// work id p = t + T*(f + S*s).  The reference's winner is the largest score;
+// among equal scores, the earliest reduction group g = p / WG (parse scans
+// groups with strict >), then inside the group the max_local_reduction tree
+// (bp.cl:167-186) keeps the lane with the smallest bit-reversed local id.
+// That total order is reduced here in any order.  Leftover work items of the
+// last group (p >= items) repeat combination p mod items; they can only win
+// inside a single-group window, where parse then rejects them (bp.cl:714-718).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t bitrev_bits(uint32_t x, uint32_t bits)
+{
+    return bits == 0 ? 0u : (__builtin_bitreverse32(x) >> (32 - bits));
+}
+
+__device__ __forceinline__ bool key_better(float a, uint64_t ka, float b, uint64_t kb)
+{
+    return a > b || (a == b && ka < kb);
+}
+
+__device__ float overlap_direct_score(const Dims &d, const float *__restrict__ sv, const float *__restrict__ smain,
+                                      const float *__restrict__ Tm, uint32_t t, uint32_t f, uint32_t s)
+{
+    // a_j = Tm[lab, c, j + M - f] and b_j = Tm[t, c, j + M - s] where in range.
+    const int32_t fa = (int32_t)d.M - (int32_t)f;
+    const int32_t sa = (int32_t)d.M - (int32_t)s;
+    const int32_t flo = fa < 0 ? -fa : 0, fhi = min((int32_t)d.N, (int32_t)d.N - fa);
+    const int32_t slo = sa < 0 ? -sa : 0, shi = min((int32_t)d.N, (int32_t)d.N - sa);
+    float acc = 0.0f;
+    for (uint32_t c = 0; c < d.C; c++) {
+        const float *tm = smain + c * d.N;
+        const float *ts = Tm + ((uint64_t)t * d.C + c) * d.N;
+        const float *vc = sv + c * d.N;
+        float ss = 0.0f;
+        for (int32_t j = 0; j < (int32_t)d.N; j++) {
+            const float fv = (j >= flo && j < fhi) ? tm[j + fa] : 0.0f;
+            const float bv = (j >= slo && j < shi) ? ts[j + sa] : 0.0f;
+            const float sum = fv + bv;
+            ss = __builtin_fmaf(sum, sum, ss);       // bp.cl:621
+            acc = __builtin_fmaf(sum, vc[j], acc);   // bp.cl:622
+        }
+        if (ss > 0.0f) acc = (float)((double)acc - 0.5 * (double)ss);   // bp.cl:627-630
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void k_overlap_direct(
+    Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
+    const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, const Counters *__restrict__ ctr,
+    float *best_L, const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab,
+    uint8_t *ov)
+{
+    extern __shared__ float dyn[];
+    float *sv = dyn;                       // [C][N] voltage at the best onset
+    float *smain = dyn + d.C * d.N;        // [C][N] template of the best label
+    __shared__ float r_sc[4];
+    __shared__ uint64_t r_key[4];
+    __shared__ uint64_t r_p[4];
+    const uint32_t n_ov = ctr->n_ov;
+    const uint32_t wg_bits = 31 - __builtin_clz(d.reduce_wg);
+    const uint64_t span = d.nl * d.reduce_wg;
+    const uint32_t S = d.S, T = d.T, N = d.N, M = d.M;
+    for (uint32_t k = blockIdx.x; k < n_ov; k += gridDim.x) {
+        const uint32_t id = ov_list[k];
+        const uint32_t lab = best_lab[id], bidx = best_idx[id];
+        const float bl = best_L[id];
+        const float Lb = L[(uint64_t)lab * d.K + bidx];
+        const bool wskip = (Lb < thr[lab]) || ((uint64_t)bidx >= (uint64_t)(d.K - d.N));   // bp.cl:518-527
+        float sc = -INFINITY;
+        uint64_t key = ~0ull, bp = 0;
+        if (!wskip) {
+            __syncthreads();
+            for (uint32_t e = threadIdx.x; e < d.C * N; e += blockDim.x) {
+                const uint32_t c = e / N, j = e % N;
+                sv[e] = V[(uint64_t)c * d.K + bidx + j];
+                smain[e] = Tm[((uint64_t)lab * d.C + c) * N + j];
+            }
+            __syncthreads();
+            const float *Lmain = L + (uint64_t)lab * d.K;
+            for (uint64_t q = threadIdx.x; q < d.items; q += blockDim.x) {
+                const uint32_t s = (uint32_t)(q % S);
+                const uint32_t f = (uint32_t)((q / S) % S);
+                const uint32_t t = (uint32_t)(q / ((uint64_t)S * S));
+                if ((uint64_t)f + bidx < M || (uint64_t)f + bidx >= (uint64_t)d.K - N + M) continue;   // bp.cl:534
+                if ((uint64_t)s + bidx < M || (uint64_t)s + bidx >= (uint64_t)d.K - N + M) continue;   // bp.cl:538
+                if ((f > s ? f - s : s - f) > N) continue;                                              // bp.cl:543-556
+                const float Lf = Lmain[bidx + f - M];
+                const float Ls = L[(uint64_t)t * d.K + bidx + s - M];
+                if (Lf <= 0.0f && Ls <= 0.0f) continue;                                                 // bp.cl:572-576
+                const float score = overlap_direct_score(d, sv, smain, Tm, t, f, s);
+                uint64_t p = (uint64_t)t + (uint64_t)T * (f + (uint64_t)S * s);
+                uint64_t g = p / d.reduce_wg;
+                uint32_t br = bitrev_bits((uint32_t)(p % d.reduce_wg), wg_bits);
+                for (uint64_t p2 = p + d.items; p2 < span && p2 / d.reduce_wg == g; p2 += d.items) {
+                    const uint32_t br2 = bitrev_bits((uint32_t)(p2 % d.reduce_wg), wg_bits);
+                    if (br2 < br) { br = br2; p = p2; }
+                }
+                const uint64_t kk = (g << 32) | br;
+                if (key_better(score, kk, sc, key)) { sc = score; key = kk; bp = p; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const float o = __shfl_xor(sc, off);
+                const uint64_t ok = __shfl_xor(key, off), op = __shfl_xor(bp, off);
+                if (key_better(o, ok, sc, key)) { sc = o; key = ok; bp = op; }
+            }
+            if (lane_of() == 0) { r_sc[threadIdx.x >> 6] = sc; r_key[threadIdx.x >> 6] = key; r_p[threadIdx.x >> 6] = bp; }
+            __syncthreads();
+            if (threadIdx.x == 0)
+                for (int q = 1; q < (int)(blockDim.x >> 6); q++)
+                    if (key_better(r_sc[q], r_key[q], sc, key)) { sc = r_sc[q]; key = r_key[q]; bp = r_p[q]; }
+        }
+        if (threadIdx.x == 0) {
+            // parse_overlap_recheck_indices (bp.cl:690-814)
+            bool keep = sc > bl && sc > 0.0f;                          // bp.cl:707-711
+            if (keep && bp % span >= d.items) keep = false;            // bp.cl:714-718
+            if (keep) {
+                const uint32_t t = (uint32_t)(bp % T);
+                const uint32_t f = (uint32_t)((bp / T) % S);
+                const uint32_t s = (uint32_t)((bp / ((uint64_t)S * T)) % S);
+                const uint32_t af = bidx + f - M, as = bidx + s - M;
+                const float Lm = L[(uint64_t)lab * d.K + af];
+                const float Lo = L[(uint64_t)t * d.K + as];
+                if (sc < Lm || sc < Lo) keep = false;                  // bp.cl:733-738
+                else if (Lm <= 0.0f || Lo <= 0.0f) keep = false;       // bp.cl:739-744
+                else if (Lm >= Lo && Lm > thr[lab]) { best_L[id] = Lm; ov_lab[id] = lab; ov_idx[id] = af; }
+                else if (Lo > Lm && Lo > thr[t]) { best_L[id] = Lo; ov_lab[id] = t; ov_idx[id] = as; }
+                else if (Lm > thr[lab]) { best_L[id] = Lm; ov_lab[id] = lab; ov_idx[id] = af; }
+                else { ov_idx[id] = bidx; ov_lab[id] = lab; keep = false; }   // bp.cl:806-814
+            }
+            if (!keep) ov[id] = 0;
+        }
+    }
+}
+
+// check_overlap_reassignments (bp.cl:822-875).  ov bit 1 marks a window that
+// gave up, so the id-2 reads below see bit 0 as parsed.
+__global__ __launch_bounds__(256) void k_check_overlap(Dims d, const uint32_t *__restrict__ ov_list,
+                                                       const Counters *__restrict__ ctr, uint8_t *ov, uint8_t *nc,
+                                                       uint32_t *best_idx, uint32_t *best_lab,
+                                                       const uint32_t *ov_idx, const uint32_t *ov_lab)
+{
+    const uint32_t n_ov = ctr->n_ov;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_ov; k += gridDim.x * blockDim.x) {
+        const uint32_t id = ov_list[k];
+        if (!(ov[id] & 1)) continue;
+        const uint64_t start_w = (uint64_t)id * d.N, end_w = start_w + d.N;
+        const uint32_t oi = ov_idx[id];
+        best_idx[id] = oi;
+        best_lab[id] = ov_lab[id];
+        if (oi < start_w && id > 1) {
+            if (ov[id - 2] & 1) {
+                ov[id] = 3;
+                best_idx[id] = 0;
+                nc[id - 1] = 2;
+                nc[id] = 2;
+                nc[id + 1] = 2;
+            } else {
+                nc[id - 2] = 2;
+            }
+        }
+        if (oi >= end_w && id + 2 < d.n_floor) nc[id + 2] = 2;
+    }
+}
+
+// binary_pursuit, add half (bp.cl:961-999).
+__global__ __launch_bounds__(256) void k_pursuit_add(Dims d, const uint32_t *__restrict__ win, uint32_t n_win,
+                                                     const float *__restrict__ best_L,
+                                                     const uint32_t *__restrict__ best_lab,
+                                                     const uint32_t *__restrict__ best_idx,
+                                                     const float *__restrict__ thr, const uint8_t *__restrict__ ov,
+                                                     uint8_t *nc, uint8_t *has, uint32_t *cand_idx, uint32_t *cand_lab)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_win) return;
+    const uint32_t id = win[k];
+    const uint64_t start_w = (uint64_t)id * d.N, end_w = start_w + d.N;
+    float ml = 0.0f;
+    uint32_t lab = 0, idx = 0;
+    if (end_w < (uint64_t)(d.K - d.N)) { ml = best_L[id]; lab = best_lab[id]; idx = best_idx[id]; }
+    const bool o = ov[id] == 1;
+    const bool hs = (ml > thr[lab] || o) && ((idx >= start_w && idx < end_w) || o);
+    if (hs) {
+        nc[id] = 3;
+        if (id > 0) nc[id - 1] = 3;
+        if (id + 1 < d.n_floor) nc[id + 1] = 3;
+    }
+    has[k] = hs ? 1 : 0;
+    cand_idx[k] = idx;
+    cand_lab[k] = lab;
+}
+
+// binary_pursuit, pruning half (bp.cl:1009-1045).  Bit 2 marks a pruned flag so
+// neighbours keep reading the value it had after the adds.
+__global__ __launch_bounds__(256) void k_pursuit_prune(Dims d, const uint32_t *__restrict__ win, uint32_t n_win,
+                                                       const float *__restrict__ best_L,
+                                                       const uint32_t *__restrict__ best_lab,
+                                                       const uint32_t *__restrict__ best_idx,
+                                                       const float *__restrict__ thr,
+                                                       const uint8_t *__restrict__ has, uint8_t *nc)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_win) return;
+    const uint32_t id = win[k];
+    const uint64_t start_w = (uint64_t)id * d.N, end_w = start_w + d.N;
+    if (!(end_w < (uint64_t)(d.K - d.N)) || has[k]) return;
+    const float ml = best_L[id];
+    const uint32_t lab = best_lab[id], idx = best_idx[id];
+    const uint8_t orig = nc[id] & 3;
+    uint8_t own = orig;
+    if (idx < start_w && own == 1 && id > 0 && (nc[id - 1] & 3) == 0) own = 0;
+    if (idx >= end_w && own == 1 && id + 1 < d.n_floor && (nc[id + 1] & 3) == 0) own = 0;
+    if (ml <= thr[lab] && own < 3) own = 0;
+    if (own == 0 && orig != 0) nc[id] = orig | 4;
+}
+
+// compute_residual (bp.cl:57-102) without lost updates.  One block per spike of
+// the pass; thread (c, j) owns sample x = onset + j if this spike is the first,
+// in (onset, label, position) order, of the pass's spikes covering x, and then
+// applies all of them in that order.  Spikes are found through the map from
+// adding window to spike: a spike added by window w lies in [wN-M, (w+1)N+M).
+constexpr int kMaxCover = 32;
+
+__global__ __launch_bounds__(256) void k_residual(Dims d, float *V, const float *__restrict__ Tm,
+                                                  const uint32_t *__restrict__ sp_idx,
+                                                  const uint32_t *__restrict__ sp_lab, uint32_t n_new,
+                                                  const uint32_t *__restrict__ win_spike)
+{
+    for (uint32_t k = blockIdx.x; k < n_new; k += gridDim.x) {
+        const uint32_t idx = sp_idx[k], lab = sp_lab[k];
+        if (lab >= d.T) continue;   // bp.cl:78-81
+        for (uint32_t e = threadIdx.x; e < d.C * d.N; e += blockDim.x) {
+            const uint32_t c = e / d.N, j = e % d.N;
+            const uint64_t x = (uint64_t)idx + j;
+            if (x >= d.K) continue;   // bp.cl:93-96
+            const int64_t lo = ((int64_t)x - 2 * (int64_t)d.N - (int64_t)d.M + 2);
+            int64_t wlo = lo < 0 ? 0 : lo / d.N;
+            int64_t whi = ((int64_t)x + d.M) / d.N;
+            if (whi > (int64_t)d.W - 1) whi = d.W - 1;
+            uint32_t ci[kMaxCover], cl[kMaxCover], cp[kMaxCover];
+            int nc = 0;
+            bool owner = true;
+            for (int64_t wd = wlo; wd <= whi; wd++) {
+                const uint32_t m = win_spike[wd];
+                if (!m) continue;
+                const uint32_t pos = m - 1;
+                const uint32_t i2 = sp_idx[pos];
+                if (!((uint64_t)i2 <= x && x < (uint64_t)i2 + d.N)) continue;
+                const uint32_t l2 = sp_lab[pos];
+                if (l2 >= d.T) continue;
+                if (i2 < idx || (i2 == idx && (l2 < lab || (l2 == lab && pos < k)))) { owner = false; break; }
+                // insertion in (onset, label, position) order
+                int q = nc;
+                while (q > 0 && (ci[q - 1] > i2 || (ci[q - 1] == i2 && (cl[q - 1] > l2 || (cl[q - 1] == l2 && cp[q - 1] > pos))))) {
+                    ci[q] = ci[q - 1]; cl[q] = cl[q - 1]; cp[q] = cp[q - 1];
+                    q--;
+                }
+                ci[q] = i2; cl[q] = l2; cp[q] = pos;
+                if (nc < kMaxCover - 1) nc++;
+            }
+            if (!owner) continue;
+            float *vp = V + (uint64_t)c * d.K + x;
+            float v = *vp;
+            for (int q = 0; q < nc; q++) v -= Tm[((uint64_t)cl[q] * d.C + c) * d.N + (uint32_t)(x - ci[q])];
+            *vp = v;
+        }
+    }
+}
+
+__global__ void k_clear_win_spike(const uint32_t *__restrict__ win, uint32_t n_win, uint32_t *win_spike)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_win) win_spike[win[k]] = 0;
+}
+
+// get_adjusted_clips (bp.cl:1088-1130): clip = residual + template, 0 past K.
+__global__ __launch_bounds__(256) void k_clips(Dims d, const float *__restrict__ V, const float *__restrict__ Tm,
+                                               const uint32_t *__restrict__ sp_idx,
+                                               const uint32_t *__restrict__ sp_lab, uint64_t n,
+                                               float *__restrict__ clips)
+{
+    const uint32_t cn = d.C * d.N;
+    for (uint64_t k = blockIdx.x; k < n; k += gridDim.x) {
+        const uint32_t idx = sp_idx[k], lab = sp_lab[k];
+        for (uint32_t e = threadIdx.x; e < cn; e += blockDim.x) {
+            const uint32_t c = e / d.N, j = e % d.N;
+            float out = 0.0f;
+            if (lab < d.T && (uint64_t)idx + j < d.K)
+                out = V[(uint64_t)c * d.K + idx + j] + Tm[((uint64_t)lab * d.C + c) * d.N + j];
+            clips[k * cn + e] = out;
+        }
+    }
+}
+
+inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 65535u * 4)
+{
+    uint64_t g = (n + block - 1) / block;
+    if (g == 0) g = 1;
+    return (uint32_t)(g > cap ? cap : g);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launch wrappers
+// ---------------------------------------------------------------------------
+hipError_t launch_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_iota_u32(uint32_t *p, uint64_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_rows(float *dst, const float *src, uint32_t rows, uint64_t cols, uint64_t src_stride,
+                            hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy_rows, dim3(grid_for((uint64_t)rows * cols, 256, 16384)), dim3(256), 0, st, dst, src,
+                       rows, cols, src_stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_templates(const Dims &d, const float *Tm, const float *bias_in, float *TmT,
+                                      float *bias_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_transpose_templates, dim3(grid_for((uint64_t)d.C * d.N * d.Tp, 256, 4096)), dim3(256), 0,
+                       st, d, Tm, bias_in, TmT, bias_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st)
+{
+    if (n_win == 0) return hipSuccess;
+    const uint32_t span = kWave + d.N - 1;
+    uint32_t cb_max = kCorrLdsFloats / span;
+    if (cb_max > d.C) cb_max = d.C;
+    if (cb_max == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_corr_fp32, dim3(n_win), dim3(64), 0, st, d, s.V, s.TmT, s.bias, s.win, s.L, cb_max);
+    return hipGetLastError();
+}
+
+hipError_t launch_argmax(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st)
+{
+    if (n_win == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_argmax, dim3((n_win + 3) / 4), dim3(256), 0, st, d, s.L, s.win, n_win, s.best_L,
+                       s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.nc, s.ov);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_ov(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st)
+{
+    return compact(OvPred{s.win, s.ov}, OvEmit{s.win, s.ov_list}, n_win, s.blk, &s.ctr->n_ov, st);
+}
+
+hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st)
+{
+    if (max_ov == 0) return hipSuccess;
+    const size_t lds = 2ull * d.C * d.N * sizeof(float);
+    const uint32_t grid = max_ov < 4096 ? max_ov : 4096;
+    hipLaunchKernelGGL(k_overlap_direct, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr, s.ov_list, s.ctr,
+                       s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_overlap(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st)
+{
+    if (max_ov == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_check_overlap, dim3(grid_for(max_ov, 256, 1024)), dim3(256), 0, st, d, s.ov_list, s.ctr,
+                       s.ov, s.nc, s.best_idx, s.best_lab, s.ov_idx, s.ov_lab);
+    return hipGetLastError();
+}
+
+hipError_t launch_pursuit_add(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st)
+{
+    if (n_win == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pursuit_add, dim3((n_win + 255) / 256), dim3(256), 0, st, d, s.win, n_win, s.best_L,
+                       s.best_lab, s.best_idx, s.thr, s.ov, s.nc, s.has, s.cand_idx, s.cand_lab);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_spikes(const Dims &d, const DeviceState &s, uint32_t n_win, uint64_t spike_base,
+                                 hipStream_t st)
+{
+    return compact(SpikePred{s.has},
+                   SpikeEmit{s.win, s.cand_idx, s.cand_lab, s.sp_idx + spike_base, s.sp_lab + spike_base, s.win_spike},
+                   n_win, s.blk, &s.ctr->n_new, st);
+}
+
+hipError_t launch_pursuit_prune(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st)
+{
+    if (n_win == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pursuit_prune, dim3((n_win + 255) / 256), dim3(256), 0, st, d, s.win, n_win, s.best_L,
+                       s.best_lab, s.best_idx, s.thr, s.has, s.nc);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_next(const Dims &d, const DeviceState &s, hipStream_t st)
+{
+    return compact(NextPred{s.nc}, NextEmit{s.win_next}, d.W, s.blk, &s.ctr->n_next, st);
+}
+
+hipError_t launch_residual(const Dims &d, const DeviceState &s, uint64_t spike_base, uint32_t n_new, hipStream_t st)
+{
+    if (n_new == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_residual, dim3(grid_for(n_new, 1, 65536)), dim3(256), 0, st, d, s.V, s.Tm,
+                       s.sp_idx + spike_base, s.sp_lab + spike_base, n_new, s.win_spike);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_win_spike(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st)
+{
+    if (n_win == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_win_spike, dim3((n_win + 255) / 256), dim3(256), 0, st, s.win, n_win, s.win_spike);
+    return hipGetLastError();
+}
+
+hipError_t launch_clips(const Dims &d, const DeviceState &s, uint64_t n_spikes, hipStream_t st)
+{
+    if (n_spikes == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clips, dim3(grid_for(n_spikes, 1, 65536)), dim3(256), 0, st, d, s.V, s.Tm, s.sp_idx,
+                       s.sp_lab, n_spikes, s.clips);
+    return hipGetLastError();
+}
+
+}  // namespace fbp

@@ -1,0 +1,423 @@
+// runtime.hip -- C-ABI (include/fbp.h) and the device-resident pass loop.
+//
+// One fbp_segment holds every buffer of one chunk in HBM for the whole call
+// and runs the reference's pass loop
+// (spikesorting_fullpursuit/parallel/binary_pursuit_parallel.py:532-736) with
+// a single 32-byte device-to-host read per pass: window lists and overlap
+// lists are compacted on the device, kernels read their counts from device
+// memory, so the host never copies flags, lists or spikes between passes.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/fbp.h"
+#include "fbp_internal.h"
+
+using namespace fbp;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return fail(FBP_ENOMEM, "%s: out of device memory (%s)", what, hipGetErrorString(e));
+    return fail(FBP_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                      \
+    do {                                         \
+        hipError_t e_ = (expr);                  \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+constexpr uint32_t kTemplateBlock = 8;   // must match the correlation kernel's kCorrTB
+
+}  // namespace
+
+struct fbp_segment {
+    int device = 0;
+    fbp_config cfg{};
+    Dims d{};
+    DeviceState s{};
+    hipStream_t own_stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    Counters *h_ctr = nullptr;   // pinned host copy of the device counters
+    uint64_t cap = 0;            // spike capacity of sp_idx / sp_lab
+    uint64_t n_spikes = 0;
+    uint64_t clips_cap = 0;
+    bool have_templates = false, have_voltage = false, ran = false;
+    fbp_stats stats{};
+};
+
+namespace {
+
+template <class T>
+hipError_t dalloc(T **p, uint64_t n)
+{
+    return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+}
+
+void free_state(fbp_segment *g)
+{
+    DeviceState &s = g->s;
+    void *ptrs[] = {s.V, s.V0, s.Tm, s.TmT, s.bias, s.thr, s.L, s.best_L, s.best_lab, s.best_idx, s.ov_idx,
+                    s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
+                    s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    s = DeviceState{};
+}
+
+int set_device(const fbp_segment *g)
+{
+    HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+    return FBP_OK;
+}
+
+int ensure_spike_capacity(fbp_segment *g, uint64_t need, hipStream_t st)
+{
+    if (need <= g->cap) return FBP_OK;
+    uint64_t cap = g->cap * 2;
+    if (cap < need) cap = need;
+    uint32_t *ni = nullptr, *nl = nullptr;
+    HIP_TRY(dalloc(&ni, cap), "spike buffer");
+    hipError_t e = dalloc(&nl, cap);
+    if (e != hipSuccess) { (void)hipFree(ni); return hip_fail(e, "spike buffer"); }
+    if (g->n_spikes > 0) {
+        HIP_TRY(hipMemcpyAsync(ni, g->s.sp_idx, g->n_spikes * 4, hipMemcpyDeviceToDevice, st), "grow spikes");
+        HIP_TRY(hipMemcpyAsync(nl, g->s.sp_lab, g->n_spikes * 4, hipMemcpyDeviceToDevice, st), "grow spikes");
+    }
+    HIP_TRY(hipStreamSynchronize(st), "grow spikes");
+    (void)hipFree(g->s.sp_idx);
+    (void)hipFree(g->s.sp_lab);
+    g->s.sp_idx = ni;
+    g->s.sp_lab = nl;
+    g->cap = cap;
+    return FBP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fbp_version(void) { return 100; }
+
+const char *fbp_last_error(void) { return g_err.c_str(); }
+
+int fbp_device_count(int *count)
+{
+    if (!count) return fail(FBP_EINVAL, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        *count = 0;
+        return fail(FBP_ENODEV, "No GPU available");
+    }
+    *count = n;
+    return FBP_OK;
+}
+
+int fbp_device_memory(int device, uint64_t *total_bytes, uint64_t *free_bytes)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(FBP_ENODEV, "No GPU available");
+    if (device < 0 || device >= n) return fail(FBP_EINVAL, "device %d out of range (%d devices)", device, n);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+    if (total_bytes) *total_bytes = tot;
+    if (free_bytes) *free_bytes = fr;
+    return FBP_OK;
+}
+
+int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
+{
+    if (!cfg || !out) return fail(FBP_EINVAL, "NULL argument");
+    *out = nullptr;
+    const uint32_t C = cfg->n_chans, K = cfg->n_samples, T = cfg->n_templates, N = cfg->template_len;
+    if (C == 0 || T == 0 || N == 0) return fail(FBP_EINVAL, "channels, templates and template length must be > 0");
+    if (K <= N) return fail(FBP_EINVAL, "chunk of %u samples is not longer than the template (%u)", K, N);
+    if (cfg->reduce_wg == 0 || (cfg->reduce_wg & (cfg->reduce_wg - 1)) || cfg->reduce_wg > (1u << 20))
+        return fail(FBP_EINVAL, "reduce_wg must be a power of two");
+    if ((uint64_t)cfg->max_shift >= 4ull * N)
+        return fail(FBP_EINVAL, "n_max_shift_inds (%u) must be below 4x the template length", cfg->max_shift);
+    if (cfg->precision != FBP_PREC_FP32)
+        return fail(FBP_EINVAL, "precision %u is not available in this build", cfg->precision);
+    if (cfg->overlap_mode != FBP_OVERLAP_DIRECT)
+        return fail(FBP_EINVAL, "overlap mode %u is not available in this build", cfg->overlap_mode);
+    if ((uint64_t)C * N * sizeof(float) * 2 > 64 * 1024 - 256)
+        return fail(FBP_EINVAL, "C*N = %u exceeds the overlap kernel's LDS staging", C * N);
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(FBP_ENODEV, "No GPU available");
+    if (device < 0 || device >= n) return fail(FBP_EINVAL, "device %d out of range", device);
+
+    fbp_segment *g = new fbp_segment();
+    g->device = device;
+    g->cfg = *cfg;
+    Dims &d = g->d;
+    d.C = C; d.K = K; d.T = T; d.N = N; d.M = cfg->max_shift;
+    d.S = 2 * cfg->max_shift + 1;
+    d.W = (uint32_t)(((uint64_t)K + N - 1) / N);
+    d.n_floor = K / N;
+    d.Tp = (T + kTemplateBlock - 1) / kTemplateBlock * kTemplateBlock;
+    d.reduce_wg = cfg->reduce_wg;
+    d.items = (uint64_t)d.S * d.S * T;
+    d.nl = (d.items - 1) / cfg->reduce_wg + 1;
+
+    int rc = FBP_OK;
+    hipError_t e = hipSetDevice(device);
+    const uint64_t W = d.W, CK = (uint64_t)C * K;
+    DeviceState &s = g->s;
+    const uint32_t nb = (uint32_t)((W + kCompactItems - 1) / kCompactItems);
+#define ALLOC(ptr, n)                                  \
+    if (e == hipSuccess) e = dalloc(&(ptr), (n));
+    ALLOC(s.V, CK)
+    ALLOC(s.V0, CK)
+    ALLOC(s.Tm, (uint64_t)T * C * N)
+    ALLOC(s.TmT, (uint64_t)C * N * d.Tp)
+    ALLOC(s.bias, d.Tp)
+    ALLOC(s.thr, T)
+    ALLOC(s.L, (uint64_t)T * K)
+    ALLOC(s.best_L, W)
+    ALLOC(s.best_lab, W)
+    ALLOC(s.best_idx, W)
+    ALLOC(s.ov_idx, W)
+    ALLOC(s.ov_lab, W)
+    ALLOC(s.nc, W + 2)
+    ALLOC(s.ov, W + 2)
+    ALLOC(s.win, W)
+    ALLOC(s.win_next, W)
+    ALLOC(s.ov_list, W)
+    ALLOC(s.has, W)
+    ALLOC(s.cand_idx, W)
+    ALLOC(s.cand_lab, W)
+    ALLOC(s.win_spike, W)
+    ALLOC(s.blk, nb + 1)
+    ALLOC(s.ctr, 1)
+#undef ALLOC
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->own_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&g->ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&g->ev_stop);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&g->h_ctr, sizeof(Counters), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "fbp_segment_create");
+        fbp_segment_destroy(g);
+        return rc;
+    }
+    rc = ensure_spike_capacity(g, W, g->own_stream);
+    if (rc) { fbp_segment_destroy(g); return rc; }
+    *out = g;
+    return FBP_OK;
+}
+
+void fbp_segment_destroy(fbp_segment *g)
+{
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
+    free_state(g);
+    if (g->ev_start) (void)hipEventDestroy(g->ev_start);
+    if (g->ev_stop) (void)hipEventDestroy(g->ev_stop);
+    if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
+    if (g->h_ctr) (void)hipHostFree(g->h_ctr);
+    delete g;
+}
+
+int fbp_segment_set_templates(fbp_segment *g, const float *templates, const float *bias, const float *thresholds)
+{
+    if (!g || !templates || !bias || !thresholds) return fail(FBP_EINVAL, "NULL argument");
+    if (int rc = set_device(g)) return rc;
+    const Dims &d = g->d;
+    hipStream_t st = g->own_stream;
+    float *tmp_bias = nullptr;
+    HIP_TRY(dalloc(&tmp_bias, d.T), "bias staging");
+    hipError_t e = hipMemcpyAsync(g->s.Tm, templates, (uint64_t)d.T * d.C * d.N * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(tmp_bias, bias, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->s.thr, thresholds, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_transpose_templates(d, g->s.Tm, tmp_bias, g->s.TmT, g->s.bias, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(tmp_bias);
+    if (e != hipSuccess) return hip_fail(e, "fbp_segment_set_templates");
+    g->have_templates = true;
+    return FBP_OK;
+}
+
+int fbp_segment_set_voltage(fbp_segment *g, const float *voltage, uint64_t row_stride)
+{
+    if (!g || !voltage) return fail(FBP_EINVAL, "NULL argument");
+    const Dims &d = g->d;
+    if (row_stride < d.K) return fail(FBP_EINVAL, "row stride %llu below chunk length %u",
+                                      (unsigned long long)row_stride, d.K);
+    if (int rc = set_device(g)) return rc;
+    HIP_TRY(hipMemcpy2DAsync(g->s.V0, (size_t)d.K * 4, voltage, (size_t)row_stride * 4, (size_t)d.K * 4, d.C,
+                             hipMemcpyHostToDevice, g->own_stream), "voltage upload");
+    HIP_TRY(hipStreamSynchronize(g->own_stream), "voltage upload");
+    g->have_voltage = true;
+    return FBP_OK;
+}
+
+int fbp_segment_set_voltage_device(fbp_segment *g, const float *d_voltage, uint64_t row_stride)
+{
+    if (!g || !d_voltage) return fail(FBP_EINVAL, "NULL argument");
+    const Dims &d = g->d;
+    if (row_stride < d.K) return fail(FBP_EINVAL, "row stride below chunk length");
+    if (int rc = set_device(g)) return rc;
+    HIP_TRY(launch_copy_rows(g->s.V0, d_voltage, d.C, d.K, row_stride, g->own_stream), "voltage copy");
+    HIP_TRY(hipStreamSynchronize(g->own_stream), "voltage copy");
+    g->have_voltage = true;
+    return FBP_OK;
+}
+
+int fbp_segment_run(fbp_segment *g, void *stream)
+{
+    if (!g) return fail(FBP_EINVAL, "NULL segment");
+    if (!g->have_templates || !g->have_voltage) return fail(FBP_ESTATE, "templates and voltage must be set first");
+    if (int rc = set_device(g)) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : g->own_stream;
+    const Dims &d = g->d;
+    DeviceState &s = g->s;
+    g->ran = false;
+    g->n_spikes = 0;
+    g->stats = fbp_stats{};
+    HIP_TRY(hipEventRecord(g->ev_start, st), "event");
+    // Fresh state for the chunk (bpp.py:386-423); L needs no clearing because
+    // pass 1 writes every position of it.
+    HIP_TRY(hipMemcpyAsync(s.V, s.V0, (uint64_t)d.C * d.K * 4, hipMemcpyDeviceToDevice, st), "voltage restore");
+    HIP_TRY(launch_fill_u32((uint32_t *)s.best_L, d.W, 0u, st), "init");
+    HIP_TRY(launch_fill_u32(s.best_lab, d.W, 0u, st), "init");
+    HIP_TRY(launch_fill_u32(s.best_idx, d.W, 0u, st), "init");
+    HIP_TRY(launch_fill_u32(s.ov_idx, d.W, 0u, st), "init");
+    HIP_TRY(launch_fill_u32(s.ov_lab, d.W, 0u, st), "init");
+    HIP_TRY(launch_fill_u32(s.win_spike, d.W, 0u, st), "init");
+    HIP_TRY(launch_fill_u8(s.nc, d.W + 2, 0, st), "init");
+    HIP_TRY(launch_fill_u8(s.ov, d.W + 2, 0, st), "init");
+    HIP_TRY(launch_iota_u32(s.win, d.W, st), "init");
+
+    uint32_t nw = d.W;
+    uint64_t total = 0;
+    uint32_t passes = 0;
+    int rc = FBP_OK;
+    for (;;) {
+        if (g->cfg.max_passes && passes >= g->cfg.max_passes) { rc = fail(FBP_EPASSES, "pass cap %u reached", passes); break; }
+        passes++;
+        if ((rc = ensure_spike_capacity(g, total + nw, st))) break;
+        HIP_TRY(launch_corr_fp32(d, s, nw, st), "correlation");                   // bpp.py:538-549
+        HIP_TRY(launch_argmax(d, s, nw, st), "argmax");                            // bpp.py:551-562
+        if (g->cfg.do_overlap) {                                                   // bpp.py:565-661
+            HIP_TRY(launch_compact_ov(d, s, nw, st), "overlap list");
+            HIP_TRY(launch_overlap_direct(d, s, nw, st), "overlap recheck");
+            HIP_TRY(launch_check_overlap(d, s, nw, st), "overlap reassign");
+        }
+        HIP_TRY(launch_pursuit_add(d, s, nw, st), "pursuit add");                  // bpp.py:663-671
+        HIP_TRY(launch_compact_spikes(d, s, nw, total, st), "spike list");
+        HIP_TRY(launch_pursuit_prune(d, s, nw, st), "pursuit prune");
+        HIP_TRY(launch_compact_next(d, s, st), "window list");                     // bpp.py:683-689
+        HIP_TRY(hipMemcpyAsync(g->h_ctr, s.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st), "counters");
+        HIP_TRY(hipStreamSynchronize(st), "pass");
+        const uint32_t n_new = g->h_ctr->n_new;
+        g->stats.windows_evaluated += nw;
+        if (g->cfg.do_overlap) g->stats.overlap_windows += g->h_ctr->n_ov;
+        if (n_new == 0) break;                                                     // bpp.py:675-676
+        HIP_TRY(launch_fill_u8(s.ov, d.W + 2, 0, st), "reset");                  // bpp.py:679-680
+        HIP_TRY(launch_residual(d, s, total, n_new, st), "residual");             // bpp.py:716-731
+        HIP_TRY(launch_clear_win_spike(d, s, nw, st), "reset");
+        HIP_TRY(launch_fill_u8(s.nc, d.W + 2, 0, st), "reset");                  // bpp.py:699-700
+        std::swap(s.win, s.win_next);
+        total += n_new;
+        nw = g->h_ctr->n_next;
+    }
+    HIP_TRY(hipEventRecord(g->ev_stop, st), "event");
+    HIP_TRY(hipEventSynchronize(g->ev_stop), "event");
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, g->ev_start, g->ev_stop);
+    g->n_spikes = total;
+    g->stats.n_passes = passes;
+    g->stats.n_spikes = total;
+    g->stats.ms_total = ms;
+    // Results stay readable when the pass cap stopped the loop.
+    g->ran = (rc == FBP_OK || rc == FBP_EPASSES);
+    return rc;
+}
+
+int fbp_segment_num_spikes(const fbp_segment *g, uint64_t *n)
+{
+    if (!g || !n) return fail(FBP_EINVAL, "NULL argument");
+    if (!g->ran) return fail(FBP_ESTATE, "segment has not run");
+    *n = g->n_spikes;
+    return FBP_OK;
+}
+
+int fbp_segment_get_spikes(fbp_segment *g, uint32_t *onsets, uint32_t *labels)
+{
+    if (!g || !onsets || !labels) return fail(FBP_EINVAL, "NULL argument");
+    if (!g->ran) return fail(FBP_ESTATE, "segment has not run");
+    if (g->n_spikes == 0) return FBP_OK;
+    if (int rc = set_device(g)) return rc;
+    HIP_TRY(hipMemcpy(onsets, g->s.sp_idx, g->n_spikes * 4, hipMemcpyDeviceToHost), "spikes");
+    HIP_TRY(hipMemcpy(labels, g->s.sp_lab, g->n_spikes * 4, hipMemcpyDeviceToHost), "spikes");
+    return FBP_OK;
+}
+
+int fbp_segment_get_clips(fbp_segment *g, float *clips)
+{
+    if (!g || !clips) return fail(FBP_EINVAL, "NULL argument");
+    if (!g->ran) return fail(FBP_ESTATE, "segment has not run");
+    if (g->n_spikes == 0) return FBP_OK;
+    if (int rc = set_device(g)) return rc;
+    const uint64_t n = g->n_spikes * g->d.C * g->d.N;
+    if (n > g->clips_cap) {
+        (void)hipFree(g->s.clips);
+        g->s.clips = nullptr;
+        g->clips_cap = 0;
+        HIP_TRY(dalloc(&g->s.clips, n), "clips");
+        g->clips_cap = n;
+    }
+    HIP_TRY(launch_clips(g->d, g->s, g->n_spikes, g->own_stream), "clips");
+    HIP_TRY(hipMemcpyAsync(clips, g->s.clips, n * 4, hipMemcpyDeviceToHost, g->own_stream), "clips");
+    HIP_TRY(hipStreamSynchronize(g->own_stream), "clips");
+    return FBP_OK;
+}
+
+int fbp_segment_get_residual(fbp_segment *g, float *residual)
+{
+    if (!g || !residual) return fail(FBP_EINVAL, "NULL argument");
+    if (!g->ran) return fail(FBP_ESTATE, "segment has not run");
+    if (int rc = set_device(g)) return rc;
+    HIP_TRY(hipMemcpy(residual, g->s.V, (uint64_t)g->d.C * g->d.K * 4, hipMemcpyDeviceToHost), "residual");
+    return FBP_OK;
+}
+
+int fbp_segment_get_likelihood(fbp_segment *g, float *likelihood)
+{
+    if (!g || !likelihood) return fail(FBP_EINVAL, "NULL argument");
+    if (!g->ran) return fail(FBP_ESTATE, "segment has not run");
+    if (int rc = set_device(g)) return rc;
+    HIP_TRY(hipMemcpy(likelihood, g->s.L, (uint64_t)g->d.T * g->d.K * 4, hipMemcpyDeviceToHost), "likelihood");
+    return FBP_OK;
+}
+
+int fbp_segment_get_stats(const fbp_segment *g, fbp_stats *stats)
+{
+    if (!g || !stats) return fail(FBP_EINVAL, "NULL argument");
+    *stats = g->stats;
+    return FBP_OK;
+}
+
+}  // extern "C"

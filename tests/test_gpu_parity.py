@@ -1,0 +1,53 @@
+"""HIP path (libfbp.so through its C-ABI) against the oracle, bit for bit.
+
+Float32 arithmetic of both sides follows the reference's fused multiply-add
+order, so the likelihood buffer, the spike list (onset, label, record order),
+the residual and the adjusted clips must be identical, not merely close.
+"""
+import numpy as np
+import pytest
+
+from helpers import run_gpu, run_oracle, small_case
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (seed, C, T, N, duration_s, M, do_overlap, reduce_wg)
+    (1, 4, 5, 61, 0.5, 30, True, 256),
+    (2, 4, 5, 61, 0.5, 60, True, 256),
+    (3, 4, 5, 61, 0.5, 30, False, 256),
+    (4, 8, 7, 41, 0.4, 20, True, 64),
+    (5, 3, 3, 81, 0.4, 40, True, 1024),
+    (6, 2, 1, 9, 0.05, 4, True, 16),
+]
+
+
+def _case(seed, C, T, N, dur):
+    gains = "ladder" if C <= 4 else "gauss"
+    return small_case(seed, C=C, T=T, N=N, duration_s=dur, gains=gains, overlap_frac=0.1 if T > 1 else 0.0)
+
+
+@pytest.mark.parametrize("seed,C,T,N,dur,M,ov,wg", CASES)
+def test_first_pass_likelihood_bitwise(oracle_mod, seed, C, T, N, dur, M, ov, wg):
+    rec = _case(seed, C, T, N, dur)
+    print("case", seed, C, T, N, dur, M, ov, wg, flush=True)
+    g = run_gpu(rec, M, ov, wg, max_passes=1)
+    o = run_oracle(oracle_mod, rec, M, ov, wg, max_passes=1)
+    np.testing.assert_array_equal(g['L'].view(np.uint32), o['L'].view(np.uint32))
+    np.testing.assert_array_equal(g['idx'], o['idx'])
+    np.testing.assert_array_equal(g['lab'], o['lab'])
+
+
+@pytest.mark.parametrize("seed,C,T,N,dur,M,ov,wg", CASES)
+def test_full_pursuit_bitwise(oracle_mod, seed, C, T, N, dur, M, ov, wg):
+    rec = _case(seed, C, T, N, dur)
+    print("case", seed, C, T, N, dur, M, ov, wg, flush=True)
+    g = run_gpu(rec, M, ov, wg)
+    o = run_oracle(oracle_mod, rec, M, ov, wg)
+    assert g['stats']['n_passes'] == o['n_passes']
+    np.testing.assert_array_equal(g['idx'], o['idx'])
+    np.testing.assert_array_equal(g['lab'], o['lab'])
+    np.testing.assert_array_equal(g['residual'].view(np.uint32), o['residual'].view(np.uint32))
+    np.testing.assert_array_equal(g['L'].view(np.uint32), o['L'].view(np.uint32))
+    clips = oracle_mod.adjusted_clips(o['residual'], rec['templates'], rec['N'], o['idx'], o['lab'])
+    np.testing.assert_array_equal(g['clips'].view(np.uint32), clips.view(np.uint32))
