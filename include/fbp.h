@@ -85,8 +85,12 @@ typedef struct fbp_stats {
     uint64_t rescored_positions;  /* low-precision paths: float32 re-scores     */
     uint64_t rescored_combos;     /* screened overlap: direct re-scores         */
     double   ms_total;            /* device time of the last run (HIP events)   */
-    double   ms_correlation;
-    double   ms_overlap;
+    double   ms_correlation;      /* summed over passes, HIP events on the run's stream */
+    double   ms_overlap;          /* overlap recheck + parse + reassignment      */
+    uint64_t corr_launches;       /* correlation kernel launches (one per pass) */
+    uint64_t overlap_launches;    /* overlap kernel launches                    */
+    uint64_t corr_positions;      /* likelihood positions computed, all passes  */
+    uint64_t overlap_combos;      /* (t,f,s) combinations scored, all passes    */
 } fbp_stats;
 
 typedef struct fbp_segment fbp_segment;

@@ -64,6 +64,10 @@ class FbpStats(ctypes.Structure):
         ("ms_total", ctypes.c_double),
         ("ms_correlation", ctypes.c_double),
         ("ms_overlap", ctypes.c_double),
+        ("corr_launches", ctypes.c_uint64),
+        ("overlap_launches", ctypes.c_uint64),
+        ("corr_positions", ctypes.c_uint64),
+        ("overlap_combos", ctypes.c_uint64),
     ]
 
     def as_dict(self):

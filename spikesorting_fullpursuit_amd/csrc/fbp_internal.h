@@ -32,6 +32,7 @@ struct Counters {
     uint32_t n_combo;    // screened overlap: combinations re-scored directly
     uint32_t n_rescore_overflow;
     uint32_t pad[2];
+    unsigned long long combos;   // overlap combinations scored since the run started
 };
 
 // Device buffers of one chunk.
@@ -69,6 +70,7 @@ constexpr int kCompactItems = 1024;   // items per block of the compaction kerne
 // Launch wrappers (kernels.hip).  All enqueue on `st` and return hipError_t.
 hipError_t launch_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_reset_counters(Counters *c, hipStream_t st);
 hipError_t launch_iota_u32(uint32_t *p, uint64_t n, hipStream_t st);
 hipError_t launch_copy_rows(float *dst, const float *src, uint32_t rows, uint64_t cols,
                             uint64_t src_stride, hipStream_t st);

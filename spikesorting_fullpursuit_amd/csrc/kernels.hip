@@ -40,6 +40,11 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
         p[i] = v;
 }
 
+__global__ void k_reset_counters(Counters *c)
+{
+    if (threadIdx.x == 0) *c = Counters{};
+}
+
 __global__ void k_iota_u32(uint32_t *p, uint64_t n)
 {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
@@ -359,10 +364,11 @@ __device__ float overlap_direct_score(const Dims &d, const float *__restrict__ s
 
 __global__ __launch_bounds__(256) void k_overlap_direct(
     Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
-    const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, const Counters *__restrict__ ctr,
+    const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw,
     float *best_L, const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab,
     uint8_t *ov)
 {
+    const Counters *ctr = ctr_rw;
     extern __shared__ float dyn[];
     float *sv = dyn;                       // [C][N] voltage at the best onset
     float *smain = dyn + d.C * d.N;        // [C][N] template of the best label
@@ -381,6 +387,7 @@ __global__ __launch_bounds__(256) void k_overlap_direct(
         const bool wskip = (Lb < thr[lab]) || ((uint64_t)bidx >= (uint64_t)(d.K - d.N));   // bp.cl:518-527
         float sc = -INFINITY;
         uint64_t key = ~0ull, bp = 0;
+        uint32_t scored = 0;
         if (!wskip) {
             __syncthreads();
             for (uint32_t e = threadIdx.x; e < d.C * N; e += blockDim.x) {
@@ -401,6 +408,7 @@ __global__ __launch_bounds__(256) void k_overlap_direct(
                 const float Ls = L[(uint64_t)t * d.K + bidx + s - M];
                 if (Lf <= 0.0f && Ls <= 0.0f) continue;                                                 // bp.cl:572-576
                 const float score = overlap_direct_score(d, sv, smain, Tm, t, f, s);
+                scored++;
                 uint64_t p = (uint64_t)t + (uint64_t)T * (f + (uint64_t)S * s);
                 uint64_t g = p / d.reduce_wg;
                 uint32_t br = bitrev_bits((uint32_t)(p % d.reduce_wg), wg_bits);
@@ -415,7 +423,9 @@ __global__ __launch_bounds__(256) void k_overlap_direct(
                 const float o = __shfl_xor(sc, off);
                 const uint64_t ok = __shfl_xor(key, off), op = __shfl_xor(bp, off);
                 if (key_better(o, ok, sc, key)) { sc = o; key = ok; bp = op; }
+                scored += __shfl_xor(scored, off);
             }
+            if (lane_of() == 0 && scored) atomicAdd(&ctr_rw->combos, (unsigned long long)scored);
             if (lane_of() == 0) { r_sc[threadIdx.x >> 6] = sc; r_key[threadIdx.x >> 6] = key; r_p[threadIdx.x >> 6] = bp; }
             __syncthreads();
             if (threadIdx.x == 0)
@@ -625,6 +635,12 @@ hipError_t launch_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t st)
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_counters(Counters *c, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c);
     return hipGetLastError();
 }
 

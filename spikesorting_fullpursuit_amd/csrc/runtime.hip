@@ -58,6 +58,7 @@ struct fbp_segment {
     DeviceState s{};
     hipStream_t own_stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipEvent_t ev_stage[4] = {nullptr, nullptr, nullptr, nullptr};   // correlation / overlap brackets
     Counters *h_ctr = nullptr;   // pinned host copy of the device counters
     uint64_t cap = 0;            // spike capacity of sp_idx / sp_lab
     uint64_t n_spikes = 0;
@@ -215,6 +216,7 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&g->ev_start);
     if (e == hipSuccess) e = hipEventCreate(&g->ev_stop);
+    for (int q = 0; q < 4 && e == hipSuccess; q++) e = hipEventCreate(&g->ev_stage[q]);
     if (e == hipSuccess) e = hipHostMalloc((void **)&g->h_ctr, sizeof(Counters), hipHostMallocDefault);
     if (e != hipSuccess) {
         rc = hip_fail(e, "fbp_segment_create");
@@ -235,6 +237,8 @@ void fbp_segment_destroy(fbp_segment *g)
     free_state(g);
     if (g->ev_start) (void)hipEventDestroy(g->ev_start);
     if (g->ev_stop) (void)hipEventDestroy(g->ev_stop);
+    for (int q = 0; q < 4; q++)
+        if (g->ev_stage[q]) (void)hipEventDestroy(g->ev_stage[q]);
     if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
     if (g->h_ctr) (void)hipHostFree(g->h_ctr);
     delete g;
@@ -309,6 +313,7 @@ int fbp_segment_run(fbp_segment *g, void *stream)
     HIP_TRY(launch_fill_u8(s.nc, d.W + 2, 0, st), "init");
     HIP_TRY(launch_fill_u8(s.ov, d.W + 2, 0, st), "init");
     HIP_TRY(launch_iota_u32(s.win, d.W, st), "init");
+    HIP_TRY(launch_reset_counters(s.ctr, st), "init");
 
     uint32_t nw = d.W;
     uint64_t total = 0;
@@ -318,12 +323,16 @@ int fbp_segment_run(fbp_segment *g, void *stream)
         if (g->cfg.max_passes && passes >= g->cfg.max_passes) { rc = fail(FBP_EPASSES, "pass cap %u reached", passes); break; }
         passes++;
         if ((rc = ensure_spike_capacity(g, total + nw, st))) break;
+        HIP_TRY(hipEventRecord(g->ev_stage[0], st), "event");
         HIP_TRY(launch_corr_fp32(d, s, nw, st), "correlation");                   // bpp.py:538-549
+        HIP_TRY(hipEventRecord(g->ev_stage[1], st), "event");
         HIP_TRY(launch_argmax(d, s, nw, st), "argmax");                            // bpp.py:551-562
         if (g->cfg.do_overlap) {                                                   // bpp.py:565-661
             HIP_TRY(launch_compact_ov(d, s, nw, st), "overlap list");
+            HIP_TRY(hipEventRecord(g->ev_stage[2], st), "event");
             HIP_TRY(launch_overlap_direct(d, s, nw, st), "overlap recheck");
             HIP_TRY(launch_check_overlap(d, s, nw, st), "overlap reassign");
+            HIP_TRY(hipEventRecord(g->ev_stage[3], st), "event");
         }
         HIP_TRY(launch_pursuit_add(d, s, nw, st), "pursuit add");                  // bpp.py:663-671
         HIP_TRY(launch_compact_spikes(d, s, nw, total, st), "spike list");
@@ -333,7 +342,17 @@ int fbp_segment_run(fbp_segment *g, void *stream)
         HIP_TRY(hipStreamSynchronize(st), "pass");
         const uint32_t n_new = g->h_ctr->n_new;
         g->stats.windows_evaluated += nw;
-        if (g->cfg.do_overlap) g->stats.overlap_windows += g->h_ctr->n_ov;
+        g->stats.corr_launches++;
+        g->stats.corr_positions += (uint64_t)nw * d.N;
+        float ms_c = 0.0f, ms_o = 0.0f;
+        (void)hipEventElapsedTime(&ms_c, g->ev_stage[0], g->ev_stage[1]);
+        g->stats.ms_correlation += ms_c;
+        if (g->cfg.do_overlap) {
+            g->stats.overlap_windows += g->h_ctr->n_ov;
+            g->stats.overlap_launches++;
+            (void)hipEventElapsedTime(&ms_o, g->ev_stage[2], g->ev_stage[3]);
+            g->stats.ms_overlap += ms_o;
+        }
         if (n_new == 0) break;                                                     // bpp.py:675-676
         HIP_TRY(launch_fill_u8(s.ov, d.W + 2, 0, st), "reset");                  // bpp.py:679-680
         HIP_TRY(launch_residual(d, s, total, n_new, st), "residual");             // bpp.py:716-731
@@ -351,6 +370,7 @@ int fbp_segment_run(fbp_segment *g, void *stream)
     g->stats.n_passes = passes;
     g->stats.n_spikes = total;
     g->stats.ms_total = ms;
+    g->stats.overlap_combos = g->h_ctr->combos;
     // Results stay readable when the pass cap stopped the loop.
     g->ran = (rc == FBP_OK || rc == FBP_EPASSES);
     return rc;

@@ -121,15 +121,26 @@ def make_recording(seed, n_chans, fs, duration_s, n_templates, n_samples_per_cha
     else:
         voltage = np.zeros((C, K), dtype=np.float32)
     tm3 = templates.reshape(T, C, N)
+    onsets, labels = [], []
     for t in range(T):
         on = events[t] - clip_init
         on = on[(on >= 0) & (on + N <= K)]
         events[t] = on + clip_init
-        idx = on[:, None] + np.arange(N)[None, :]
+        onsets.append(on)
+        labels.append(np.full(on.size, t, dtype=np.int64))
+    if onsets:
+        on_all = np.concatenate(onsets)
+        lab_all = np.concatenate(labels)
+        # Sum every spike's contribution per channel in float64 (bincount is
+        # deterministic), then add it to the noise once.
         for c in range(C):
-            if not np.any(tm3[t, c]):
+            vals = tm3[lab_all, c, :].astype(np.float64)
+            keep = np.any(vals != 0.0, axis=1)
+            if not np.any(keep):
                 continue
-            np.add.at(voltage[c], idx.ravel(), np.tile(tm3[t, c], on.size))
+            idx = (on_all[keep, None] + np.arange(N)[None, :]).ravel()
+            add = np.bincount(idx, weights=vals[keep].ravel(), minlength=K)
+            voltage[c] += add.astype(np.float32)
     voltage = voltage.astype(dtype, copy=False)
 
     ss = np.sum(templates.astype(np.float64) ** 2, axis=1)
