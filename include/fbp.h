@@ -83,14 +83,15 @@ typedef struct fbp_stats {
     uint64_t windows_evaluated;   /* sum over passes of W_p                     */
     uint64_t overlap_windows;     /* sum over passes of O_p                     */
     uint64_t rescored_positions;  /* low-precision paths: float32 re-scores     */
-    uint64_t rescored_combos;     /* screened overlap: direct re-scores         */
+    uint64_t screened_combos;     /* screened overlap: float64 screen evaluations */
     double   ms_total;            /* device time of the last run (HIP events)   */
     double   ms_correlation;      /* summed over passes, HIP events on the run's stream */
     double   ms_overlap;          /* overlap recheck + parse + reassignment      */
     uint64_t corr_launches;       /* correlation kernel launches (one per pass) */
     uint64_t overlap_launches;    /* overlap kernel launches                    */
     uint64_t corr_positions;      /* likelihood positions computed, all passes  */
-    uint64_t overlap_combos;      /* (t,f,s) combinations scored, all passes    */
+    uint64_t overlap_combos;      /* (t,f,s) combinations scored with the reference's
+                                     float32 chain, all passes                    */
 } fbp_stats;
 
 typedef struct fbp_segment fbp_segment;

@@ -60,7 +60,7 @@ class FbpStats(ctypes.Structure):
         ("windows_evaluated", ctypes.c_uint64),
         ("overlap_windows", ctypes.c_uint64),
         ("rescored_positions", ctypes.c_uint64),
-        ("rescored_combos", ctypes.c_uint64),
+        ("screened_combos", ctypes.c_uint64),
         ("ms_total", ctypes.c_double),
         ("ms_correlation", ctypes.c_double),
         ("ms_overlap", ctypes.c_double),

@@ -32,7 +32,8 @@ struct Counters {
     uint32_t n_combo;    // screened overlap: combinations re-scored directly
     uint32_t n_rescore_overflow;
     uint32_t pad[2];
-    unsigned long long combos;   // overlap combinations scored since the run started
+    unsigned long long combos;   // overlap combinations scored exactly since the run started
+    unsigned long long screened; // combinations screened in float64 (screened overlap)
 };
 
 // Device buffers of one chunk.
@@ -63,6 +64,9 @@ struct DeviceState {
     uint32_t *sp_idx;    // [cap] spike onsets, record order
     uint32_t *sp_lab;    // [cap]
     float *clips;        // [n][C*N] adjusted clips (allocated on demand)
+    float *tabAB;        // [T][S][T][S] screened overlap: shifted-template cross products
+    float *tabB2;        // [T][S]       squared norms of shifted, truncated templates
+    float *tnorm;        // [T]          template norms (upper bounds)
 };
 
 constexpr int kCompactItems = 1024;   // items per block of the compaction kernels
@@ -81,6 +85,9 @@ hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win,
 hipError_t launch_argmax(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
 hipError_t launch_compact_ov(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
 hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);
+size_t overlap_screened_lds(const Dims &d);
+hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_t st);
+hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);
 hipError_t launch_check_overlap(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);
 hipError_t launch_pursuit_add(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
 hipError_t launch_compact_spikes(const Dims &d, const DeviceState &s, uint32_t n_win,

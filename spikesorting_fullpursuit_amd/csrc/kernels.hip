@@ -336,6 +336,67 @@ __device__ __forceinline__ bool key_better(float a, uint64_t ka, float b, uint64
     return a > b || (a == b && ka < kb);
 }
 
+// Winner-order key of one scored combination: (reduction group, bit-reversed
+// lane) -- smaller is better among equal scores.  p_out is the reference work id
+// that carries the key (a leftover duplicate of a single-group window may
+// carry it, which parse then rejects).
+__device__ __forceinline__ uint64_t combo_key(const Dims &d, uint32_t wg_bits, uint64_t span, uint32_t t,
+                                              uint32_t f, uint32_t s, uint64_t &p_out)
+{
+    uint64_t p = (uint64_t)t + (uint64_t)d.T * (f + (uint64_t)d.S * s);
+    const uint64_t g = p / d.reduce_wg;
+    uint32_t br = bitrev_bits((uint32_t)(p % d.reduce_wg), wg_bits);
+    for (uint64_t p2 = p + d.items; p2 < span && p2 / d.reduce_wg == g; p2 += d.items) {
+        const uint32_t br2 = bitrev_bits((uint32_t)(p2 % d.reduce_wg), wg_bits);
+        if (br2 < br) { br = br2; p = p2; }
+    }
+    p_out = p;
+    return (g << 32) | br;
+}
+
+// parse_overlap_recheck_indices (bp.cl:690-814) for one window, given the best
+// scored combination (score sc, reference work id bp).  Thread 0 only.
+__device__ void parse_overlap(const Dims &d, const float *__restrict__ L, const float *__restrict__ thr, uint32_t id,
+                              uint32_t lab, uint32_t bidx, float bl, float sc, uint64_t bp, float *best_L,
+                              uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov)
+{
+    const uint32_t S = d.S, T = d.T, M = d.M;
+    const uint64_t span = d.nl * d.reduce_wg;
+    bool keep = sc > bl && sc > 0.0f;                          // bp.cl:707-711
+    if (keep && bp % span >= d.items) keep = false;            // bp.cl:714-718
+    if (keep) {
+        const uint32_t t = (uint32_t)(bp % T);
+        const uint32_t f = (uint32_t)((bp / T) % S);
+        const uint32_t s = (uint32_t)((bp / ((uint64_t)S * T)) % S);
+        const uint32_t af = bidx + f - M, as = bidx + s - M;
+        const float Lm = L[(uint64_t)lab * d.K + af];
+        const float Lo = L[(uint64_t)t * d.K + as];
+        if (sc < Lm || sc < Lo) keep = false;                  // bp.cl:733-738
+        else if (Lm <= 0.0f || Lo <= 0.0f) keep = false;       // bp.cl:739-744
+        else if (Lm >= Lo && Lm > thr[lab]) { best_L[id] = Lm; ov_lab[id] = lab; ov_idx[id] = af; }
+        else if (Lo > Lm && Lo > thr[t]) { best_L[id] = Lo; ov_lab[id] = t; ov_idx[id] = as; }
+        else if (Lm > thr[lab]) { best_L[id] = Lm; ov_lab[id] = lab; ov_idx[id] = af; }
+        else { ov_idx[id] = bidx; ov_lab[id] = lab; keep = false; }   // bp.cl:806-814
+    }
+    if (!keep) ov[id] = 0;
+}
+
+// Block-wide reduction of (score, key, work id) in winner order; the result is
+// valid in thread 0.
+__device__ void block_key_reduce(float &sc, uint64_t &key, uint64_t &bp, float *r_sc, uint64_t *r_key, uint64_t *r_p)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const float o = __shfl_xor(sc, off);
+        const uint64_t ok = __shfl_xor(key, off), op = __shfl_xor(bp, off);
+        if (key_better(o, ok, sc, key)) { sc = o; key = ok; bp = op; }
+    }
+    if (lane_of() == 0) { r_sc[threadIdx.x >> 6] = sc; r_key[threadIdx.x >> 6] = key; r_p[threadIdx.x >> 6] = bp; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++)
+            if (key_better(r_sc[q], r_key[q], sc, key)) { sc = r_sc[q]; key = r_key[q]; bp = r_p[q]; }
+}
+
 __device__ float overlap_direct_score(const Dims &d, const float *__restrict__ sv, const float *__restrict__ smain,
                                       const float *__restrict__ Tm, uint32_t t, uint32_t f, uint32_t s)
 {
@@ -378,7 +439,7 @@ __global__ __launch_bounds__(256) void k_overlap_direct(
     const uint32_t n_ov = ctr->n_ov;
     const uint32_t wg_bits = 31 - __builtin_clz(d.reduce_wg);
     const uint64_t span = d.nl * d.reduce_wg;
-    const uint32_t S = d.S, T = d.T, N = d.N, M = d.M;
+    const uint32_t S = d.S, N = d.N, M = d.M;
     for (uint32_t k = blockIdx.x; k < n_ov; k += gridDim.x) {
         const uint32_t id = ov_list[k];
         const uint32_t lab = best_lab[id], bidx = best_idx[id];
@@ -409,49 +470,209 @@ __global__ __launch_bounds__(256) void k_overlap_direct(
                 if (Lf <= 0.0f && Ls <= 0.0f) continue;                                                 // bp.cl:572-576
                 const float score = overlap_direct_score(d, sv, smain, Tm, t, f, s);
                 scored++;
-                uint64_t p = (uint64_t)t + (uint64_t)T * (f + (uint64_t)S * s);
-                uint64_t g = p / d.reduce_wg;
-                uint32_t br = bitrev_bits((uint32_t)(p % d.reduce_wg), wg_bits);
-                for (uint64_t p2 = p + d.items; p2 < span && p2 / d.reduce_wg == g; p2 += d.items) {
-                    const uint32_t br2 = bitrev_bits((uint32_t)(p2 % d.reduce_wg), wg_bits);
-                    if (br2 < br) { br = br2; p = p2; }
-                }
-                const uint64_t kk = (g << 32) | br;
+                uint64_t p;
+                const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
                 if (key_better(score, kk, sc, key)) { sc = score; key = kk; bp = p; }
             }
-            for (int off = 32; off > 0; off >>= 1) {
-                const float o = __shfl_xor(sc, off);
-                const uint64_t ok = __shfl_xor(key, off), op = __shfl_xor(bp, off);
-                if (key_better(o, ok, sc, key)) { sc = o; key = ok; bp = op; }
-                scored += __shfl_xor(scored, off);
-            }
+            for (int off = 32; off > 0; off >>= 1) scored += __shfl_xor(scored, off);
             if (lane_of() == 0 && scored) atomicAdd(&ctr_rw->combos, (unsigned long long)scored);
-            if (lane_of() == 0) { r_sc[threadIdx.x >> 6] = sc; r_key[threadIdx.x >> 6] = key; r_p[threadIdx.x >> 6] = bp; }
-            __syncthreads();
-            if (threadIdx.x == 0)
-                for (int q = 1; q < (int)(blockDim.x >> 6); q++)
-                    if (key_better(r_sc[q], r_key[q], sc, key)) { sc = r_sc[q]; key = r_key[q]; bp = r_p[q]; }
+            block_key_reduce(sc, key, bp, r_sc, r_key, r_p);
         }
-        if (threadIdx.x == 0) {
-            // parse_overlap_recheck_indices (bp.cl:690-814)
-            bool keep = sc > bl && sc > 0.0f;                          // bp.cl:707-711
-            if (keep && bp % span >= d.items) keep = false;            // bp.cl:714-718
-            if (keep) {
-                const uint32_t t = (uint32_t)(bp % T);
-                const uint32_t f = (uint32_t)((bp / T) % S);
-                const uint32_t s = (uint32_t)((bp / ((uint64_t)S * T)) % S);
-                const uint32_t af = bidx + f - M, as = bidx + s - M;
-                const float Lm = L[(uint64_t)lab * d.K + af];
-                const float Lo = L[(uint64_t)t * d.K + as];
-                if (sc < Lm || sc < Lo) keep = false;                  // bp.cl:733-738
-                else if (Lm <= 0.0f || Lo <= 0.0f) keep = false;       // bp.cl:739-744
-                else if (Lm >= Lo && Lm > thr[lab]) { best_L[id] = Lm; ov_lab[id] = lab; ov_idx[id] = af; }
-                else if (Lo > Lm && Lo > thr[t]) { best_L[id] = Lo; ov_lab[id] = t; ov_idx[id] = as; }
-                else if (Lm > thr[lab]) { best_L[id] = Lm; ov_lab[id] = lab; ov_idx[id] = af; }
-                else { ov_idx[id] = bidx; ov_lab[id] = lab; keep = false; }   // bp.cl:806-814
+        if (threadIdx.x == 0) parse_overlap(d, L, thr, id, lab, bidx, bl, sc, bp, best_L, ov_idx, ov_lab, ov);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Overlap recheck, screened form.  The pair score of bp.cl:599-631 is, in exact
+// arithmetic, a sum of terms that separate:
+//   score(t,f,s) = Q(lab,f) + Q(t,s) - B2(lab,f)/2 - B2(t,s)/2 - AB(lab,f,t,s)
+// with Q(t,s) = <template t shifted by s-M, truncated to the window ; voltage>
+// (per window, T*S values), B2(t,s) its squared norm and AB the cross product of
+// two shifted, truncated templates (both voltage-free tables built once per
+// template set).  The screen evaluates that sum in float64 for every
+// combination the reference does not skip, and bounds |screen - float32 score|
+// by E = ecoef * ((|t_lab|+|t_t|)*|v| + (|t_lab|+|t_t|)^2), which covers the
+// float32 rounding of the reference's fused chain (gamma_{CN+2C+8} with a
+// doubled unit roundoff) and the float32 storage of Q and the tables.  Every
+// combination whose upper bound reaches the largest lower bound is then scored
+// exactly with the reference's float32 chain (overlap_direct_score) and reduced
+// in the reference's winner order, so the result equals the direct form's.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_overlap_tables(Dims d, const float *__restrict__ Tm, float *__restrict__ AB,
+                                                       float *__restrict__ B2, float *__restrict__ tnorm)
+{
+    const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
+    const uint64_t nab = (uint64_t)T * S * T * S;
+    const uint64_t total = nab + (uint64_t)T * S + T;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        if (e < nab) {
+            const uint32_t s = (uint32_t)(e % S);
+            const uint32_t t = (uint32_t)((e / S) % T);
+            const uint32_t f = (uint32_t)((e / ((uint64_t)S * T)) % S);
+            const uint32_t lab = (uint32_t)(e / ((uint64_t)S * T * S));
+            const int32_t fa = (int32_t)M - (int32_t)f, sa = (int32_t)M - (int32_t)s;
+            const int32_t lo = max(0, max(-fa, -sa));
+            const int32_t hi = min((int32_t)N, min((int32_t)N - fa, (int32_t)N - sa));
+            double acc = 0.0;
+            for (uint32_t c = 0; c < C; c++) {
+                const float *a = Tm + ((uint64_t)lab * C + c) * N;
+                const float *b = Tm + ((uint64_t)t * C + c) * N;
+                for (int32_t j = lo; j < hi; j++) acc = __builtin_fma((double)a[j + fa], (double)b[j + sa], acc);
             }
-            if (!keep) ov[id] = 0;
+            AB[e] = (float)acc;
+        } else if (e < nab + (uint64_t)T * S) {
+            const uint64_t q = e - nab;
+            const uint32_t s = (uint32_t)(q % S), t = (uint32_t)(q / S);
+            const int32_t sa = (int32_t)M - (int32_t)s;
+            const int32_t lo = max(0, -sa), hi = min((int32_t)N, (int32_t)N - sa);
+            double acc = 0.0;
+            for (uint32_t c = 0; c < C; c++) {
+                const float *b = Tm + ((uint64_t)t * C + c) * N;
+                for (int32_t j = lo; j < hi; j++) acc = __builtin_fma((double)b[j + sa], (double)b[j + sa], acc);
+            }
+            B2[q] = (float)acc;
+        } else {
+            const uint32_t t = (uint32_t)(e - nab - (uint64_t)T * S);
+            double acc = 0.0;
+            for (uint64_t k = 0; k < (uint64_t)C * N; k++) {
+                const double x = Tm[(uint64_t)t * C * N + k];
+                acc = __builtin_fma(x, x, acc);
+            }
+            // upper bound of the norm, rounded up to float
+            const double nrm = sqrt(acc) * (1.0 + 1e-12);
+            float up = (float)nrm;
+            if ((double)up < nrm) up = __uint_as_float(__float_as_uint(up) + 1u);
+            tnorm[t] = up;
         }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_overlap_screened(
+    Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
+    const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw, float *best_L,
+    const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov,
+    const float *__restrict__ AB, const float *__restrict__ B2, const float *__restrict__ tnorm)
+{
+    extern __shared__ float dyn[];
+    const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
+    float *sv = dyn;                                   // [C][N] voltage at the best onset
+    float *smain = sv + C * N;                         // [C][N] template of the best label
+    float *sQ = smain + C * N;                         // [T][S] truncated correlations
+    uint8_t *spos = (uint8_t *)(sQ + (uint64_t)T * S); // [T][S] L[t, bidx+s-M] > 0 (in bounds only)
+    __shared__ float r_sc[4];
+    __shared__ uint64_t r_key[4], r_p[4];
+    __shared__ double r_d[4][2];
+    const Counters *ctr = ctr_rw;
+    const uint32_t n_ov = ctr->n_ov;
+    const uint32_t wg_bits = 31 - __builtin_clz(d.reduce_wg);
+    const uint64_t span = d.nl * d.reduce_wg;
+    // E coefficient: float32 chain bound plus float32 storage of Q and tables.
+    const double nops = (double)C * N + 2.0 * C + 8.0;
+    const double u2 = 0x1p-23;
+    const double g = nops * u2 / (1.0 - nops * u2);
+    const double ecoef = (g + 2.0 * u2) * 1.01;
+    const uint64_t ST = (uint64_t)S * T;
+    for (uint32_t k = blockIdx.x; k < n_ov; k += gridDim.x) {
+        const uint32_t id = ov_list[k];
+        const uint32_t lab = best_lab[id], bidx = best_idx[id];
+        const float bl = best_L[id];
+        const float Lb = L[(uint64_t)lab * d.K + bidx];
+        const bool wskip = (Lb < thr[lab]) || ((uint64_t)bidx >= (uint64_t)(d.K - d.N));   // bp.cl:518-527
+        float sc = -INFINITY;
+        uint64_t key = ~0ull, bp = 0;
+        if (!wskip) {
+            __syncthreads();
+            double vv = 0.0;
+            for (uint32_t e = threadIdx.x; e < C * N; e += blockDim.x) {
+                const uint32_t c = e / N, j = e % N;
+                const float x = V[(uint64_t)c * d.K + bidx + j];
+                sv[e] = x;
+                vv = __builtin_fma((double)x, (double)x, vv);
+                smain[e] = Tm[((uint64_t)lab * C + c) * N + j];
+            }
+            for (uint32_t e = threadIdx.x; e < T * S; e += blockDim.x) {
+                const uint32_t t = e / S, s = e % S;
+                const bool inb = !((uint64_t)s + bidx < M || (uint64_t)s + bidx >= (uint64_t)d.K - N + M);   // bp.cl:534,538
+                spos[e] = (inb ? 2 : 0) | ((inb && L[(uint64_t)t * d.K + bidx + s - M] > 0.0f) ? 1 : 0);
+            }
+            for (int off = 32; off > 0; off >>= 1) vv += __shfl_xor(vv, off);
+            if (lane_of() == 0) r_d[threadIdx.x >> 6][0] = vv;
+            __syncthreads();
+            const double vn = sqrt(r_d[0][0] + r_d[1][0] + r_d[2][0] + r_d[3][0]) * (1.0 + 1e-12);
+            for (uint32_t e = threadIdx.x; e < T * S; e += blockDim.x) {
+                const uint32_t t = e / S, s = e % S;
+                const int32_t sa = (int32_t)M - (int32_t)s;
+                const int32_t lo = max(0, -sa), hi = min((int32_t)N, (int32_t)N - sa);
+                double q = 0.0;
+                for (uint32_t c = 0; c < C; c++) {
+                    const float *b = Tm + ((uint64_t)t * C + c) * N + sa;
+                    const float *vc = sv + c * N;
+                    for (int32_t j = lo; j < hi; j++) q = __builtin_fma((double)b[j], (double)vc[j], q);
+                }
+                sQ[e] = (float)q;
+            }
+            __syncthreads();
+            const double nlab = tnorm[lab];
+            const float *ABl = AB + (uint64_t)lab * S * ST;
+            // pass A: largest lower bound and largest upper bound of the score
+            double lo_max = -INFINITY, hi_max = -INFINITY;
+            uint32_t screened = 0;
+            for (uint64_t q = threadIdx.x; q < d.items; q += blockDim.x) {
+                const uint32_t s = (uint32_t)(q % S);
+                const uint32_t t = (uint32_t)((q / S) % T);
+                const uint32_t f = (uint32_t)(q / ST);
+                const uint8_t pf = spos[lab * S + f], ps = spos[t * S + s];
+                if (!(pf & 2) || !(ps & 2)) continue;                          // bp.cl:534-541
+                if ((f > s ? f - s : s - f) > N) continue;                     // bp.cl:543-556
+                if (!(pf & 1) && !(ps & 1)) continue;                          // bp.cl:572-576
+                const double nsum = nlab + (double)tnorm[t];
+                const double E = ecoef * (nsum * vn + nsum * nsum);
+                const double sh = (double)sQ[lab * S + f] + (double)sQ[t * S + s]
+                                  - 0.5 * ((double)B2[lab * S + f] + (double)B2[t * S + s]) - (double)ABl[q];
+                lo_max = fmax(lo_max, sh - E);
+                hi_max = fmax(hi_max, sh + E);
+                screened++;
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                lo_max = fmax(lo_max, __shfl_xor(lo_max, off));
+                hi_max = fmax(hi_max, __shfl_xor(hi_max, off));
+                screened += __shfl_xor(screened, off);
+            }
+            __syncthreads();
+            if (lane_of() == 0) { r_d[threadIdx.x >> 6][0] = lo_max; r_d[threadIdx.x >> 6][1] = hi_max; }
+            if (lane_of() == 0 && screened) atomicAdd(&ctr_rw->screened, (unsigned long long)screened);
+            __syncthreads();
+            lo_max = fmax(fmax(r_d[0][0], r_d[1][0]), fmax(r_d[2][0], r_d[3][0]));
+            hi_max = fmax(fmax(r_d[0][1], r_d[1][1]), fmax(r_d[2][1], r_d[3][1]));
+            // Nothing can beat best_L: parse rejects without any exact score.
+            if (hi_max > (double)bl) {
+                uint32_t scored = 0;
+                for (uint64_t q = threadIdx.x; q < d.items; q += blockDim.x) {
+                    const uint32_t s = (uint32_t)(q % S);
+                    const uint32_t t = (uint32_t)((q / S) % T);
+                    const uint32_t f = (uint32_t)(q / ST);
+                    const uint8_t pf = spos[lab * S + f], ps = spos[t * S + s];
+                    if (!(pf & 2) || !(ps & 2)) continue;
+                    if ((f > s ? f - s : s - f) > N) continue;
+                    if (!(pf & 1) && !(ps & 1)) continue;
+                    const double nsum = nlab + (double)tnorm[t];
+                    const double E = ecoef * (nsum * vn + nsum * nsum);
+                    const double sh = (double)sQ[lab * S + f] + (double)sQ[t * S + s]
+                                      - 0.5 * ((double)B2[lab * S + f] + (double)B2[t * S + s]) - (double)ABl[q];
+                    if (sh + E < lo_max) continue;
+                    const float score = overlap_direct_score(d, sv, smain, Tm, t, f, s);
+                    scored++;
+                    uint64_t p;
+                    const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
+                    if (key_better(score, kk, sc, key)) { sc = score; key = kk; bp = p; }
+                }
+                for (int off = 32; off > 0; off >>= 1) scored += __shfl_xor(scored, off);
+                if (lane_of() == 0 && scored) atomicAdd(&ctr_rw->combos, (unsigned long long)scored);
+                block_key_reduce(sc, key, bp, r_sc, r_key, r_p);
+            }
+        }
+        if (threadIdx.x == 0) parse_overlap(d, L, thr, id, lab, bidx, bl, sc, bp, best_L, ov_idx, ov_lab, ov);
     }
 }
 
@@ -694,9 +915,42 @@ hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t m
 {
     if (max_ov == 0) return hipSuccess;
     const size_t lds = 2ull * d.C * d.N * sizeof(float);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_overlap_direct,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     const uint32_t grid = max_ov < 4096 ? max_ov : 4096;
     hipLaunchKernelGGL(k_overlap_direct, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr, s.ov_list, s.ctr,
                        s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov);
+    return hipGetLastError();
+}
+
+size_t overlap_screened_lds(const Dims &d)
+{
+    return 2ull * d.C * d.N * sizeof(float) + (uint64_t)d.T * d.S * (sizeof(float) + 1) + 16;
+}
+
+hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_t st)
+{
+    const uint64_t total = (uint64_t)d.T * d.S * d.T * d.S + (uint64_t)d.T * d.S + d.T;
+    hipLaunchKernelGGL(k_overlap_tables, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st, d, s.Tm, s.tabAB,
+                       s.tabB2, s.tnorm);
+    return hipGetLastError();
+}
+
+hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st)
+{
+    if (max_ov == 0) return hipSuccess;
+    const size_t lds = overlap_screened_lds(d);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_overlap_screened,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const uint32_t grid = max_ov < 4096 ? max_ov : 4096;
+    hipLaunchKernelGGL(k_overlap_screened, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr, s.ov_list,
+                       s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB, s.tabB2, s.tnorm);
     return hipGetLastError();
 }
 

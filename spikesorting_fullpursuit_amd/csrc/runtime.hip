@@ -48,6 +48,7 @@ int hip_fail(hipError_t e, const char *what)
     } while (0)
 
 constexpr uint32_t kTemplateBlock = 8;   // must match the correlation kernel's kCorrTB
+constexpr uint64_t kMaxLds = 160 * 1024 - 1024;   // LDS a gfx950 work-group may allocate
 
 }  // namespace
 
@@ -80,7 +81,7 @@ void free_state(fbp_segment *g)
     DeviceState &s = g->s;
     void *ptrs[] = {s.V, s.V0, s.Tm, s.TmT, s.bias, s.thr, s.L, s.best_L, s.best_lab, s.best_idx, s.ov_idx,
                     s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
-                    s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips};
+                    s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabB2, s.tnorm};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     s = DeviceState{};
@@ -92,7 +93,8 @@ int set_device(const fbp_segment *g)
     return FBP_OK;
 }
 
-int ensure_spike_capacity(fbp_segment *g, uint64_t need, hipStream_t st)
+// Grow the spike list to hold `need` entries, keeping the first `keep`.
+int ensure_spike_capacity(fbp_segment *g, uint64_t need, uint64_t keep, hipStream_t st)
 {
     if (need <= g->cap) return FBP_OK;
     uint64_t cap = g->cap * 2;
@@ -101,9 +103,9 @@ int ensure_spike_capacity(fbp_segment *g, uint64_t need, hipStream_t st)
     HIP_TRY(dalloc(&ni, cap), "spike buffer");
     hipError_t e = dalloc(&nl, cap);
     if (e != hipSuccess) { (void)hipFree(ni); return hip_fail(e, "spike buffer"); }
-    if (g->n_spikes > 0) {
-        HIP_TRY(hipMemcpyAsync(ni, g->s.sp_idx, g->n_spikes * 4, hipMemcpyDeviceToDevice, st), "grow spikes");
-        HIP_TRY(hipMemcpyAsync(nl, g->s.sp_lab, g->n_spikes * 4, hipMemcpyDeviceToDevice, st), "grow spikes");
+    if (keep > 0) {
+        HIP_TRY(hipMemcpyAsync(ni, g->s.sp_idx, keep * 4, hipMemcpyDeviceToDevice, st), "grow spikes");
+        HIP_TRY(hipMemcpyAsync(nl, g->s.sp_lab, keep * 4, hipMemcpyDeviceToDevice, st), "grow spikes");
     }
     HIP_TRY(hipStreamSynchronize(st), "grow spikes");
     (void)hipFree(g->s.sp_idx);
@@ -161,9 +163,9 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         return fail(FBP_EINVAL, "n_max_shift_inds (%u) must be below 4x the template length", cfg->max_shift);
     if (cfg->precision != FBP_PREC_FP32)
         return fail(FBP_EINVAL, "precision %u is not available in this build", cfg->precision);
-    if (cfg->overlap_mode != FBP_OVERLAP_DIRECT)
-        return fail(FBP_EINVAL, "overlap mode %u is not available in this build", cfg->overlap_mode);
-    if ((uint64_t)C * N * sizeof(float) * 2 > 64 * 1024 - 256)
+    if (cfg->overlap_mode != FBP_OVERLAP_DIRECT && cfg->overlap_mode != FBP_OVERLAP_SCREENED)
+        return fail(FBP_EINVAL, "unknown overlap mode %u", cfg->overlap_mode);
+    if ((uint64_t)C * N * sizeof(float) * 2 > kMaxLds)
         return fail(FBP_EINVAL, "C*N = %u exceeds the overlap kernel's LDS staging", C * N);
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(FBP_ENODEV, "No GPU available");
@@ -181,6 +183,10 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     d.reduce_wg = cfg->reduce_wg;
     d.items = (uint64_t)d.S * d.S * T;
     d.nl = (d.items - 1) / cfg->reduce_wg + 1;
+    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED && overlap_screened_lds(d) > kMaxLds) {
+        delete g;
+        return fail(FBP_EINVAL, "screened overlap needs %zu bytes of LDS (T*S too large)", overlap_screened_lds(d));
+    }
 
     int rc = FBP_OK;
     hipError_t e = hipSetDevice(device);
@@ -212,6 +218,11 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     ALLOC(s.win_spike, W)
     ALLOC(s.blk, nb + 1)
     ALLOC(s.ctr, 1)
+    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED) {
+        ALLOC(s.tabAB, (uint64_t)T * d.S * T * d.S)
+        ALLOC(s.tabB2, (uint64_t)T * d.S)
+        ALLOC(s.tnorm, T)
+    }
 #undef ALLOC
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&g->ev_start);
@@ -223,7 +234,7 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         fbp_segment_destroy(g);
         return rc;
     }
-    rc = ensure_spike_capacity(g, W, g->own_stream);
+    rc = ensure_spike_capacity(g, W, 0, g->own_stream);
     if (rc) { fbp_segment_destroy(g); return rc; }
     *out = g;
     return FBP_OK;
@@ -256,6 +267,7 @@ int fbp_segment_set_templates(fbp_segment *g, const float *templates, const floa
     if (e == hipSuccess) e = hipMemcpyAsync(tmp_bias, bias, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(g->s.thr, thresholds, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_transpose_templates(d, g->s.Tm, tmp_bias, g->s.TmT, g->s.bias, st);
+    if (e == hipSuccess && g->cfg.overlap_mode == FBP_OVERLAP_SCREENED) e = launch_overlap_tables(d, g->s, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(tmp_bias);
     if (e != hipSuccess) return hip_fail(e, "fbp_segment_set_templates");
@@ -322,7 +334,7 @@ int fbp_segment_run(fbp_segment *g, void *stream)
     for (;;) {
         if (g->cfg.max_passes && passes >= g->cfg.max_passes) { rc = fail(FBP_EPASSES, "pass cap %u reached", passes); break; }
         passes++;
-        if ((rc = ensure_spike_capacity(g, total + nw, st))) break;
+        if ((rc = ensure_spike_capacity(g, total + nw, total, st))) break;
         HIP_TRY(hipEventRecord(g->ev_stage[0], st), "event");
         HIP_TRY(launch_corr_fp32(d, s, nw, st), "correlation");                   // bpp.py:538-549
         HIP_TRY(hipEventRecord(g->ev_stage[1], st), "event");
@@ -330,7 +342,10 @@ int fbp_segment_run(fbp_segment *g, void *stream)
         if (g->cfg.do_overlap) {                                                   // bpp.py:565-661
             HIP_TRY(launch_compact_ov(d, s, nw, st), "overlap list");
             HIP_TRY(hipEventRecord(g->ev_stage[2], st), "event");
-            HIP_TRY(launch_overlap_direct(d, s, nw, st), "overlap recheck");
+            if (g->cfg.overlap_mode == FBP_OVERLAP_SCREENED)
+                HIP_TRY(launch_overlap_screened(d, s, nw, st), "overlap recheck (screened)");
+            else
+                HIP_TRY(launch_overlap_direct(d, s, nw, st), "overlap recheck");
             HIP_TRY(launch_check_overlap(d, s, nw, st), "overlap reassign");
             HIP_TRY(hipEventRecord(g->ev_stage[3], st), "event");
         }
@@ -371,6 +386,7 @@ int fbp_segment_run(fbp_segment *g, void *stream)
     g->stats.n_spikes = total;
     g->stats.ms_total = ms;
     g->stats.overlap_combos = g->h_ctr->combos;
+    g->stats.screened_combos = g->h_ctr->screened;
     // Results stay readable when the pass cap stopped the loop.
     g->ran = (rc == FBP_OK || rc == FBP_EPASSES);
     return rc;

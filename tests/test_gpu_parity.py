@@ -38,11 +38,12 @@ def test_first_pass_likelihood_bitwise(oracle_mod, seed, C, T, N, dur, M, ov, wg
     np.testing.assert_array_equal(g['lab'], o['lab'])
 
 
+@pytest.mark.parametrize("mode", ["direct", "screened"])
 @pytest.mark.parametrize("seed,C,T,N,dur,M,ov,wg", CASES)
-def test_full_pursuit_bitwise(oracle_mod, seed, C, T, N, dur, M, ov, wg):
+def test_full_pursuit_bitwise(oracle_mod, seed, C, T, N, dur, M, ov, wg, mode):
     rec = _case(seed, C, T, N, dur)
-    print("case", seed, C, T, N, dur, M, ov, wg, flush=True)
-    g = run_gpu(rec, M, ov, wg)
+    print("case", seed, C, T, N, dur, M, ov, wg, mode, flush=True)
+    g = run_gpu(rec, M, ov, wg, overlap_mode=mode)
     o = run_oracle(oracle_mod, rec, M, ov, wg)
     assert g['stats']['n_passes'] == o['n_passes']
     np.testing.assert_array_equal(g['idx'], o['idx'])
@@ -51,3 +52,25 @@ def test_full_pursuit_bitwise(oracle_mod, seed, C, T, N, dur, M, ov, wg):
     np.testing.assert_array_equal(g['L'].view(np.uint32), o['L'].view(np.uint32))
     clips = oracle_mod.adjusted_clips(o['residual'], rec['templates'], rec['N'], o['idx'], o['lab'])
     np.testing.assert_array_equal(g['clips'].view(np.uint32), clips.view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", ["direct", "screened"])
+def test_dense_16ch_spike_list_grows(oracle_mod, mode):
+    # 16 channels, 20 templates, 1 s at 40 kHz: more spikes than windows, so
+    # the device spike list grows during the run.
+    from spikesorting_fullpursuit_amd import synthetic
+    rec = synthetic.config("16ch-300s", seed=3, duration_s=1.0)
+    sm = rec['separability_metrics']
+    rec['bias'] = np.float32(-0.5 * sm['template_SS'])
+    rec['thr'] = np.float32(sm['neuron_lower_thresholds'])
+    rec['N'] = 61
+    print("case 16ch 1s", mode, flush=True)
+    g = run_gpu(rec, 30, True, 256, overlap_mode=mode)
+    o = run_oracle(oracle_mod, rec, 30, True, 256)
+    W = -(-rec['voltage'].shape[1] // 61)
+    assert o['idx'].size > W
+    np.testing.assert_array_equal(g['idx'], o['idx'])
+    np.testing.assert_array_equal(g['lab'], o['lab'])
+    np.testing.assert_array_equal(g['residual'].view(np.uint32), o['residual'].view(np.uint32))
+    if mode == "screened":
+        assert g['stats']['screened_combos'] > 20 * g['stats']['overlap_combos']
