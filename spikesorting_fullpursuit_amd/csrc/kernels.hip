@@ -548,6 +548,35 @@ __global__ __launch_bounds__(256) void k_overlap_tables(Dims d, const float *__r
     }
 }
 
+// Calls fn(f, t, s, e = t*S + s) for every combination of one window that the
+// reference does not skip (bp.cl:534-576), in no particular order: the main
+// shifts f whose likelihood is positive against every in-bounds (t, s) within
+// N of f, then the other in-bounds main shifts against the (t, s) whose
+// likelihood is positive.
+template <class Fn>
+__device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t *spos, const uint32_t *flist,
+                                                   uint32_t n_fpos, uint32_t n_fneg, const uint32_t *plist,
+                                                   uint32_t n_ppos, Fn fn)
+{
+    const uint32_t S = d.S, N = d.N, TS = d.T * d.S;
+    for (uint32_t a = 0; a < n_fpos; a++) {
+        const uint32_t f = flist[a];
+        uint32_t t = threadIdx.x / S, s = threadIdx.x - t * S;
+        for (uint32_t e = threadIdx.x; e < TS; e += blockDim.x) {
+            if ((spos[e] & 2) && (f > s ? f - s : s - f) <= N) fn(f, t, s, e);
+            s += blockDim.x;
+            while (s >= S) { s -= S; t++; }
+        }
+    }
+    const uint32_t total = n_fneg * n_ppos;
+    for (uint32_t r = threadIdx.x; r < total; r += blockDim.x) {
+        const uint32_t a = r / n_ppos, b = r - a * n_ppos;
+        const uint32_t f = flist[n_fpos + a], e = plist[b];
+        const uint32_t t = e / S, s = e - t * S;
+        if ((f > s ? f - s : s - f) <= N) fn(f, t, s, e);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_overlap_screened(
     Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
     const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw, float *best_L,
@@ -556,13 +585,17 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
 {
     extern __shared__ float dyn[];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
-    float *sv = dyn;                                   // [C][N] voltage at the best onset
-    float *smain = sv + C * N;                         // [C][N] template of the best label
-    float *sQ = smain + C * N;                         // [T][S] truncated correlations
-    uint8_t *spos = (uint8_t *)(sQ + (uint64_t)T * S); // [T][S] L[t, bidx+s-M] > 0 (in bounds only)
+    const uint32_t TS = T * S;
+    float *sv = dyn;                                    // [C][N] voltage at the best onset
+    float *sQ = sv + C * N;                             // [T][S] truncated correlations
+    uint32_t *plist = (uint32_t *)(sQ + TS);            // (t,s) with positive likelihood
+    uint32_t *flist = plist + TS;                       // main shifts: positive first, then the rest
+    uint8_t *spos = (uint8_t *)(flist + S);             // [T][S] bit1 in bounds, bit0 likelihood > 0
     __shared__ float r_sc[4];
     __shared__ uint64_t r_key[4], r_p[4];
     __shared__ double r_d[4][2];
+    __shared__ double r_vv[4];
+    __shared__ uint32_t n_ppos, n_fpos, n_fneg;
     const Counters *ctr = ctr_rw;
     const uint32_t n_ov = ctr->n_ov;
     const uint32_t wg_bits = 31 - __builtin_clz(d.reduce_wg);
@@ -572,7 +605,6 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
     const double u2 = 0x1p-23;
     const double g = nops * u2 / (1.0 - nops * u2);
     const double ecoef = (g + 2.0 * u2) * 1.01;
-    const uint64_t ST = (uint64_t)S * T;
     for (uint32_t k = blockIdx.x; k < n_ov; k += gridDim.x) {
         const uint32_t id = ov_list[k];
         const uint32_t lab = best_lab[id], bidx = best_idx[id];
@@ -583,25 +615,26 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
         uint64_t key = ~0ull, bp = 0;
         if (!wskip) {
             __syncthreads();
+            if (threadIdx.x == 0) { n_ppos = 0; n_fpos = 0; n_fneg = 0; }
             double vv = 0.0;
             for (uint32_t e = threadIdx.x; e < C * N; e += blockDim.x) {
                 const uint32_t c = e / N, j = e % N;
                 const float x = V[(uint64_t)c * d.K + bidx + j];
                 sv[e] = x;
                 vv = __builtin_fma((double)x, (double)x, vv);
-                smain[e] = Tm[((uint64_t)lab * C + c) * N + j];
-            }
-            for (uint32_t e = threadIdx.x; e < T * S; e += blockDim.x) {
-                const uint32_t t = e / S, s = e % S;
-                const bool inb = !((uint64_t)s + bidx < M || (uint64_t)s + bidx >= (uint64_t)d.K - N + M);   // bp.cl:534,538
-                spos[e] = (inb ? 2 : 0) | ((inb && L[(uint64_t)t * d.K + bidx + s - M] > 0.0f) ? 1 : 0);
             }
             for (int off = 32; off > 0; off >>= 1) vv += __shfl_xor(vv, off);
-            if (lane_of() == 0) r_d[threadIdx.x >> 6][0] = vv;
+            if (lane_of() == 0) r_vv[threadIdx.x >> 6] = vv;
             __syncthreads();
-            const double vn = sqrt(r_d[0][0] + r_d[1][0] + r_d[2][0] + r_d[3][0]) * (1.0 + 1e-12);
-            for (uint32_t e = threadIdx.x; e < T * S; e += blockDim.x) {
+            for (uint32_t e = threadIdx.x; e < TS; e += blockDim.x) {
                 const uint32_t t = e / S, s = e % S;
+                const bool inb = !((uint64_t)s + bidx < M || (uint64_t)s + bidx >= (uint64_t)d.K - N + M);   // bp.cl:534,538
+                const bool pos = inb && L[(uint64_t)t * d.K + bidx + s - M] > 0.0f;
+                spos[e] = (inb ? 2 : 0) | (pos ? 1 : 0);
+                if (pos) plist[atomicAdd(&n_ppos, 1u)] = e;
+                if (t == lab && inb) {
+                    if (pos) flist[atomicAdd(&n_fpos, 1u)] = s;
+                }
                 const int32_t sa = (int32_t)M - (int32_t)s;
                 const int32_t lo = max(0, -sa), hi = min((int32_t)N, (int32_t)N - sa);
                 double q = 0.0;
@@ -613,33 +646,41 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
                 sQ[e] = (float)q;
             }
             __syncthreads();
+            // in-bounds main shifts without a positive likelihood go after the positive ones
+            if (threadIdx.x == 0) {
+                uint32_t nn = 0;
+                for (uint32_t f = 0; f < S; f++)
+                    if ((spos[lab * S + f] & 3) == 2) flist[n_fpos + nn++] = f;
+                n_fneg = nn;
+            }
+            __syncthreads();
+            const double vn = sqrt(r_vv[0] + r_vv[1] + r_vv[2] + r_vv[3]) * (1.0 + 1e-12);
             const double nlab = tnorm[lab];
-            const float *ABl = AB + (uint64_t)lab * S * ST;
+            const float *ABl = AB + (uint64_t)lab * S * TS;
+            const float *B2l = B2 + (uint64_t)lab * S;
+            const float *sQl = sQ + lab * S;
+            const uint32_t nfp = n_fpos, nfn = n_fneg, npp = n_ppos;
+            auto screen = [&](uint32_t f, uint32_t t, uint32_t e, double &E) {
+                const double nsum = nlab + (double)tnorm[t];
+                E = ecoef * (nsum * vn + nsum * nsum);
+                return (double)sQl[f] + (double)sQ[e] - 0.5 * ((double)B2l[f] + (double)B2[e])
+                       - (double)ABl[(uint64_t)f * TS + e];
+            };
             // pass A: largest lower bound and largest upper bound of the score
             double lo_max = -INFINITY, hi_max = -INFINITY;
             uint32_t screened = 0;
-            for (uint64_t q = threadIdx.x; q < d.items; q += blockDim.x) {
-                const uint32_t s = (uint32_t)(q % S);
-                const uint32_t t = (uint32_t)((q / S) % T);
-                const uint32_t f = (uint32_t)(q / ST);
-                const uint8_t pf = spos[lab * S + f], ps = spos[t * S + s];
-                if (!(pf & 2) || !(ps & 2)) continue;                          // bp.cl:534-541
-                if ((f > s ? f - s : s - f) > N) continue;                     // bp.cl:543-556
-                if (!(pf & 1) && !(ps & 1)) continue;                          // bp.cl:572-576
-                const double nsum = nlab + (double)tnorm[t];
-                const double E = ecoef * (nsum * vn + nsum * nsum);
-                const double sh = (double)sQ[lab * S + f] + (double)sQ[t * S + s]
-                                  - 0.5 * ((double)B2[lab * S + f] + (double)B2[t * S + s]) - (double)ABl[q];
+            for_each_unskipped(d, spos, flist, nfp, nfn, plist, npp, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e) {
+                double E;
+                const double sh = screen(f, t, e, E);
                 lo_max = fmax(lo_max, sh - E);
                 hi_max = fmax(hi_max, sh + E);
                 screened++;
-            }
+            });
             for (int off = 32; off > 0; off >>= 1) {
                 lo_max = fmax(lo_max, __shfl_xor(lo_max, off));
                 hi_max = fmax(hi_max, __shfl_xor(hi_max, off));
                 screened += __shfl_xor(screened, off);
             }
-            __syncthreads();
             if (lane_of() == 0) { r_d[threadIdx.x >> 6][0] = lo_max; r_d[threadIdx.x >> 6][1] = hi_max; }
             if (lane_of() == 0 && screened) atomicAdd(&ctr_rw->screened, (unsigned long long)screened);
             __syncthreads();
@@ -648,25 +689,17 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
             // Nothing can beat best_L: parse rejects without any exact score.
             if (hi_max > (double)bl) {
                 uint32_t scored = 0;
-                for (uint64_t q = threadIdx.x; q < d.items; q += blockDim.x) {
-                    const uint32_t s = (uint32_t)(q % S);
-                    const uint32_t t = (uint32_t)((q / S) % T);
-                    const uint32_t f = (uint32_t)(q / ST);
-                    const uint8_t pf = spos[lab * S + f], ps = spos[t * S + s];
-                    if (!(pf & 2) || !(ps & 2)) continue;
-                    if ((f > s ? f - s : s - f) > N) continue;
-                    if (!(pf & 1) && !(ps & 1)) continue;
-                    const double nsum = nlab + (double)tnorm[t];
-                    const double E = ecoef * (nsum * vn + nsum * nsum);
-                    const double sh = (double)sQ[lab * S + f] + (double)sQ[t * S + s]
-                                      - 0.5 * ((double)B2[lab * S + f] + (double)B2[t * S + s]) - (double)ABl[q];
-                    if (sh + E < lo_max) continue;
-                    const float score = overlap_direct_score(d, sv, smain, Tm, t, f, s);
+                const float *tmain = Tm + (uint64_t)lab * C * N;
+                for_each_unskipped(d, spos, flist, nfp, nfn, plist, npp, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e) {
+                    double E;
+                    const double sh = screen(f, t, e, E);
+                    if (sh + E < lo_max) return;
+                    const float score = overlap_direct_score(d, sv, tmain, Tm, t, f, s);
                     scored++;
                     uint64_t p;
                     const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
                     if (key_better(score, kk, sc, key)) { sc = score; key = kk; bp = p; }
-                }
+                });
                 for (int off = 32; off > 0; off >>= 1) scored += __shfl_xor(scored, off);
                 if (lane_of() == 0 && scored) atomicAdd(&ctr_rw->combos, (unsigned long long)scored);
                 block_key_reduce(sc, key, bp, r_sc, r_key, r_p);
@@ -928,7 +961,8 @@ hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t m
 
 size_t overlap_screened_lds(const Dims &d)
 {
-    return 2ull * d.C * d.N * sizeof(float) + (uint64_t)d.T * d.S * (sizeof(float) + 1) + 16;
+    const uint64_t ts = (uint64_t)d.T * d.S;
+    return (uint64_t)d.C * d.N * 4 + ts * 4 + ts * 4 + (uint64_t)d.S * 4 + ts + 16;
 }
 
 hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_t st)
