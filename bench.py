@@ -43,8 +43,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--duration", type=float, default=None, help="override the recording length (s)")
     ap.add_argument("--max-shift", type=int, default=None, help="n_max_shift_inds (default N//2)")
-    ap.add_argument("--precision", default="fp32")
-    ap.add_argument("--overlap-mode", default="direct")
+    ap.add_argument("--precision", default="bf16", help="correlation arithmetic of the timed path")
+    ap.add_argument("--overlap-mode", default="screened", help="overlap arithmetic of the timed path")
+    ap.add_argument("--plain-steps", type=int, default=1,
+                    help="steps of the plain float32 path (fp32 correlation, direct overlap) timed for comparison")
     ap.add_argument("--no-overlap", action="store_true")
     ap.add_argument("--cpu-target-s", type=float, default=12.0, help="CPU baseline budget (s)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -81,7 +83,8 @@ def main():
         log("warmup %d: %.3f s, %d spikes, %d passes" % (w, time.perf_counter() - t, seg.num_spikes(),
                                                           seg.stats()['n_passes']))
     sums = {"ms_correlation": 0.0, "ms_overlap": 0.0, "ms_total": 0.0, "corr_launches": 0,
-            "overlap_launches": 0, "corr_positions": 0, "overlap_combos": 0}
+            "overlap_launches": 0, "corr_positions": 0, "overlap_combos": 0, "screened_combos": 0,
+            "overlap_windows": 0, "rescored_positions": 0}
     t_start = time.perf_counter()
     for k in range(args.steps):
         seg.run()
@@ -95,30 +98,24 @@ def main():
     ms_per_step = 1000.0 * elapsed / args.steps
     value = C * K * args.steps / elapsed
 
-    # Roofline of the dominant kernel (algorithmic work / HIP-event time).
-    corr_flop = 2.0 * T * C * N * sums["corr_positions"]
-    ovl_flop = (5.0 * C * N + C) * sums["overlap_combos"]
-    stages = {
-        "correlation": {
-            "ms": sums["ms_correlation"], "launches": sums["corr_launches"], "flop": corr_flop,
-            "tflops": corr_flop / (sums["ms_correlation"] * 1e-3) / 1e12 if sums["ms_correlation"] > 0 else 0.0,
-            "flop_per_unit": "2*T*C*N per likelihood position",
-        },
-        "overlap": {
-            "ms": sums["ms_overlap"], "launches": sums["overlap_launches"], "flop": ovl_flop,
-            "tflops": ovl_flop / (sums["ms_overlap"] * 1e-3) / 1e12 if sums["ms_overlap"] > 0 else 0.0,
-            "flop_per_unit": "5*C*N + C per scored (t,f,s) combination",
-        },
-    }
-    dom = "overlap" if sums["ms_overlap"] > sums["ms_correlation"] else "correlation"
-    ds = stages[dom]
-    peak = PEAK_TFLOPS["fp32"] if args.precision == "fp32" else PEAK_TFLOPS[args.precision]
-    roofline = {
-        "bound": "mfma", "kernel": "k_overlap_direct" if dom == "overlap" else "k_corr_fp32",
-        "achieved": ds["tflops"], "peak": peak, "unit": "TFLOP/s", "frac": ds["tflops"] / peak,
-        "traffic": None,
-        "note": "float32 VALU fused multiply-adds (v_fmac_f32); MI355X float32 peak 157.3 TFLOP/s",
-    }
+    stages, roofline = roofline_of(args, sums, C, T, N, M, K)
+
+    plain = None
+    if args.plain_steps > 0 and (args.precision != "fp32" or args.overlap_mode != "direct"):
+        idx_fast, lab_fast = seg.spikes()
+        with _lib.Segment(C, K, T, N, M, do_overlap, 256, "fp32", "direct") as ps:
+            ps.set_templates(sm['templates'], bias, thr)
+            ps.set_voltage(V)
+            t = time.perf_counter()
+            for _ in range(args.plain_steps):
+                ps.run()
+            dt = (time.perf_counter() - t) / args.plain_steps
+            pi, pl = ps.spikes()
+        equal = bool(np.array_equal(pi, idx_fast) and np.array_equal(pl, lab_fast))
+        plain = {"path": "fp32 correlation + direct overlap (reference arithmetic)", "value": C * K / dt,
+                 "ms_per_step": 1000.0 * dt, "steps": args.plain_steps, "events_equal": equal,
+                 "speedup_of_timed_path": (C * K / (elapsed / args.steps)) / (C * K / dt)}
+        log("plain fp32 path: %.1f ms per step, spike lists identical: %s" % (1000 * dt, equal))
 
     cpu = None
     if not args.no_cpu:
@@ -129,7 +126,7 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "ch·samples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded N(0,1) noise + DoG templates)",
+        "vs_baseline": None, "dtype": args.precision if args.precision != "fp32" else "f32", "data": "synthetic (seeded N(0,1) noise + DoG templates)",
         "config": {"workload": args.workload, "n_chans": C, "sampling_rate": fs, "duration_s": K / fs,
                    "n_templates": T, "template_len": N, "n_max_shift_inds": M, "do_overlap_recheck": do_overlap,
                    "precision": args.precision, "overlap_mode": args.overlap_mode, "seed": args.seed,
@@ -138,11 +135,57 @@ def main():
         "cpu_baseline": cpu,
         "result": {"n_spikes": int(n_spikes), "n_passes": int(last["n_passes"]),
                    "windows_evaluated": int(last["windows_evaluated"]),
-                   "overlap_windows": int(last["overlap_windows"]), "device_ms_per_step": sums["ms_total"] / args.steps},
+                   "overlap_windows": int(last["overlap_windows"]), "device_ms_per_step": sums["ms_total"] / args.steps,
+                   "rescored_fraction": sums["rescored_positions"] / max(1, sums["corr_positions"] * T)},
         "stages": stages,
+        "fp32_plain": plain,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def roofline_of(args, sums, C, T, N, M, K):
+    """Algorithmic work of the two dominant stages and the roofline entry of
+    the slower one (work / HIP-event time summed over the timed steps)."""
+    S = 2 * M + 1
+    pos = sums["corr_positions"]
+    stages = {}
+    t_corr = sums["ms_correlation"] * 1e-3
+    if args.precision == "fp32":
+        flop = 2.0 * T * C * N * pos
+        stages["correlation"] = {"kernel": "k_corr_fp32", "ms": sums["ms_correlation"],
+                                 "launches": sums["corr_launches"], "bound": "mfma", "unit": "TFLOP/s",
+                                 "achieved": flop / t_corr / 1e12 if t_corr else 0.0, "peak": PEAK_TFLOPS["fp32"],
+                                 "work": "2*T*C*N flop per likelihood position (float32 FMA chain)"}
+    else:
+        nbytes = (4.0 * C + 4.0 * T) * pos
+        stages["correlation"] = {"kernel": "k_corr_mfma + k_rescore", "ms": sums["ms_correlation"],
+                                 "launches": sums["corr_launches"], "bound": "hbm", "unit": "GB/s",
+                                 "achieved": nbytes / t_corr / 1e9 if t_corr else 0.0, "peak": PEAK_HBM_GBS,
+                                 "work": "4*C bytes of voltage read + 4*T bytes of likelihood written per position"}
+    t_ov = sums["ms_overlap"] * 1e-3
+    if args.overlap_mode == "direct":
+        flop = (5.0 * C * N + C) * sums["overlap_combos"]
+        stages["overlap"] = {"kernel": "k_overlap_direct", "ms": sums["ms_overlap"],
+                             "launches": sums["overlap_launches"], "bound": "mfma", "unit": "TFLOP/s",
+                             "achieved": flop / t_ov / 1e12 if t_ov else 0.0, "peak": PEAK_TFLOPS["fp32"],
+                             "work": "5*C*N + C flop per scored (t,f,s) combination"}
+    else:
+        # per overlap window: Q table 2*C*sum_s(valid j) per template, plus the
+        # float64 screen (~12 flop) per unskipped combination and the exact
+        # float32 chain of the re-scored ones.
+        valid = sum(min(N, N + s - M) - max(0, s - M) for s in range(S))
+        flop = (2.0 * C * T * valid * sums.get("overlap_windows", 0) + 12.0 * sums["screened_combos"]
+                + (5.0 * C * N + C) * sums["overlap_combos"])
+        stages["overlap"] = {"kernel": "k_overlap_screened", "ms": sums["ms_overlap"],
+                             "launches": sums["overlap_launches"], "bound": "mfma", "unit": "TFLOP/s",
+                             "achieved": flop / t_ov / 1e12 if t_ov else 0.0, "peak": PEAK_TFLOPS["fp64"],
+                             "work": "float64 VALU: truncated correlations + screen + exact re-scores"}
+    dom = max(stages, key=lambda k: stages[k]["ms"])
+    st = stages[dom]
+    roofline = {"bound": st["bound"], "kernel": st["kernel"], "achieved": st["achieved"], "peak": st["peak"],
+                "unit": st["unit"], "frac": st["achieved"] / st["peak"], "traffic": None, "work": st["work"]}
+    return stages, roofline
 
 
 def cpu_baseline(rec, bias, thr, N, M, do_overlap, target_s, seg_cls, precision, overlap_mode):

@@ -74,7 +74,14 @@ typedef struct fbp_config {
     uint32_t precision;      /* FBP_PREC_*                                      */
     uint32_t overlap_mode;   /* FBP_OVERLAP_*                                   */
     uint32_t max_passes;     /* 0 = unbounded (the reference has no bound)      */
+    uint32_t debug_flags;    /* FBP_DEBUG_*; 0 in normal use                    */
 } fbp_config;
+
+/* Test-only switches. */
+enum {
+    FBP_DEBUG_NO_RESCORE = 1  /* low-precision paths keep the screened likelihood
+                                 (no float32 re-score): checks the error bound */
+};
 
 typedef struct fbp_stats {
     uint32_t n_passes;

@@ -49,6 +49,7 @@ class FbpConfig(ctypes.Structure):
         ("precision", ctypes.c_uint32),
         ("overlap_mode", ctypes.c_uint32),
         ("max_passes", ctypes.c_uint32),
+        ("debug_flags", ctypes.c_uint32),
     ]
 
 
@@ -166,11 +167,12 @@ class Segment:
     """Device-resident state of one chunk (owner of an fbp_segment*)."""
 
     def __init__(self, n_chans, n_samples, n_templates, template_len, max_shift, do_overlap=True,
-                 reduce_wg=256, precision="fp32", overlap_mode="direct", max_passes=0, device=0):
+                 reduce_wg=256, precision="fp32", overlap_mode="direct", max_passes=0, device=0,
+                 debug_flags=0):
         self._lib = load()
         self.cfg = FbpConfig(n_chans, n_samples, n_templates, template_len, max_shift,
                              1 if do_overlap else 0, reduce_wg, PRECISIONS[precision],
-                             OVERLAP_MODES[overlap_mode], max_passes)
+                             OVERLAP_MODES[overlap_mode], max_passes, debug_flags)
         self._h = ctypes.c_void_p(None)
         check(self._lib.fbp_segment_create(device, ctypes.byref(self.cfg), ctypes.byref(self._h)),
               "fbp_segment_create")

@@ -146,6 +146,280 @@ __global__ __launch_bounds__(64) void k_corr_fp32(Dims d, const float *__restric
 }
 
 // ---------------------------------------------------------------------------
+// Correlation screen on the matrix cores (bf16 or fp16 inputs, float32
+// accumulation) followed by an exact float32 re-score of every position whose
+// likelihood could be positive.
+//
+// The likelihood decides events only through its sign (compared with 0 by the
+// argmax, the check flags and the overlap skip rules) and, where it is
+// positive, through its exact float32 value (bp.cl:402-417, :572, :733-775).
+// The screen computes Lhat = bias + sum over (c, j) of q(T[t,c,j]) q(V[c,i+j])
+// with a bound e >= |Lhat - L32| (L32 = the reference's float32 chain):
+//   e = k1*|t|*|v_i| + k2*(|t| + |v_i|) + k3 + k4*|bias|
+// from the input rounding (unit roundoff u of the format, absolute floor delta
+// for flushed subnormals) and the float32 accumulation of both sums (gamma_n
+// with a doubled unit roundoff).  Entries with Lhat > -e (or a non-finite
+// Lhat) are queued and re-scored with the float32 chain; the others are stored
+// as Lhat <= -e <= 0.  Every stored value then takes every branch of the
+// reference exactly as the float32 value would.
+//
+// Matrix layout: rows = positions of the window (16 per tile), columns =
+// templates (16 per tile), k = (channel, template sample padded to 16).  The
+// A fragment (Hankel of the voltage) is read from four copies of the window's
+// converted voltage, shifted by 0..3 samples, so every lane's four samples are
+// one aligned 8-byte LDS read.  B fragments (templates) are packed once per
+// template set in the exact lane order of v_mfma_f32_16x16x16{bf16,f16}.
+// ---------------------------------------------------------------------------
+typedef short short4v __attribute__((ext_vector_type(4)));
+typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t to_bf16(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even (inputs are finite)
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ uint16_t to_f16(float x)
+{
+    const _Float16 h = (_Float16)x;
+    return __builtin_bit_cast(uint16_t, h);
+}
+
+template <int PREC>
+__device__ __forceinline__ uint16_t to_narrow(float x) { return PREC == 1 ? to_bf16(x) : to_f16(x); }
+
+template <int PREC>
+__device__ __forceinline__ float4v mfma16(short4v a, short4v b, float4v c)
+{
+    if constexpr (PREC == 1) {
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    } else {
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(half4v, a), __builtin_bit_cast(half4v, b),
+                                                      c, 0, 0, 0);
+    }
+}
+
+// TmB[((c*KS + ks)*CT + ct)*64 + lane] = 4 narrow values of template
+// t = ct*16 + lane%16 at samples j = ks*16 + 4*(lane/16) + 0..3 (0 if padded).
+template <int PREC>
+__global__ void k_pack_templates(Dims d, const float *__restrict__ Tm, short4v *__restrict__ TmB, uint32_t KS,
+                                 uint32_t CT)
+{
+    const uint64_t total = (uint64_t)d.C * KS * CT * 64;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t lane = (uint32_t)(e % 64);
+        const uint64_t r = e / 64;
+        const uint32_t ct = (uint32_t)(r % CT);
+        const uint32_t ks = (uint32_t)((r / CT) % KS);
+        const uint32_t c = (uint32_t)(r / ((uint64_t)CT * KS));
+        const uint32_t t = ct * 16 + (lane & 15);
+        short4v v;
+        for (int q = 0; q < 4; q++) {
+            const uint32_t j = ks * 16 + 4 * (lane >> 4) + q;
+            const float x = (t < d.T && j < d.N) ? Tm[((uint64_t)t * d.C + c) * d.N + j] : 0.0f;
+            v[q] = (short)to_narrow<PREC>(x);
+        }
+        TmB[e] = v;
+    }
+}
+
+// Template norms (upper bounds), used by both screens.
+__global__ void k_template_norms(Dims d, const float *__restrict__ Tm, float *__restrict__ tnorm)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.T) return;
+    double acc = 0.0;
+    for (uint64_t k = 0; k < (uint64_t)d.C * d.N; k++) {
+        const double x = Tm[(uint64_t)t * d.C * d.N + k];
+        acc = __builtin_fma(x, x, acc);
+    }
+    const double nrm = sqrt(acc) * (1.0 + 1e-12);
+    float up = (float)nrm;
+    if ((double)up < nrm) up = __uint_as_float(__float_as_uint(up) + 1u);
+    tnorm[t] = up;
+}
+
+struct ScreenCoef {
+    float k1, k2, k3, k4;
+};
+
+__device__ __forceinline__ float exact_likelihood(const Dims &d, const float *__restrict__ V,
+                                                  const float *__restrict__ Tm, const float *__restrict__ bias,
+                                                  uint32_t t, uint64_t i)
+{
+    float acc = bias[t];
+    for (uint32_t c = 0; c < d.C; c++) {
+        const float *tc = Tm + ((uint64_t)t * d.C + c) * d.N;
+        const float *vc = V + (uint64_t)c * d.K + i;
+        for (uint32_t j = 0; j < d.N; j++) acc = __builtin_fmaf(tc[j], vc[j], acc);
+    }
+    return acc;
+}
+
+constexpr int kMfmaMaxRT = 6;   // row tiles per window: N <= 96
+constexpr int kMfmaCTB = 4;     // template column tiles per accumulation chunk
+
+template <int PREC, int RT>
+__global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restrict__ V, const float *__restrict__ Tm,
+                                                  const short4v *__restrict__ TmB, const float *__restrict__ bias,
+                                                  const float *__restrict__ tnorm, const uint32_t *__restrict__ win,
+                                                  float *__restrict__ L, Counters *ctr, uint64_t *__restrict__ rq,
+                                                  uint32_t rq_cap, uint32_t cb_max, ScreenCoef co, uint32_t no_rescore)
+{
+    extern __shared__ uint16_t vb[];            // [4 shifts][cb][LA] narrow voltage
+    __shared__ double s2[kMfmaMaxRT * 16 * 2 + 16];
+    __shared__ float vnb[kMfmaMaxRT * 16];
+    const int lane = threadIdx.x;
+    const uint32_t KS = (d.N + 15) / 16;
+    const uint32_t CT = (d.T + 15) / 16;
+    const uint32_t LA = (RT + KS) * 16;        // samples staged per channel
+    const uint32_t w = win[blockIdx.x];
+    const uint64_t base = (uint64_t)w * d.N;
+    const uint64_t wend = min(base + d.N, (uint64_t)d.K);
+    const uint32_t np = (uint32_t)(wend - base);   // positions of this window
+
+    // Per-position window norm bound |v_i| (float64 sums of the float32 samples).
+    for (uint32_t x = lane; x < RT * 16 + d.N; x += 64) {
+        double a = 0.0;
+        const uint64_t g = base + x;
+        if (g < d.K)
+            for (uint32_t c = 0; c < d.C; c++) {
+                const double v = V[(uint64_t)c * d.K + g];
+                a = __builtin_fma(v, v, a);
+            }
+        s2[x] = a;
+    }
+    __syncthreads();
+    for (uint32_t p = lane; p < RT * 16; p += 64) {
+        double a = 0.0;
+        for (uint32_t x = 0; x < d.N; x++) a += s2[p + x];
+        const double nv = sqrt(a) * (1.0 + 1e-9);
+        float up = (float)nv;
+        if ((double)up < nv) up = __uint_as_float(__float_as_uint(up) + 1u);
+        vnb[p] = up;
+    }
+
+    const bool staged_once = cb_max >= d.C;
+    bool have = false;
+    for (uint32_t ct0 = 0; ct0 < CT; ct0 += kMfmaCTB) {
+        const uint32_t nct = min((uint32_t)kMfmaCTB, CT - ct0);
+        float4v acc[RT][kMfmaCTB];
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+            for (int ct = 0; ct < kMfmaCTB; ct++) acc[rt][ct] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t c0 = 0; c0 < d.C; c0 += cb_max) {
+            const uint32_t cb = min(cb_max, d.C - c0);
+            if (!(staged_once && have)) {
+                __syncthreads();
+                for (uint32_t e = lane; e < cb * LA; e += 64) {
+                    const uint32_t c = e / LA, y = e % LA;
+                    const float *vr = V + (uint64_t)(c0 + c) * d.K;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const uint64_t g = base + y + r;
+                        vb[((uint64_t)r * cb + c) * LA + y] = to_narrow<PREC>(g < d.K ? vr[g] : 0.0f);
+                    }
+                }
+                __syncthreads();
+                have = true;
+            }
+            for (uint32_t c = 0; c < cb; c++) {
+                for (uint32_t ks = 0; ks < KS; ks++) {
+                    short4v b[kMfmaCTB];
+#pragma unroll
+                    for (int ct = 0; ct < kMfmaCTB; ct++)
+                        if (ct < (int)nct) b[ct] = TmB[(((uint64_t)(c0 + c) * KS + ks) * CT + ct0 + ct) * 64 + lane];
+#pragma unroll
+                    for (int rt = 0; rt < RT; rt++) {
+                        const uint32_t o = rt * 16 + ks * 16 + 4 * (lane >> 4) + (lane & 15);
+                        const uint32_t r = o & 3;
+                        const short4v a = *(const short4v *)(vb + ((uint64_t)r * cb + c) * LA + (o - r));
+#pragma unroll
+                        for (int ct = 0; ct < kMfmaCTB; ct++)
+                            if (ct < (int)nct) acc[rt][ct] = mfma16<PREC>(a, b[ct], acc[rt][ct]);
+                    }
+                }
+            }
+        }
+        // Epilogue: bias, bound, store, queue the entries that need float32.
+        uint32_t nflag = 0;
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+            for (int ct = 0; ct < kMfmaCTB; ct++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t p = rt * 16 + 4 * (lane >> 4) + q;
+                    const uint32_t t = (ct0 + ct) * 16 + (lane & 15);
+                    if (ct >= (int)nct || p >= np || t >= d.T) continue;
+                    const uint64_t i = base + p;
+                    if (i + d.N >= d.K) { L[(uint64_t)t * d.K + i] = 0.0f; continue; }   // bp.cl:231-234
+                    const float lh = bias[t] + acc[rt][ct][q];
+                    const float tn = tnorm[t], vn = vnb[p];
+                    const float e = co.k1 * tn * vn + co.k2 * (tn + vn) + co.k3 + co.k4 * fabsf(bias[t]);
+                    L[(uint64_t)t * d.K + i] = lh;
+                    if (!(lh <= -e)) nflag++;
+                }
+        if (no_rescore) continue;
+        // wave-wide reservation in the re-score queue
+        uint32_t incl = nflag;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+        }
+        const uint32_t tot = __shfl(incl, 63);
+        uint32_t qbase = 0;
+        if (tot) {
+            if (lane == 63) qbase = atomicAdd(&ctr->n_rescore, tot);
+            qbase = __shfl(qbase, 63);
+        }
+        uint32_t pos = qbase + incl - nflag;
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+            for (int ct = 0; ct < kMfmaCTB; ct++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t p = rt * 16 + 4 * (lane >> 4) + q;
+                    const uint32_t t = (ct0 + ct) * 16 + (lane & 15);
+                    if (ct >= (int)nct || p >= np || t >= d.T) continue;
+                    const uint64_t i = base + p;
+                    if (i + d.N >= d.K) continue;
+                    const float lh = bias[t] + acc[rt][ct][q];
+                    const float tn = tnorm[t], vn = vnb[p];
+                    const float e = co.k1 * tn * vn + co.k2 * (tn + vn) + co.k3 + co.k4 * fabsf(bias[t]);
+                    if (lh <= -e) continue;
+                    if (pos < rq_cap) rq[pos] = ((uint64_t)t << 32) | (uint64_t)i;
+                    else L[(uint64_t)t * d.K + i] = exact_likelihood(d, V, Tm, bias, t, i);   // queue full
+                    pos++;
+                }
+    }
+}
+
+// Float32 re-score of the queued (template, position) entries.
+__global__ __launch_bounds__(256) void k_rescore(Dims d, const float *__restrict__ V, const float *__restrict__ Tm,
+                                                 const float *__restrict__ bias, const uint64_t *__restrict__ rq,
+                                                 const Counters *__restrict__ ctr, uint32_t rq_cap, float *L)
+{
+    const uint32_t n = min(ctr->n_rescore, rq_cap);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint64_t ent = rq[k];
+        const uint32_t t = (uint32_t)(ent >> 32);
+        const uint64_t i = ent & 0xffffffffull;
+        L[(uint64_t)t * d.K + i] = exact_likelihood(d, V, Tm, bias, t, i);
+    }
+}
+
+__global__ void k_zero_rescore(Counters *ctr)
+{
+    if (threadIdx.x == 0) ctr->n_rescore = 0;
+}
+
+// ---------------------------------------------------------------------------
 // Windowed argmax over all templates (bp.cl:350-441, with the host's T
 // sequential launches, bpp.py:551-562, fused).  One wave per listed window.
 // The reference scans t ascending, then position ascending, with strict >,
@@ -928,6 +1202,72 @@ hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win,
     if (cb_max > d.C) cb_max = d.C;
     if (cb_max == 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_corr_fp32, dim3(n_win), dim3(64), 0, st, d, s.V, s.TmT, s.bias, s.win, s.L, cb_max);
+    return hipGetLastError();
+}
+
+uint32_t mfma_ks(const Dims &d) { return (d.N + 15) / 16; }
+uint32_t mfma_ct(const Dims &d) { return (d.T + 15) / 16; }
+
+hipError_t launch_template_norms(const Dims &d, const DeviceState &s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_template_norms, dim3((d.T + 63) / 64), dim3(64), 0, st, d, s.Tm, s.tnorm);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_templates(const Dims &d, const DeviceState &s, uint32_t precision, hipStream_t st)
+{
+    const uint64_t total = (uint64_t)d.C * mfma_ks(d) * mfma_ct(d) * 64;
+    if (precision == 1)
+        hipLaunchKernelGGL(k_pack_templates<1>, dim3(grid_for(total, 256, 4096)), dim3(256), 0, st, d, s.Tm,
+                           (short4v *)s.TmB, mfma_ks(d), mfma_ct(d));
+    else
+        hipLaunchKernelGGL(k_pack_templates<2>, dim3(grid_for(total, 256, 4096)), dim3(256), 0, st, d, s.Tm,
+                           (short4v *)s.TmB, mfma_ks(d), mfma_ct(d));
+    return hipGetLastError();
+}
+
+template <int PREC, int RT>
+static hipError_t launch_corr_mfma_rt(const Dims &d, const DeviceState &s, uint32_t n_win, const float *coef,
+                                      uint32_t rq_cap, uint32_t no_rescore, hipStream_t st)
+{
+    const uint32_t LA = (RT + mfma_ks(d)) * 16;
+    const uint32_t budget = 48 * 1024;
+    uint32_t cb_max = budget / (4 * LA * 2);
+    if (cb_max > d.C) cb_max = d.C;
+    if (cb_max == 0) return hipErrorInvalidValue;
+    const size_t lds = 4ull * cb_max * LA * 2;
+    ScreenCoef co{coef[0], coef[1], coef[2], coef[3]};
+    hipLaunchKernelGGL((k_corr_mfma<PREC, RT>), dim3(n_win), dim3(64), lds, st, d, s.V, s.Tm,
+                       (const short4v *)s.TmB, s.bias, s.tnorm, s.win, s.L, s.ctr, s.rq, rq_cap, cb_max, co,
+                       no_rescore);
+    return hipGetLastError();
+}
+
+template <int PREC>
+static hipError_t launch_corr_mfma_p(const Dims &d, const DeviceState &s, uint32_t n_win, const float *coef,
+                                     uint32_t rq_cap, uint32_t no_rescore, hipStream_t st)
+{
+    switch ((d.N + 15) / 16) {
+    case 1: return launch_corr_mfma_rt<PREC, 1>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    case 2: return launch_corr_mfma_rt<PREC, 2>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    case 3: return launch_corr_mfma_rt<PREC, 3>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    case 4: return launch_corr_mfma_rt<PREC, 4>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    case 5: return launch_corr_mfma_rt<PREC, 5>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    case 6: return launch_corr_mfma_rt<PREC, 6>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_win, uint32_t precision,
+                              const float *coef, uint32_t rq_cap, uint32_t no_rescore, hipStream_t st)
+{
+    if (n_win == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_rescore, dim3(1), dim3(64), 0, st, s.ctr);
+    hipError_t e = precision == 1 ? launch_corr_mfma_p<1>(d, s, n_win, coef, rq_cap, no_rescore, st)
+                                  : launch_corr_mfma_p<2>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    if (e != hipSuccess) return e;
+    if (!no_rescore)
+        hipLaunchKernelGGL(k_rescore, dim3(2048), dim3(256), 0, st, d, s.V, s.Tm, s.bias, s.rq, s.ctr, rq_cap, s.L);
     return hipGetLastError();
 }
 

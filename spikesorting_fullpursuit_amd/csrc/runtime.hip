@@ -66,6 +66,8 @@ struct fbp_segment {
     uint64_t clips_cap = 0;
     bool have_templates = false, have_voltage = false, ran = false;
     fbp_stats stats{};
+    uint32_t rq_cap = 0;         // capacity of the re-score queue
+    float coef[4] = {0, 0, 0, 0};  // screen bound coefficients k1..k4
 };
 
 namespace {
@@ -81,7 +83,8 @@ void free_state(fbp_segment *g)
     DeviceState &s = g->s;
     void *ptrs[] = {s.V, s.V0, s.Tm, s.TmT, s.bias, s.thr, s.L, s.best_L, s.best_lab, s.best_idx, s.ov_idx,
                     s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
-                    s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabB2, s.tnorm};
+                    s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabB2, s.tnorm,
+                    s.TmB, s.rq};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     s = DeviceState{};
@@ -161,8 +164,10 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         return fail(FBP_EINVAL, "reduce_wg must be a power of two");
     if ((uint64_t)cfg->max_shift >= 4ull * N)
         return fail(FBP_EINVAL, "n_max_shift_inds (%u) must be below 4x the template length", cfg->max_shift);
-    if (cfg->precision != FBP_PREC_FP32)
+    if (cfg->precision != FBP_PREC_FP32 && cfg->precision != FBP_PREC_BF16 && cfg->precision != FBP_PREC_FP16)
         return fail(FBP_EINVAL, "precision %u is not available in this build", cfg->precision);
+    if (cfg->precision != FBP_PREC_FP32 && N > 96)
+        return fail(FBP_EINVAL, "the matrix-core screen supports templates of up to 96 samples (N=%u)", N);
     if (cfg->overlap_mode != FBP_OVERLAP_DIRECT && cfg->overlap_mode != FBP_OVERLAP_SCREENED)
         return fail(FBP_EINVAL, "unknown overlap mode %u", cfg->overlap_mode);
     if ((uint64_t)C * N * sizeof(float) * 2 > kMaxLds)
@@ -218,10 +223,30 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     ALLOC(s.win_spike, W)
     ALLOC(s.blk, nb + 1)
     ALLOC(s.ctr, 1)
+    ALLOC(s.tnorm, T)
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED) {
         ALLOC(s.tabAB, (uint64_t)T * d.S * T * d.S)
         ALLOC(s.tabB2, (uint64_t)T * d.S)
-        ALLOC(s.tnorm, T)
+    }
+    if (cfg->precision != FBP_PREC_FP32) {
+        uint64_t cap = (uint64_t)T * K / 8;
+        if (cap < (1u << 20)) cap = 1u << 20;
+        if (cap > (64u << 20)) cap = 64u << 20;
+        g->rq_cap = (uint32_t)cap;
+        ALLOC(s.rq, cap)
+        uint16_t *tmb = nullptr;
+        ALLOC(tmb, (uint64_t)C * mfma_ks(d) * mfma_ct(d) * 64 * 4)
+        s.TmB = tmb;
+        // Screen bound coefficients (see the correlation screen in kernels.hip).
+        const double u = cfg->precision == FBP_PREC_BF16 ? 0x1p-8 : 0x1p-11;
+        const double delta = cfg->precision == FBP_PREC_BF16 ? 0x1p-126 : 0x1p-14;
+        const double n = (double)C * mfma_ks(d) * 16 + 2.0;
+        const double gp = n * 0x1p-23 / (1.0 - n * 0x1p-23);
+        const double cn = (double)C * N;
+        g->coef[0] = (float)((2 * u + u * u + 2 * gp * (1 + u) * (1 + u)) * 1.01);
+        g->coef[1] = (float)(delta * (1 + u) * sqrt(cn) * (1 + gp) * 1.01);
+        g->coef[2] = (float)(cn * delta * delta * (1 + gp) * 1.01 + 1e-30);
+        g->coef[3] = (float)(2 * gp * 1.01);
     }
 #undef ALLOC
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->own_stream, hipStreamNonBlocking);
@@ -267,7 +292,9 @@ int fbp_segment_set_templates(fbp_segment *g, const float *templates, const floa
     if (e == hipSuccess) e = hipMemcpyAsync(tmp_bias, bias, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(g->s.thr, thresholds, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_transpose_templates(d, g->s.Tm, tmp_bias, g->s.TmT, g->s.bias, st);
+    if (e == hipSuccess) e = launch_template_norms(d, g->s, st);
     if (e == hipSuccess && g->cfg.overlap_mode == FBP_OVERLAP_SCREENED) e = launch_overlap_tables(d, g->s, st);
+    if (e == hipSuccess && g->cfg.precision != FBP_PREC_FP32) e = launch_pack_templates(d, g->s, g->cfg.precision, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(tmp_bias);
     if (e != hipSuccess) return hip_fail(e, "fbp_segment_set_templates");
@@ -336,7 +363,11 @@ int fbp_segment_run(fbp_segment *g, void *stream)
         passes++;
         if ((rc = ensure_spike_capacity(g, total + nw, total, st))) break;
         HIP_TRY(hipEventRecord(g->ev_stage[0], st), "event");
-        HIP_TRY(launch_corr_fp32(d, s, nw, st), "correlation");                   // bpp.py:538-549
+        if (g->cfg.precision == FBP_PREC_FP32)
+            HIP_TRY(launch_corr_fp32(d, s, nw, st), "correlation");               // bpp.py:538-549
+        else
+            HIP_TRY(launch_corr_screen(d, s, nw, g->cfg.precision, g->coef, g->rq_cap,
+                                       g->cfg.debug_flags & FBP_DEBUG_NO_RESCORE, st), "correlation screen");
         HIP_TRY(hipEventRecord(g->ev_stage[1], st), "event");
         HIP_TRY(launch_argmax(d, s, nw, st), "argmax");                            // bpp.py:551-562
         if (g->cfg.do_overlap) {                                                   // bpp.py:565-661
@@ -359,6 +390,7 @@ int fbp_segment_run(fbp_segment *g, void *stream)
         g->stats.windows_evaluated += nw;
         g->stats.corr_launches++;
         g->stats.corr_positions += (uint64_t)nw * d.N;
+        if (g->cfg.precision != FBP_PREC_FP32) g->stats.rescored_positions += g->h_ctr->n_rescore;
         float ms_c = 0.0f, ms_o = 0.0f;
         (void)hipEventElapsedTime(&ms_c, g->ev_stage[0], g->ev_stage[1]);
         g->stats.ms_correlation += ms_c;

@@ -16,12 +16,14 @@ def small_case(seed, C=4, T=5, N=61, duration_s=0.5, fs=40000, peak=(6.0, 15.0),
     return rec
 
 
-def run_gpu(rec, M, do_overlap=True, reduce_wg=256, max_passes=0, precision="fp32", overlap_mode="direct"):
+def run_gpu(rec, M, do_overlap=True, reduce_wg=256, max_passes=0, precision="fp32", overlap_mode="direct",
+            debug_flags=0):
     from spikesorting_fullpursuit_amd import _lib
     V = rec['voltage']
     C, K = V.shape
     T = rec['templates'].shape[0]
-    with _lib.Segment(C, K, T, rec['N'], M, do_overlap, reduce_wg, precision, overlap_mode, max_passes) as seg:
+    with _lib.Segment(C, K, T, rec['N'], M, do_overlap, reduce_wg, precision, overlap_mode, max_passes,
+                      debug_flags=debug_flags) as seg:
         seg.set_templates(rec['templates'], rec['bias'], rec['thr'])
         seg.set_voltage(V)
         seg.run(allow_cap=max_passes > 0)
@@ -33,3 +35,27 @@ def run_gpu(rec, M, do_overlap=True, reduce_wg=256, max_passes=0, precision="fp3
 def run_oracle(oracle, rec, M, do_overlap=True, reduce_wg=256, max_passes=0):
     return oracle.pursuit_chunk(rec['voltage'], rec['templates'], rec['bias'], rec['thr'], rec['N'], M,
                                 do_overlap=do_overlap, reduce_wg=reduce_wg, max_passes=max_passes, want_L=True)
+
+
+def screen_bound(rec, precision):
+    """Float64 restatement of the correlation screen's bound e[t, i] (the
+    kernel's k1..k4 coefficients, exact template norms and per-position window
+    norms)."""
+    V = rec['voltage'].astype(np.float64)
+    C, K = V.shape
+    N = rec['N']
+    T = rec['templates'].shape[0]
+    u, delta = (2.0 ** -8, 2.0 ** -126) if precision == "bf16" else (2.0 ** -11, 2.0 ** -14)
+    n = C * 16 * ((N + 15) // 16) + 2.0
+    gp = n * 2.0 ** -23 / (1 - n * 2.0 ** -23)
+    cn = C * N
+    k1 = (2 * u + u * u + 2 * gp * (1 + u) ** 2) * 1.01
+    k2 = delta * (1 + u) * np.sqrt(cn) * (1 + gp) * 1.01
+    k3 = cn * delta * delta * (1 + gp) * 1.01
+    k4 = 2 * gp * 1.01
+    tn = np.sqrt(np.sum(rec['templates'].astype(np.float64) ** 2, axis=1))
+    s2 = np.concatenate([np.sum(V ** 2, axis=0), np.zeros(N)])
+    cs = np.concatenate([[0.0], np.cumsum(s2)])
+    vn = np.sqrt(cs[N:N + K] - cs[:K])
+    bias = np.abs(rec['bias'].astype(np.float64))
+    return k1 * tn[:, None] * vn[None, :] + k2 * (tn[:, None] + vn[None, :]) + k3 + k4 * bias[:, None]

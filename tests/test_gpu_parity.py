@@ -7,7 +7,7 @@ the residual and the adjusted clips must be identical, not merely close.
 import numpy as np
 import pytest
 
-from helpers import run_gpu, run_oracle, small_case
+from helpers import run_gpu, run_oracle, screen_bound, small_case
 
 pytestmark = pytest.mark.gpu
 
@@ -74,3 +74,59 @@ def test_dense_16ch_spike_list_grows(oracle_mod, mode):
     np.testing.assert_array_equal(g['residual'].view(np.uint32), o['residual'].view(np.uint32))
     if mode == "screened":
         assert g['stats']['screened_combos'] > 20 * g['stats']['overlap_combos']
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+@pytest.mark.parametrize("seed,C,T,N,dur", [(31, 4, 5, 61, 0.5), (32, 8, 20, 41, 0.3), (33, 3, 3, 81, 0.3)])
+def test_screen_error_bound_holds(precision, seed, C, T, N, dur):
+    # Without the float32 re-score, the matrix-core likelihood must lie within
+    # the stated bound of the float32 likelihood at every position.
+    rec = _case(seed, C, T, N, dur)
+    print("case", precision, seed, C, T, N, dur, flush=True)
+    lo = run_gpu(rec, N // 2, True, 256, max_passes=1, precision=precision, debug_flags=1)
+    hi = run_gpu(rec, N // 2, True, 256, max_passes=1)
+    K = rec['voltage'].shape[1]
+    err = np.abs(lo['L'].astype(np.float64) - hi['L'].astype(np.float64))
+    e = screen_bound(rec, precision)
+    valid = np.arange(K) + N < K
+    assert np.all(err[:, valid] <= e[:, valid]), float(np.max(err[:, valid] / e[:, valid]))
+    assert np.all(lo['L'][:, ~valid] == 0.0)
+    # the bound is not vacuous: it is within a small factor of the observed error
+    assert np.max(err[:, valid] / e[:, valid]) > 1e-3
+
+
+@pytest.mark.parametrize("mode", ["direct", "screened"])
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+@pytest.mark.parametrize("seed,C,T,N,dur,M,ov,wg", CASES[:5])
+def test_low_precision_decisions_equal_fp32(oracle_mod, seed, C, T, N, dur, M, ov, wg, precision, mode):
+    rec = _case(seed, C, T, N, dur)
+    print("case", seed, C, T, N, dur, M, ov, wg, precision, mode, flush=True)
+    g = run_gpu(rec, M, ov, wg, precision=precision, overlap_mode=mode)
+    o = run_oracle(oracle_mod, rec, M, ov, wg)
+    assert g['stats']['n_passes'] == o['n_passes']
+    np.testing.assert_array_equal(g['idx'], o['idx'])
+    np.testing.assert_array_equal(g['lab'], o['lab'])
+    np.testing.assert_array_equal(g['residual'].view(np.uint32), o['residual'].view(np.uint32))
+    # stored likelihood: positive values exact, every other value non-positive
+    pos = o['L'] > 0
+    np.testing.assert_array_equal(g['L'][pos].view(np.uint32), o['L'][pos].view(np.uint32))
+    assert np.all(g['L'][~pos] <= 0.0)
+    assert g['stats']['rescored_positions'] > 0
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_low_precision_dense_16ch(oracle_mod, precision):
+    from spikesorting_fullpursuit_amd import synthetic
+    rec = synthetic.config("16ch-300s", seed=5, duration_s=1.0)
+    sm = rec['separability_metrics']
+    rec['bias'] = np.float32(-0.5 * sm['template_SS'])
+    rec['thr'] = np.float32(sm['neuron_lower_thresholds'])
+    rec['N'] = 61
+    print("case 16ch 1s", precision, flush=True)
+    g = run_gpu(rec, 30, True, 256, precision=precision, overlap_mode="screened")
+    o = run_oracle(oracle_mod, rec, 30, True, 256)
+    np.testing.assert_array_equal(g['idx'], o['idx'])
+    np.testing.assert_array_equal(g['lab'], o['lab'])
+    np.testing.assert_array_equal(g['residual'].view(np.uint32), o['residual'].view(np.uint32))
+    frac = g['stats']['rescored_positions'] / g['stats']['corr_positions'] / 20
+    print("re-scored fraction of (t, i):", frac, flush=True)
