@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--no-overlap", action="store_true")
     ap.add_argument("--cpu-target-s", type=float, default=12.0, help="CPU baseline budget (s)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=64, help="requests of the served mix (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate of the mix (1/s)")
+    ap.add_argument("--serve-inflight", type=int, default=8)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -117,6 +120,10 @@ def main():
                  "speedup_of_timed_path": (C * K / (elapsed / args.steps)) / (C * K / dt)}
         log("plain fp32 path: %.1f ms per step, spike lists identical: %s" % (1000 * dt, equal))
 
+    serve = None
+    if args.serve_requests > 0:
+        serve = serving_mix_run(args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(rec, bias, thr, N, M, do_overlap, args.cpu_target_s, seg_cls=_lib.Segment,
@@ -139,9 +146,50 @@ def main():
                    "rescored_fraction": sums["rescored_positions"] / max(1, sums["corr_positions"] * T)},
         "stages": stages,
         "fp32_plain": plain,
+        "serving": serve,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def serving_mix_run(args):
+    """The served mix: Poisson arrivals of segments of 5-120 s at 30/40 kHz,
+    4-32 channels, 3-40 templates of 41-81 samples; admitted by HBM budget and
+    run concurrently.  Latency = arrival to result on the host."""
+    from spikesorting_fullpursuit_amd import serving, synthetic
+    specs = synthetic.serving_mix(seed=args.seed + 100, n_requests=args.serve_requests, rate_hz=args.serve_rate)
+    t0 = time.time()
+    reqs = []
+    for sp in specs:
+        rec = synthetic.recording_for(sp)
+        sm = rec['separability_metrics']
+        reqs.append(serving.SegmentRequest(rec['voltage'], sm['templates'], sm['template_SS'],
+                                           sm['neuron_lower_thresholds'], sp['n_samples_per_chan'], tag=len(reqs)))
+    chs = sum(r.C * r.K for r in reqs)
+    log("served mix: %d requests, %.2f G ch*samples, generated in %.1f s" % (len(reqs), chs / 1e9, time.time() - t0))
+    # warm the kernels of every shape class once, outside the timed mix
+    serving.run_segment(reqs[0], args.precision, args.overlap_mode)
+    with serving.PursuitServer(max_inflight=args.serve_inflight, precision=args.precision,
+                               overlap_mode=args.overlap_mode) as srv:
+        futs = []
+        t_start = time.perf_counter()
+        for sp, r in zip(specs, reqs):
+            delay = sp['arrival_s'] - (time.perf_counter() - t_start)
+            if delay > 0:
+                time.sleep(delay)
+            futs.append(srv.submit(r))
+        done = [f.result() for f in futs]
+        makespan = time.perf_counter() - t_start
+        summ = srv.latency_summary()
+    spikes = sum(int(d.onsets.size) for d in done)
+    log("served mix: makespan %.2f s, p50 %.3f s, p99 %.3f s, peak %d in flight"
+        % (makespan, summ['p50_s'], summ['p99_s'], summ['peak_inflight']))
+    return {"requests": len(reqs), "arrival_rate_hz": args.serve_rate, "max_inflight": args.serve_inflight,
+            "ch_samples": chs, "spikes": spikes, "makespan_s": makespan, "throughput": chs / makespan,
+            "unit": "ch·samples/s", "latency_p50_s": summ['p50_s'], "latency_p99_s": summ['p99_s'],
+            "latency_mean_s": summ['mean_s'], "queue_p99_s": summ['queue_p99_s'],
+            "peak_inflight": summ['peak_inflight'], "peak_hbm_bytes": summ['peak_bytes'],
+            "arrivals_span_s": specs[-1]['arrival_s']}
 
 
 def roofline_of(args, sums, C, T, N, M, K):

@@ -120,6 +120,11 @@ int fbp_device_memory(int device, uint64_t *total_bytes, uint64_t *free_bytes);
 /* Device-resident state of one chunk: voltage/residual, likelihood buffer,
  * per-window pass state and spike lists (bpp.py:355-423). */
 int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out);
+
+/* Device bytes fbp_segment_create would allocate for cfg (validates cfg).
+ * Plays the role of the reference's per-chunk memory estimate (bpp.py:306-326)
+ * for callers that admit several segments against an HBM budget. */
+int fbp_segment_bytes(const fbp_config *cfg, uint64_t *bytes);
 void fbp_segment_destroy(fbp_segment *seg);
 
 /* Upload templates [T][C][N], bias [T] and thresholds [T] from host memory

@@ -30,7 +30,7 @@ OVERLAP_MODES = {"direct": 0, "screened": 1}
 # Names every build of libfbp.so must export (checked by the CPU test suite).
 EXPORTS = (
     "fbp_version", "fbp_last_error", "fbp_device_count", "fbp_device_memory",
-    "fbp_segment_create", "fbp_segment_destroy", "fbp_segment_set_templates",
+    "fbp_segment_create", "fbp_segment_bytes", "fbp_segment_destroy", "fbp_segment_set_templates",
     "fbp_segment_set_voltage", "fbp_segment_set_voltage_device", "fbp_segment_run",
     "fbp_segment_num_spikes", "fbp_segment_get_spikes", "fbp_segment_get_clips",
     "fbp_segment_get_residual", "fbp_segment_get_likelihood", "fbp_segment_get_stats",
@@ -96,6 +96,7 @@ def _declare(lib):
     lib.fbp_device_count.argtypes = [P(ctypes.c_int)]
     lib.fbp_device_memory.argtypes = [ctypes.c_int, u64p, u64p]
     lib.fbp_segment_create.argtypes = [ctypes.c_int, P(FbpConfig), P(vp)]
+    lib.fbp_segment_bytes.argtypes = [P(FbpConfig), u64p]
     lib.fbp_segment_destroy.argtypes = [vp]
     lib.fbp_segment_destroy.restype = None
     lib.fbp_segment_set_templates.argtypes = [vp, f32p, f32p, f32p]
@@ -161,6 +162,18 @@ def device_memory(device=0):
     tot, fr = ctypes.c_uint64(0), ctypes.c_uint64(0)
     check(lib.fbp_device_memory(device, ctypes.byref(tot), ctypes.byref(fr)), "fbp_device_memory")
     return tot.value, fr.value
+
+
+def segment_bytes(n_chans, n_samples, n_templates, template_len, max_shift, do_overlap=True, reduce_wg=256,
+                  precision="fp32", overlap_mode="direct"):
+    """Device bytes a Segment of this shape allocates (raises ValueError if
+    the shape is not supported).  Needs no GPU."""
+    lib = load()
+    cfg = FbpConfig(n_chans, n_samples, n_templates, template_len, max_shift, 1 if do_overlap else 0, reduce_wg,
+                    PRECISIONS[precision], OVERLAP_MODES[overlap_mode], 0, 0)
+    b = ctypes.c_uint64(0)
+    check(lib.fbp_segment_bytes(ctypes.byref(cfg), ctypes.byref(b)), "fbp_segment_bytes")
+    return b.value
 
 
 class Segment:

@@ -72,10 +72,31 @@ struct fbp_segment {
 
 namespace {
 
-template <class T>
-hipError_t dalloc(T **p, uint64_t n)
+// Device memory of a segment comes from the device's stream-ordered pool on
+// the segment's own stream: creating and destroying segments while others run
+// (the serving path) then never synchronises the device, and freed blocks are
+// reused by the next segment instead of going back to the driver.
+void keep_pool_memory(int device)
 {
-    return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+    static bool done[64] = {false};
+    if (device < 0 || device >= 64 || done[device]) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        uint64_t keep = ~0ull;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    done[device] = true;
+}
+
+template <class T>
+hipError_t dalloc(T **p, uint64_t n, hipStream_t st)
+{
+    return hipMallocAsync((void **)p, (n ? n : 1) * sizeof(T), st);
+}
+
+void dfree(void *p, hipStream_t st)
+{
+    if (p) (void)hipFreeAsync(p, st);
 }
 
 void free_state(fbp_segment *g)
@@ -85,9 +106,29 @@ void free_state(fbp_segment *g)
                     s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
                     s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabB2, s.tnorm,
                     s.TmB, s.rq};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    for (void *p : ptrs) dfree(p, g->own_stream);
+    if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
     s = DeviceState{};
+}
+
+// Bytes a segment allocates on the device (each block rounded to 256 bytes),
+// including the initial spike list; clips are allocated on demand on top.
+uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
+{
+    auto r = [](uint64_t b) { return (b + 255) / 256 * 256; };
+    const uint64_t W = d.W, CK = (uint64_t)d.C * d.K, T = d.T;
+    const uint64_t nb = (W + kCompactItems - 1) / kCompactItems;
+    uint64_t b = 2 * r(CK * 4) + r(T * d.C * d.N * 4) + r((uint64_t)d.C * d.N * d.Tp * 4) + r(d.Tp * 4) +
+                 r(T * 4) + r(T * d.K * 4) + 5 * r(W * 4) + 2 * r(W + 2) + 3 * r(W * 4) + r(W) + 3 * r(W * 4) +
+                 r((nb + 1) * 4) + r(sizeof(Counters)) + r(T * 4) + 2 * r(W * 4);
+    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED) b += r(T * d.S * T * d.S * 4) + r(T * d.S * 4);
+    if (cfg->precision != FBP_PREC_FP32) {
+        uint64_t cap = T * d.K / 8;
+        if (cap < (1u << 20)) cap = 1u << 20;
+        if (cap > (64u << 20)) cap = 64u << 20;
+        b += r(cap * 8) + r((uint64_t)d.C * mfma_ks(d) * mfma_ct(d) * 64 * 8);
+    }
+    return b;
 }
 
 int set_device(const fbp_segment *g)
@@ -103,16 +144,15 @@ int ensure_spike_capacity(fbp_segment *g, uint64_t need, uint64_t keep, hipStrea
     uint64_t cap = g->cap * 2;
     if (cap < need) cap = need;
     uint32_t *ni = nullptr, *nl = nullptr;
-    HIP_TRY(dalloc(&ni, cap), "spike buffer");
-    hipError_t e = dalloc(&nl, cap);
-    if (e != hipSuccess) { (void)hipFree(ni); return hip_fail(e, "spike buffer"); }
+    HIP_TRY(dalloc(&ni, cap, st), "spike buffer");
+    hipError_t e = dalloc(&nl, cap, st);
+    if (e != hipSuccess) { dfree(ni, st); return hip_fail(e, "spike buffer"); }
     if (keep > 0) {
         HIP_TRY(hipMemcpyAsync(ni, g->s.sp_idx, keep * 4, hipMemcpyDeviceToDevice, st), "grow spikes");
         HIP_TRY(hipMemcpyAsync(nl, g->s.sp_lab, keep * 4, hipMemcpyDeviceToDevice, st), "grow spikes");
     }
-    HIP_TRY(hipStreamSynchronize(st), "grow spikes");
-    (void)hipFree(g->s.sp_idx);
-    (void)hipFree(g->s.sp_lab);
+    dfree(g->s.sp_idx, st);
+    dfree(g->s.sp_lab, st);
     g->s.sp_idx = ni;
     g->s.sp_lab = nl;
     g->cap = cap;
@@ -153,10 +193,19 @@ int fbp_device_memory(int device, uint64_t *total_bytes, uint64_t *free_bytes)
     return FBP_OK;
 }
 
-int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
+static int make_dims(const fbp_config *cfg, Dims &d);
+
+int fbp_segment_bytes(const fbp_config *cfg, uint64_t *bytes)
 {
-    if (!cfg || !out) return fail(FBP_EINVAL, "NULL argument");
-    *out = nullptr;
+    if (!cfg || !bytes) return fail(FBP_EINVAL, "NULL argument");
+    Dims d{};
+    if (int rc = make_dims(cfg, d)) return rc;
+    *bytes = segment_bytes(cfg, d);
+    return FBP_OK;
+}
+
+static int make_dims(const fbp_config *cfg, Dims &d)
+{
     const uint32_t C = cfg->n_chans, K = cfg->n_samples, T = cfg->n_templates, N = cfg->template_len;
     if (C == 0 || T == 0 || N == 0) return fail(FBP_EINVAL, "channels, templates and template length must be > 0");
     if (K <= N) return fail(FBP_EINVAL, "chunk of %u samples is not longer than the template (%u)", K, N);
@@ -172,14 +221,6 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         return fail(FBP_EINVAL, "unknown overlap mode %u", cfg->overlap_mode);
     if ((uint64_t)C * N * sizeof(float) * 2 > kMaxLds)
         return fail(FBP_EINVAL, "C*N = %u exceeds the overlap kernel's LDS staging", C * N);
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(FBP_ENODEV, "No GPU available");
-    if (device < 0 || device >= n) return fail(FBP_EINVAL, "device %d out of range", device);
-
-    fbp_segment *g = new fbp_segment();
-    g->device = device;
-    g->cfg = *cfg;
-    Dims &d = g->d;
     d.C = C; d.K = K; d.T = T; d.N = N; d.M = cfg->max_shift;
     d.S = 2 * cfg->max_shift + 1;
     d.W = (uint32_t)(((uint64_t)K + N - 1) / N);
@@ -188,18 +229,36 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     d.reduce_wg = cfg->reduce_wg;
     d.items = (uint64_t)d.S * d.S * T;
     d.nl = (d.items - 1) / cfg->reduce_wg + 1;
-    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED && overlap_screened_lds(d) > kMaxLds) {
-        delete g;
+    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED && overlap_screened_lds(d) > kMaxLds)
         return fail(FBP_EINVAL, "screened overlap needs %zu bytes of LDS (T*S too large)", overlap_screened_lds(d));
-    }
+    return FBP_OK;
+}
+
+int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
+{
+    if (!cfg || !out) return fail(FBP_EINVAL, "NULL argument");
+    *out = nullptr;
+    Dims dd{};
+    if (int rc = make_dims(cfg, dd)) return rc;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(FBP_ENODEV, "No GPU available");
+    if (device < 0 || device >= n) return fail(FBP_EINVAL, "device %d out of range", device);
+    fbp_segment *g = new fbp_segment();
+    g->device = device;
+    g->cfg = *cfg;
+    g->d = dd;
+    const Dims &d = g->d;
+    const uint32_t C = d.C, K = d.K, T = d.T, N = d.N;
 
     int rc = FBP_OK;
     hipError_t e = hipSetDevice(device);
+    keep_pool_memory(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->own_stream, hipStreamNonBlocking);
     const uint64_t W = d.W, CK = (uint64_t)C * K;
     DeviceState &s = g->s;
     const uint32_t nb = (uint32_t)((W + kCompactItems - 1) / kCompactItems);
 #define ALLOC(ptr, n)                                  \
-    if (e == hipSuccess) e = dalloc(&(ptr), (n));
+    if (e == hipSuccess) e = dalloc(&(ptr), (n), g->own_stream);
     ALLOC(s.V, CK)
     ALLOC(s.V0, CK)
     ALLOC(s.Tm, (uint64_t)T * C * N)
@@ -249,7 +308,6 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         g->coef[3] = (float)(2 * gp * 1.01);
     }
 #undef ALLOC
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&g->ev_start);
     if (e == hipSuccess) e = hipEventCreate(&g->ev_stop);
     for (int q = 0; q < 4 && e == hipSuccess; q++) e = hipEventCreate(&g->ev_stage[q]);
@@ -259,7 +317,8 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         fbp_segment_destroy(g);
         return rc;
     }
-    rc = ensure_spike_capacity(g, W, 0, g->own_stream);
+    rc = ensure_spike_capacity(g, 2 * W, 0, g->own_stream);
+    if (rc == FBP_OK && hipStreamSynchronize(g->own_stream) != hipSuccess) rc = fail(FBP_EHIP, "allocation failed");
     if (rc) { fbp_segment_destroy(g); return rc; }
     *out = g;
     return FBP_OK;
@@ -270,7 +329,7 @@ void fbp_segment_destroy(fbp_segment *g)
     if (!g) return;
     (void)hipSetDevice(g->device);
     if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
-    free_state(g);
+    free_state(g);   // stream-ordered frees, then a wait on the segment's stream
     if (g->ev_start) (void)hipEventDestroy(g->ev_start);
     if (g->ev_stop) (void)hipEventDestroy(g->ev_stop);
     for (int q = 0; q < 4; q++)
@@ -287,7 +346,7 @@ int fbp_segment_set_templates(fbp_segment *g, const float *templates, const floa
     const Dims &d = g->d;
     hipStream_t st = g->own_stream;
     float *tmp_bias = nullptr;
-    HIP_TRY(dalloc(&tmp_bias, d.T), "bias staging");
+    HIP_TRY(dalloc(&tmp_bias, d.T, st), "bias staging");
     hipError_t e = hipMemcpyAsync(g->s.Tm, templates, (uint64_t)d.T * d.C * d.N * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(tmp_bias, bias, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(g->s.thr, thresholds, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
@@ -295,8 +354,8 @@ int fbp_segment_set_templates(fbp_segment *g, const float *templates, const floa
     if (e == hipSuccess) e = launch_template_norms(d, g->s, st);
     if (e == hipSuccess && g->cfg.overlap_mode == FBP_OVERLAP_SCREENED) e = launch_overlap_tables(d, g->s, st);
     if (e == hipSuccess && g->cfg.precision != FBP_PREC_FP32) e = launch_pack_templates(d, g->s, g->cfg.precision, st);
+    dfree(tmp_bias, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(tmp_bias);
     if (e != hipSuccess) return hip_fail(e, "fbp_segment_set_templates");
     g->have_templates = true;
     return FBP_OK;
@@ -451,10 +510,10 @@ int fbp_segment_get_clips(fbp_segment *g, float *clips)
     if (int rc = set_device(g)) return rc;
     const uint64_t n = g->n_spikes * g->d.C * g->d.N;
     if (n > g->clips_cap) {
-        (void)hipFree(g->s.clips);
+        dfree(g->s.clips, g->own_stream);
         g->s.clips = nullptr;
         g->clips_cap = 0;
-        HIP_TRY(dalloc(&g->s.clips, n), "clips");
+        HIP_TRY(dalloc(&g->s.clips, n, g->own_stream), "clips");
         g->clips_cap = n;
     }
     HIP_TRY(launch_clips(g->d, g->s, g->n_spikes, g->own_stream), "clips");

@@ -183,3 +183,33 @@ def config(name, seed=0, duration_s=None):
         return make_recording(seed, 128, 30000, duration_s or 600.0, 200, 61, (6.0, 25.0), (0.5, 100.0),
                               gains="gauss", spatial_sigma=2.0, support=8)
     raise ValueError("unknown workload %r" % name)
+
+
+def serving_mix(seed=0, n_requests=64, rate_hz=16.0, duration_range=(5.0, 120.0), channel_range=(4, 32),
+                template_range=(3, 40), length_range=(41, 81), peak_range=(4.0, 20.0), rates=(1.0, 60.0)):
+    """Request specs of the served mix: Poisson arrivals at rate_hz; per
+    request a length uniform in duration_range at 30 or 40 kHz, channels and
+    templates uniform in their ranges, an odd template length in
+    length_range, DoG peaks in peak_range (sigma) and Poisson trains at rates
+    (Hz).  Returns a list of dicts (arrival_s, seed and make_recording
+    arguments); build the data with recording_for(spec)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    specs = []
+    for k in range(n_requests):
+        t += rng.exponential(1.0 / rate_hz)
+        n = int(rng.integers(length_range[0] // 2, length_range[1] // 2 + 1)) * 2 + 1
+        n = min(max(n, length_range[0] | 1), length_range[1])
+        specs.append(dict(arrival_s=t, seed=int(rng.integers(0, 2 ** 31)),
+                          n_chans=int(rng.integers(channel_range[0], channel_range[1] + 1)),
+                          fs=int(rng.choice([30000, 40000])),
+                          duration_s=float(rng.uniform(*duration_range)),
+                          n_templates=int(rng.integers(template_range[0], template_range[1] + 1)),
+                          n_samples_per_chan=n, peak_range=peak_range, rate_range=rates))
+    return specs
+
+
+def recording_for(spec):
+    return make_recording(spec['seed'], spec['n_chans'], spec['fs'], spec['duration_s'], spec['n_templates'],
+                          spec['n_samples_per_chan'], spec['peak_range'], spec['rate_range'], gains="gauss",
+                          spatial_sigma=2.0)
