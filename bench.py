@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=64, help="requests of the served mix (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate of the mix (1/s)")
     ap.add_argument("--serve-inflight", type=int, default=8)
+    ap.add_argument("--no-precision-report", action="store_true",
+                    help="skip the likelihood error report against the float64 oracle (4 ch x 60 s)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -120,6 +122,10 @@ def main():
                  "speedup_of_timed_path": (C * K / (elapsed / args.steps)) / (C * K / dt)}
         log("plain fp32 path: %.1f ms per step, spike lists identical: %s" % (1000 * dt, equal))
 
+    prec = None
+    if not args.no_precision_report:
+        prec = precision_report(args)
+
     serve = None
     if args.serve_requests > 0:
         serve = serving_mix_run(args)
@@ -147,9 +153,68 @@ def main():
         "stages": stages,
         "fp32_plain": plain,
         "serving": serve,
+        "precision_errors": prec,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def precision_report(args):
+    """Errors of the first-pass likelihood against float64 on the 4-channel,
+    60 s workload (5 templates x 61 samples): the float32 chain, the raw bf16
+    and fp16 matrix-core screens (before any float32 re-score) and the screen
+    bound; and whether each path's full run returns the float32 spike list."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from spikesorting_fullpursuit_amd import _lib, synthetic
+    rec = synthetic.config("4ch-60s", seed=args.seed)
+    sm = rec['separability_metrics']
+    V = rec['voltage']
+    C, K = V.shape
+    T = sm['templates'].shape[0]
+    N = 61
+    M = N // 2
+    bias = np.float32(-0.5 * sm['template_SS'])
+    thr = np.float32(sm['neuron_lower_thresholds'])
+    t = time.perf_counter()
+    L64 = oracle.likelihood_f64(V, sm['templates'], sm['template_SS'].astype(np.float32), N)
+    log("float64 likelihood (oracle, host) in %.1f s" % (time.perf_counter() - t))
+    valid = np.arange(K) + N < K
+
+    def run(precision, debug=0, max_passes=0, overlap="screened"):
+        with _lib.Segment(C, K, T, N, M, True, 256, precision, overlap, max_passes, debug_flags=debug) as seg:
+            seg.set_templates(sm['templates'], bias, thr)
+            seg.set_voltage(V)
+            seg.run(allow_cap=max_passes > 0)
+            return seg.likelihood(), seg.spikes(), seg.stats()
+
+    L32, ev32, _ = run("fp32", overlap="direct")
+    L32p1, _, _ = run("fp32", max_passes=1)
+    out = {"workload": "4ch-60s (4 ch, 40 kHz, 60 s, 5 templates x 61)", "first_pass": {}}
+    tn = np.sqrt(np.sum(sm['templates'].astype(np.float64) ** 2, axis=1))
+    s2 = np.concatenate([np.sum(V.astype(np.float64) ** 2, axis=0), np.zeros(N)])
+    cs = np.concatenate([[0.0], np.cumsum(s2)])
+    vn = np.sqrt(cs[N:N + K] - cs[:K])
+    n32 = C * N + 1
+    g32 = n32 * 2.0 ** -24 / (1 - n32 * 2.0 ** -24)
+    b32 = g32 * (np.abs(bias.astype(np.float64))[:, None] + tn[:, None] * vn[None, :])
+    e32 = np.abs(L32p1.astype(np.float64) - L64)[:, valid]
+    out["first_pass"]["fp32"] = {"max_abs_err": float(e32.max()), "median_abs_err": float(np.median(e32)),
+                                 "max_err_over_gamma_bound": float((e32 / b32[:, valid]).max())}
+    for prec in ("bf16", "fp16", "fp8"):
+        Lr, _, _ = run(prec, debug=1, max_passes=1)
+        e = np.abs(Lr.astype(np.float64) - L64)[:, valid]
+        e_vs32 = np.abs(Lr.astype(np.float64) - L32p1.astype(np.float64))[:, valid]
+        _, evp, stp = run(prec)
+        same = bool(np.array_equal(evp[0], ev32[0]) and np.array_equal(evp[1], ev32[1]))
+        out["first_pass"][prec + "_screen"] = {
+            "max_abs_err": float(e.max()), "median_abs_err": float(np.median(e)),
+            "max_abs_err_vs_fp32": float(e_vs32.max()),
+            "events_equal_fp32_after_rescore": same,
+            "rescored_fraction": stp["rescored_positions"] / max(1, stp["corr_positions"] * T)}
+        log("%s screen: max |L - L64| = %.4g, events equal after re-score: %s" % (prec, e.max(), same))
+    out["spikes_fp32"] = int(ev32[0].size)
+    return out
 
 
 def serving_mix_run(args):

@@ -67,7 +67,8 @@ struct DeviceState {
     float *tabAB;        // [T][S][T][S] screened overlap: shifted-template cross products
     float *tabB2;        // [T][S]       squared norms of shifted, truncated templates
     float *tnorm;        // [T]          template norms (upper bounds)
-    void *TmB;           // packed bf16/fp16 template fragments for the matrix cores
+    float *tscale;       // [T]          fp8 screen: per-template power-of-two scale (1 otherwise)
+    void *TmB;           // packed bf16/fp16/fp8 template fragments for the matrix cores
     uint64_t *rq;        // re-score queue: (template << 32) | position
 };
 
@@ -84,9 +85,10 @@ hipError_t launch_transpose_templates(const Dims &d, const float *Tm, const floa
                                       float *TmT, float *bias_out, hipStream_t st);
 
 hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
-uint32_t mfma_ks(const Dims &d);
+uint32_t mfma_kstep(uint32_t precision);
+uint32_t mfma_ks(const Dims &d, uint32_t precision);
 uint32_t mfma_ct(const Dims &d);
-hipError_t launch_template_norms(const Dims &d, const DeviceState &s, hipStream_t st);
+hipError_t launch_template_norms(const Dims &d, const DeviceState &s, uint32_t precision, hipStream_t st);
 hipError_t launch_pack_templates(const Dims &d, const DeviceState &s, uint32_t precision, hipStream_t st);
 hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_win, uint32_t precision,
                               const float *coef, uint32_t rq_cap, uint32_t no_rescore, hipStream_t st);

@@ -187,26 +187,54 @@ __device__ __forceinline__ uint16_t to_f16(float x)
     return __builtin_bit_cast(uint16_t, h);
 }
 
+// Element traits of the three screens: bf16 and fp16 use
+// v_mfma_f32_16x16x16{bf16,f16} (k = 16, 4 two-byte elements per lane), fp8
+// e4m3 uses v_mfma_f32_16x16x32_fp8_fp8 (k = 32, 8 one-byte elements per
+// lane).  Both read one aligned 8-byte fragment per lane.
 template <int PREC>
-__device__ __forceinline__ uint16_t to_narrow(float x) { return PREC == 1 ? to_bf16(x) : to_f16(x); }
+struct NarrowTraits {
+    static constexpr uint32_t kBytes = PREC == 3 ? 1 : 2;   // bytes per element
+    static constexpr uint32_t kK = PREC == 3 ? 32 : 16;       // k per MFMA
+    static constexpr uint32_t kPer = 8 / kBytes;              // elements per lane fragment
+    static constexpr uint32_t kCopies = kPer;                  // shifted voltage copies in LDS
+};
+
+// Round a float32 to the screen's element format (round to nearest even).
+// fp8 returns the e4m3 byte in the low 8 bits.
+template <int PREC>
+__device__ __forceinline__ uint32_t to_narrow(float x)
+{
+    if constexpr (PREC == 1) return to_bf16(x);
+    else if constexpr (PREC == 2) return to_f16(x);
+    else return (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xffu;
+}
 
 template <int PREC>
-__device__ __forceinline__ float4v mfma16(short4v a, short4v b, float4v c)
+__device__ __forceinline__ float4v mfma_narrow(uint64_t a, uint64_t b, float4v c)
 {
     if constexpr (PREC == 1) {
-        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-    } else {
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4v, a), __builtin_bit_cast(short4v, b),
+                                                        c, 0, 0, 0);
+    } else if constexpr (PREC == 2) {
         return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(half4v, a), __builtin_bit_cast(half4v, b),
                                                       c, 0, 0, 0);
+    } else {
+        return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)a, (long)b, c, 0, 0, 0);
     }
 }
 
-// TmB[((c*KS + ks)*CT + ct)*64 + lane] = 4 narrow values of template
-// t = ct*16 + lane%16 at samples j = ks*16 + 4*(lane/16) + 0..3 (0 if padded).
+// Largest |value| the fp8 screen feeds to the matrix cores unscaled; a window
+// holding a larger voltage sample has all its entries re-scored.
+constexpr float kFp8Limit = 440.0f;
+
+// TmB[((c*KS + ks)*CT + ct)*64 + lane] = the 8-byte fragment of template
+// t = ct*16 + lane%16 at samples j = ks*K + per*(lane/16) + 0..per-1 (0 when
+// padded), scaled by tscale[t] (a power of two; 1 except for fp8).
 template <int PREC>
-__global__ void k_pack_templates(Dims d, const float *__restrict__ Tm, short4v *__restrict__ TmB, uint32_t KS,
-                                 uint32_t CT)
+__global__ void k_pack_templates(Dims d, const float *__restrict__ Tm, const float *__restrict__ tscale,
+                                 uint64_t *__restrict__ TmB, uint32_t KS, uint32_t CT)
 {
+    using Tr = NarrowTraits<PREC>;
     const uint64_t total = (uint64_t)d.C * KS * CT * 64;
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
          e += (uint64_t)gridDim.x * blockDim.x) {
@@ -216,30 +244,43 @@ __global__ void k_pack_templates(Dims d, const float *__restrict__ Tm, short4v *
         const uint32_t ks = (uint32_t)((r / CT) % KS);
         const uint32_t c = (uint32_t)(r / ((uint64_t)CT * KS));
         const uint32_t t = ct * 16 + (lane & 15);
-        short4v v;
-        for (int q = 0; q < 4; q++) {
-            const uint32_t j = ks * 16 + 4 * (lane >> 4) + q;
-            const float x = (t < d.T && j < d.N) ? Tm[((uint64_t)t * d.C + c) * d.N + j] : 0.0f;
-            v[q] = (short)to_narrow<PREC>(x);
+        const float sc = t < d.T ? tscale[t] : 1.0f;
+        uint64_t v = 0;
+        for (uint32_t q = 0; q < Tr::kPer; q++) {
+            const uint32_t j = ks * Tr::kK + Tr::kPer * (lane >> 4) + q;
+            const float x = (t < d.T && j < d.N) ? Tm[((uint64_t)t * d.C + c) * d.N + j] * sc : 0.0f;
+            v |= (uint64_t)to_narrow<PREC>(x) << (8 * Tr::kBytes * q);
         }
         TmB[e] = v;
     }
 }
 
-// Template norms (upper bounds), used by both screens.
-__global__ void k_template_norms(Dims d, const float *__restrict__ Tm, float *__restrict__ tnorm)
+// Template norms (upper bounds), used by both screens, and the fp8 scales:
+// the power of two that brings the template's largest |value| to (112, 224].
+__global__ void k_template_norms(Dims d, const float *__restrict__ Tm, float *__restrict__ tnorm,
+                                 float *__restrict__ tscale, uint32_t fp8)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.T) return;
     double acc = 0.0;
+    float mx = 0.0f;
     for (uint64_t k = 0; k < (uint64_t)d.C * d.N; k++) {
-        const double x = Tm[(uint64_t)t * d.C * d.N + k];
+        const float xf = Tm[(uint64_t)t * d.C * d.N + k];
+        const double x = xf;
         acc = __builtin_fma(x, x, acc);
+        mx = fmaxf(mx, fabsf(xf));
     }
     const double nrm = sqrt(acc) * (1.0 + 1e-12);
     float up = (float)nrm;
     if ((double)up < nrm) up = __uint_as_float(__float_as_uint(up) + 1u);
     tnorm[t] = up;
+    float sc = 1.0f;
+    if (fp8 && mx > 0.0f) {
+        int ex;
+        frexpf(224.0f / mx, &ex);   // 224/mx = m * 2^ex with m in [0.5, 1)
+        sc = ldexpf(1.0f, ex - 1);  // largest power of two <= 224/mx
+    }
+    tscale[t] = sc;
 }
 
 struct ScreenCoef {
@@ -264,24 +305,29 @@ constexpr int kMfmaCTB = 4;     // template column tiles per accumulation chunk
 
 template <int PREC, int RT>
 __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restrict__ V, const float *__restrict__ Tm,
-                                                  const short4v *__restrict__ TmB, const float *__restrict__ bias,
-                                                  const float *__restrict__ tnorm, const uint32_t *__restrict__ win,
-                                                  float *__restrict__ L, Counters *ctr, uint64_t *__restrict__ rq,
-                                                  uint32_t rq_cap, uint32_t cb_max, ScreenCoef co, uint32_t no_rescore)
+                                                  const uint64_t *__restrict__ TmB, const float *__restrict__ bias,
+                                                  const float *__restrict__ tnorm, const float *__restrict__ tscale,
+                                                  const uint32_t *__restrict__ win, float *__restrict__ L,
+                                                  Counters *ctr, uint64_t *__restrict__ rq, uint32_t rq_cap,
+                                                  uint32_t cb_max, ScreenCoef co, uint32_t no_rescore)
 {
-    extern __shared__ uint16_t vb[];            // [4 shifts][cb][LA] narrow voltage
+    using Tr = NarrowTraits<PREC>;
+    extern __shared__ uint64_t vb_raw[];
+    uint8_t *vb = (uint8_t *)vb_raw;            // [copies][cb][LA] narrow voltage, shifted by 0..copies-1
     __shared__ double s2[kMfmaMaxRT * 16 * 2 + 16];
     __shared__ float vnb[kMfmaMaxRT * 16];
+    __shared__ int over;
     const int lane = threadIdx.x;
-    const uint32_t KS = (d.N + 15) / 16;
+    const uint32_t KS = (d.N + Tr::kK - 1) / Tr::kK;
     const uint32_t CT = (d.T + 15) / 16;
-    const uint32_t LA = (RT + KS) * 16;        // samples staged per channel
+    const uint32_t LA = RT * 16 + KS * Tr::kK;  // elements staged per channel and copy
     const uint32_t w = win[blockIdx.x];
     const uint64_t base = (uint64_t)w * d.N;
     const uint64_t wend = min(base + d.N, (uint64_t)d.K);
     const uint32_t np = (uint32_t)(wend - base);   // positions of this window
 
     // Per-position window norm bound |v_i| (float64 sums of the float32 samples).
+    if (lane == 0) over = 0;
     for (uint32_t x = lane; x < RT * 16 + d.N; x += 64) {
         double a = 0.0;
         const uint64_t g = base + x;
@@ -319,9 +365,14 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
                     const uint32_t c = e / LA, y = e % LA;
                     const float *vr = V + (uint64_t)(c0 + c) * d.K;
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
+                    for (int r = 0; r < (int)Tr::kCopies; r++) {
                         const uint64_t g = base + y + r;
-                        vb[((uint64_t)r * cb + c) * LA + y] = to_narrow<PREC>(g < d.K ? vr[g] : 0.0f);
+                        const float x = g < d.K ? vr[g] : 0.0f;
+                        if (PREC == 3 && fabsf(x) > kFp8Limit) over = 1;
+                        const uint32_t q = to_narrow<PREC>(x);
+                        uint8_t *dst = vb + (((uint64_t)r * cb + c) * LA + y) * Tr::kBytes;
+                        if constexpr (Tr::kBytes == 2) *(uint16_t *)dst = (uint16_t)q;
+                        else *dst = (uint8_t)q;
                     }
                 }
                 __syncthreads();
@@ -329,22 +380,24 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
             }
             for (uint32_t c = 0; c < cb; c++) {
                 for (uint32_t ks = 0; ks < KS; ks++) {
-                    short4v b[kMfmaCTB];
+                    uint64_t b[kMfmaCTB];
 #pragma unroll
                     for (int ct = 0; ct < kMfmaCTB; ct++)
                         if (ct < (int)nct) b[ct] = TmB[(((uint64_t)(c0 + c) * KS + ks) * CT + ct0 + ct) * 64 + lane];
 #pragma unroll
                     for (int rt = 0; rt < RT; rt++) {
-                        const uint32_t o = rt * 16 + ks * 16 + 4 * (lane >> 4) + (lane & 15);
-                        const uint32_t r = o & 3;
-                        const short4v a = *(const short4v *)(vb + ((uint64_t)r * cb + c) * LA + (o - r));
+                        const uint32_t o = rt * 16 + ks * Tr::kK + Tr::kPer * (lane >> 4) + (lane & 15);
+                        const uint32_t r = o % Tr::kCopies;
+                        const uint64_t a =
+                            *(const uint64_t *)(vb + (((uint64_t)r * cb + c) * LA + (o - r)) * Tr::kBytes);
 #pragma unroll
                         for (int ct = 0; ct < kMfmaCTB; ct++)
-                            if (ct < (int)nct) acc[rt][ct] = mfma16<PREC>(a, b[ct], acc[rt][ct]);
+                            if (ct < (int)nct) acc[rt][ct] = mfma_narrow<PREC>(a, b[ct], acc[rt][ct]);
                     }
                 }
             }
         }
+        const bool force = PREC == 3 && over;
         // Epilogue: bias, bound, store, queue the entries that need float32.
         uint32_t nflag = 0;
 #pragma unroll
@@ -358,11 +411,12 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
                     if (ct >= (int)nct || p >= np || t >= d.T) continue;
                     const uint64_t i = base + p;
                     if (i + d.N >= d.K) { L[(uint64_t)t * d.K + i] = 0.0f; continue; }   // bp.cl:231-234
-                    const float lh = bias[t] + acc[rt][ct][q];
+                    const float inv = 1.0f / tscale[t];
+                    const float lh = bias[t] + acc[rt][ct][q] * inv;
                     const float tn = tnorm[t], vn = vnb[p];
-                    const float e = co.k1 * tn * vn + co.k2 * (tn + vn) + co.k3 + co.k4 * fabsf(bias[t]);
+                    const float e = co.k1 * tn * vn + co.k2 * (tn + vn * inv) + co.k3 * inv + co.k4 * fabsf(bias[t]);
                     L[(uint64_t)t * d.K + i] = lh;
-                    if (!(lh <= -e)) nflag++;
+                    if (force || !(lh <= -e)) nflag++;
                 }
         if (no_rescore) continue;
         // wave-wide reservation in the re-score queue
@@ -389,10 +443,11 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
                     if (ct >= (int)nct || p >= np || t >= d.T) continue;
                     const uint64_t i = base + p;
                     if (i + d.N >= d.K) continue;
-                    const float lh = bias[t] + acc[rt][ct][q];
+                    const float inv = 1.0f / tscale[t];
+                    const float lh = bias[t] + acc[rt][ct][q] * inv;
                     const float tn = tnorm[t], vn = vnb[p];
-                    const float e = co.k1 * tn * vn + co.k2 * (tn + vn) + co.k3 + co.k4 * fabsf(bias[t]);
-                    if (lh <= -e) continue;
+                    const float e = co.k1 * tn * vn + co.k2 * (tn + vn * inv) + co.k3 * inv + co.k4 * fabsf(bias[t]);
+                    if (!force && lh <= -e) continue;
                     if (pos < rq_cap) rq[pos] = ((uint64_t)t << 32) | (uint64_t)i;
                     else L[(uint64_t)t * d.K + i] = exact_likelihood(d, V, Tm, bias, t, i);   // queue full
                     pos++;
@@ -1205,41 +1260,49 @@ hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win,
     return hipGetLastError();
 }
 
-uint32_t mfma_ks(const Dims &d) { return (d.N + 15) / 16; }
+uint32_t mfma_kstep(uint32_t precision) { return precision == 3 ? 32 : 16; }
+uint32_t mfma_ks(const Dims &d, uint32_t precision) { return (d.N + mfma_kstep(precision) - 1) / mfma_kstep(precision); }
 uint32_t mfma_ct(const Dims &d) { return (d.T + 15) / 16; }
 
-hipError_t launch_template_norms(const Dims &d, const DeviceState &s, hipStream_t st)
+hipError_t launch_template_norms(const Dims &d, const DeviceState &s, uint32_t precision, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_template_norms, dim3((d.T + 63) / 64), dim3(64), 0, st, d, s.Tm, s.tnorm);
+    hipLaunchKernelGGL(k_template_norms, dim3((d.T + 63) / 64), dim3(64), 0, st, d, s.Tm, s.tnorm, s.tscale,
+                       precision == 3 ? 1u : 0u);
+    return hipGetLastError();
+}
+
+template <int PREC>
+static hipError_t launch_pack_p(const Dims &d, const DeviceState &s, hipStream_t st)
+{
+    const uint64_t total = (uint64_t)d.C * mfma_ks(d, PREC) * mfma_ct(d) * 64;
+    hipLaunchKernelGGL(k_pack_templates<PREC>, dim3(grid_for(total, 256, 4096)), dim3(256), 0, st, d, s.Tm, s.tscale,
+                       (uint64_t *)s.TmB, mfma_ks(d, PREC), mfma_ct(d));
     return hipGetLastError();
 }
 
 hipError_t launch_pack_templates(const Dims &d, const DeviceState &s, uint32_t precision, hipStream_t st)
 {
-    const uint64_t total = (uint64_t)d.C * mfma_ks(d) * mfma_ct(d) * 64;
-    if (precision == 1)
-        hipLaunchKernelGGL(k_pack_templates<1>, dim3(grid_for(total, 256, 4096)), dim3(256), 0, st, d, s.Tm,
-                           (short4v *)s.TmB, mfma_ks(d), mfma_ct(d));
-    else
-        hipLaunchKernelGGL(k_pack_templates<2>, dim3(grid_for(total, 256, 4096)), dim3(256), 0, st, d, s.Tm,
-                           (short4v *)s.TmB, mfma_ks(d), mfma_ct(d));
-    return hipGetLastError();
+    if (precision == 1) return launch_pack_p<1>(d, s, st);
+    if (precision == 2) return launch_pack_p<2>(d, s, st);
+    return launch_pack_p<3>(d, s, st);
 }
 
 template <int PREC, int RT>
 static hipError_t launch_corr_mfma_rt(const Dims &d, const DeviceState &s, uint32_t n_win, const float *coef,
                                       uint32_t rq_cap, uint32_t no_rescore, hipStream_t st)
 {
-    const uint32_t LA = (RT + mfma_ks(d)) * 16;
+    using Tr = NarrowTraits<PREC>;
+    const uint32_t LA = RT * 16 + mfma_ks(d, PREC) * Tr::kK;
+    const uint32_t per_channel = Tr::kCopies * LA * Tr::kBytes;
     const uint32_t budget = 48 * 1024;
-    uint32_t cb_max = budget / (4 * LA * 2);
+    uint32_t cb_max = budget / per_channel;
     if (cb_max > d.C) cb_max = d.C;
     if (cb_max == 0) return hipErrorInvalidValue;
-    const size_t lds = 4ull * cb_max * LA * 2;
+    const size_t lds = (size_t)cb_max * per_channel;
     ScreenCoef co{coef[0], coef[1], coef[2], coef[3]};
     hipLaunchKernelGGL((k_corr_mfma<PREC, RT>), dim3(n_win), dim3(64), lds, st, d, s.V, s.Tm,
-                       (const short4v *)s.TmB, s.bias, s.tnorm, s.win, s.L, s.ctr, s.rq, rq_cap, cb_max, co,
-                       no_rescore);
+                       (const uint64_t *)s.TmB, s.bias, s.tnorm, s.tscale, s.win, s.L, s.ctr, s.rq, rq_cap, cb_max,
+                       co, no_rescore);
     return hipGetLastError();
 }
 
@@ -1263,8 +1326,10 @@ hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_wi
 {
     if (n_win == 0) return hipSuccess;
     hipLaunchKernelGGL(k_zero_rescore, dim3(1), dim3(64), 0, st, s.ctr);
-    hipError_t e = precision == 1 ? launch_corr_mfma_p<1>(d, s, n_win, coef, rq_cap, no_rescore, st)
-                                  : launch_corr_mfma_p<2>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    hipError_t e;
+    if (precision == 1) e = launch_corr_mfma_p<1>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    else if (precision == 2) e = launch_corr_mfma_p<2>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    else e = launch_corr_mfma_p<3>(d, s, n_win, coef, rq_cap, no_rescore, st);
     if (e != hipSuccess) return e;
     if (!no_rescore)
         hipLaunchKernelGGL(k_rescore, dim3(2048), dim3(256), 0, st, d, s.V, s.Tm, s.bias, s.rq, s.ctr, rq_cap, s.L);

@@ -105,7 +105,7 @@ void free_state(fbp_segment *g)
     void *ptrs[] = {s.V, s.V0, s.Tm, s.TmT, s.bias, s.thr, s.L, s.best_L, s.best_lab, s.best_idx, s.ov_idx,
                     s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
                     s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabB2, s.tnorm,
-                    s.TmB, s.rq};
+                    s.TmB, s.rq, s.tscale};
     for (void *p : ptrs) dfree(p, g->own_stream);
     if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
     s = DeviceState{};
@@ -126,7 +126,7 @@ uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
         uint64_t cap = T * d.K / 8;
         if (cap < (1u << 20)) cap = 1u << 20;
         if (cap > (64u << 20)) cap = 64u << 20;
-        b += r(cap * 8) + r((uint64_t)d.C * mfma_ks(d) * mfma_ct(d) * 64 * 8);
+        b += r(cap * 8) + r((uint64_t)d.C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64 * 8);
     }
     return b;
 }
@@ -213,7 +213,7 @@ static int make_dims(const fbp_config *cfg, Dims &d)
         return fail(FBP_EINVAL, "reduce_wg must be a power of two");
     if ((uint64_t)cfg->max_shift >= 4ull * N)
         return fail(FBP_EINVAL, "n_max_shift_inds (%u) must be below 4x the template length", cfg->max_shift);
-    if (cfg->precision != FBP_PREC_FP32 && cfg->precision != FBP_PREC_BF16 && cfg->precision != FBP_PREC_FP16)
+    if (cfg->precision > FBP_PREC_FP8)
         return fail(FBP_EINVAL, "precision %u is not available in this build", cfg->precision);
     if (cfg->precision != FBP_PREC_FP32 && N > 96)
         return fail(FBP_EINVAL, "the matrix-core screen supports templates of up to 96 samples (N=%u)", N);
@@ -283,6 +283,7 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     ALLOC(s.blk, nb + 1)
     ALLOC(s.ctr, 1)
     ALLOC(s.tnorm, T)
+    ALLOC(s.tscale, T)
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED) {
         ALLOC(s.tabAB, (uint64_t)T * d.S * T * d.S)
         ALLOC(s.tabB2, (uint64_t)T * d.S)
@@ -293,13 +294,15 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         if (cap > (64u << 20)) cap = 64u << 20;
         g->rq_cap = (uint32_t)cap;
         ALLOC(s.rq, cap)
-        uint16_t *tmb = nullptr;
-        ALLOC(tmb, (uint64_t)C * mfma_ks(d) * mfma_ct(d) * 64 * 4)
+        uint64_t *tmb = nullptr;
+        ALLOC(tmb, (uint64_t)C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64)
         s.TmB = tmb;
-        // Screen bound coefficients (see the correlation screen in kernels.hip).
-        const double u = cfg->precision == FBP_PREC_BF16 ? 0x1p-8 : 0x1p-11;
-        const double delta = cfg->precision == FBP_PREC_BF16 ? 0x1p-126 : 0x1p-14;
-        const double n = (double)C * mfma_ks(d) * 16 + 2.0;
+        // Screen bound coefficients (see the correlation screen in kernels.hip):
+        // unit roundoff u and flushed-subnormal floor delta of the format.
+        const double u = cfg->precision == FBP_PREC_BF16 ? 0x1p-8 : cfg->precision == FBP_PREC_FP16 ? 0x1p-11 : 0x1p-4;
+        const double delta = cfg->precision == FBP_PREC_BF16 ? 0x1p-126 : cfg->precision == FBP_PREC_FP16 ? 0x1p-14
+                                                                                                        : 0x1p-6;
+        const double n = (double)C * mfma_ks(d, cfg->precision) * mfma_kstep(cfg->precision) + 2.0;
         const double gp = n * 0x1p-23 / (1.0 - n * 0x1p-23);
         const double cn = (double)C * N;
         g->coef[0] = (float)((2 * u + u * u + 2 * gp * (1 + u) * (1 + u)) * 1.01);
@@ -351,7 +354,7 @@ int fbp_segment_set_templates(fbp_segment *g, const float *templates, const floa
     if (e == hipSuccess) e = hipMemcpyAsync(tmp_bias, bias, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(g->s.thr, thresholds, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_transpose_templates(d, g->s.Tm, tmp_bias, g->s.TmT, g->s.bias, st);
-    if (e == hipSuccess) e = launch_template_norms(d, g->s, st);
+    if (e == hipSuccess) e = launch_template_norms(d, g->s, g->cfg.precision, st);
     if (e == hipSuccess && g->cfg.overlap_mode == FBP_OVERLAP_SCREENED) e = launch_overlap_tables(d, g->s, st);
     if (e == hipSuccess && g->cfg.precision != FBP_PREC_FP32) e = launch_pack_templates(d, g->s, g->cfg.precision, st);
     dfree(tmp_bias, st);

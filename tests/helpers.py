@@ -45,8 +45,10 @@ def screen_bound(rec, precision):
     C, K = V.shape
     N = rec['N']
     T = rec['templates'].shape[0]
-    u, delta = (2.0 ** -8, 2.0 ** -126) if precision == "bf16" else (2.0 ** -11, 2.0 ** -14)
-    n = C * 16 * ((N + 15) // 16) + 2.0
+    u, delta = {"bf16": (2.0 ** -8, 2.0 ** -126), "fp16": (2.0 ** -11, 2.0 ** -14),
+                "fp8": (2.0 ** -4, 2.0 ** -6)}[precision]
+    kstep = 32 if precision == "fp8" else 16
+    n = C * kstep * ((N + kstep - 1) // kstep) + 2.0
     gp = n * 2.0 ** -23 / (1 - n * 2.0 ** -23)
     cn = C * N
     k1 = (2 * u + u * u + 2 * gp * (1 + u) ** 2) * 1.01
@@ -58,4 +60,9 @@ def screen_bound(rec, precision):
     cs = np.concatenate([[0.0], np.cumsum(s2)])
     vn = np.sqrt(cs[N:N + K] - cs[:K])
     bias = np.abs(rec['bias'].astype(np.float64))
-    return k1 * tn[:, None] * vn[None, :] + k2 * (tn[:, None] + vn[None, :]) + k3 + k4 * bias[:, None]
+    inv = np.ones(T)
+    if precision == "fp8":
+        mx = np.max(np.abs(rec['templates']), axis=1).astype(np.float64)
+        inv = np.where(mx > 0, 1.0 / 2.0 ** np.floor(np.log2(224.0 / np.where(mx > 0, mx, 1.0))), 1.0)
+    return (k1 * tn[:, None] * vn[None, :] + k2 * (tn[:, None] + vn[None, :] * inv[:, None]) + k3 * inv[:, None]
+            + k4 * bias[:, None])

@@ -76,7 +76,7 @@ def test_dense_16ch_spike_list_grows(oracle_mod, mode):
         assert g['stats']['screened_combos'] > 20 * g['stats']['overlap_combos']
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "fp8"])
 @pytest.mark.parametrize("seed,C,T,N,dur", [(31, 4, 5, 61, 0.5), (32, 8, 20, 41, 0.3), (33, 3, 3, 81, 0.3)])
 def test_screen_error_bound_holds(precision, seed, C, T, N, dur):
     # Without the float32 re-score, the matrix-core likelihood must lie within
@@ -96,7 +96,7 @@ def test_screen_error_bound_holds(precision, seed, C, T, N, dur):
 
 
 @pytest.mark.parametrize("mode", ["direct", "screened"])
-@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "fp8"])
 @pytest.mark.parametrize("seed,C,T,N,dur,M,ov,wg", CASES[:5])
 def test_low_precision_decisions_equal_fp32(oracle_mod, seed, C, T, N, dur, M, ov, wg, precision, mode):
     rec = _case(seed, C, T, N, dur)
@@ -114,7 +114,7 @@ def test_low_precision_decisions_equal_fp32(oracle_mod, seed, C, T, N, dur, M, o
     assert g['stats']['rescored_positions'] > 0
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "fp8"])
 def test_low_precision_dense_16ch(oracle_mod, precision):
     from spikesorting_fullpursuit_amd import synthetic
     rec = synthetic.config("16ch-300s", seed=5, duration_s=1.0)
