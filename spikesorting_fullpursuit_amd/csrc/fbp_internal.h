@@ -67,9 +67,11 @@ struct DeviceState {
     float *tabAB;        // [T][S][T][S] screened overlap: shifted-template cross products
     float *tabB2;        // [T][S]       squared norms of shifted, truncated templates
     float *tnorm;        // [T]          template norms (upper bounds)
+    float *TmQ;          // screened overlap: float32 A fragments of the matrix-core Q table
     float *tscale;       // [T]          fp8 screen: per-template power-of-two scale (1 otherwise)
     void *TmB;           // packed bf16/fp16/fp8 template fragments for the matrix cores
     uint64_t *rq;        // re-score queue: (template << 32) | position
+    void *rdesc;         // [W * chunks] (queue start, count) per (window, template chunk)
 };
 
 constexpr int kCompactItems = 1024;   // items per block of the compaction kernels
@@ -88,6 +90,7 @@ hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win,
 uint32_t mfma_kstep(uint32_t precision);
 uint32_t mfma_ks(const Dims &d, uint32_t precision);
 uint32_t mfma_ct(const Dims &d);
+uint32_t mfma_nchunk(const Dims &d);
 hipError_t launch_template_norms(const Dims &d, const DeviceState &s, uint32_t precision, hipStream_t st);
 hipError_t launch_pack_templates(const Dims &d, const DeviceState &s, uint32_t precision, hipStream_t st);
 hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_win, uint32_t precision,
@@ -96,6 +99,7 @@ hipError_t launch_argmax(const Dims &d, const DeviceState &s, uint32_t n_win, hi
 hipError_t launch_compact_ov(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
 hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);
 size_t overlap_screened_lds(const Dims &d);
+uint64_t overlap_qtemplate_floats(const Dims &d);
 hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_t st);
 hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);
 hipError_t launch_check_overlap(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);

@@ -309,7 +309,8 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
                                                   const float *__restrict__ tnorm, const float *__restrict__ tscale,
                                                   const uint32_t *__restrict__ win, float *__restrict__ L,
                                                   Counters *ctr, uint64_t *__restrict__ rq, uint32_t rq_cap,
-                                                  uint32_t cb_max, ScreenCoef co, uint32_t no_rescore)
+                                                  uint2 *__restrict__ rdesc, uint32_t cb_max, ScreenCoef co,
+                                                  uint32_t no_rescore)
 {
     using Tr = NarrowTraits<PREC>;
     extern __shared__ uint64_t vb_raw[];
@@ -432,6 +433,12 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
             qbase = __shfl(qbase, 63);
         }
         uint32_t pos = qbase + incl - nflag;
+        // this (window, template chunk)'s slice of the queue, for k_rescore_win
+        if (lane == 0) {
+            const uint32_t nchunk = (CT + kMfmaCTB - 1) / kMfmaCTB;
+            const uint32_t kept = qbase >= rq_cap ? 0u : min(tot, rq_cap - qbase);
+            rdesc[(uint64_t)blockIdx.x * nchunk + ct0 / kMfmaCTB] = make_uint2(qbase, kept);
+        }
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
@@ -452,6 +459,58 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
                     else L[(uint64_t)t * d.K + i] = exact_likelihood(d, V, Tm, bias, t, i);   // queue full
                     pos++;
                 }
+    }
+}
+
+// Float32 re-score of one (window, template chunk)'s queued entries: one wave,
+// each lane carrying up to kRescoreR chains.  The window's voltage is staged in
+// LDS one channel at a time and template values come from the [C][N][Tp] copy,
+// so for every (c, j) a wave reads one LDS sample per chain and one short run
+// of template values.  Each chain is the reference's: bias, then c = 0..C-1,
+// j = 0..N-1, fused multiply-adds.
+constexpr int kRescoreR = 4;
+
+__global__ __launch_bounds__(64) void k_rescore_win(Dims d, const float *__restrict__ V,
+                                                    const float *__restrict__ TmT, const float *__restrict__ bias,
+                                                    const uint64_t *__restrict__ rq, const uint2 *__restrict__ rdesc,
+                                                    const uint32_t *__restrict__ win, uint32_t nchunk, float *L)
+{
+    __shared__ float sv[2 * kMfmaMaxRT * 16 + 8];
+    const uint2 dsc = rdesc[blockIdx.x];
+    const uint32_t cnt = dsc.y;
+    if (cnt == 0) return;
+    const int lane = threadIdx.x;
+    const uint32_t w = win[blockIdx.x / nchunk];
+    const uint64_t base = (uint64_t)w * d.N;
+    const uint32_t span = ((d.N + 15) / 16) * 16 + d.N;
+    for (uint32_t e0 = 0; e0 < cnt; e0 += 64 * kRescoreR) {
+        uint32_t tt[kRescoreR], pp[kRescoreR];
+        float acc[kRescoreR];
+        bool ok[kRescoreR];
+#pragma unroll
+        for (int r = 0; r < kRescoreR; r++) {
+            const uint32_t e = e0 + lane + 64 * r;
+            ok[r] = e < cnt;
+            const uint64_t ent = ok[r] ? rq[dsc.x + e] : 0;
+            tt[r] = ok[r] ? (uint32_t)(ent >> 32) : 0u;
+            pp[r] = ok[r] ? (uint32_t)((ent & 0xffffffffull) - base) : 0u;
+            acc[r] = bias[tt[r]];
+        }
+        for (uint32_t c = 0; c < d.C; c++) {
+            __syncthreads();
+            const float *vr = V + (uint64_t)c * d.K;
+            for (uint32_t x = lane; x < span; x += 64) sv[x] = base + x < d.K ? vr[base + x] : 0.0f;
+            __syncthreads();
+            const float *tc = TmT + (uint64_t)c * d.N * d.Tp;
+            for (uint32_t j = 0; j < d.N; j++) {
+                const float *tj = tc + (uint64_t)j * d.Tp;
+#pragma unroll
+                for (int r = 0; r < kRescoreR; r++) acc[r] = __builtin_fmaf(tj[tt[r]], sv[pp[r] + j], acc[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kRescoreR; r++)
+            if (ok[r]) L[(uint64_t)tt[r] * d.K + base + pp[r]] = acc[r];
     }
 }
 
@@ -906,17 +965,72 @@ __device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t 
     }
 }
 
+// Q table on the matrix cores: Q[t][s] = sum over (c, x) of T[t][c][x] *
+// vpad[c][x + s - M] (vpad = window voltage, zero outside [0, N)), one
+// v_mfma_f32_16x16x4f32 per (16 templates, 16 shifts, 4 samples).  Template
+// A fragments come pre-packed in lane order (TmQ); the zero-padded voltage
+// svp[c][y] = v[c][y - M] is in LDS.
+__device__ void q_table_mfma(const Dims &d, const float *__restrict__ TmQ, const float *svp, uint32_t LP,
+                             float *sQ)
+{
+    const uint32_t S = d.S, T = d.T;
+    const uint32_t TT = (T + 15) / 16, ST = (S + 15) / 16, NK4 = (d.N + 3) / 4;
+    const int lane = lane_of();
+    const uint32_t wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    for (uint32_t tile = wave; tile < TT * ST; tile += nwave) {
+        const uint32_t tt = tile / ST, st = tile % ST;
+        float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const uint32_t scol = st * 16 + (lane & 15);
+        for (uint32_t c = 0; c < d.C; c++) {
+            const float *ap = TmQ + ((uint64_t)c * NK4 * TT + tt) * 64 + lane;
+            const float *bp = svp + (uint64_t)c * LP + scol + (lane >> 4);
+            for (uint32_t kk = 0; kk < NK4; kk++) {
+                const float a = ap[(uint64_t)kk * TT * 64];
+                const float b = bp[kk * 4];
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const uint32_t t = tt * 16 + 4 * (lane >> 4) + r;
+            if (t < T && scol < S) sQ[t * S + scol] = acc[r];
+        }
+    }
+}
+
+// TmQ[((c*NK4 + kk)*TT + tt)*64 + lane] = T[tt*16 + lane%16][c][4*kk + lane/16]
+// (0 when padded): the A fragments of q_table_mfma.
+__global__ void k_pack_qtemplates(Dims d, const float *__restrict__ Tm, float *__restrict__ TmQ)
+{
+    const uint32_t TT = (d.T + 15) / 16, NK4 = (d.N + 3) / 4;
+    const uint64_t total = (uint64_t)d.C * NK4 * TT * 64;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t lane = (uint32_t)(e % 64);
+        const uint64_t r = e / 64;
+        const uint32_t tt = (uint32_t)(r % TT);
+        const uint32_t kk = (uint32_t)((r / TT) % NK4);
+        const uint32_t c = (uint32_t)(r / ((uint64_t)TT * NK4));
+        const uint32_t t = tt * 16 + (lane & 15), x = 4 * kk + (lane >> 4);
+        TmQ[e] = (t < d.T && x < d.N) ? Tm[((uint64_t)t * d.C + c) * d.N + x] : 0.0f;
+    }
+}
+
+template <bool MQ>
 __global__ __launch_bounds__(256) void k_overlap_screened(
     Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
     const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw, float *best_L,
     const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov,
-    const float *__restrict__ AB, const float *__restrict__ B2, const float *__restrict__ tnorm)
+    const float *__restrict__ AB, const float *__restrict__ B2, const float *__restrict__ tnorm,
+    const float *__restrict__ TmQ)
 {
     extern __shared__ float dyn[];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
     const uint32_t TS = T * S;
+    const uint32_t LP = ((N + 3) / 4) * 4 + ((S + 15) / 16) * 16;   // padded voltage row (MQ)
     float *sv = dyn;                                    // [C][N] voltage at the best onset
-    float *sQ = sv + C * N;                             // [T][S] truncated correlations
+    float *svp = sv + C * N;                            // [C][LP] zero-padded voltage (MQ only)
+    float *sQ = svp + (MQ ? C * LP : 0);                // [T][S] truncated correlations
     uint32_t *plist = (uint32_t *)(sQ + TS);            // (t,s) with positive likelihood
     uint32_t *flist = plist + TS;                       // main shifts: positive first, then the rest
     uint8_t *spos = (uint8_t *)(flist + S);             // [T][S] bit1 in bounds, bit0 likelihood > 0
@@ -933,7 +1047,10 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
     const double nops = (double)C * N + 2.0 * C + 8.0;
     const double u2 = 0x1p-23;
     const double g = nops * u2 / (1.0 - nops * u2);
-    const double ecoef = (g + 2.0 * u2) * 1.01;
+    // matrix-core Q: float32 products and sums over C*NK4*4 + 1 terms, for both Q terms
+    const double nq = (double)C * ((N + 3) / 4) * 4 + 1.0;
+    const double gq = MQ ? nq * u2 / (1.0 - nq * u2) : 0.0;
+    const double ecoef = (g + 2.0 * gq + 2.0 * u2) * 1.01;
     for (uint32_t k = blockIdx.x; k < n_ov; k += gridDim.x) {
         const uint32_t id = ov_list[k];
         const uint32_t lab = best_lab[id], bidx = best_idx[id];
@@ -952,6 +1069,11 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
                 sv[e] = x;
                 vv = __builtin_fma((double)x, (double)x, vv);
             }
+            if (MQ)
+                for (uint32_t e = threadIdx.x; e < C * LP; e += blockDim.x) {
+                    const uint32_t c = e / LP, y = e % LP;
+                    svp[e] = (y >= M && y - M < N) ? V[(uint64_t)c * d.K + bidx + (y - M)] : 0.0f;
+                }
             for (int off = 32; off > 0; off >>= 1) vv += __shfl_xor(vv, off);
             if (lane_of() == 0) r_vv[threadIdx.x >> 6] = vv;
             __syncthreads();
@@ -964,15 +1086,21 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
                 if (t == lab && inb) {
                     if (pos) flist[atomicAdd(&n_fpos, 1u)] = s;
                 }
-                const int32_t sa = (int32_t)M - (int32_t)s;
-                const int32_t lo = max(0, -sa), hi = min((int32_t)N, (int32_t)N - sa);
-                double q = 0.0;
-                for (uint32_t c = 0; c < C; c++) {
-                    const float *b = Tm + ((uint64_t)t * C + c) * N + sa;
-                    const float *vc = sv + c * N;
-                    for (int32_t j = lo; j < hi; j++) q = __builtin_fma((double)b[j], (double)vc[j], q);
+                if (!MQ) {
+                    const int32_t sa = (int32_t)M - (int32_t)s;
+                    const int32_t lo = max(0, -sa), hi = min((int32_t)N, (int32_t)N - sa);
+                    double q = 0.0;
+                    for (uint32_t c = 0; c < C; c++) {
+                        const float *b = Tm + ((uint64_t)t * C + c) * N + sa;
+                        const float *vc = sv + c * N;
+                        for (int32_t j = lo; j < hi; j++) q = __builtin_fma((double)b[j], (double)vc[j], q);
+                    }
+                    sQ[e] = (float)q;
                 }
-                sQ[e] = (float)q;
+            }
+            if (MQ) {
+                __syncthreads();
+                q_table_mfma(d, TmQ, svp, LP, sQ);
             }
             __syncthreads();
             // in-bounds main shifts without a positive likelihood go after the positive ones
@@ -1263,6 +1391,7 @@ hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win,
 uint32_t mfma_kstep(uint32_t precision) { return precision == 3 ? 32 : 16; }
 uint32_t mfma_ks(const Dims &d, uint32_t precision) { return (d.N + mfma_kstep(precision) - 1) / mfma_kstep(precision); }
 uint32_t mfma_ct(const Dims &d) { return (d.T + 15) / 16; }
+uint32_t mfma_nchunk(const Dims &d) { return (mfma_ct(d) + kMfmaCTB - 1) / kMfmaCTB; }
 
 hipError_t launch_template_norms(const Dims &d, const DeviceState &s, uint32_t precision, hipStream_t st)
 {
@@ -1301,8 +1430,8 @@ static hipError_t launch_corr_mfma_rt(const Dims &d, const DeviceState &s, uint3
     const size_t lds = (size_t)cb_max * per_channel;
     ScreenCoef co{coef[0], coef[1], coef[2], coef[3]};
     hipLaunchKernelGGL((k_corr_mfma<PREC, RT>), dim3(n_win), dim3(64), lds, st, d, s.V, s.Tm,
-                       (const uint64_t *)s.TmB, s.bias, s.tnorm, s.tscale, s.win, s.L, s.ctr, s.rq, rq_cap, cb_max,
-                       co, no_rescore);
+                       (const uint64_t *)s.TmB, s.bias, s.tnorm, s.tscale, s.win, s.L, s.ctr, s.rq, rq_cap,
+                       (uint2 *)s.rdesc, cb_max, co, no_rescore);
     return hipGetLastError();
 }
 
@@ -1331,8 +1460,11 @@ hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_wi
     else if (precision == 2) e = launch_corr_mfma_p<2>(d, s, n_win, coef, rq_cap, no_rescore, st);
     else e = launch_corr_mfma_p<3>(d, s, n_win, coef, rq_cap, no_rescore, st);
     if (e != hipSuccess) return e;
-    if (!no_rescore)
-        hipLaunchKernelGGL(k_rescore, dim3(2048), dim3(256), 0, st, d, s.V, s.Tm, s.bias, s.rq, s.ctr, rq_cap, s.L);
+    if (!no_rescore) {
+        const uint32_t nchunk = (mfma_ct(d) + kMfmaCTB - 1) / kMfmaCTB;
+        hipLaunchKernelGGL(k_rescore_win, dim3(n_win * nchunk), dim3(64), 0, st, d, s.V, s.TmT, s.bias, s.rq,
+                           (const uint2 *)s.rdesc, s.win, nchunk, s.L);
+    }
     return hipGetLastError();
 }
 
@@ -1364,10 +1496,21 @@ hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t m
     return hipGetLastError();
 }
 
-size_t overlap_screened_lds(const Dims &d)
+static size_t overlap_screened_lds_mode(const Dims &d, bool mq)
 {
     const uint64_t ts = (uint64_t)d.T * d.S;
-    return (uint64_t)d.C * d.N * 4 + ts * 4 + ts * 4 + (uint64_t)d.S * 4 + ts + 16;
+    const uint64_t LP = ((d.N + 3) / 4) * 4 + ((d.S + 15) / 16) * 16;
+    return (uint64_t)d.C * d.N * 4 + (mq ? (uint64_t)d.C * LP * 4 : 0) + ts * 4 + ts * 4 + (uint64_t)d.S * 4 + ts + 16;
+}
+
+// The matrix-core Q table is used whenever its padded voltage fits in LDS.
+static bool overlap_uses_mfma_q(const Dims &d) { return overlap_screened_lds_mode(d, true) <= 150 * 1024; }
+
+size_t overlap_screened_lds(const Dims &d) { return overlap_screened_lds_mode(d, overlap_uses_mfma_q(d)); }
+
+uint64_t overlap_qtemplate_floats(const Dims &d)
+{
+    return (uint64_t)d.C * ((d.N + 3) / 4) * ((d.T + 15) / 16) * 64;
 }
 
 hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_t st)
@@ -1375,21 +1518,30 @@ hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_
     const uint64_t total = (uint64_t)d.T * d.S * d.T * d.S + (uint64_t)d.T * d.S + d.T;
     hipLaunchKernelGGL(k_overlap_tables, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st, d, s.Tm, s.tabAB,
                        s.tabB2, s.tnorm);
+    const uint64_t nq = overlap_qtemplate_floats(d);
+    hipLaunchKernelGGL(k_pack_qtemplates, dim3(grid_for(nq, 256, 4096)), dim3(256), 0, st, d, s.Tm, s.TmQ);
     return hipGetLastError();
 }
 
 hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st)
 {
     if (max_ov == 0) return hipSuccess;
+    const bool mq = overlap_uses_mfma_q(d);
     const size_t lds = overlap_screened_lds(d);
+    const void *fn = mq ? (const void *)k_overlap_screened<true> : (const void *)k_overlap_screened<false>;
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_overlap_screened,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const uint32_t grid = max_ov < 4096 ? max_ov : 4096;
-    hipLaunchKernelGGL(k_overlap_screened, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr, s.ov_list,
-                       s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB, s.tabB2, s.tnorm);
+    if (mq)
+        hipLaunchKernelGGL(k_overlap_screened<true>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
+                           s.ov_list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
+                           s.tabB2, s.tnorm, s.TmQ);
+    else
+        hipLaunchKernelGGL(k_overlap_screened<false>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
+                           s.ov_list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
+                           s.tabB2, s.tnorm, s.TmQ);
     return hipGetLastError();
 }
 

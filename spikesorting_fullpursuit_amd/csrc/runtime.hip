@@ -105,7 +105,7 @@ void free_state(fbp_segment *g)
     void *ptrs[] = {s.V, s.V0, s.Tm, s.TmT, s.bias, s.thr, s.L, s.best_L, s.best_lab, s.best_idx, s.ov_idx,
                     s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
                     s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabB2, s.tnorm,
-                    s.TmB, s.rq, s.tscale};
+                    s.TmB, s.rq, s.tscale, s.rdesc, s.TmQ};
     for (void *p : ptrs) dfree(p, g->own_stream);
     if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
     s = DeviceState{};
@@ -121,12 +121,14 @@ uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
     uint64_t b = 2 * r(CK * 4) + r(T * d.C * d.N * 4) + r((uint64_t)d.C * d.N * d.Tp * 4) + r(d.Tp * 4) +
                  r(T * 4) + r(T * d.K * 4) + 5 * r(W * 4) + 2 * r(W + 2) + 3 * r(W * 4) + r(W) + 3 * r(W * 4) +
                  r((nb + 1) * 4) + r(sizeof(Counters)) + r(T * 4) + 2 * r(W * 4);
-    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED) b += r(T * d.S * T * d.S * 4) + r(T * d.S * 4);
+    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED)
+        b += r(T * d.S * T * d.S * 4) + r(T * d.S * 4) + r(overlap_qtemplate_floats(d) * 4);
     if (cfg->precision != FBP_PREC_FP32) {
         uint64_t cap = T * d.K / 8;
         if (cap < (1u << 20)) cap = 1u << 20;
         if (cap > (64u << 20)) cap = 64u << 20;
-        b += r(cap * 8) + r((uint64_t)d.C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64 * 8);
+        b += r(cap * 8) + r((uint64_t)d.C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64 * 8) +
+             r(W * mfma_nchunk(d) * 8);
     }
     return b;
 }
@@ -287,6 +289,7 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED) {
         ALLOC(s.tabAB, (uint64_t)T * d.S * T * d.S)
         ALLOC(s.tabB2, (uint64_t)T * d.S)
+        ALLOC(s.TmQ, overlap_qtemplate_floats(d))
     }
     if (cfg->precision != FBP_PREC_FP32) {
         uint64_t cap = (uint64_t)T * K / 8;
@@ -294,6 +297,9 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         if (cap > (64u << 20)) cap = 64u << 20;
         g->rq_cap = (uint32_t)cap;
         ALLOC(s.rq, cap)
+        uint64_t *rd = nullptr;
+        ALLOC(rd, (uint64_t)W * mfma_nchunk(d))
+        s.rdesc = rd;
         uint64_t *tmb = nullptr;
         ALLOC(tmb, (uint64_t)C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64)
         s.TmB = tmb;
