@@ -301,9 +301,8 @@ __device__ __forceinline__ float exact_likelihood(const Dims &d, const float *__
 }
 
 constexpr int kMfmaMaxRT = 6;   // row tiles per window: N <= 96
-constexpr int kMfmaCTB = 4;     // template column tiles per accumulation chunk
 
-template <int PREC, int RT>
+template <int PREC, int RT, int CTB>
 __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restrict__ V, const float *__restrict__ Tm,
                                                   const uint64_t *__restrict__ TmB, const float *__restrict__ bias,
                                                   const float *__restrict__ tnorm, const float *__restrict__ tscale,
@@ -313,15 +312,15 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
                                                   uint32_t no_rescore)
 {
     using Tr = NarrowTraits<PREC>;
+    constexpr uint32_t KS = (RT * 16 + Tr::kK - 1) / Tr::kK;   // k-steps per channel (N padded to RT*16)
+    constexpr uint32_t LA = RT * 16 + KS * Tr::kK;            // elements staged per channel and copy
     extern __shared__ uint64_t vb_raw[];
     uint8_t *vb = (uint8_t *)vb_raw;            // [copies][cb][LA] narrow voltage, shifted by 0..copies-1
     __shared__ double s2[kMfmaMaxRT * 16 * 2 + 16];
     __shared__ float vnb[kMfmaMaxRT * 16];
     __shared__ int over;
     const int lane = threadIdx.x;
-    const uint32_t KS = (d.N + Tr::kK - 1) / Tr::kK;
     const uint32_t CT = (d.T + 15) / 16;
-    const uint32_t LA = RT * 16 + KS * Tr::kK;  // elements staged per channel and copy
     const uint32_t w = win[blockIdx.x];
     const uint64_t base = (uint64_t)w * d.N;
     const uint64_t wend = min(base + d.N, (uint64_t)d.K);
@@ -351,26 +350,29 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
 
     const bool staged_once = cb_max >= d.C;
     bool have = false;
-    for (uint32_t ct0 = 0; ct0 < CT; ct0 += kMfmaCTB) {
-        const uint32_t nct = min((uint32_t)kMfmaCTB, CT - ct0);
-        float4v acc[RT][kMfmaCTB];
+    for (uint32_t ct0 = 0; ct0 < CT; ct0 += CTB) {
+        const uint32_t nct = min((uint32_t)CTB, CT - ct0);
+        float4v acc[RT][CTB];
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
-            for (int ct = 0; ct < kMfmaCTB; ct++) acc[rt][ct] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int ct = 0; ct < CTB; ct++) acc[rt][ct] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
         for (uint32_t c0 = 0; c0 < d.C; c0 += cb_max) {
             const uint32_t cb = min(cb_max, d.C - c0);
             if (!(staged_once && have)) {
                 __syncthreads();
-                for (uint32_t e = lane; e < cb * LA; e += 64) {
-                    const uint32_t c = e / LA, y = e % LA;
-                    const float *vr = V + (uint64_t)(c0 + c) * d.K;
+                // one load per sample; copy r holds sample base + y + r at y
+                const uint32_t LS = LA + Tr::kCopies;
+                for (uint32_t e = lane; e < cb * LS; e += 64) {
+                    const uint32_t c = e / LS, g = e % LS;
+                    const uint64_t gi = base + g;
+                    const float x = gi < d.K ? V[(uint64_t)(c0 + c) * d.K + gi] : 0.0f;
+                    if (PREC == 3 && fabsf(x) > kFp8Limit) over = 1;
+                    const uint32_t q = to_narrow<PREC>(x);
 #pragma unroll
                     for (int r = 0; r < (int)Tr::kCopies; r++) {
-                        const uint64_t g = base + y + r;
-                        const float x = g < d.K ? vr[g] : 0.0f;
-                        if (PREC == 3 && fabsf(x) > kFp8Limit) over = 1;
-                        const uint32_t q = to_narrow<PREC>(x);
+                        const int y = (int)g - r;
+                        if (y < 0 || y >= (int)LA) continue;
                         uint8_t *dst = vb + (((uint64_t)r * cb + c) * LA + y) * Tr::kBytes;
                         if constexpr (Tr::kBytes == 2) *(uint16_t *)dst = (uint16_t)q;
                         else *dst = (uint8_t)q;
@@ -380,11 +382,16 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
                 have = true;
             }
             for (uint32_t c = 0; c < cb; c++) {
-                for (uint32_t ks = 0; ks < KS; ks++) {
-                    uint64_t b[kMfmaCTB];
+                uint64_t b[KS][CTB];
 #pragma unroll
-                    for (int ct = 0; ct < kMfmaCTB; ct++)
-                        if (ct < (int)nct) b[ct] = TmB[(((uint64_t)(c0 + c) * KS + ks) * CT + ct0 + ct) * 64 + lane];
+                for (uint32_t ks = 0; ks < KS; ks++)
+#pragma unroll
+                    for (int ct = 0; ct < CTB; ct++)
+                        b[ks][ct] = ct < (int)nct
+                                        ? TmB[(((uint64_t)(c0 + c) * KS + ks) * CT + ct0 + ct) * 64 + lane]
+                                        : 0ull;
+#pragma unroll
+                for (uint32_t ks = 0; ks < KS; ks++) {
 #pragma unroll
                     for (int rt = 0; rt < RT; rt++) {
                         const uint32_t o = rt * 16 + ks * Tr::kK + Tr::kPer * (lane >> 4) + (lane & 15);
@@ -392,32 +399,44 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
                         const uint64_t a =
                             *(const uint64_t *)(vb + (((uint64_t)r * cb + c) * LA + (o - r)) * Tr::kBytes);
 #pragma unroll
-                        for (int ct = 0; ct < kMfmaCTB; ct++)
-                            if (ct < (int)nct) acc[rt][ct] = mfma_narrow<PREC>(a, b[ct], acc[rt][ct]);
+                        for (int ct = 0; ct < CTB; ct++) acc[rt][ct] = mfma_narrow<PREC>(a, b[ks][ct], acc[rt][ct]);
                     }
                 }
             }
         }
         const bool force = PREC == 3 && over;
-        // Epilogue: bias, bound, store, queue the entries that need float32.
+        // per-template constants of this lane's columns
+        float cb_bias[CTB], cb_inv[CTB], cb_tn[CTB];
+        bool cb_ok[CTB];
+#pragma unroll
+        for (int ct = 0; ct < CTB; ct++) {
+            const uint32_t t = (ct0 + ct) * 16 + (lane & 15);
+            cb_ok[ct] = ct < (int)nct && t < d.T;
+            cb_bias[ct] = cb_ok[ct] ? bias[t] : 0.0f;
+            cb_inv[ct] = cb_ok[ct] ? 1.0f / tscale[t] : 1.0f;
+            cb_tn[ct] = cb_ok[ct] ? tnorm[t] : 0.0f;
+        }
+        // Epilogue: bias, bound, store; count the entries that need float32.
         uint32_t nflag = 0;
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
-            for (int ct = 0; ct < kMfmaCTB; ct++)
+            for (int ct = 0; ct < CTB; ct++)
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const uint32_t p = rt * 16 + 4 * (lane >> 4) + q;
                     const uint32_t t = (ct0 + ct) * 16 + (lane & 15);
-                    if (ct >= (int)nct || p >= np || t >= d.T) continue;
+                    if (!cb_ok[ct] || p >= np) continue;
                     const uint64_t i = base + p;
                     if (i + d.N >= d.K) { L[(uint64_t)t * d.K + i] = 0.0f; continue; }   // bp.cl:231-234
-                    const float inv = 1.0f / tscale[t];
-                    const float lh = bias[t] + acc[rt][ct][q] * inv;
-                    const float tn = tnorm[t], vn = vnb[p];
-                    const float e = co.k1 * tn * vn + co.k2 * (tn + vn * inv) + co.k3 * inv + co.k4 * fabsf(bias[t]);
+                    const float lh = cb_bias[ct] + acc[rt][ct][q] * cb_inv[ct];
+                    const float vn = vnb[p];
+                    const float e = co.k1 * cb_tn[ct] * vn + co.k2 * (cb_tn[ct] + vn * cb_inv[ct]) + co.k3 * cb_inv[ct]
+                                    + co.k4 * fabsf(cb_bias[ct]);
                     L[(uint64_t)t * d.K + i] = lh;
-                    if (force || !(lh <= -e)) nflag++;
+                    const bool fl = force || !(lh <= -e);
+                    nflag += fl ? 1u : 0u;
+                    acc[rt][ct][q] = fl ? 1.0f : 0.0f;   // keep the decision for the queue pass
                 }
         if (no_rescore) continue;
         // wave-wide reservation in the re-score queue
@@ -435,26 +454,23 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
         uint32_t pos = qbase + incl - nflag;
         // this (window, template chunk)'s slice of the queue, for k_rescore_win
         if (lane == 0) {
-            const uint32_t nchunk = (CT + kMfmaCTB - 1) / kMfmaCTB;
+            const uint32_t nchunk = (CT + CTB - 1) / CTB;
             const uint32_t kept = qbase >= rq_cap ? 0u : min(tot, rq_cap - qbase);
-            rdesc[(uint64_t)blockIdx.x * nchunk + ct0 / kMfmaCTB] = make_uint2(qbase, kept);
+            rdesc[(uint64_t)blockIdx.x * nchunk + ct0 / CTB] = make_uint2(qbase, kept);
         }
+        if (!nflag) continue;
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
-            for (int ct = 0; ct < kMfmaCTB; ct++)
+            for (int ct = 0; ct < CTB; ct++)
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
+                    if (acc[rt][ct][q] == 0.0f || !cb_ok[ct]) continue;
                     const uint32_t p = rt * 16 + 4 * (lane >> 4) + q;
+                    if (p >= np) continue;
                     const uint32_t t = (ct0 + ct) * 16 + (lane & 15);
-                    if (ct >= (int)nct || p >= np || t >= d.T) continue;
                     const uint64_t i = base + p;
                     if (i + d.N >= d.K) continue;
-                    const float inv = 1.0f / tscale[t];
-                    const float lh = bias[t] + acc[rt][ct][q] * inv;
-                    const float tn = tnorm[t], vn = vnb[p];
-                    const float e = co.k1 * tn * vn + co.k2 * (tn + vn * inv) + co.k3 * inv + co.k4 * fabsf(bias[t]);
-                    if (!force && lh <= -e) continue;
                     if (pos < rq_cap) rq[pos] = ((uint64_t)t << 32) | (uint64_t)i;
                     else L[(uint64_t)t * d.K + i] = exact_likelihood(d, V, Tm, bias, t, i);   // queue full
                     pos++;
@@ -1389,9 +1405,15 @@ hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win,
 }
 
 uint32_t mfma_kstep(uint32_t precision) { return precision == 3 ? 32 : 16; }
-uint32_t mfma_ks(const Dims &d, uint32_t precision) { return (d.N + mfma_kstep(precision) - 1) / mfma_kstep(precision); }
+// k-steps per channel: the template length padded to whole 16-sample row tiles
+uint32_t mfma_ks(const Dims &d, uint32_t precision)
+{
+    const uint32_t padded = (d.N + 15) / 16 * 16;
+    return (padded + mfma_kstep(precision) - 1) / mfma_kstep(precision);
+}
 uint32_t mfma_ct(const Dims &d) { return (d.T + 15) / 16; }
-uint32_t mfma_nchunk(const Dims &d) { return (mfma_ct(d) + kMfmaCTB - 1) / kMfmaCTB; }
+uint32_t mfma_ctb(const Dims &d) { const uint32_t ct = mfma_ct(d); return ct >= 4 ? 4 : (ct == 3 ? 4 : ct); }
+uint32_t mfma_nchunk(const Dims &d) { return (mfma_ct(d) + mfma_ctb(d) - 1) / mfma_ctb(d); }
 
 hipError_t launch_template_norms(const Dims &d, const DeviceState &s, uint32_t precision, hipStream_t st)
 {
@@ -1416,23 +1438,33 @@ hipError_t launch_pack_templates(const Dims &d, const DeviceState &s, uint32_t p
     return launch_pack_p<3>(d, s, st);
 }
 
-template <int PREC, int RT>
+template <int PREC, int RT, int CTB>
 static hipError_t launch_corr_mfma_rt(const Dims &d, const DeviceState &s, uint32_t n_win, const float *coef,
                                       uint32_t rq_cap, uint32_t no_rescore, hipStream_t st)
 {
     using Tr = NarrowTraits<PREC>;
     const uint32_t LA = RT * 16 + mfma_ks(d, PREC) * Tr::kK;
     const uint32_t per_channel = Tr::kCopies * LA * Tr::kBytes;
-    const uint32_t budget = 48 * 1024;
+    const uint32_t budget = 32 * 1024;
     uint32_t cb_max = budget / per_channel;
     if (cb_max > d.C) cb_max = d.C;
     if (cb_max == 0) return hipErrorInvalidValue;
     const size_t lds = (size_t)cb_max * per_channel;
     ScreenCoef co{coef[0], coef[1], coef[2], coef[3]};
-    hipLaunchKernelGGL((k_corr_mfma<PREC, RT>), dim3(n_win), dim3(64), lds, st, d, s.V, s.Tm,
+    hipLaunchKernelGGL((k_corr_mfma<PREC, RT, CTB>), dim3(n_win), dim3(64), lds, st, d, s.V, s.Tm,
                        (const uint64_t *)s.TmB, s.bias, s.tnorm, s.tscale, s.win, s.L, s.ctr, s.rq, rq_cap,
                        (uint2 *)s.rdesc, cb_max, co, no_rescore);
     return hipGetLastError();
+}
+
+template <int PREC, int RT>
+static hipError_t launch_corr_mfma_ct(const Dims &d, const DeviceState &s, uint32_t n_win, const float *coef,
+                                      uint32_t rq_cap, uint32_t no_rescore, hipStream_t st)
+{
+    const uint32_t ctb = mfma_ctb(d);
+    if (ctb == 1) return launch_corr_mfma_rt<PREC, RT, 1>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    if (ctb == 2) return launch_corr_mfma_rt<PREC, RT, 2>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    return launch_corr_mfma_rt<PREC, RT, 4>(d, s, n_win, coef, rq_cap, no_rescore, st);
 }
 
 template <int PREC>
@@ -1440,12 +1472,12 @@ static hipError_t launch_corr_mfma_p(const Dims &d, const DeviceState &s, uint32
                                      uint32_t rq_cap, uint32_t no_rescore, hipStream_t st)
 {
     switch ((d.N + 15) / 16) {
-    case 1: return launch_corr_mfma_rt<PREC, 1>(d, s, n_win, coef, rq_cap, no_rescore, st);
-    case 2: return launch_corr_mfma_rt<PREC, 2>(d, s, n_win, coef, rq_cap, no_rescore, st);
-    case 3: return launch_corr_mfma_rt<PREC, 3>(d, s, n_win, coef, rq_cap, no_rescore, st);
-    case 4: return launch_corr_mfma_rt<PREC, 4>(d, s, n_win, coef, rq_cap, no_rescore, st);
-    case 5: return launch_corr_mfma_rt<PREC, 5>(d, s, n_win, coef, rq_cap, no_rescore, st);
-    case 6: return launch_corr_mfma_rt<PREC, 6>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    case 1: return launch_corr_mfma_ct<PREC, 1>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    case 2: return launch_corr_mfma_ct<PREC, 2>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    case 3: return launch_corr_mfma_ct<PREC, 3>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    case 4: return launch_corr_mfma_ct<PREC, 4>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    case 5: return launch_corr_mfma_ct<PREC, 5>(d, s, n_win, coef, rq_cap, no_rescore, st);
+    case 6: return launch_corr_mfma_ct<PREC, 6>(d, s, n_win, coef, rq_cap, no_rescore, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1461,7 +1493,7 @@ hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_wi
     else e = launch_corr_mfma_p<3>(d, s, n_win, coef, rq_cap, no_rescore, st);
     if (e != hipSuccess) return e;
     if (!no_rescore) {
-        const uint32_t nchunk = (mfma_ct(d) + kMfmaCTB - 1) / kMfmaCTB;
+        const uint32_t nchunk = mfma_nchunk(d);
         hipLaunchKernelGGL(k_rescore_win, dim3(n_win * nchunk), dim3(64), 0, st, d, s.V, s.TmT, s.bias, s.rq,
                            (const uint2 *)s.rdesc, s.win, nchunk, s.L);
     }
