@@ -484,14 +484,13 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
 // so for every (c, j) a wave reads one LDS sample per chain and one short run
 // of template values.  Each chain is the reference's: bias, then c = 0..C-1,
 // j = 0..N-1, fused multiply-adds.
-constexpr int kRescoreR = 4;
-
 __global__ __launch_bounds__(64) void k_rescore_win(Dims d, const float *__restrict__ V,
                                                     const float *__restrict__ TmT, const float *__restrict__ bias,
                                                     const uint64_t *__restrict__ rq, const uint2 *__restrict__ rdesc,
                                                     const uint32_t *__restrict__ win, uint32_t nchunk, float *L)
 {
-    __shared__ float sv[2 * kMfmaMaxRT * 16 + 8];
+    extern __shared__ float st_tm[];                  // [N][Tp] templates of channel c
+    __shared__ float sv[2 * kMfmaMaxRT * 16 + 8];     // window voltage of channel c
     const uint2 dsc = rdesc[blockIdx.x];
     const uint32_t cnt = dsc.y;
     if (cnt == 0) return;
@@ -499,34 +498,26 @@ __global__ __launch_bounds__(64) void k_rescore_win(Dims d, const float *__restr
     const uint32_t w = win[blockIdx.x / nchunk];
     const uint64_t base = (uint64_t)w * d.N;
     const uint32_t span = ((d.N + 15) / 16) * 16 + d.N;
-    for (uint32_t e0 = 0; e0 < cnt; e0 += 64 * kRescoreR) {
-        uint32_t tt[kRescoreR], pp[kRescoreR];
-        float acc[kRescoreR];
-        bool ok[kRescoreR];
-#pragma unroll
-        for (int r = 0; r < kRescoreR; r++) {
-            const uint32_t e = e0 + lane + 64 * r;
-            ok[r] = e < cnt;
-            const uint64_t ent = ok[r] ? rq[dsc.x + e] : 0;
-            tt[r] = ok[r] ? (uint32_t)(ent >> 32) : 0u;
-            pp[r] = ok[r] ? (uint32_t)((ent & 0xffffffffull) - base) : 0u;
-            acc[r] = bias[tt[r]];
-        }
+    const uint32_t ntm = d.N * d.Tp;
+    for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
+        const uint32_t e = e0 + lane;
+        const bool ok = e < cnt;
+        const uint64_t ent = ok ? rq[dsc.x + e] : 0;
+        const uint32_t t = ok ? (uint32_t)(ent >> 32) : 0u;
+        const uint32_t p = ok ? (uint32_t)((ent & 0xffffffffull) - base) : 0u;
+        float acc = bias[t];
         for (uint32_t c = 0; c < d.C; c++) {
             __syncthreads();
             const float *vr = V + (uint64_t)c * d.K;
             for (uint32_t x = lane; x < span; x += 64) sv[x] = base + x < d.K ? vr[base + x] : 0.0f;
+            const float *tc = TmT + (uint64_t)c * ntm;
+            for (uint32_t x = lane; x < ntm; x += 64) st_tm[x] = tc[x];
             __syncthreads();
-            const float *tc = TmT + (uint64_t)c * d.N * d.Tp;
-            for (uint32_t j = 0; j < d.N; j++) {
-                const float *tj = tc + (uint64_t)j * d.Tp;
-#pragma unroll
-                for (int r = 0; r < kRescoreR; r++) acc[r] = __builtin_fmaf(tj[tt[r]], sv[pp[r] + j], acc[r]);
-            }
+            const float *vp = sv + p;
+            const float *tp = st_tm + t;
+            for (uint32_t j = 0; j < d.N; j++) acc = __builtin_fmaf(tp[j * d.Tp], vp[j], acc);
         }
-#pragma unroll
-        for (int r = 0; r < kRescoreR; r++)
-            if (ok[r]) L[(uint64_t)tt[r] * d.K + base + pp[r]] = acc[r];
+        if (ok) L[(uint64_t)t * d.K + base + p] = acc;
     }
 }
 
@@ -1044,9 +1035,11 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
     const uint32_t TS = T * S;
     const uint32_t LP = ((N + 3) / 4) * 4 + ((S + 15) / 16) * 16;   // padded voltage row (MQ)
-    float *sv = dyn;                                    // [C][N] voltage at the best onset
-    float *svp = sv + C * N;                            // [C][LP] zero-padded voltage (MQ only)
-    float *sQ = svp + (MQ ? C * LP : 0);                // [T][S] truncated correlations
+    double *sE = (double *)dyn;                         // [T] screen bound per template (this window)
+    float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset
+    float *smain = sv + C * N;                          // [C][N] template of the best label
+    float *svp = smain + C * N;                         // [C][LP] zero-padded voltage (MQ only)
+    float *sQ = svp + (MQ ? C * LP : 0);                // [T][S] Q - B2/2
     uint32_t *plist = (uint32_t *)(sQ + TS);            // (t,s) with positive likelihood
     uint32_t *flist = plist + TS;                       // main shifts: positive first, then the rest
     uint8_t *spos = (uint8_t *)(flist + S);             // [T][S] bit1 in bounds, bit0 likelihood > 0
@@ -1066,7 +1059,8 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
     // matrix-core Q: float32 products and sums over C*NK4*4 + 1 terms, for both Q terms
     const double nq = (double)C * ((N + 3) / 4) * 4 + 1.0;
     const double gq = MQ ? nq * u2 / (1.0 - nq * u2) : 0.0;
-    const double ecoef = (g + 2.0 * gq + 2.0 * u2) * 1.01;
+    // + float32 storage of Q - B2/2 and of the AB table (unit roundoff each)
+    const double ecoef = (g + 2.0 * gq + 4.0 * u2) * 1.01;
     for (uint32_t k = blockIdx.x; k < n_ov; k += gridDim.x) {
         const uint32_t id = ov_list[k];
         const uint32_t lab = best_lab[id], bidx = best_idx[id];
@@ -1083,6 +1077,7 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
                 const uint32_t c = e / N, j = e % N;
                 const float x = V[(uint64_t)c * d.K + bidx + j];
                 sv[e] = x;
+                smain[e] = Tm[((uint64_t)lab * C + c) * N + j];
                 vv = __builtin_fma((double)x, (double)x, vv);
             }
             if (MQ)
@@ -1119,6 +1114,15 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
                 q_table_mfma(d, TmQ, svp, LP, sQ);
             }
             __syncthreads();
+            const double vn = sqrt(r_vv[0] + r_vv[1] + r_vv[2] + r_vv[3]) * (1.0 + 1e-12);
+            const double nlab = tnorm[lab];
+            // fold the voltage-free half norms into Q, and the bound per template
+            for (uint32_t e = threadIdx.x; e < TS; e += blockDim.x)
+                sQ[e] = (float)((double)sQ[e] - 0.5 * (double)B2[e]);
+            for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+                const double nsum = nlab + (double)tnorm[t];
+                sE[t] = ecoef * (nsum * vn + nsum * nsum);
+            }
             // in-bounds main shifts without a positive likelihood go after the positive ones
             if (threadIdx.x == 0) {
                 uint32_t nn = 0;
@@ -1127,17 +1131,12 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
                 n_fneg = nn;
             }
             __syncthreads();
-            const double vn = sqrt(r_vv[0] + r_vv[1] + r_vv[2] + r_vv[3]) * (1.0 + 1e-12);
-            const double nlab = tnorm[lab];
             const float *ABl = AB + (uint64_t)lab * S * TS;
-            const float *B2l = B2 + (uint64_t)lab * S;
             const float *sQl = sQ + lab * S;
             const uint32_t nfp = n_fpos, nfn = n_fneg, npp = n_ppos;
             auto screen = [&](uint32_t f, uint32_t t, uint32_t e, double &E) {
-                const double nsum = nlab + (double)tnorm[t];
-                E = ecoef * (nsum * vn + nsum * nsum);
-                return (double)sQl[f] + (double)sQ[e] - 0.5 * ((double)B2l[f] + (double)B2[e])
-                       - (double)ABl[(uint64_t)f * TS + e];
+                E = sE[t];
+                return (double)sQl[f] + (double)sQ[e] - (double)ABl[(uint64_t)f * TS + e];
             };
             // pass A: largest lower bound and largest upper bound of the score
             double lo_max = -INFINITY, hi_max = -INFINITY;
@@ -1162,7 +1161,7 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
             // Nothing can beat best_L: parse rejects without any exact score.
             if (hi_max > (double)bl) {
                 uint32_t scored = 0;
-                const float *tmain = Tm + (uint64_t)lab * C * N;
+                const float *tmain = smain;
                 for_each_unskipped(d, spos, flist, nfp, nfn, plist, npp, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e) {
                     double E;
                     const double sh = screen(f, t, e, E);
@@ -1275,42 +1274,53 @@ __global__ __launch_bounds__(256) void k_residual(Dims d, float *V, const float 
                                                   const uint32_t *__restrict__ sp_lab, uint32_t n_new,
                                                   const uint32_t *__restrict__ win_spike)
 {
+    // spikes of the pass overlapping this spike's span, in (onset, label, position) order
+    __shared__ uint32_t ci[kMaxCover], cl[kMaxCover], cp[kMaxCover];
+    __shared__ int ncand;
     for (uint32_t k = blockIdx.x; k < n_new; k += gridDim.x) {
         const uint32_t idx = sp_idx[k], lab = sp_lab[k];
-        if (lab >= d.T) continue;   // bp.cl:78-81
-        for (uint32_t e = threadIdx.x; e < d.C * d.N; e += blockDim.x) {
-            const uint32_t c = e / d.N, j = e % d.N;
-            const uint64_t x = (uint64_t)idx + j;
-            if (x >= d.K) continue;   // bp.cl:93-96
-            const int64_t lo = ((int64_t)x - 2 * (int64_t)d.N - (int64_t)d.M + 2);
-            int64_t wlo = lo < 0 ? 0 : lo / d.N;
-            int64_t whi = ((int64_t)x + d.M) / d.N;
+        if (lab >= d.T) continue;   // bp.cl:78-81 (block-uniform)
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // windows whose spike can overlap [idx, idx + N): a spike added by
+            // window w lies in [wN - M, (w+1)N + M)
+            const int64_t lo = (int64_t)idx - 2 * (int64_t)d.N - (int64_t)d.M + 2;
+            const int64_t wlo = lo < 0 ? 0 : lo / d.N;
+            int64_t whi = ((int64_t)idx + d.N - 1 + d.M) / d.N;
             if (whi > (int64_t)d.W - 1) whi = d.W - 1;
-            uint32_t ci[kMaxCover], cl[kMaxCover], cp[kMaxCover];
-            int nc = 0;
-            bool owner = true;
+            int n = 0;
             for (int64_t wd = wlo; wd <= whi; wd++) {
                 const uint32_t m = win_spike[wd];
                 if (!m) continue;
                 const uint32_t pos = m - 1;
-                const uint32_t i2 = sp_idx[pos];
-                if (!((uint64_t)i2 <= x && x < (uint64_t)i2 + d.N)) continue;
-                const uint32_t l2 = sp_lab[pos];
-                if (l2 >= d.T) continue;
-                if (i2 < idx || (i2 == idx && (l2 < lab || (l2 == lab && pos < k)))) { owner = false; break; }
-                // insertion in (onset, label, position) order
-                int q = nc;
+                const uint32_t i2 = sp_idx[pos], l2 = sp_lab[pos];
+                if (l2 >= d.T || (uint64_t)i2 + d.N <= idx || i2 >= (uint64_t)idx + d.N) continue;
+                int q = n;
                 while (q > 0 && (ci[q - 1] > i2 || (ci[q - 1] == i2 && (cl[q - 1] > l2 || (cl[q - 1] == l2 && cp[q - 1] > pos))))) {
                     ci[q] = ci[q - 1]; cl[q] = cl[q - 1]; cp[q] = cp[q - 1];
                     q--;
                 }
                 ci[q] = i2; cl[q] = l2; cp[q] = pos;
-                if (nc < kMaxCover - 1) nc++;
+                if (n < kMaxCover - 1) n++;
             }
-            if (!owner) continue;
+            ncand = n;
+        }
+        __syncthreads();
+        const int n = ncand;
+        for (uint32_t e = threadIdx.x; e < d.C * d.N; e += blockDim.x) {
+            const uint32_t c = e / d.N, j = e % d.N;
+            const uint64_t x = (uint64_t)idx + j;
+            if (x >= d.K) continue;   // bp.cl:93-96
+            // owner = first candidate (in order) covering x
+            int first = -1;
+            for (int q = 0; q < n; q++)
+                if ((uint64_t)ci[q] <= x && x < (uint64_t)ci[q] + d.N) { first = q; break; }
+            if (first < 0 || cp[first] != k) continue;
             float *vp = V + (uint64_t)c * d.K + x;
             float v = *vp;
-            for (int q = 0; q < nc; q++) v -= Tm[((uint64_t)cl[q] * d.C + c) * d.N + (uint32_t)(x - ci[q])];
+            for (int q = first; q < n; q++)
+                if ((uint64_t)ci[q] <= x && x < (uint64_t)ci[q] + d.N)
+                    v -= Tm[((uint64_t)cl[q] * d.C + c) * d.N + (uint32_t)(x - ci[q])];
             *vp = v;
         }
     }
@@ -1494,7 +1504,13 @@ hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_wi
     if (e != hipSuccess) return e;
     if (!no_rescore) {
         const uint32_t nchunk = mfma_nchunk(d);
-        hipLaunchKernelGGL(k_rescore_win, dim3(n_win * nchunk), dim3(64), 0, st, d, s.V, s.TmT, s.bias, s.rq,
+        const size_t lds = (size_t)d.N * d.Tp * sizeof(float);
+        if (lds > 64 * 1024) {
+            hipError_t e2 = hipFuncSetAttribute((const void *)k_rescore_win,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e2 != hipSuccess) return e2;
+        }
+        hipLaunchKernelGGL(k_rescore_win, dim3(n_win * nchunk), dim3(64), lds, st, d, s.V, s.TmT, s.bias, s.rq,
                            (const uint2 *)s.rdesc, s.win, nchunk, s.L);
     }
     return hipGetLastError();
@@ -1532,7 +1548,8 @@ static size_t overlap_screened_lds_mode(const Dims &d, bool mq)
 {
     const uint64_t ts = (uint64_t)d.T * d.S;
     const uint64_t LP = ((d.N + 3) / 4) * 4 + ((d.S + 15) / 16) * 16;
-    return (uint64_t)d.C * d.N * 4 + (mq ? (uint64_t)d.C * LP * 4 : 0) + ts * 4 + ts * 4 + (uint64_t)d.S * 4 + ts + 16;
+    return (uint64_t)d.T * 8 + 2ull * d.C * d.N * 4 + (mq ? (uint64_t)d.C * LP * 4 : 0) + ts * 4 + ts * 4 +
+           (uint64_t)d.S * 4 + ts + 16;
 }
 
 // The matrix-core Q table is used whenever its padded voltage fits in LDS.
