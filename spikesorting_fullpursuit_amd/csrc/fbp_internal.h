@@ -68,6 +68,7 @@ struct DeviceState {
     float *tabB2;        // [T][S]       squared norms of shifted, truncated templates
     float *tnorm;        // [T]          template norms (upper bounds)
     float *TmQ;          // screened overlap: float32 A fragments of the matrix-core Q table
+    float *Tpad;         // [T][C][N + 2M] templates at [M, M + N), +0.0 around (overlap scores)
     float *tscale;       // [T]          fp8 screen: per-template power-of-two scale (1 otherwise)
     void *TmB;           // packed bf16/fp16/fp8 template fragments for the matrix cores
     uint64_t *rq;        // re-score queue: (template << 32) | position
@@ -100,6 +101,7 @@ hipError_t launch_compact_ov(const Dims &d, const DeviceState &s, uint32_t n_win
 hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);
 size_t overlap_screened_lds(const Dims &d);
 uint64_t overlap_qtemplate_floats(const Dims &d);
+hipError_t launch_pad_templates(const Dims &d, const DeviceState &s, hipStream_t st);
 hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_t st);
 hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);
 hipError_t launch_check_overlap(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);

@@ -792,24 +792,24 @@ __device__ void block_key_reduce(float &sc, uint64_t &key, uint64_t &bp, float *
             if (key_better(r_sc[q], r_key[q], sc, key)) { sc = r_sc[q]; key = r_key[q]; bp = r_p[q]; }
 }
 
-__device__ float overlap_direct_score(const Dims &d, const float *__restrict__ sv, const float *__restrict__ smain,
-                                      const float *__restrict__ Tm, uint32_t t, uint32_t f, uint32_t s)
+// Reference pair score (bp.cl:599-631) from zero-padded template rows:
+// a_j = pad_lab[c][j + 2M - f], b_j = pad_t[c][j + 2M - s], where a row holds
+// the template at [M, M + N) and +0.0 elsewhere -- the value the reference's
+// out-of-range branches use -- so every sum a_j + b_j is the reference's.
+__device__ __forceinline__ float overlap_direct_score(const Dims &d, const float *__restrict__ sv,
+                                                      const float *__restrict__ amain, const float *__restrict__ bt,
+                                                      uint32_t f, uint32_t s)
 {
-    // a_j = Tm[lab, c, j + M - f] and b_j = Tm[t, c, j + M - s] where in range.
-    const int32_t fa = (int32_t)d.M - (int32_t)f;
-    const int32_t sa = (int32_t)d.M - (int32_t)s;
-    const int32_t flo = fa < 0 ? -fa : 0, fhi = min((int32_t)d.N, (int32_t)d.N - fa);
-    const int32_t slo = sa < 0 ? -sa : 0, shi = min((int32_t)d.N, (int32_t)d.N - sa);
+    const uint32_t R = d.N + 2 * d.M;
     float acc = 0.0f;
     for (uint32_t c = 0; c < d.C; c++) {
-        const float *tm = smain + c * d.N;
-        const float *ts = Tm + ((uint64_t)t * d.C + c) * d.N;
+        const float *a = amain + c * R + 2 * d.M - f;
+        const float *b = bt + (uint64_t)c * R + 2 * d.M - s;
         const float *vc = sv + c * d.N;
         float ss = 0.0f;
-        for (int32_t j = 0; j < (int32_t)d.N; j++) {
-            const float fv = (j >= flo && j < fhi) ? tm[j + fa] : 0.0f;
-            const float bv = (j >= slo && j < shi) ? ts[j + sa] : 0.0f;
-            const float sum = fv + bv;
+#pragma unroll 16
+        for (uint32_t j = 0; j < d.N; j++) {
+            const float sum = a[j] + b[j];
             ss = __builtin_fmaf(sum, sum, ss);       // bp.cl:621
             acc = __builtin_fmaf(sum, vc[j], acc);   // bp.cl:622
         }
@@ -818,8 +818,21 @@ __device__ float overlap_direct_score(const Dims &d, const float *__restrict__ s
     return acc;
 }
 
+// Tpad[t][c][y] = T[t][c][y - M] for y - M in [0, N), +0.0 otherwise (y < N + 2M).
+__global__ void k_pad_templates(Dims d, const float *__restrict__ Tm, float *__restrict__ Tpad)
+{
+    const uint32_t R = d.N + 2 * d.M;
+    const uint64_t total = (uint64_t)d.T * d.C * R;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t y = (uint32_t)(e % R);
+        const uint64_t tc = e / R;
+        Tpad[e] = (y >= d.M && y - d.M < d.N) ? Tm[tc * d.N + (y - d.M)] : 0.0f;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_overlap_direct(
-    Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
+    Dims d, const float *__restrict__ V, const float *__restrict__ Tpad, const float *__restrict__ L,
     const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw,
     float *best_L, const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab,
     uint8_t *ov)
@@ -827,7 +840,8 @@ __global__ __launch_bounds__(256) void k_overlap_direct(
     const Counters *ctr = ctr_rw;
     extern __shared__ float dyn[];
     float *sv = dyn;                       // [C][N] voltage at the best onset
-    float *smain = dyn + d.C * d.N;        // [C][N] template of the best label
+    float *smain = dyn + d.C * d.N;        // [C][N + 2M] padded template of the best label
+    const uint32_t R = d.N + 2 * d.M;
     __shared__ float r_sc[4];
     __shared__ uint64_t r_key[4];
     __shared__ uint64_t r_p[4];
@@ -849,8 +863,8 @@ __global__ __launch_bounds__(256) void k_overlap_direct(
             for (uint32_t e = threadIdx.x; e < d.C * N; e += blockDim.x) {
                 const uint32_t c = e / N, j = e % N;
                 sv[e] = V[(uint64_t)c * d.K + bidx + j];
-                smain[e] = Tm[((uint64_t)lab * d.C + c) * N + j];
             }
+            for (uint32_t e = threadIdx.x; e < d.C * R; e += blockDim.x) smain[e] = Tpad[(uint64_t)lab * d.C * R + e];
             __syncthreads();
             const float *Lmain = L + (uint64_t)lab * d.K;
             for (uint64_t q = threadIdx.x; q < d.items; q += blockDim.x) {
@@ -863,7 +877,7 @@ __global__ __launch_bounds__(256) void k_overlap_direct(
                 const float Lf = Lmain[bidx + f - M];
                 const float Ls = L[(uint64_t)t * d.K + bidx + s - M];
                 if (Lf <= 0.0f && Ls <= 0.0f) continue;                                                 // bp.cl:572-576
-                const float score = overlap_direct_score(d, sv, smain, Tm, t, f, s);
+                const float score = overlap_direct_score(d, sv, smain, Tpad + (uint64_t)t * d.C * R, f, s);
                 scored++;
                 uint64_t p;
                 const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
@@ -1029,7 +1043,7 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
     const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw, float *best_L,
     const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov,
     const float *__restrict__ AB, const float *__restrict__ B2, const float *__restrict__ tnorm,
-    const float *__restrict__ TmQ)
+    const float *__restrict__ TmQ, const float *__restrict__ Tpad)
 {
     extern __shared__ float dyn[];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
@@ -1037,8 +1051,8 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
     const uint32_t LP = ((N + 3) / 4) * 4 + ((S + 15) / 16) * 16;   // padded voltage row (MQ)
     double *sE = (double *)dyn;                         // [T] screen bound per template (this window)
     float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset
-    float *smain = sv + C * N;                          // [C][N] template of the best label
-    float *svp = smain + C * N;                         // [C][LP] zero-padded voltage (MQ only)
+    float *smain = sv + C * N;                          // [C][N + 2M] padded template of the best label
+    float *svp = smain + C * (N + 2 * M);               // [C][LP] zero-padded voltage (MQ only)
     float *sQ = svp + (MQ ? C * LP : 0);                // [T][S] Q - B2/2
     uint32_t *plist = (uint32_t *)(sQ + TS);            // (t,s) with positive likelihood
     uint32_t *flist = plist + TS;                       // main shifts: positive first, then the rest
@@ -1077,9 +1091,10 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
                 const uint32_t c = e / N, j = e % N;
                 const float x = V[(uint64_t)c * d.K + bidx + j];
                 sv[e] = x;
-                smain[e] = Tm[((uint64_t)lab * C + c) * N + j];
                 vv = __builtin_fma((double)x, (double)x, vv);
             }
+            for (uint32_t e = threadIdx.x; e < C * (N + 2 * M); e += blockDim.x)
+                smain[e] = Tpad[(uint64_t)lab * C * (N + 2 * M) + e];
             if (MQ)
                 for (uint32_t e = threadIdx.x; e < C * LP; e += blockDim.x) {
                     const uint32_t c = e / LP, y = e % LP;
@@ -1166,7 +1181,7 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
                     double E;
                     const double sh = screen(f, t, e, E);
                     if (sh + E < lo_max) return;
-                    const float score = overlap_direct_score(d, sv, tmain, Tm, t, f, s);
+                    const float score = overlap_direct_score(d, sv, tmain, Tpad + (uint64_t)t * C * (N + 2 * M), f, s);
                     scored++;
                     uint64_t p;
                     const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
@@ -1532,14 +1547,14 @@ hipError_t launch_compact_ov(const Dims &d, const DeviceState &s, uint32_t n_win
 hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st)
 {
     if (max_ov == 0) return hipSuccess;
-    const size_t lds = 2ull * d.C * d.N * sizeof(float);
+    const size_t lds = ((uint64_t)d.C * d.N + (uint64_t)d.C * (d.N + 2 * d.M)) * sizeof(float);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_overlap_direct,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const uint32_t grid = max_ov < 4096 ? max_ov : 4096;
-    hipLaunchKernelGGL(k_overlap_direct, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr, s.ov_list, s.ctr,
+    hipLaunchKernelGGL(k_overlap_direct, dim3(grid), dim3(256), lds, st, d, s.V, s.Tpad, s.L, s.thr, s.ov_list, s.ctr,
                        s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov);
     return hipGetLastError();
 }
@@ -1548,8 +1563,8 @@ static size_t overlap_screened_lds_mode(const Dims &d, bool mq)
 {
     const uint64_t ts = (uint64_t)d.T * d.S;
     const uint64_t LP = ((d.N + 3) / 4) * 4 + ((d.S + 15) / 16) * 16;
-    return (uint64_t)d.T * 8 + 2ull * d.C * d.N * 4 + (mq ? (uint64_t)d.C * LP * 4 : 0) + ts * 4 + ts * 4 +
-           (uint64_t)d.S * 4 + ts + 16;
+    return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + (uint64_t)d.C * (d.N + 2 * d.M) * 4 +
+           (mq ? (uint64_t)d.C * LP * 4 : 0) + ts * 4 + ts * 4 + (uint64_t)d.S * 4 + ts + 16;
 }
 
 // The matrix-core Q table is used whenever its padded voltage fits in LDS.
@@ -1560,6 +1575,13 @@ size_t overlap_screened_lds(const Dims &d) { return overlap_screened_lds_mode(d,
 uint64_t overlap_qtemplate_floats(const Dims &d)
 {
     return (uint64_t)d.C * ((d.N + 3) / 4) * ((d.T + 15) / 16) * 64;
+}
+
+hipError_t launch_pad_templates(const Dims &d, const DeviceState &s, hipStream_t st)
+{
+    const uint64_t total = (uint64_t)d.T * d.C * (d.N + 2 * d.M);
+    hipLaunchKernelGGL(k_pad_templates, dim3(grid_for(total, 256, 4096)), dim3(256), 0, st, d, s.Tm, s.Tpad);
+    return hipGetLastError();
 }
 
 hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_t st)
@@ -1586,11 +1608,11 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
     if (mq)
         hipLaunchKernelGGL(k_overlap_screened<true>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                            s.ov_list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                           s.tabB2, s.tnorm, s.TmQ);
+                           s.tabB2, s.tnorm, s.TmQ, s.Tpad);
     else
         hipLaunchKernelGGL(k_overlap_screened<false>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                            s.ov_list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                           s.tabB2, s.tnorm, s.TmQ);
+                           s.tabB2, s.tnorm, s.TmQ, s.Tpad);
     return hipGetLastError();
 }
 

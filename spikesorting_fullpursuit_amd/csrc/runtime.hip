@@ -105,7 +105,7 @@ void free_state(fbp_segment *g)
     void *ptrs[] = {s.V, s.V0, s.Tm, s.TmT, s.bias, s.thr, s.L, s.best_L, s.best_lab, s.best_idx, s.ov_idx,
                     s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
                     s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabB2, s.tnorm,
-                    s.TmB, s.rq, s.tscale, s.rdesc, s.TmQ};
+                    s.TmB, s.rq, s.tscale, s.rdesc, s.TmQ, s.Tpad};
     for (void *p : ptrs) dfree(p, g->own_stream);
     if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
     s = DeviceState{};
@@ -120,7 +120,8 @@ uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
     const uint64_t nb = (W + kCompactItems - 1) / kCompactItems;
     uint64_t b = 2 * r(CK * 4) + r(T * d.C * d.N * 4) + r((uint64_t)d.C * d.N * d.Tp * 4) + r(d.Tp * 4) +
                  r(T * 4) + r(T * d.K * 4) + 5 * r(W * 4) + 2 * r(W + 2) + 3 * r(W * 4) + r(W) + 3 * r(W * 4) +
-                 r((nb + 1) * 4) + r(sizeof(Counters)) + r(T * 4) + 2 * r(W * 4);
+                 r((nb + 1) * 4) + r(sizeof(Counters)) + r(T * 4) + 2 * r(W * 4) + r(T * 4) +
+                 r(T * d.C * (d.N + 2 * d.M) * 4);
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED)
         b += r(T * d.S * T * d.S * 4) + r(T * d.S * 4) + r(overlap_qtemplate_floats(d) * 4);
     if (cfg->precision != FBP_PREC_FP32) {
@@ -221,8 +222,9 @@ static int make_dims(const fbp_config *cfg, Dims &d)
         return fail(FBP_EINVAL, "the matrix-core screen supports templates of up to 96 samples (N=%u)", N);
     if (cfg->overlap_mode != FBP_OVERLAP_DIRECT && cfg->overlap_mode != FBP_OVERLAP_SCREENED)
         return fail(FBP_EINVAL, "unknown overlap mode %u", cfg->overlap_mode);
-    if ((uint64_t)C * N * sizeof(float) * 2 > kMaxLds)
-        return fail(FBP_EINVAL, "C*N = %u exceeds the overlap kernel's LDS staging", C * N);
+    if ((uint64_t)C * (2ull * N + 2ull * cfg->max_shift) * sizeof(float) > kMaxLds)
+        return fail(FBP_EINVAL, "C*(2N+2M) = %llu exceeds the overlap kernel's LDS staging",
+                    (unsigned long long)C * (2ull * N + 2ull * cfg->max_shift));
     d.C = C; d.K = K; d.T = T; d.N = N; d.M = cfg->max_shift;
     d.S = 2 * cfg->max_shift + 1;
     d.W = (uint32_t)(((uint64_t)K + N - 1) / N);
@@ -286,6 +288,7 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     ALLOC(s.ctr, 1)
     ALLOC(s.tnorm, T)
     ALLOC(s.tscale, T)
+    ALLOC(s.Tpad, (uint64_t)T * C * (N + 2 * d.M))
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED) {
         ALLOC(s.tabAB, (uint64_t)T * d.S * T * d.S)
         ALLOC(s.tabB2, (uint64_t)T * d.S)
@@ -361,6 +364,7 @@ int fbp_segment_set_templates(fbp_segment *g, const float *templates, const floa
     if (e == hipSuccess) e = hipMemcpyAsync(g->s.thr, thresholds, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_transpose_templates(d, g->s.Tm, tmp_bias, g->s.TmT, g->s.bias, st);
     if (e == hipSuccess) e = launch_template_norms(d, g->s, g->cfg.precision, st);
+    if (e == hipSuccess) e = launch_pad_templates(d, g->s, st);
     if (e == hipSuccess && g->cfg.overlap_mode == FBP_OVERLAP_SCREENED) e = launch_overlap_tables(d, g->s, st);
     if (e == hipSuccess && g->cfg.precision != FBP_PREC_FP32) e = launch_pack_templates(d, g->s, g->cfg.precision, st);
     dfree(tmp_bias, st);
