@@ -1005,10 +1005,18 @@ __device__ void q_table_mfma(const Dims &d, const float *__restrict__ TmQ, const
         for (uint32_t c = 0; c < d.C; c++) {
             const float *ap = TmQ + ((uint64_t)c * NK4 * TT + tt) * 64 + lane;
             const float *bp = svp + (uint64_t)c * LP + scol + (lane >> 4);
-            for (uint32_t kk = 0; kk < NK4; kk++) {
-                const float a = ap[(uint64_t)kk * TT * 64];
-                const float b = bp[kk * 4];
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+            // fragments in batches of 16: all loads issue before the MFMAs;
+            // a padded step multiplies zeros and leaves acc unchanged
+            for (uint32_t k0 = 0; k0 < NK4; k0 += 16) {
+                float a[16], b[16];
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const uint32_t kk = k0 + q;
+                    a[q] = kk < NK4 ? ap[(uint64_t)kk * TT * 64] : 0.0f;
+                    b[q] = kk < NK4 ? bp[kk * 4] : 0.0f;
+                }
+#pragma unroll
+                for (int q = 0; q < 16; q++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], b[q], acc, 0, 0, 0);
             }
         }
 #pragma unroll
