@@ -67,7 +67,7 @@ struct DeviceState {
     float *tabAB;        // [T][S][T][S] screened overlap: shifted-template cross products
     float *tabB2;        // [T][S]       squared norms of shifted, truncated templates
     float *tnorm;        // [T]          template norms (upper bounds)
-    float *TmQ;          // screened overlap: float32 A fragments of the matrix-core Q table
+    float *TmQ;          // screened overlap: split-bf16 A fragments of the matrix-core Q table (uint4 each)
     float *Tpad;         // [T][C][N + 2M] templates at [M, M + N), +0.0 around (overlap scores)
     float *tscale;       // [T]          fp8 screen: per-template power-of-two scale (1 otherwise)
     void *TmB;           // packed bf16/fp16/fp8 template fragments for the matrix cores

@@ -484,6 +484,8 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
 // so for every (c, j) a wave reads one LDS sample per chain and one short run
 // of template values.  Each chain is the reference's: bias, then c = 0..C-1,
 // j = 0..N-1, fused multiply-adds.
+constexpr int kRescorePre = 32;   // per-lane prefetch slots for one channel's templates
+
 __global__ __launch_bounds__(64) void k_rescore_win(Dims d, const float *__restrict__ V,
                                                     const float *__restrict__ TmT, const float *__restrict__ bias,
                                                     const uint64_t *__restrict__ rq, const uint2 *__restrict__ rdesc,
@@ -499,6 +501,7 @@ __global__ __launch_bounds__(64) void k_rescore_win(Dims d, const float *__restr
     const uint64_t base = (uint64_t)w * d.N;
     const uint32_t span = ((d.N + 15) / 16) * 16 + d.N;
     const uint32_t ntm = d.N * d.Tp;
+    const bool pre = ntm <= 64u * kRescorePre && span <= 64u * 4;
     for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
         const uint32_t e = e0 + lane;
         const bool ok = e < cnt;
@@ -506,13 +509,45 @@ __global__ __launch_bounds__(64) void k_rescore_win(Dims d, const float *__restr
         const uint32_t t = ok ? (uint32_t)(ent >> 32) : 0u;
         const uint32_t p = ok ? (uint32_t)((ent & 0xffffffffull) - base) : 0u;
         float acc = bias[t];
+        // channel c's staging values are loaded into registers while channel
+        // c-1 runs its chains
+        float rt_[kRescorePre], rv[4];
+        auto fetch = [&](uint32_t c) {
+            const float *tc = TmT + (uint64_t)c * ntm;
+            const float *vr = V + (uint64_t)c * d.K;
+#pragma unroll
+            for (int q = 0; q < kRescorePre; q++) {
+                const uint32_t x = lane + 64u * q;
+                rt_[q] = x < ntm ? tc[x] : 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t x = lane + 64u * q;
+                rv[q] = (x < span && base + x < d.K) ? vr[base + x] : 0.0f;
+            }
+        };
+        if (pre) fetch(0);
         for (uint32_t c = 0; c < d.C; c++) {
             __syncthreads();
-            const float *vr = V + (uint64_t)c * d.K;
-            for (uint32_t x = lane; x < span; x += 64) sv[x] = base + x < d.K ? vr[base + x] : 0.0f;
-            const float *tc = TmT + (uint64_t)c * ntm;
-            for (uint32_t x = lane; x < ntm; x += 64) st_tm[x] = tc[x];
+            if (pre) {
+#pragma unroll
+                for (int q = 0; q < kRescorePre; q++) {
+                    const uint32_t x = lane + 64u * q;
+                    if (x < ntm) st_tm[x] = rt_[q];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t x = lane + 64u * q;
+                    if (x < span) sv[x] = rv[q];
+                }
+            } else {
+                const float *vr = V + (uint64_t)c * d.K;
+                for (uint32_t x = lane; x < span; x += 64) sv[x] = base + x < d.K ? vr[base + x] : 0.0f;
+                const float *tc = TmT + (uint64_t)c * ntm;
+                for (uint32_t x = lane; x < ntm; x += 64) st_tm[x] = tc[x];
+            }
             __syncthreads();
+            if (pre && c + 1 < d.C) fetch(c + 1);
             const float *vp = sv + p;
             const float *tp = st_tm + t;
             for (uint32_t j = 0; j < d.N; j++) acc = __builtin_fmaf(tp[j * d.Tp], vp[j], acc);
@@ -957,111 +992,239 @@ __global__ __launch_bounds__(256) void k_overlap_tables(Dims d, const float *__r
     }
 }
 
-// Calls fn(f, t, s, e = t*S + s) for every combination of one window that the
-// reference does not skip (bp.cl:534-576), in no particular order: the main
-// shifts f whose likelihood is positive against every in-bounds (t, s) within
-// N of f, then the other in-bounds main shifts against the (t, s) whose
-// likelihood is positive.
-template <class Fn>
+// Calls fn(f, t, s, e = t*S + s, AB[lab][f][t][s]) for every combination of
+// one window that the reference does not skip (bp.cl:534-576), in no
+// particular order: the main shifts f whose likelihood is positive against
+// every in-bounds (t, s) within N of f, then the other in-bounds main shifts
+// against the (t, s) whose likelihood is positive.  Each thread fetches the AB
+// values of B combinations before it evaluates them, so their latencies
+// overlap.  Quotients advance incrementally; no division per element.
+template <int B, class Fn>
 __device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t *spos, const uint32_t *flist,
                                                    uint32_t n_fpos, uint32_t n_fneg, const uint32_t *plist,
-                                                   uint32_t n_ppos, Fn fn)
+                                                   uint32_t n_ppos, const float *__restrict__ ABl, Fn fn)
 {
     const uint32_t S = d.S, N = d.N, TS = d.T * d.S;
+    const uint32_t step = blockDim.x;
+    const uint32_t stT = step / S, stS = step - stT * S;
+    const uint32_t t00 = threadIdx.x / S, s00 = threadIdx.x - t00 * S;
+    // e / S for e < T*S: e * ceil(2^32 / S) / 2^32 is exact while e * S < 2^32
+    const uint64_t mS = ((1ull << 32) + S - 1) / S;
     for (uint32_t a = 0; a < n_fpos; a++) {
         const uint32_t f = flist[a];
-        uint32_t t = threadIdx.x / S, s = threadIdx.x - t * S;
-        for (uint32_t e = threadIdx.x; e < TS; e += blockDim.x) {
-            if ((spos[e] & 2) && (f > s ? f - s : s - f) <= N) fn(f, t, s, e);
-            s += blockDim.x;
-            while (s >= S) { s -= S; t++; }
+        const float *abf = ABl + (uint64_t)f * TS;
+        uint32_t t = t00, s = s00;
+        for (uint32_t e0 = threadIdx.x; e0 < TS; e0 += B * step) {
+            float ab[B];
+#pragma unroll
+            for (int q = 0; q < B; q++) {
+                const uint32_t e = e0 + q * step;
+                ab[q] = e < TS ? abf[e] : 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < B; q++) {
+                const uint32_t e = e0 + q * step;
+                if (e < TS && (spos[e] & 2) && (f > s ? f - s : s - f) <= N) fn(f, t, s, e, ab[q]);
+                s += stS;
+                t += stT;
+                if (s >= S) { s -= S; t++; }
+            }
         }
     }
     const uint32_t total = n_fneg * n_ppos;
-    for (uint32_t r = threadIdx.x; r < total; r += blockDim.x) {
-        const uint32_t a = r / n_ppos, b = r - a * n_ppos;
-        const uint32_t f = flist[n_fpos + a], e = plist[b];
-        const uint32_t t = e / S, s = e - t * S;
-        if ((f > s ? f - s : s - f) <= N) fn(f, t, s, e);
+    if (total == 0) return;
+    const uint32_t stA = step / n_ppos, stB = step - stA * n_ppos;
+    uint32_t ra = threadIdx.x / n_ppos, rb = threadIdx.x - ra * n_ppos;
+    for (uint32_t r0 = threadIdx.x; r0 < total; r0 += B * step) {
+        float ab[B];
+        uint32_t ff[B], ee[B];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const bool ok = r0 + q * step < total;
+            ff[q] = ok ? flist[n_fpos + ra] : 0u;
+            ee[q] = ok ? plist[rb] : 0u;
+            ab[q] = ok ? ABl[(uint64_t)ff[q] * TS + ee[q]] : 0.0f;
+            rb += stB;
+            ra += stA;
+            if (rb >= n_ppos) { rb -= n_ppos; ra++; }
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            if (r0 + q * step >= total) break;
+            const uint32_t f = ff[q], e = ee[q];
+            const uint32_t t = (uint32_t)(((uint64_t)e * mS) >> 32), s = e - t * S;
+            if ((f > s ? f - s : s - f) <= N) fn(f, t, s, e, ab[q]);
+        }
     }
 }
 
-// Q table on the matrix cores: Q[t][s] = sum over (c, x) of T[t][c][x] *
-// vpad[c][x + s - M] (vpad = window voltage, zero outside [0, N)), one
-// v_mfma_f32_16x16x4f32 per (16 templates, 16 shifts, 4 samples).  Template
-// A fragments come pre-packed in lane order (TmQ); the zero-padded voltage
-// svp[c][y] = v[c][y - M] is in LDS.
-__device__ void q_table_mfma(const Dims &d, const float *__restrict__ TmQ, const float *svp, uint32_t LP,
-                             float *sQ)
+// Q table on the matrix cores in split bfloat16:
+//   Q[t][s] = sum over (c, x < N) of T[t][c][x] * vpad[c][x + s],
+// vpad[c][y] = v[c][y - M] for y - M in [0, N), 0 elsewhere.  Every float32
+// operand is split into hi = bf16(x) and lo = bf16(x - hi) (both round to
+// nearest even; x - hi is exact in float32), and
+//   Q ~= sum hi_T*hi_v  +  sum (hi_T*lo_v + lo_T*hi_v)
+// on v_mfma_f32_16x16x16bf16_1k (bf16 products are exact in float32), in two
+// accumulators.  The omitted terms lo*lo and the representation error of lo
+// are below 3.1 * 2^-16 * |T| * |v| per product.
+//
+// Work: tiles of 16 templates x 16 shifts.  One task = one 16-template row
+// tile, up to four shift tiles (each A fragment feeds up to 12 MFMAs), and one
+// contiguous slice of the channels when there are too few tiles to occupy the
+// block's waves; slice partials are added in slice order afterwards.
+constexpr uint32_t kOvWaves = 4;   // waves of a k_overlap_screened block
+
+struct QPlan {
+    uint32_t TT, ST, NKQ, SG, NSG, KS, LPH;
+};
+
+__host__ __device__ inline QPlan q_plan(const Dims &d)
 {
-    const uint32_t S = d.S, T = d.T;
-    const uint32_t TT = (T + 15) / 16, ST = (S + 15) / 16, NK4 = (d.N + 3) / 4;
-    const int lane = lane_of();
+    QPlan p;
+    p.TT = (d.T + 15) / 16;
+    p.ST = (d.S + 15) / 16;
+    p.NKQ = (d.N + 15) / 16;
+    p.SG = p.ST < 4 ? p.ST : 4;
+    p.NSG = (p.ST + p.SG - 1) / p.SG;
+    const uint32_t base = p.TT * p.NSG;
+    uint32_t ks = base < kOvWaves ? kOvWaves / base : 1;
+    p.KS = ks < d.C ? ks : d.C;
+    // bf16 elements per channel row of the split voltage: every lane reads the
+    // three dwords holding samples y..y+3, y < 16*(NKQ + ST) - 4
+    p.LPH = 16 * (p.NKQ + p.ST) + 8;
+    return p;
+}
+
+__device__ __forceinline__ uint64_t q_bfrag(const uint32_t *row, uint32_t y)
+{
+    // samples y..y+3 of a bf16 row, sample y in the low half of the low dword
+    const uint32_t w = y >> 1, sh = 16u * (y & 1u);
+    const uint32_t d0 = row[w], d1 = row[w + 1], d2 = row[w + 2];
+    const uint32_t lo = __builtin_amdgcn_alignbit(d1, d0, sh);
+    const uint32_t hi = __builtin_amdgcn_alignbit(d2, d1, sh);
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+__device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__restrict__ TmQ,
+                              const uint16_t *svh, const uint16_t *svl, float *sQ, float *sQp)
+{
+    const uint32_t S = d.S, T = d.T, TS = T * S;
+    const uint32_t lane = lane_of();
     const uint32_t wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    for (uint32_t tile = wave; tile < TT * ST; tile += nwave) {
-        const uint32_t tt = tile / ST, st = tile % ST;
-        float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        const uint32_t scol = st * 16 + (lane & 15);
-        for (uint32_t c = 0; c < d.C; c++) {
-            const float *ap = TmQ + ((uint64_t)c * NK4 * TT + tt) * 64 + lane;
-            const float *bp = svp + (uint64_t)c * LP + scol + (lane >> 4);
-            // fragments in batches of 16: all loads issue before the MFMAs;
-            // a padded step multiplies zeros and leaves acc unchanged
-            for (uint32_t k0 = 0; k0 < NK4; k0 += 16) {
-                float a[16], b[16];
+    const uint32_t ntask = qp.TT * qp.NSG * qp.KS;
+    for (uint32_t task = wave; task < ntask; task += nwave) {
+        const uint32_t ks = task % qp.KS, rest = task / qp.KS;
+        const uint32_t g = rest % qp.NSG, tt = rest / qp.NSG;
+        const uint32_t c0 = ks * d.C / qp.KS, c1 = (ks + 1) * d.C / qp.KS;
+        const uint32_t st0 = g * qp.SG;
+        const uint32_t nst = qp.ST - st0 < qp.SG ? qp.ST - st0 : qp.SG;
+        float4v acch[4], accl[4];
 #pragma unroll
-                for (int q = 0; q < 16; q++) {
-                    const uint32_t kk = k0 + q;
-                    a[q] = kk < NK4 ? ap[(uint64_t)kk * TT * 64] : 0.0f;
-                    b[q] = kk < NK4 ? bp[kk * 4] : 0.0f;
+        for (int j = 0; j < 4; j++) {
+            acch[j] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+            accl[j] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+        // steps (c, kq) in c-major order; the A fragment of step st is
+        // TmQ[(st*TT + tt)*64 + lane].  Fragments are fetched four steps ahead.
+        const uint32_t sbeg = c0 * qp.NKQ, send = c1 * qp.NKQ;
+        const uint4 *ap = TmQ + (uint64_t)tt * 64 + lane;
+        const uint64_t astride = (uint64_t)qp.TT * 64;
+        const uint32_t ybase = 4 * (lane >> 4) + st0 * 16 + (lane & 15);
+        uint4 an[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) an[q] = sbeg + q < send ? ap[(sbeg + q) * astride] : uint4{0u, 0u, 0u, 0u};
+        uint32_t c = c0, kq = 0;
+        for (uint32_t s0 = sbeg; s0 < send; s0 += 4) {
+            uint4 a[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) a[q] = an[q];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                an[q] = s0 + 4 + q < send ? ap[(s0 + 4 + q) * astride] : uint4{0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (s0 + q >= send) break;
+                const uint64_t ah = (uint64_t)a[q].x | ((uint64_t)a[q].y << 32);
+                const uint64_t al = (uint64_t)a[q].z | ((uint64_t)a[q].w << 32);
+                const uint32_t *rh = (const uint32_t *)(svh + c * qp.LPH);
+                const uint32_t *rl = (const uint32_t *)(svl + c * qp.LPH);
+                const uint32_t y0 = kq * 16 + ybase;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if ((uint32_t)j < nst) {
+                        const uint64_t bh = q_bfrag(rh, y0 + 16 * j), bl = q_bfrag(rl, y0 + 16 * j);
+                        acch[j] = mfma_narrow<1>(ah, bh, acch[j]);
+                        accl[j] = mfma_narrow<1>(ah, bl, accl[j]);
+                        accl[j] = mfma_narrow<1>(al, bh, accl[j]);
+                    }
                 }
-#pragma unroll
-                for (int q = 0; q < 16; q++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], b[q], acc, 0, 0, 0);
+                if (++kq == qp.NKQ) { kq = 0; c++; }
             }
         }
+        float *dst = ks == 0 ? sQ : sQp + (uint64_t)(ks - 1) * TS;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const uint32_t t = tt * 16 + 4 * (lane >> 4) + r;
-            if (t < T && scol < S) sQ[t * S + scol] = acc[r];
+        for (int j = 0; j < 4; j++) {
+            if ((uint32_t)j >= nst) break;
+            const uint32_t scol = (st0 + j) * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint32_t t = tt * 16 + 4 * (lane >> 4) + r;
+                if (t < T && scol < S) dst[t * S + scol] = acch[j][r] + accl[j][r];
+            }
         }
     }
 }
 
-// TmQ[((c*NK4 + kk)*TT + tt)*64 + lane] = T[tt*16 + lane%16][c][4*kk + lane/16]
-// (0 when padded): the A fragments of q_table_mfma.
-__global__ void k_pack_qtemplates(Dims d, const float *__restrict__ Tm, float *__restrict__ TmQ)
+// TmQ[(c*NKQ + kq)*TT + tt][lane] = {hi, lo} fragments (4 bf16 each, sample
+// kq*16 + 4*(lane/16) + i in bits 16i) of template t = tt*16 + lane%16 on
+// channel c; 0 when padded.  The A fragments of q_table_split.
+__global__ void k_pack_qtemplates(Dims d, const float *__restrict__ Tm, uint4 *__restrict__ TmQ)
 {
-    const uint32_t TT = (d.T + 15) / 16, NK4 = (d.N + 3) / 4;
-    const uint64_t total = (uint64_t)d.C * NK4 * TT * 64;
+    const uint32_t TT = (d.T + 15) / 16, NKQ = (d.N + 15) / 16;
+    const uint64_t total = (uint64_t)d.C * NKQ * TT * 64;
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
          e += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t lane = (uint32_t)(e % 64);
         const uint64_t r = e / 64;
         const uint32_t tt = (uint32_t)(r % TT);
-        const uint32_t kk = (uint32_t)((r / TT) % NK4);
-        const uint32_t c = (uint32_t)(r / ((uint64_t)TT * NK4));
-        const uint32_t t = tt * 16 + (lane & 15), x = 4 * kk + (lane >> 4);
-        TmQ[e] = (t < d.T && x < d.N) ? Tm[((uint64_t)t * d.C + c) * d.N + x] : 0.0f;
+        const uint32_t kq = (uint32_t)((r / TT) % NKQ);
+        const uint32_t c = (uint32_t)(r / ((uint64_t)TT * NKQ));
+        const uint32_t t = tt * 16 + (lane & 15);
+        uint32_t h[2] = {0u, 0u}, l[2] = {0u, 0u};
+        for (uint32_t i = 0; i < 4; i++) {
+            const uint32_t x = kq * 16 + 4 * (lane >> 4) + i;
+            const float v = (t < d.T && x < d.N) ? Tm[((uint64_t)t * d.C + c) * d.N + x] : 0.0f;
+            const uint32_t vh = to_bf16(v);
+            const uint32_t vl = to_bf16(v - __uint_as_float(vh << 16));
+            h[i >> 1] |= vh << (16 * (i & 1));
+            l[i >> 1] |= vl << (16 * (i & 1));
+        }
+        TmQ[e] = uint4{h[0], h[1], l[0], l[1]};
     }
 }
 
+// Register budget of three waves per SIMD (no spills); LDS allows about four
+// blocks per CU at the 16-channel shapes.
 template <bool MQ>
-__global__ __launch_bounds__(256) void k_overlap_screened(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_overlap_screened(
     Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
     const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw, float *best_L,
     const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov,
     const float *__restrict__ AB, const float *__restrict__ B2, const float *__restrict__ tnorm,
-    const float *__restrict__ TmQ, const float *__restrict__ Tpad)
+    const uint4 *__restrict__ TmQ, const float *__restrict__ Tpad)
 {
     extern __shared__ float dyn[];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
     const uint32_t TS = T * S;
-    const uint32_t LP = ((N + 3) / 4) * 4 + ((S + 15) / 16) * 16;   // padded voltage row (MQ)
+    const QPlan qp = q_plan(d);
+    const uint32_t LPH = qp.LPH;
     double *sE = (double *)dyn;                         // [T] screen bound per template (this window)
     float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset
     float *smain = sv + C * N;                          // [C][N + 2M] padded template of the best label
-    float *svp = smain + C * (N + 2 * M);               // [C][LP] zero-padded voltage (MQ only)
-    float *sQ = svp + (MQ ? C * LP : 0);                // [T][S] Q - B2/2
+    uint16_t *svh = (uint16_t *)(smain + C * (N + 2 * M));   // [C][LPH] zero-padded voltage, bf16 hi (MQ)
+    uint16_t *svl = svh + (MQ ? C * LPH : 0);           // [C][LPH] bf16 lo (MQ)
+    float *sQp = (float *)(svl + (MQ ? C * LPH : 0));   // [KS-1][T][S] channel-slice partials of Q (MQ)
+    float *sQ = sQp + (MQ ? (qp.KS - 1) * TS : 0);      // [T][S] Q - B2/2
     uint32_t *plist = (uint32_t *)(sQ + TS);            // (t,s) with positive likelihood
     uint32_t *flist = plist + TS;                       // main shifts: positive first, then the rest
     uint8_t *spos = (uint8_t *)(flist + S);             // [T][S] bit1 in bounds, bit0 likelihood > 0
@@ -1078,9 +1241,15 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
     const double nops = (double)C * N + 2.0 * C + 8.0;
     const double u2 = 0x1p-23;
     const double g = nops * u2 / (1.0 - nops * u2);
-    // matrix-core Q: float32 products and sums over C*NK4*4 + 1 terms, for both Q terms
-    const double nq = (double)C * ((N + 3) / 4) * 4 + 1.0;
-    const double gq = MQ ? nq * u2 / (1.0 - nq * u2) : 0.0;
+    // Split-bf16 matrix-core Q, relative to sum |T||v| <= |T_t| |v|: the hi*hi
+    // chain (depth nq: C*NKQ*16 products, the hi + lo add and KS-1 slice
+    // adds), the hi*lo + lo*hi chain (depth 2nq, terms <= 2^-7.98 of the
+    // products) and the omitted lo*lo terms and lo representation error.
+    const double nq = (double)C * qp.NKQ * 16 + qp.KS + 2.0;
+    const double gq1 = nq * u2 / (1.0 - nq * u2), gq2 = 2.0 * nq * u2 / (1.0 - 2.0 * nq * u2);
+    const double gq = MQ ? gq1 * 1.004 + gq2 * 0.0079 + 3.1 * 0x1p-16 : 0.0;
+    // products flushed below the float32 normal range: 2^-126 each, both Q terms
+    const double eabs = MQ ? 2.0 * 3.0 * nq * 0x1p-126 : 0.0;
     // + float32 storage of Q - B2/2 and of the AB table (unit roundoff each)
     const double ecoef = (g + 2.0 * gq + 4.0 * u2) * 1.01;
     for (uint32_t k = blockIdx.x; k < n_ov; k += gridDim.x) {
@@ -1104,9 +1273,12 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
             for (uint32_t e = threadIdx.x; e < C * (N + 2 * M); e += blockDim.x)
                 smain[e] = Tpad[(uint64_t)lab * C * (N + 2 * M) + e];
             if (MQ)
-                for (uint32_t e = threadIdx.x; e < C * LP; e += blockDim.x) {
-                    const uint32_t c = e / LP, y = e % LP;
-                    svp[e] = (y >= M && y - M < N) ? V[(uint64_t)c * d.K + bidx + (y - M)] : 0.0f;
+                for (uint32_t e = threadIdx.x; e < C * LPH; e += blockDim.x) {
+                    const uint32_t c = e / LPH, y = e % LPH;
+                    const float x = (y >= M && y - M < N) ? V[(uint64_t)c * d.K + bidx + (y - M)] : 0.0f;
+                    const uint32_t h = to_bf16(x);
+                    svh[e] = (uint16_t)h;
+                    svl[e] = to_bf16(x - __uint_as_float(h << 16));
                 }
             for (int off = 32; off > 0; off >>= 1) vv += __shfl_xor(vv, off);
             if (lane_of() == 0) r_vv[threadIdx.x >> 6] = vv;
@@ -1134,7 +1306,15 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
             }
             if (MQ) {
                 __syncthreads();
-                q_table_mfma(d, TmQ, svp, LP, sQ);
+                q_table_split(d, qp, TmQ, svh, svl, sQ, sQp);
+                if (qp.KS > 1) {
+                    __syncthreads();
+                    for (uint32_t e = threadIdx.x; e < TS; e += blockDim.x) {
+                        float q = sQ[e];
+                        for (uint32_t k = 1; k < qp.KS; k++) q += sQp[(k - 1) * TS + e];
+                        sQ[e] = q;
+                    }
+                }
             }
             __syncthreads();
             const double vn = sqrt(r_vv[0] + r_vv[1] + r_vv[2] + r_vv[3]) * (1.0 + 1e-12);
@@ -1144,7 +1324,7 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
                 sQ[e] = (float)((double)sQ[e] - 0.5 * (double)B2[e]);
             for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
                 const double nsum = nlab + (double)tnorm[t];
-                sE[t] = ecoef * (nsum * vn + nsum * nsum);
+                sE[t] = ecoef * (nsum * vn + nsum * nsum) + eabs;
             }
             // in-bounds main shifts without a positive likelihood go after the positive ones
             if (threadIdx.x == 0) {
@@ -1157,16 +1337,16 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
             const float *ABl = AB + (uint64_t)lab * S * TS;
             const float *sQl = sQ + lab * S;
             const uint32_t nfp = n_fpos, nfn = n_fneg, npp = n_ppos;
-            auto screen = [&](uint32_t f, uint32_t t, uint32_t e, double &E) {
+            auto screen = [&](uint32_t f, uint32_t t, uint32_t e, float ab, double &E) {
                 E = sE[t];
-                return (double)sQl[f] + (double)sQ[e] - (double)ABl[(uint64_t)f * TS + e];
+                return (double)sQl[f] + (double)sQ[e] - (double)ab;
             };
             // pass A: largest lower bound and largest upper bound of the score
             double lo_max = -INFINITY, hi_max = -INFINITY;
             uint32_t screened = 0;
-            for_each_unskipped(d, spos, flist, nfp, nfn, plist, npp, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e) {
+            for_each_unskipped<4>(d, spos, flist, nfp, nfn, plist, npp, ABl, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab) {
                 double E;
-                const double sh = screen(f, t, e, E);
+                const double sh = screen(f, t, e, ab, E);
                 lo_max = fmax(lo_max, sh - E);
                 hi_max = fmax(hi_max, sh + E);
                 screened++;
@@ -1185,9 +1365,9 @@ __global__ __launch_bounds__(256) void k_overlap_screened(
             if (hi_max > (double)bl) {
                 uint32_t scored = 0;
                 const float *tmain = smain;
-                for_each_unskipped(d, spos, flist, nfp, nfn, plist, npp, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e) {
+                for_each_unskipped<1>(d, spos, flist, nfp, nfn, plist, npp, ABl, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab) {
                     double E;
-                    const double sh = screen(f, t, e, E);
+                    const double sh = screen(f, t, e, ab, E);
                     if (sh + E < lo_max) return;
                     const float score = overlap_direct_score(d, sv, tmain, Tpad + (uint64_t)t * C * (N + 2 * M), f, s);
                     scored++;
@@ -1570,9 +1750,10 @@ hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t m
 static size_t overlap_screened_lds_mode(const Dims &d, bool mq)
 {
     const uint64_t ts = (uint64_t)d.T * d.S;
-    const uint64_t LP = ((d.N + 3) / 4) * 4 + ((d.S + 15) / 16) * 16;
+    const QPlan qp = q_plan(d);
     return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + (uint64_t)d.C * (d.N + 2 * d.M) * 4 +
-           (mq ? (uint64_t)d.C * LP * 4 : 0) + ts * 4 + ts * 4 + (uint64_t)d.S * 4 + ts + 16;
+           (mq ? (uint64_t)d.C * qp.LPH * 4 + (uint64_t)(qp.KS - 1) * ts * 4 : 0) + ts * 4 + ts * 4 +
+           (uint64_t)d.S * 4 + ts + 16;
 }
 
 // The matrix-core Q table is used whenever its padded voltage fits in LDS.
@@ -1582,7 +1763,7 @@ size_t overlap_screened_lds(const Dims &d) { return overlap_screened_lds_mode(d,
 
 uint64_t overlap_qtemplate_floats(const Dims &d)
 {
-    return (uint64_t)d.C * ((d.N + 3) / 4) * ((d.T + 15) / 16) * 64;
+    return (uint64_t)d.C * ((d.N + 15) / 16) * ((d.T + 15) / 16) * 64 * 4;   // one uint4 per entry
 }
 
 hipError_t launch_pad_templates(const Dims &d, const DeviceState &s, hipStream_t st)
@@ -1598,7 +1779,8 @@ hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_
     hipLaunchKernelGGL(k_overlap_tables, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st, d, s.Tm, s.tabAB,
                        s.tabB2, s.tnorm);
     const uint64_t nq = overlap_qtemplate_floats(d);
-    hipLaunchKernelGGL(k_pack_qtemplates, dim3(grid_for(nq, 256, 4096)), dim3(256), 0, st, d, s.Tm, s.TmQ);
+    hipLaunchKernelGGL(k_pack_qtemplates, dim3(grid_for(nq / 4, 256, 4096)), dim3(256), 0, st, d, s.Tm,
+                       (uint4 *)s.TmQ);
     return hipGetLastError();
 }
 
@@ -1616,11 +1798,11 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
     if (mq)
         hipLaunchKernelGGL(k_overlap_screened<true>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                            s.ov_list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                           s.tabB2, s.tnorm, s.TmQ, s.Tpad);
+                           s.tabB2, s.tnorm, (const uint4 *)s.TmQ, s.Tpad);
     else
         hipLaunchKernelGGL(k_overlap_screened<false>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                            s.ov_list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                           s.tabB2, s.tnorm, s.TmQ, s.Tpad);
+                           s.tabB2, s.tnorm, (const uint4 *)s.TmQ, s.Tpad);
     return hipGetLastError();
 }
 
