@@ -303,7 +303,7 @@ __device__ __forceinline__ float exact_likelihood(const Dims &d, const float *__
 constexpr int kMfmaMaxRT = 6;   // row tiles per window: N <= 96
 
 template <int PREC, int RT, int CTB>
-__global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restrict__ V, const float *__restrict__ Tm,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT <= 4 && CTB <= 2 ? 3 : (RT * CTB > 16 ? 1 : 2)))) void k_corr_mfma(Dims d, const float *__restrict__ V, const float *__restrict__ Tm,
                                                   const uint64_t *__restrict__ TmB, const float *__restrict__ bias,
                                                   const float *__restrict__ tnorm, const float *__restrict__ tscale,
                                                   const uint32_t *__restrict__ win, float *__restrict__ L,
@@ -326,29 +326,73 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
     const uint64_t wend = min(base + d.N, (uint64_t)d.K);
     const uint32_t np = (uint32_t)(wend - base);   // positions of this window
 
-    // Per-position window norm bound |v_i| (float64 sums of the float32 samples).
-    if (lane == 0) over = 0;
-    for (uint32_t x = lane; x < RT * 16 + d.N; x += 64) {
-        double a = 0.0;
-        const uint64_t g = base + x;
-        if (g < d.K)
-            for (uint32_t c = 0; c < d.C; c++) {
-                const double v = V[(uint64_t)c * d.K + g];
-                a = __builtin_fma(v, v, a);
-            }
-        s2[x] = a;
-    }
-    __syncthreads();
-    for (uint32_t p = lane; p < RT * 16; p += 64) {
-        double a = 0.0;
-        for (uint32_t x = 0; x < d.N; x++) a += s2[p + x];
-        const double nv = sqrt(a) * (1.0 + 1e-9);
-        float up = (float)nv;
-        if ((double)up < nv) up = __uint_as_float(__float_as_uint(up) + 1u);
-        vnb[p] = up;
-    }
-
+    // Per-position window norm bound |v_i| (float64 sums of the float32 samples):
+    // s2[x] = sum over channels of v[c][x]^2, accumulated while staging when all
+    // channels are staged at once, else in a pass of its own.
     const bool staged_once = cb_max >= d.C;
+    const uint32_t nsq = RT * 16 + d.N;
+    if (lane == 0) over = 0;
+    if (!staged_once) {
+        for (uint32_t x = lane; x < nsq; x += 64) {
+            double a = 0.0;
+            const uint64_t g = base + x;
+            if (g < d.K)
+                for (uint32_t c = 0; c < d.C; c++) {
+                    const double v = V[(uint64_t)c * d.K + g];
+                    a = __builtin_fma(v, v, a);
+                }
+            s2[x] = a;
+        }
+    }
+    auto window_norms = [&]() {
+        for (uint32_t p = lane; p < RT * 16; p += 64) {
+            double a = 0.0;
+            for (uint32_t x = 0; x < d.N; x++) a += s2[p + x];
+            const double nv = sqrt(a) * (1.0 + 1e-9);
+            float up = (float)nv;
+            if ((double)up < nv) up = __uint_as_float(__float_as_uint(up) + 1u);
+            vnb[p] = up;
+        }
+    };
+    if (!staged_once) {
+        __syncthreads();
+        window_norms();
+    }
+    // Stage channels [c0, c0 + cb): lane owns samples g = lane + 64*slot, and
+    // the loads of up to 16 channels of a sample are issued together.  Copy r
+    // holds sample base + y + r at y.
+    auto stage = [&](uint32_t c0, uint32_t cb) {
+        constexpr uint32_t LS = LA + Tr::kCopies;
+        for (uint32_t g = lane; g < LS; g += 64) {
+            const uint64_t gi = base + g;
+            double a = 0.0;
+            for (uint32_t cc = 0; cc < cb; cc += 16) {
+                float xs[16];
+#pragma unroll
+                for (int q = 0; q < 16; q++)
+                    xs[q] = (cc + q < cb && gi < d.K) ? V[(uint64_t)(c0 + cc + q) * d.K + gi] : 0.0f;
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    if (cc + q >= cb) break;
+                    const float x = xs[q];
+                    if (PREC == 3 && fabsf(x) > kFp8Limit) over = 1;
+                    a = __builtin_fma((double)x, (double)x, a);
+                    const uint32_t qv = to_narrow<PREC>(x);
+                    const uint32_t c = cc + q;
+#pragma unroll
+                    for (int r = 0; r < (int)Tr::kCopies; r++) {
+                        const int y = (int)g - r;
+                        if (y < 0 || y >= (int)LA) continue;
+                        uint8_t *dst = vb + (((uint64_t)r * cb + c) * LA + y) * Tr::kBytes;
+                        if constexpr (Tr::kBytes == 2) *(uint16_t *)dst = (uint16_t)qv;
+                        else *dst = (uint8_t)qv;
+                    }
+                }
+            }
+            if (staged_once && g < nsq) s2[g] = a;
+        }
+    };
+
     bool have = false;
     for (uint32_t ct0 = 0; ct0 < CT; ct0 += CTB) {
         const uint32_t nct = min((uint32_t)CTB, CT - ct0);
@@ -361,35 +405,33 @@ __global__ __launch_bounds__(64) void k_corr_mfma(Dims d, const float *__restric
             const uint32_t cb = min(cb_max, d.C - c0);
             if (!(staged_once && have)) {
                 __syncthreads();
-                // one load per sample; copy r holds sample base + y + r at y
-                const uint32_t LS = LA + Tr::kCopies;
-                for (uint32_t e = lane; e < cb * LS; e += 64) {
-                    const uint32_t c = e / LS, g = e % LS;
-                    const uint64_t gi = base + g;
-                    const float x = gi < d.K ? V[(uint64_t)(c0 + c) * d.K + gi] : 0.0f;
-                    if (PREC == 3 && fabsf(x) > kFp8Limit) over = 1;
-                    const uint32_t q = to_narrow<PREC>(x);
-#pragma unroll
-                    for (int r = 0; r < (int)Tr::kCopies; r++) {
-                        const int y = (int)g - r;
-                        if (y < 0 || y >= (int)LA) continue;
-                        uint8_t *dst = vb + (((uint64_t)r * cb + c) * LA + y) * Tr::kBytes;
-                        if constexpr (Tr::kBytes == 2) *(uint16_t *)dst = (uint16_t)q;
-                        else *dst = (uint8_t)q;
-                    }
-                }
+                stage(c0, cb);
                 __syncthreads();
+                if (staged_once) {
+                    window_norms();
+                    __syncthreads();
+                }
                 have = true;
             }
+            // template fragments of channel c + 1 are fetched during channel c
+            uint64_t bn[KS][CTB];
+            auto fetch_b = [&](uint32_t c) {
+#pragma unroll
+                for (uint32_t ks = 0; ks < KS; ks++)
+#pragma unroll
+                    for (int ct = 0; ct < CTB; ct++)
+                        bn[ks][ct] = ct < (int)nct
+                                         ? TmB[(((uint64_t)(c0 + c) * KS + ks) * CT + ct0 + ct) * 64 + lane]
+                                         : 0ull;
+            };
+            fetch_b(0);
             for (uint32_t c = 0; c < cb; c++) {
                 uint64_t b[KS][CTB];
 #pragma unroll
                 for (uint32_t ks = 0; ks < KS; ks++)
 #pragma unroll
-                    for (int ct = 0; ct < CTB; ct++)
-                        b[ks][ct] = ct < (int)nct
-                                        ? TmB[(((uint64_t)(c0 + c) * KS + ks) * CT + ct0 + ct) * 64 + lane]
-                                        : 0ull;
+                    for (int ct = 0; ct < CTB; ct++) b[ks][ct] = bn[ks][ct];
+                if (c + 1 < cb) fetch_b(c + 1);
 #pragma unroll
                 for (uint32_t ks = 0; ks < KS; ks++) {
 #pragma unroll
