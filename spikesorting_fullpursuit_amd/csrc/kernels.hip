@@ -520,6 +520,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT <= 4 && C
     }
 }
 
+// acc += tp[j * ts] * vp[j] for j = 0..n-1, in order, fused; operands are
+// read in batches of 8 ahead of their multiply-adds.
+__device__ __forceinline__ float fma_chain(float acc, const float *tp, uint32_t ts, const float *vp, uint32_t n)
+{
+    uint32_t j = 0;
+    for (; j + 8 <= n; j += 8) {
+        float a[8], b[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            a[q] = tp[(j + q) * ts];
+            b[q] = vp[j + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc = __builtin_fmaf(a[q], b[q], acc);
+    }
+    for (; j < n; j++) acc = __builtin_fmaf(tp[j * ts], vp[j], acc);
+    return acc;
+}
+
 // Float32 re-score of one (window, template chunk)'s queued entries: one wave,
 // each lane carrying up to kRescoreR chains.  The window's voltage is staged in
 // LDS one channel at a time and template values come from the [C][N][Tp] copy,
@@ -590,12 +609,83 @@ __global__ __launch_bounds__(64) void k_rescore_win(Dims d, const float *__restr
             }
             __syncthreads();
             if (pre && c + 1 < d.C) fetch(c + 1);
-            const float *vp = sv + p;
-            const float *tp = st_tm + t;
-            for (uint32_t j = 0; j < d.N; j++) acc = __builtin_fmaf(tp[j * d.Tp], vp[j], acc);
+            acc = fma_chain(acc, st_tm + t, d.Tp, sv + p, d.N);
         }
         if (ok) L[(uint64_t)t * d.K + base + p] = acc;
     }
+}
+
+// Same re-score with every template resident in LDS: one block per CU holds
+// the [C][N][Tp] copy for its whole life, and each of its waves walks the
+// (window, chunk) descriptors with a stride, staging the window's voltage one
+// channel at a time in a region of its own (wave-level ordering only; the
+// next channel's samples are fetched while the current one runs).  Used when
+// the templates and sixteen voltage rows fit in LDS.
+constexpr uint32_t kRescoreResWaves = 16;
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(1024) void k_rescore_res(Dims d, const float *__restrict__ V,
+                                                      const float *__restrict__ TmT, const float *__restrict__ bias,
+                                                      const uint64_t *__restrict__ rq, const uint2 *__restrict__ rdesc,
+                                                      const uint32_t *__restrict__ win, uint32_t ndesc,
+                                                      uint32_t nchunk, float *L)
+{
+    extern __shared__ float lds[];   // [C][N][Tp] templates, then [waves][span] voltage rows
+    const uint32_t N = d.N, Tp = d.Tp, C = d.C;
+    const uint32_t ntm = C * N * Tp;
+    const uint32_t span = ((N + 15) / 16) * 16 + N;   // window positions (padded) + template length
+    for (uint32_t x = threadIdx.x; x < ntm; x += blockDim.x) lds[x] = TmT[x];
+    __syncthreads();
+    const uint32_t lane = lane_of(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    float *vb = lds + ntm + wave * span;
+    for (uint32_t di = blockIdx.x * nw + wave; di < ndesc; di += gridDim.x * nw) {
+        const uint2 dsc = rdesc[di];
+        const uint32_t cnt = dsc.y;
+        if (cnt == 0) continue;
+        const uint64_t base = (uint64_t)win[di / nchunk] * N;
+        for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
+            const uint32_t e = e0 + lane;
+            const bool ok = e < cnt;
+            const uint64_t ent = ok ? rq[dsc.x + e] : 0;
+            const uint32_t t = ok ? (uint32_t)(ent >> 32) : 0u;
+            const uint32_t p = ok ? (uint32_t)((ent & 0xffffffffull) - base) : 0u;
+            float acc = bias[t];
+            float rv[4];
+            auto fetch = [&](uint32_t c) {
+                const float *vr = V + (uint64_t)c * d.K;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t x = lane + 64u * q;
+                    rv[q] = (x < span && base + x < d.K) ? vr[base + x] : 0.0f;
+                }
+            };
+            fetch(0);
+            for (uint32_t c = 0; c < C; c++) {
+                wave_sync();   // every lane is done with channel c - 1
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t x = lane + 64u * q;
+                    if (x < span) vb[x] = rv[q];
+                }
+                wave_sync();
+                if (c + 1 < C) fetch(c + 1);
+                acc = fma_chain(acc, lds + (uint64_t)c * N * Tp + t, Tp, vb + p, N);
+            }
+            if (ok) L[(uint64_t)t * d.K + base + p] = acc;
+        }
+    }
+}
+
+static size_t rescore_res_lds(const Dims &d)
+{
+    const uint64_t span = ((d.N + 15) / 16) * 16 + d.N;
+    return ((uint64_t)d.C * d.N * d.Tp + kRescoreResWaves * span) * sizeof(float);
 }
 
 // Float32 re-score of the queued (template, position) entries.
@@ -1034,6 +1124,10 @@ __global__ __launch_bounds__(256) void k_overlap_tables(Dims d, const float *__r
     }
 }
 
+// Screen candidates listed per window for the exact scores (beyond that, the
+// screen is traversed a second time).
+constexpr uint32_t kOvCand = 1024;
+
 // Calls fn(f, t, s, e = t*S + s, AB[lab][f][t][s]) for every combination of
 // one window that the reference does not skip (bp.cl:534-576), in no
 // particular order: the main shifts f whose likelihood is positive against
@@ -1269,7 +1363,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     float *sQ = sQp + (MQ ? (qp.KS - 1) * TS : 0);      // [T][S] Q - B2/2
     uint32_t *plist = (uint32_t *)(sQ + TS);            // (t,s) with positive likelihood
     uint32_t *flist = plist + TS;                       // main shifts: positive first, then the rest
-    uint8_t *spos = (uint8_t *)(flist + S);             // [T][S] bit1 in bounds, bit0 likelihood > 0
+    uint32_t *cand = flist + S;                         // [kOvCand] screen candidates (f << 16 | e)
+    uint8_t *spos = (uint8_t *)(cand + kOvCand);        // [T][S] bit1 in bounds, bit0 likelihood > 0
+    __shared__ uint32_t n_cand;
     __shared__ float r_sc[4];
     __shared__ uint64_t r_key[4], r_p[4];
     __shared__ double r_d[4][2];
@@ -1304,7 +1400,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         uint64_t key = ~0ull, bp = 0;
         if (!wskip) {
             __syncthreads();
-            if (threadIdx.x == 0) { n_ppos = 0; n_fpos = 0; n_fneg = 0; }
+            if (threadIdx.x == 0) {
+                n_ppos = 0;
+                n_fpos = 0;
+                n_fneg = 0;
+                n_cand = 0;
+            }
             double vv = 0.0;
             for (uint32_t e = threadIdx.x; e < C * N; e += blockDim.x) {
                 const uint32_t c = e / N, j = e % N;
@@ -1407,7 +1508,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             if (hi_max > (double)bl) {
                 uint32_t scored = 0;
                 const float *tmain = smain;
-                for_each_unskipped<1>(d, spos, flist, nfp, nfn, plist, npp, ABl, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab) {
+                auto rescore = [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab) {
                     double E;
                     const double sh = screen(f, t, e, ab, E);
                     if (sh + E < lo_max) return;
@@ -1416,7 +1517,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     uint64_t p;
                     const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
                     if (key_better(score, kk, sc, key)) { sc = score; key = kk; bp = p; }
+                };
+                // pass B: list the combinations whose upper bound reaches
+                // lo_max, then score them exactly
+                for_each_unskipped<4>(d, spos, flist, nfp, nfn, plist, npp, ABl, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab) {
+                    double E;
+                    const double sh = screen(f, t, e, ab, E);
+                    if (sh + E < lo_max) return;
+                    const uint32_t slot = atomicAdd(&n_cand, 1u);
+                    if (slot < kOvCand) cand[slot] = (f << 16) | e;
                 });
+                __syncthreads();
+                const uint32_t nc = n_cand;
+                if (nc <= kOvCand) {
+                    for (uint32_t k2 = threadIdx.x; k2 < nc; k2 += blockDim.x) {
+                        const uint32_t f = cand[k2] >> 16, e = cand[k2] & 0xffffu;
+                        const uint32_t t = e / S, s = e - t * S;
+                        rescore(f, t, s, e, ABl[(uint64_t)f * TS + e]);
+                    }
+                } else {
+                    for_each_unskipped<1>(d, spos, flist, nfp, nfn, plist, npp, ABl, rescore);
+                }
                 for (int off = 32; off > 0; off >>= 1) scored += __shfl_xor(scored, off);
                 if (lane_of() == 0 && scored) atomicAdd(&ctr_rw->combos, (unsigned long long)scored);
                 block_key_reduce(sc, key, bp, r_sc, r_key, r_p);
@@ -1749,6 +1870,22 @@ hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_wi
     if (e != hipSuccess) return e;
     if (!no_rescore) {
         const uint32_t nchunk = mfma_nchunk(d);
+        const size_t rlds = rescore_res_lds(d);
+        if (rlds <= 150 * 1024) {
+            hipError_t e2 = hipFuncSetAttribute((const void *)k_rescore_res,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds);
+            if (e2 != hipSuccess) return e2;
+            int dev = 0, ncu = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+                ncu = 256;
+            const uint32_t ndesc = n_win * nchunk;
+            const uint32_t need = (ndesc + kRescoreResWaves - 1) / kRescoreResWaves;
+            const uint32_t grid = need < (uint32_t)ncu ? need : (uint32_t)ncu;
+            hipLaunchKernelGGL(k_rescore_res, dim3(grid), dim3(64 * kRescoreResWaves), rlds, st, d, s.V, s.TmT,
+                               s.bias, s.rq, (const uint2 *)s.rdesc, s.win, ndesc, nchunk, s.L);
+            return hipGetLastError();
+        }
         const size_t lds = (size_t)d.N * d.Tp * sizeof(float);
         if (lds > 64 * 1024) {
             hipError_t e2 = hipFuncSetAttribute((const void *)k_rescore_win,
@@ -1795,7 +1932,7 @@ static size_t overlap_screened_lds_mode(const Dims &d, bool mq)
     const QPlan qp = q_plan(d);
     return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + (uint64_t)d.C * (d.N + 2 * d.M) * 4 +
            (mq ? (uint64_t)d.C * qp.LPH * 4 + (uint64_t)(qp.KS - 1) * ts * 4 : 0) + ts * 4 + ts * 4 +
-           (uint64_t)d.S * 4 + ts + 16;
+           (uint64_t)d.S * 4 + (uint64_t)kOvCand * 4 + ts + 16;
 }
 
 // The matrix-core Q table is used whenever its padded voltage fits in LDS.
