@@ -79,8 +79,12 @@ typedef struct fbp_config {
 
 /* Test-only switches. */
 enum {
-    FBP_DEBUG_NO_RESCORE = 1  /* low-precision paths keep the screened likelihood
-                                 (no float32 re-score): checks the error bound */
+    FBP_DEBUG_NO_RESCORE = 1,      /* low-precision paths keep the screened likelihood
+                                      (no float32 re-score): checks the error bound */
+    FBP_DEBUG_OVERLAP_FULL = 2,    /* screened overlap: find the exact-score
+                                      candidates by a second full traversal     */
+    FBP_DEBUG_OVERLAP_GLOBAL = 4   /* screened overlap: exact scores read the
+                                      partner template from global memory       */
 };
 
 typedef struct fbp_stats {

@@ -103,7 +103,8 @@ size_t overlap_screened_lds(const Dims &d);
 uint64_t overlap_qtemplate_floats(const Dims &d);
 hipError_t launch_pad_templates(const Dims &d, const DeviceState &s, hipStream_t st);
 hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_t st);
-hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);
+hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t max_ov, uint32_t debug,
+                                  hipStream_t st);
 hipError_t launch_check_overlap(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);
 hipError_t launch_pursuit_add(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
 hipError_t launch_compact_spikes(const Dims &d, const DeviceState &s, uint32_t n_win,

@@ -974,11 +974,27 @@ __device__ __forceinline__ float overlap_direct_score(const Dims &d, const float
         const float *b = bt + (uint64_t)c * R + 2 * d.M - s;
         const float *vc = sv + c * d.N;
         float ss = 0.0f;
-#pragma unroll 16
-        for (uint32_t j = 0; j < d.N; j++) {
+        // operands are loaded eight samples at a time ahead of the chains
+        uint32_t j = 0;
+        for (; j + 8 <= d.N; j += 8) {
+            float av[8], bv[8], vv[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                av[q] = a[j + q];
+                bv[q] = b[j + q];
+                vv[q] = vc[j + q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const float sum = av[q] + bv[q];
+                ss = __builtin_fmaf(sum, sum, ss);       // bp.cl:621
+                acc = __builtin_fmaf(sum, vv[q], acc);   // bp.cl:622
+            }
+        }
+        for (; j < d.N; j++) {
             const float sum = a[j] + b[j];
-            ss = __builtin_fmaf(sum, sum, ss);       // bp.cl:621
-            acc = __builtin_fmaf(sum, vc[j], acc);   // bp.cl:622
+            ss = __builtin_fmaf(sum, sum, ss);
+            acc = __builtin_fmaf(sum, vc[j], acc);
         }
         if (ss > 0.0f) acc = (float)((double)acc - 0.5 * (double)ss);   // bp.cl:627-630
     }
@@ -1124,23 +1140,52 @@ __global__ __launch_bounds__(256) void k_overlap_tables(Dims d, const float *__r
     }
 }
 
-// Screen candidates listed per window for the exact scores (beyond that, the
-// screen is traversed a second time).
-constexpr uint32_t kOvCand = 1024;
+// Block-wide staging loop: for x in [0, n) with the block's threads striding,
+// U loads are issued before their U stores, so their latencies overlap.
+template <int U, class Ld, class St>
+__device__ __forceinline__ void batched_stage(uint32_t n, Ld ld, St st)
+{
+    for (uint32_t b0 = threadIdx.x; b0 < n; b0 += U * blockDim.x) {
+        float v[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const uint32_t x = b0 + q * blockDim.x;
+            v[q] = x < n ? ld(x) : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const uint32_t x = b0 + q * blockDim.x;
+            if (x < n) st(x, v[q]);
+        }
+    }
+}
 
-// Calls fn(f, t, s, e = t*S + s, AB[lab][f][t][s]) for every combination of
-// one window that the reference does not skip (bp.cl:534-576), in no
-// particular order: the main shifts f whose likelihood is positive against
-// every in-bounds (t, s) within N of f, then the other in-bounds main shifts
-// against the (t, s) whose likelihood is positive.  Each thread fetches the AB
-// values of B combinations before it evaluates them, so their latencies
-// overlap.  Quotients advance incrementally; no division per element.
+// x / d as floor(x * ceil(2^32 / d) / 2^32): exact while x * d < 2^32.
+struct FastDiv {
+    uint64_t m;
+    uint32_t d;
+    __device__ explicit FastDiv(uint32_t dv) : m(((1ull << 32) + dv - 1) / dv), d(dv) {}
+    __device__ uint32_t div(uint32_t x) const { return (uint32_t)(((uint64_t)x * m) >> 32); }
+};
+
+// Screen candidates listed per window for the exact scores: four per thread.
+constexpr uint32_t kOvCand = 4 * 256;
+
+// Calls fn(f, t, s, e = t*S + s, AB[lab][f][t][s], sQ[e], sE[t]) for every
+// combination of one window that the reference does not skip (bp.cl:534-576),
+// in no particular order: the main shifts f whose likelihood is positive
+// against every in-bounds (t, s) within N of f, then the other in-bounds main
+// shifts against the (t, s) whose likelihood is positive.  Each thread loads
+// the operands of B combinations (global AB, LDS sQ, sE, spos) before it
+// evaluates them, so their latencies overlap.  Quotients advance
+// incrementally; no division per element.
 template <int B, class Fn>
 __device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t *spos, const uint32_t *flist,
                                                    uint32_t n_fpos, uint32_t n_fneg, const uint32_t *plist,
-                                                   uint32_t n_ppos, const float *__restrict__ ABl, Fn fn)
+                                                   uint32_t n_ppos, const float *__restrict__ ABl,
+                                                   const float *sQ, const double *sE, Fn fn)
 {
-    const uint32_t S = d.S, N = d.N, TS = d.T * d.S;
+    const uint32_t S = d.S, N = d.N, T = d.T, TS = d.T * d.S;
     const uint32_t step = blockDim.x;
     const uint32_t stT = step / S, stS = step - stT * S;
     const uint32_t t00 = threadIdx.x / S, s00 = threadIdx.x - t00 * S;
@@ -1151,19 +1196,28 @@ __device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t 
         const float *abf = ABl + (uint64_t)f * TS;
         uint32_t t = t00, s = s00;
         for (uint32_t e0 = threadIdx.x; e0 < TS; e0 += B * step) {
-            float ab[B];
+            float ab[B], qe[B];
+            double Et[B];
+            uint32_t tq[B], sq[B];
+            uint8_t sp[B];
 #pragma unroll
             for (int q = 0; q < B; q++) {
                 const uint32_t e = e0 + q * step;
-                ab[q] = e < TS ? abf[e] : 0.0f;
-            }
-#pragma unroll
-            for (int q = 0; q < B; q++) {
-                const uint32_t e = e0 + q * step;
-                if (e < TS && (spos[e] & 2) && (f > s ? f - s : s - f) <= N) fn(f, t, s, e, ab[q]);
+                const bool ok = e < TS;
+                tq[q] = t;
+                sq[q] = s;
+                ab[q] = ok ? abf[e] : 0.0f;
+                sp[q] = ok ? spos[e] : (uint8_t)0;
+                qe[q] = ok ? sQ[e] : 0.0f;
+                Et[q] = (ok && t < T) ? sE[t] : 0.0;
                 s += stS;
                 t += stT;
                 if (s >= S) { s -= S; t++; }
+            }
+#pragma unroll
+            for (int q = 0; q < B; q++) {
+                const uint32_t e = e0 + q * step, sv = sq[q];
+                if ((sp[q] & 2) && (f > sv ? f - sv : sv - f) <= N) fn(f, tq[q], sv, e, ab[q], qe[q], Et[q]);
             }
         }
     }
@@ -1172,14 +1226,18 @@ __device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t 
     const uint32_t stA = step / n_ppos, stB = step - stA * n_ppos;
     uint32_t ra = threadIdx.x / n_ppos, rb = threadIdx.x - ra * n_ppos;
     for (uint32_t r0 = threadIdx.x; r0 < total; r0 += B * step) {
-        float ab[B];
-        uint32_t ff[B], ee[B];
+        float ab[B], qe[B];
+        double Et[B];
+        uint32_t ff[B], ee[B], tq[B];
 #pragma unroll
         for (int q = 0; q < B; q++) {
             const bool ok = r0 + q * step < total;
             ff[q] = ok ? flist[n_fpos + ra] : 0u;
             ee[q] = ok ? plist[rb] : 0u;
+            tq[q] = (uint32_t)(((uint64_t)ee[q] * mS) >> 32);
             ab[q] = ok ? ABl[(uint64_t)ff[q] * TS + ee[q]] : 0.0f;
+            qe[q] = ok ? sQ[ee[q]] : 0.0f;
+            Et[q] = ok ? sE[tq[q]] : 0.0;
             rb += stB;
             ra += stA;
             if (rb >= n_ppos) { rb -= n_ppos; ra++; }
@@ -1187,9 +1245,8 @@ __device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t 
 #pragma unroll
         for (int q = 0; q < B; q++) {
             if (r0 + q * step >= total) break;
-            const uint32_t f = ff[q], e = ee[q];
-            const uint32_t t = (uint32_t)(((uint64_t)e * mS) >> 32), s = e - t * S;
-            if ((f > s ? f - s : s - f) <= N) fn(f, t, s, e, ab[q]);
+            const uint32_t f = ff[q], e = ee[q], t = tq[q], s = e - t * S;
+            if ((f > s ? f - s : s - f) <= N) fn(f, t, s, e, ab[q], qe[q], Et[q]);
         }
     }
 }
@@ -1347,13 +1404,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw, float *best_L,
     const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov,
     const float *__restrict__ AB, const float *__restrict__ B2, const float *__restrict__ tnorm,
-    const uint4 *__restrict__ TmQ, const float *__restrict__ Tpad)
+    const uint4 *__restrict__ TmQ, const float *__restrict__ Tpad, uint32_t debug)
 {
     extern __shared__ float dyn[];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
     const uint32_t TS = T * S;
     const QPlan qp = q_plan(d);
     const uint32_t LPH = qp.LPH;
+    const FastDiv divN(N), divLPH(LPH), divS(S);
     double *sE = (double *)dyn;                         // [T] screen bound per template (this window)
     float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset
     float *smain = sv + C * N;                          // [C][N + 2M] padded template of the best label
@@ -1406,35 +1464,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 n_fneg = 0;
                 n_cand = 0;
             }
+            // staging (index quotients by FastDiv: every index is below the
+            // LDS size)
+            const uint32_t R = N + 2 * M;
             double vv = 0.0;
-            for (uint32_t e = threadIdx.x; e < C * N; e += blockDim.x) {
-                const uint32_t c = e / N, j = e % N;
-                const float x = V[(uint64_t)c * d.K + bidx + j];
-                sv[e] = x;
-                vv = __builtin_fma((double)x, (double)x, vv);
-            }
-            for (uint32_t e = threadIdx.x; e < C * (N + 2 * M); e += blockDim.x)
-                smain[e] = Tpad[(uint64_t)lab * C * (N + 2 * M) + e];
+            batched_stage<8>(C * N, [&](uint32_t e) { const uint32_t c = divN.div(e);
+                                                       return V[(uint64_t)c * d.K + bidx + (e - c * N)]; },
+                             [&](uint32_t e, float x) { sv[e] = x; vv = __builtin_fma((double)x, (double)x, vv); });
+            batched_stage<8>(C * R, [&](uint32_t e) { return Tpad[(uint64_t)lab * C * R + e]; },
+                             [&](uint32_t e, float x) { smain[e] = x; });
             if (MQ)
-                for (uint32_t e = threadIdx.x; e < C * LPH; e += blockDim.x) {
-                    const uint32_t c = e / LPH, y = e % LPH;
-                    const float x = (y >= M && y - M < N) ? V[(uint64_t)c * d.K + bidx + (y - M)] : 0.0f;
+                batched_stage<8>(C * LPH, [&](uint32_t e) {
+                    const uint32_t c = divLPH.div(e), y = e - c * LPH;
+                    return (y >= M && y - M < N) ? V[(uint64_t)c * d.K + bidx + (y - M)] : 0.0f;
+                }, [&](uint32_t e, float x) {
                     const uint32_t h = to_bf16(x);
                     svh[e] = (uint16_t)h;
                     svl[e] = to_bf16(x - __uint_as_float(h << 16));
-                }
+                });
             for (int off = 32; off > 0; off >>= 1) vv += __shfl_xor(vv, off);
             if (lane_of() == 0) r_vv[threadIdx.x >> 6] = vv;
             __syncthreads();
-            for (uint32_t e = threadIdx.x; e < TS; e += blockDim.x) {
-                const uint32_t t = e / S, s = e % S;
+            // likelihood signs at every in-bounds (t, s); the loaded value is
+            // -1 where (t, s) is out of bounds
+            batched_stage<8>(TS, [&](uint32_t e) {
+                const uint32_t t = divS.div(e), s = e - t * S;
                 const bool inb = !((uint64_t)s + bidx < M || (uint64_t)s + bidx >= (uint64_t)d.K - N + M);   // bp.cl:534,538
-                const bool pos = inb && L[(uint64_t)t * d.K + bidx + s - M] > 0.0f;
+                return inb ? L[(uint64_t)t * d.K + bidx + s - M] : -1.0f;
+            }, [&](uint32_t e, float lv) {
+                const uint32_t t = divS.div(e), s = e - t * S;
+                const bool inb = !((uint64_t)s + bidx < M || (uint64_t)s + bidx >= (uint64_t)d.K - N + M);
+                const bool pos = inb && lv > 0.0f;
                 spos[e] = (inb ? 2 : 0) | (pos ? 1 : 0);
                 if (pos) plist[atomicAdd(&n_ppos, 1u)] = e;
-                if (t == lab && inb) {
-                    if (pos) flist[atomicAdd(&n_fpos, 1u)] = s;
-                }
+                if (t == lab && pos) flist[atomicAdd(&n_fpos, 1u)] = s;
                 if (!MQ) {
                     const int32_t sa = (int32_t)M - (int32_t)s;
                     const int32_t lo = max(0, -sa), hi = min((int32_t)N, (int32_t)N - sa);
@@ -1446,7 +1509,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     }
                     sQ[e] = (float)q;
                 }
-            }
+            });
             if (MQ) {
                 __syncthreads();
                 q_table_split(d, qp, TmQ, svh, svl, sQ, sQp);
@@ -1463,8 +1526,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             const double vn = sqrt(r_vv[0] + r_vv[1] + r_vv[2] + r_vv[3]) * (1.0 + 1e-12);
             const double nlab = tnorm[lab];
             // fold the voltage-free half norms into Q, and the bound per template
-            for (uint32_t e = threadIdx.x; e < TS; e += blockDim.x)
-                sQ[e] = (float)((double)sQ[e] - 0.5 * (double)B2[e]);
+            batched_stage<8>(TS, [&](uint32_t e) { return B2[e]; },
+                             [&](uint32_t e, float b2) { sQ[e] = (float)((double)sQ[e] - 0.5 * (double)b2); });
             for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
                 const double nsum = nlab + (double)tnorm[t];
                 sE[t] = ecoef * (nsum * vn + nsum * nsum) + eabs;
@@ -1480,19 +1543,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             const float *ABl = AB + (uint64_t)lab * S * TS;
             const float *sQl = sQ + lab * S;
             const uint32_t nfp = n_fpos, nfn = n_fneg, npp = n_ppos;
-            auto screen = [&](uint32_t f, uint32_t t, uint32_t e, float ab, double &E) {
-                E = sE[t];
-                return (double)sQl[f] + (double)sQ[e] - (double)ab;
+            auto screen = [&](uint32_t f, float qe, float ab) {
+                return (double)sQl[f] + (double)qe - (double)ab;
             };
-            // pass A: largest lower bound and largest upper bound of the score
-            double lo_max = -INFINITY, hi_max = -INFINITY;
+            // pass A: largest lower and upper bounds of the score.  Each thread
+            // also keeps its four combinations with the largest upper bounds
+            // (kv0 >= kv1 >= kv2 >= kv3, packed f << 16 | e in kc*) and the
+            // largest upper bound it did not keep (kdrop).
+            double lo_max = -INFINITY, hi_max = -INFINITY, kdrop = -INFINITY;
+            double kv0 = -INFINITY, kv1 = -INFINITY, kv2 = -INFINITY, kv3 = -INFINITY;
+            uint32_t kc0 = 0, kc1 = 0, kc2 = 0, kc3 = 0;
             uint32_t screened = 0;
-            for_each_unskipped<4>(d, spos, flist, nfp, nfn, plist, npp, ABl, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab) {
-                double E;
-                const double sh = screen(f, t, e, ab, E);
+            for_each_unskipped<4>(d, spos, flist, nfp, nfn, plist, npp, ABl, sQ, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
+                const double sh = screen(f, qe, ab);
+                const double hi = sh + E;
                 lo_max = fmax(lo_max, sh - E);
-                hi_max = fmax(hi_max, sh + E);
+                hi_max = fmax(hi_max, hi);
                 screened++;
+                if (hi > kv3) {
+                    kdrop = fmax(kdrop, kv3);
+                    kv3 = hi;
+                    kc3 = (f << 16) | e;
+                    if (kv3 > kv2) { const double x = kv2; kv2 = kv3; kv3 = x; const uint32_t y = kc2; kc2 = kc3; kc3 = y; }
+                    if (kv2 > kv1) { const double x = kv1; kv1 = kv2; kv2 = x; const uint32_t y = kc1; kc1 = kc2; kc2 = y; }
+                    if (kv1 > kv0) { const double x = kv0; kv0 = kv1; kv1 = x; const uint32_t y = kc0; kc0 = kc1; kc1 = y; }
+                } else {
+                    kdrop = fmax(kdrop, hi);
+                }
             });
             for (int off = 32; off > 0; off >>= 1) {
                 lo_max = fmax(lo_max, __shfl_xor(lo_max, off));
@@ -1508,35 +1585,53 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             if (hi_max > (double)bl) {
                 uint32_t scored = 0;
                 const float *tmain = smain;
-                auto rescore = [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab) {
-                    double E;
-                    const double sh = screen(f, t, e, ab, E);
-                    if (sh + E < lo_max) return;
-                    const float score = overlap_direct_score(d, sv, tmain, Tpad + (uint64_t)t * C * (N + 2 * M), f, s);
+                auto exact = [&](uint32_t f, uint32_t t, uint32_t s, const float *bt) {
+                    const float score = overlap_direct_score(d, sv, tmain, bt, f, s);
                     scored++;
                     uint64_t p;
                     const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
                     if (key_better(score, kk, sc, key)) { sc = score; key = kk; bp = p; }
                 };
-                // pass B: list the combinations whose upper bound reaches
-                // lo_max, then score them exactly
-                for_each_unskipped<4>(d, spos, flist, nfp, nfn, plist, npp, ABl, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab) {
-                    double E;
-                    const double sh = screen(f, t, e, ab, E);
-                    if (sh + E < lo_max) return;
-                    const uint32_t slot = atomicAdd(&n_cand, 1u);
-                    if (slot < kOvCand) cand[slot] = (f << 16) | e;
-                });
-                __syncthreads();
-                const uint32_t nc = n_cand;
-                if (nc <= kOvCand) {
-                    for (uint32_t k2 = threadIdx.x; k2 < nc; k2 += blockDim.x) {
-                        const uint32_t f = cand[k2] >> 16, e = cand[k2] & 0xffffu;
-                        const uint32_t t = e / S, s = e - t * S;
-                        rescore(f, t, s, e, ABl[(uint64_t)f * TS + e]);
+                // Exact scores of every combination with sh + E >= lo_max
+                // (bp.cl:599-640).  They are all among the kept ones unless a
+                // thread dropped one that reaches lo_max; then the screen is
+                // traversed again.
+                if (__syncthreads_or(kdrop >= lo_max || (debug & 2u)) == 0) {
+                    auto push = [&](double v, uint32_t c) {
+                        if (v >= lo_max) cand[atomicAdd(&n_cand, 1u)] = c;
+                    };
+                    push(kv0, kc0);
+                    push(kv1, kc1);
+                    push(kv2, kc2);
+                    push(kv3, kc3);
+                    __syncthreads();
+                    const uint32_t nc = n_cand;   // <= 4 * blockDim.x = kOvCand
+                    if (MQ && nc <= 16 && !(debug & 4u)) {
+                        // one at a time, the partner template staged in LDS
+                        float *sb = (float *)svh;   // [C][N + 2M] (the Q-table voltage is no longer needed)
+                        for (uint32_t k2 = 0; k2 < nc; k2++) {
+                            const uint32_t f = cand[k2] >> 16, e = cand[k2] & 0xffffu;
+                            const uint32_t t = e / S, s = e - t * S;
+                            const float *src = Tpad + (uint64_t)t * C * R;
+                            __syncthreads();
+                            batched_stage<8>(C * R, [&](uint32_t x) { return src[x]; },
+                                             [&](uint32_t x, float v) { sb[x] = v; });
+                            __syncthreads();
+                            if (threadIdx.x == 0) exact(f, t, s, sb);
+                        }
+                    } else {
+                        for (uint32_t k2 = threadIdx.x; k2 < nc; k2 += blockDim.x) {
+                            const uint32_t f = cand[k2] >> 16, e = cand[k2] & 0xffffu;
+                            const uint32_t t = e / S, s = e - t * S;
+                            exact(f, t, s, Tpad + (uint64_t)t * C * R);
+                        }
                     }
                 } else {
-                    for_each_unskipped<1>(d, spos, flist, nfp, nfn, plist, npp, ABl, rescore);
+                    for_each_unskipped<1>(d, spos, flist, nfp, nfn, plist, npp, ABl, sQ, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
+                        const double sh = screen(f, qe, ab);
+                        if (sh + E < lo_max) return;
+                        exact(f, t, s, Tpad + (uint64_t)t * C * R);
+                    });
                 }
                 for (int off = 32; off > 0; off >>= 1) scored += __shfl_xor(scored, off);
                 if (lane_of() == 0 && scored) atomicAdd(&ctr_rw->combos, (unsigned long long)scored);
@@ -1963,7 +2058,8 @@ hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_
     return hipGetLastError();
 }
 
-hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st)
+hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t max_ov, uint32_t debug,
+                                  hipStream_t st)
 {
     if (max_ov == 0) return hipSuccess;
     const bool mq = overlap_uses_mfma_q(d);
@@ -1977,11 +2073,11 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
     if (mq)
         hipLaunchKernelGGL(k_overlap_screened<true>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                            s.ov_list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                           s.tabB2, s.tnorm, (const uint4 *)s.TmQ, s.Tpad);
+                           s.tabB2, s.tnorm, (const uint4 *)s.TmQ, s.Tpad, debug);
     else
         hipLaunchKernelGGL(k_overlap_screened<false>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                            s.ov_list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                           s.tabB2, s.tnorm, (const uint4 *)s.TmQ, s.Tpad);
+                           s.tabB2, s.tnorm, (const uint4 *)s.TmQ, s.Tpad, debug);
     return hipGetLastError();
 }
 

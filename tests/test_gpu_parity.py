@@ -76,6 +76,27 @@ def test_dense_16ch_spike_list_grows(oracle_mod, mode):
         assert g['stats']['screened_combos'] > 20 * g['stats']['overlap_combos']
 
 
+@pytest.mark.parametrize("debug", [2, 4, 6])
+def test_screened_overlap_candidate_paths(oracle_mod, debug):
+    # The screened overlap finds its exact-score candidates from per-thread
+    # lists (partner templates staged in LDS) unless forced onto the second
+    # traversal (2) or global partner reads (4); every route gives the oracle's
+    # events.
+    from spikesorting_fullpursuit_amd import synthetic
+    rec = synthetic.config("16ch-300s", seed=7, duration_s=0.5)
+    sm = rec['separability_metrics']
+    rec['bias'] = np.float32(-0.5 * sm['template_SS'])
+    rec['thr'] = np.float32(sm['neuron_lower_thresholds'])
+    rec['N'] = 61
+    print("case 16ch 0.5s debug", debug, flush=True)
+    g = run_gpu(rec, 30, True, 256, precision="bf16", overlap_mode="screened", debug_flags=debug)
+    o = run_oracle(oracle_mod, rec, 30, True, 256)
+    np.testing.assert_array_equal(g['idx'], o['idx'])
+    np.testing.assert_array_equal(g['lab'], o['lab'])
+    np.testing.assert_array_equal(g['residual'].view(np.uint32), o['residual'].view(np.uint32))
+    assert g['stats']['overlap_combos'] > 0
+
+
 @pytest.mark.parametrize("precision", ["bf16", "fp16", "fp8"])
 @pytest.mark.parametrize("seed,C,T,N,dur", [(31, 4, 5, 61, 0.5), (32, 8, 20, 41, 0.3), (33, 3, 3, 81, 0.3)])
 def test_screen_error_bound_holds(precision, seed, C, T, N, dur):
