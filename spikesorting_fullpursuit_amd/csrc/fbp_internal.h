@@ -55,6 +55,7 @@ struct DeviceState {
     uint32_t *win;       // [W] window list of the pass
     uint32_t *win_next;  // [W] window list of the next pass
     uint32_t *ov_list;   // [W] overlap windows of the pass
+    uint32_t *ov_sorted; // [W] the same, grouped by best label (screened overlap)
     uint8_t *has;        // [W] add decision per entry of the pass list
     uint32_t *cand_idx;  // [W] spike onset per entry of the pass list
     uint32_t *cand_lab;  // [W]

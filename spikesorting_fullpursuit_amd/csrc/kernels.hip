@@ -2006,6 +2006,37 @@ hipError_t launch_compact_ov(const Dims &d, const DeviceState &s, uint32_t n_win
     return compact(OvPred{s.win, s.ov}, OvEmit{s.win, s.ov_list}, n_win, s.blk, &s.ctr->n_ov, st);
 }
 
+// Overlap windows grouped by best label (counting sort, one block).  Blocks
+// that run at the same time then mostly share a label, so the label's slice of
+// the AB table and its padded template stay in each XCD's L2.  The order of
+// the windows does not change any result.
+constexpr uint32_t kSortMaxLabels = 8192;
+
+__global__ __launch_bounds__(1024) void k_sort_ov_by_label(Dims d, const uint32_t *__restrict__ ov_list,
+                                                           const Counters *__restrict__ ctr,
+                                                           const uint32_t *__restrict__ best_lab, uint32_t *ov_sorted)
+{
+    __shared__ uint32_t cnt[kSortMaxLabels];
+    const uint32_t n = ctr->n_ov, T = d.T;
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) cnt[t] = 0;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) atomicAdd(&cnt[best_lab[ov_list[k]]], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t t = 0; t < T; t++) {
+            const uint32_t c = cnt[t];
+            cnt[t] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        const uint32_t id = ov_list[k];
+        ov_sorted[atomicAdd(&cnt[best_lab[id]], 1u)] = id;
+    }
+}
+
 hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st)
 {
     if (max_ov == 0) return hipSuccess;
@@ -2070,13 +2101,19 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
         if (e != hipSuccess) return e;
     }
     const uint32_t grid = max_ov < 4096 ? max_ov : 4096;
+    const uint32_t *list = s.ov_list;
+    if (d.T <= kSortMaxLabels) {
+        hipLaunchKernelGGL(k_sort_ov_by_label, dim3(1), dim3(1024), 0, st, d, s.ov_list, s.ctr, s.best_lab,
+                           s.ov_sorted);
+        list = s.ov_sorted;
+    }
     if (mq)
         hipLaunchKernelGGL(k_overlap_screened<true>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
-                           s.ov_list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
+                           list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
                            s.tabB2, s.tnorm, (const uint4 *)s.TmQ, s.Tpad, debug);
     else
         hipLaunchKernelGGL(k_overlap_screened<false>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
-                           s.ov_list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
+                           list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
                            s.tabB2, s.tnorm, (const uint4 *)s.TmQ, s.Tpad, debug);
     return hipGetLastError();
 }
