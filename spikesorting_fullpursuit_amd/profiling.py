@@ -19,7 +19,7 @@ import re
 
 
 def kernel_name(raw):
-    s = raw.replace("void ", "").replace("fbp::(anonymous namespace)::", "")
+    s = raw.replace("void ", "").replace("fbp::(anonymous namespace)::", "").replace("fbp::", "")
     m = re.match(r"([A-Za-z_0-9]+)(<[^(]*>)?", s)
     return (m.group(1) + (m.group(2) or "")) if m else s
 
