@@ -90,6 +90,7 @@ hipError_t launch_transpose_templates(const Dims &d, const float *Tm, const floa
 
 hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
 uint32_t mfma_kstep(uint32_t precision);
+uint32_t mfma_words(uint32_t precision);   // 8-byte words per template fragment
 uint32_t mfma_ks(const Dims &d, uint32_t precision);
 uint32_t mfma_ct(const Dims &d);
 uint32_t mfma_nchunk(const Dims &d);

@@ -188,16 +188,27 @@ __device__ __forceinline__ uint16_t to_f16(float x)
 }
 
 // Element traits of the three screens: bf16 and fp16 use
-// v_mfma_f32_16x16x16{bf16,f16} (k = 16, 4 two-byte elements per lane), fp8
-// e4m3 uses v_mfma_f32_16x16x32_fp8_fp8 (k = 32, 8 one-byte elements per
-// lane).  Both read one aligned 8-byte fragment per lane.
+// v_mfma_f32_16x16x32{bf16,f16} (k = 32, 8 two-byte elements per lane: two
+// aligned 8-byte LDS reads from four shifted voltage copies), fp8 e4m3 uses
+// v_mfma_f32_16x16x32_fp8_fp8 (k = 32, 8 one-byte elements per lane: one
+// aligned 8-byte read from eight copies).  Lane l holds row (A) or column (B)
+// l % 16 at k = 8 * (l / 16) + 0..7.
 template <int PREC>
 struct NarrowTraits {
     static constexpr uint32_t kBytes = PREC == 3 ? 1 : 2;   // bytes per element
-    static constexpr uint32_t kK = PREC == 3 ? 32 : 16;       // k per MFMA
-    static constexpr uint32_t kPer = 8 / kBytes;              // elements per lane fragment
-    static constexpr uint32_t kCopies = kPer;                  // shifted voltage copies in LDS
+    static constexpr uint32_t kK = 32;                        // k per MFMA
+    static constexpr uint32_t kPer = 8;                       // elements per lane fragment
+    static constexpr uint32_t kWords = kPer * kBytes / 8;     // 8-byte words per fragment
+    static constexpr uint32_t kCopies = 8 / kBytes;           // shifted voltage copies in LDS
 };
+
+template <int PREC>
+struct Frag {
+    uint64_t w[NarrowTraits<PREC>::kWords];
+};
+
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8v __attribute__((ext_vector_type(8)));
 
 // Round a float32 to the screen's element format (round to nearest even).
 // fp8 returns the e4m3 byte in the low 8 bits.
@@ -210,29 +221,37 @@ __device__ __forceinline__ uint32_t to_narrow(float x)
 }
 
 template <int PREC>
-__device__ __forceinline__ float4v mfma_narrow(uint64_t a, uint64_t b, float4v c)
+__device__ __forceinline__ float4v mfma_narrow(const Frag<PREC> &a, const Frag<PREC> &b, float4v c)
 {
     if constexpr (PREC == 1) {
-        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4v, a), __builtin_bit_cast(short4v, b),
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8v, a), __builtin_bit_cast(bf16x8v, b),
                                                         c, 0, 0, 0);
     } else if constexpr (PREC == 2) {
-        return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(half4v, a), __builtin_bit_cast(half4v, b),
-                                                      c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(halfx8v, a), __builtin_bit_cast(halfx8v, b),
+                                                       c, 0, 0, 0);
     } else {
-        return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)a, (long)b, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)a.w[0], (long)b.w[0], c, 0, 0, 0);
     }
+}
+
+// v_mfma_f32_16x16x16bf16_1k: lane l holds row (A) or column (B) l % 16 at
+// k = 4 * (l / 16) + 0..3 (the split-bf16 Q table of the overlap screen).
+__device__ __forceinline__ float4v mfma_bf16_k16(uint64_t a, uint64_t b, float4v c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4v, a), __builtin_bit_cast(short4v, b),
+                                                    c, 0, 0, 0);
 }
 
 // Largest |value| the fp8 screen feeds to the matrix cores unscaled; a window
 // holding a larger voltage sample has all its entries re-scored.
 constexpr float kFp8Limit = 440.0f;
 
-// TmB[((c*KS + ks)*CT + ct)*64 + lane] = the 8-byte fragment of template
-// t = ct*16 + lane%16 at samples j = ks*K + per*(lane/16) + 0..per-1 (0 when
-// padded), scaled by tscale[t] (a power of two; 1 except for fp8).
+// TmB[((c*KS + ks)*CT + ct)*64 + lane] = the fragment (kWords 8-byte words) of
+// template t = ct*16 + lane%16 at samples j = ks*K + per*(lane/16) + 0..per-1
+// (0 when padded), scaled by tscale[t] (a power of two; 1 except for fp8).
 template <int PREC>
 __global__ void k_pack_templates(Dims d, const float *__restrict__ Tm, const float *__restrict__ tscale,
-                                 uint64_t *__restrict__ TmB, uint32_t KS, uint32_t CT)
+                                 Frag<PREC> *__restrict__ TmB, uint32_t KS, uint32_t CT)
 {
     using Tr = NarrowTraits<PREC>;
     const uint64_t total = (uint64_t)d.C * KS * CT * 64;
@@ -245,11 +264,13 @@ __global__ void k_pack_templates(Dims d, const float *__restrict__ Tm, const flo
         const uint32_t c = (uint32_t)(r / ((uint64_t)CT * KS));
         const uint32_t t = ct * 16 + (lane & 15);
         const float sc = t < d.T ? tscale[t] : 1.0f;
-        uint64_t v = 0;
+        Frag<PREC> v;
+        for (uint32_t w = 0; w < Tr::kWords; w++) v.w[w] = 0;
         for (uint32_t q = 0; q < Tr::kPer; q++) {
             const uint32_t j = ks * Tr::kK + Tr::kPer * (lane >> 4) + q;
             const float x = (t < d.T && j < d.N) ? Tm[((uint64_t)t * d.C + c) * d.N + j] * sc : 0.0f;
-            v |= (uint64_t)to_narrow<PREC>(x) << (8 * Tr::kBytes * q);
+            const uint32_t bit = 8 * Tr::kBytes * q;
+            v.w[bit / 64] |= (uint64_t)to_narrow<PREC>(x) << (bit % 64);
         }
         TmB[e] = v;
     }
@@ -303,8 +324,8 @@ __device__ __forceinline__ float exact_likelihood(const Dims &d, const float *__
 constexpr int kMfmaMaxRT = 6;   // row tiles per window: N <= 96
 
 template <int PREC, int RT, int CTB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT <= 4 && CTB <= 2 ? 3 : (RT * CTB > 16 ? 1 : 2)))) void k_corr_mfma(Dims d, const float *__restrict__ V, const float *__restrict__ Tm,
-                                                  const uint64_t *__restrict__ TmB, const float *__restrict__ bias,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 && RT <= 4 && CTB <= 2 ? 3 : (RT * CTB > 16 ? 1 : 2)))) void k_corr_mfma(Dims d, const float *__restrict__ V, const float *__restrict__ Tm,
+                                                  const Frag<PREC> *__restrict__ TmB, const float *__restrict__ bias,
                                                   const float *__restrict__ tnorm, const float *__restrict__ tscale,
                                                   const uint32_t *__restrict__ win, float *__restrict__ L,
                                                   Counters *ctr, uint64_t *__restrict__ rq, uint32_t rq_cap,
@@ -361,35 +382,47 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT <= 4 && C
     // Stage channels [c0, c0 + cb): lane owns samples g = lane + 64*slot, and
     // the loads of up to 16 channels of a sample are issued together.  Copy r
     // holds sample base + y + r at y.
+    // Samples at or beyond np + N - 1 meet only zero template weights or
+    // discarded positions: they are staged as 0 without a load.  Two 64-sample
+    // slots are loaded together, up to 16 channels at a time.
+    const uint32_t gmax = np + d.N - 1;
     auto stage = [&](uint32_t c0, uint32_t cb) {
         constexpr uint32_t LS = LA + Tr::kCopies;
-        for (uint32_t g = lane; g < LS; g += 64) {
-            const uint64_t gi = base + g;
-            double a = 0.0;
-            for (uint32_t cc = 0; cc < cb; cc += 16) {
-                float xs[16];
+        auto put = [&](uint32_t g, uint32_t c, float x, double &a) {
+            if (PREC == 3 && fabsf(x) > kFp8Limit) over = 1;
+            a = __builtin_fma((double)x, (double)x, a);
+            const uint32_t qv = to_narrow<PREC>(x);
 #pragma unroll
-                for (int q = 0; q < 16; q++)
-                    xs[q] = (cc + q < cb && gi < d.K) ? V[(uint64_t)(c0 + cc + q) * d.K + gi] : 0.0f;
+            for (int r = 0; r < (int)Tr::kCopies; r++) {
+                const int y = (int)g - r;
+                if (y < 0 || y >= (int)LA) continue;
+                uint8_t *dst = vb + (((uint64_t)r * cb + c) * LA + y) * Tr::kBytes;
+                if constexpr (Tr::kBytes == 2) *(uint16_t *)dst = (uint16_t)qv;
+                else *dst = (uint8_t)qv;
+            }
+        };
+        for (uint32_t g0 = lane; g0 < LS; g0 += 128) {
+            const uint32_t g1 = g0 + 64;
+            const bool l0 = g0 < gmax && base + g0 < d.K, l1 = g1 < gmax && base + g1 < d.K;
+            double a0 = 0.0, a1 = 0.0;
+            for (uint32_t cc = 0; cc < cb; cc += 16) {
+                float x0[16], x1[16];
 #pragma unroll
                 for (int q = 0; q < 16; q++) {
-                    if (cc + q >= cb) break;
-                    const float x = xs[q];
-                    if (PREC == 3 && fabsf(x) > kFp8Limit) over = 1;
-                    a = __builtin_fma((double)x, (double)x, a);
-                    const uint32_t qv = to_narrow<PREC>(x);
-                    const uint32_t c = cc + q;
+                    const float *vr = V + (uint64_t)(c0 + cc + q) * d.K + base;
+                    x0[q] = (cc + q < cb && l0) ? vr[g0] : 0.0f;
+                    x1[q] = (cc + q < cb && l1) ? vr[g1] : 0.0f;
+                }
 #pragma unroll
-                    for (int r = 0; r < (int)Tr::kCopies; r++) {
-                        const int y = (int)g - r;
-                        if (y < 0 || y >= (int)LA) continue;
-                        uint8_t *dst = vb + (((uint64_t)r * cb + c) * LA + y) * Tr::kBytes;
-                        if constexpr (Tr::kBytes == 2) *(uint16_t *)dst = (uint16_t)qv;
-                        else *dst = (uint8_t)qv;
+                for (int q = 0; q < 16; q++) {
+                    if (cc + q < cb) {
+                        put(g0, cc + q, x0[q], a0);
+                        if (g1 < LS) put(g1, cc + q, x1[q], a1);
                     }
                 }
             }
-            if (staged_once && g < nsq) s2[g] = a;
+            if (staged_once && g0 < nsq) s2[g0] = a0;
+            if (staged_once && g1 < nsq) s2[g1] = a1;
         }
     };
 
@@ -414,19 +447,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT <= 4 && C
                 have = true;
             }
             // template fragments of channel c + 1 are fetched during channel c
-            uint64_t bn[KS][CTB];
+            Frag<PREC> bn[KS][CTB];
             auto fetch_b = [&](uint32_t c) {
 #pragma unroll
                 for (uint32_t ks = 0; ks < KS; ks++)
 #pragma unroll
                     for (int ct = 0; ct < CTB; ct++)
-                        bn[ks][ct] = ct < (int)nct
-                                         ? TmB[(((uint64_t)(c0 + c) * KS + ks) * CT + ct0 + ct) * 64 + lane]
-                                         : 0ull;
+                        if (ct < (int)nct) bn[ks][ct] = TmB[(((uint64_t)(c0 + c) * KS + ks) * CT + ct0 + ct) * 64 + lane];
+                        else
+#pragma unroll
+                            for (uint32_t w = 0; w < Tr::kWords; w++) bn[ks][ct].w[w] = 0;
             };
             fetch_b(0);
             for (uint32_t c = 0; c < cb; c++) {
-                uint64_t b[KS][CTB];
+                Frag<PREC> b[KS][CTB];
 #pragma unroll
                 for (uint32_t ks = 0; ks < KS; ks++)
 #pragma unroll
@@ -438,8 +472,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT <= 4 && C
                     for (int rt = 0; rt < RT; rt++) {
                         const uint32_t o = rt * 16 + ks * Tr::kK + Tr::kPer * (lane >> 4) + (lane & 15);
                         const uint32_t r = o % Tr::kCopies;
-                        const uint64_t a =
-                            *(const uint64_t *)(vb + (((uint64_t)r * cb + c) * LA + (o - r)) * Tr::kBytes);
+                        const uint64_t *src =
+                            (const uint64_t *)(vb + (((uint64_t)r * cb + c) * LA + (o - r)) * Tr::kBytes);
+                        Frag<PREC> a;
+#pragma unroll
+                        for (uint32_t w = 0; w < Tr::kWords; w++) a.w[w] = src[w];
 #pragma unroll
                         for (int ct = 0; ct < CTB; ct++) acc[rt][ct] = mfma_narrow<PREC>(a, b[ks][ct], acc[rt][ct]);
                     }
@@ -1346,9 +1383,9 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
                 for (int j = 0; j < 4; j++) {
                     if ((uint32_t)j < nst) {
                         const uint64_t bh = q_bfrag(rh, y0 + 16 * j), bl = q_bfrag(rl, y0 + 16 * j);
-                        acch[j] = mfma_narrow<1>(ah, bh, acch[j]);
-                        accl[j] = mfma_narrow<1>(ah, bl, accl[j]);
-                        accl[j] = mfma_narrow<1>(al, bh, accl[j]);
+                        acch[j] = mfma_bf16_k16(ah, bh, acch[j]);
+                        accl[j] = mfma_bf16_k16(ah, bl, accl[j]);
+                        accl[j] = mfma_bf16_k16(al, bh, accl[j]);
                     }
                 }
                 if (++kq == qp.NKQ) { kq = 0; c++; }
@@ -1875,7 +1912,8 @@ hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win,
     return hipGetLastError();
 }
 
-uint32_t mfma_kstep(uint32_t precision) { return precision == 3 ? 32 : 16; }
+uint32_t mfma_kstep(uint32_t) { return 32; }
+uint32_t mfma_words(uint32_t precision) { return precision == 3 ? 1 : 2; }
 // k-steps per channel: the template length padded to whole 16-sample row tiles
 uint32_t mfma_ks(const Dims &d, uint32_t precision)
 {
@@ -1898,7 +1936,7 @@ static hipError_t launch_pack_p(const Dims &d, const DeviceState &s, hipStream_t
 {
     const uint64_t total = (uint64_t)d.C * mfma_ks(d, PREC) * mfma_ct(d) * 64;
     hipLaunchKernelGGL(k_pack_templates<PREC>, dim3(grid_for(total, 256, 4096)), dim3(256), 0, st, d, s.Tm, s.tscale,
-                       (uint64_t *)s.TmB, mfma_ks(d, PREC), mfma_ct(d));
+                       (Frag<PREC> *)s.TmB, mfma_ks(d, PREC), mfma_ct(d));
     return hipGetLastError();
 }
 
@@ -1923,7 +1961,7 @@ static hipError_t launch_corr_mfma_rt(const Dims &d, const DeviceState &s, uint3
     const size_t lds = (size_t)cb_max * per_channel;
     ScreenCoef co{coef[0], coef[1], coef[2], coef[3]};
     hipLaunchKernelGGL((k_corr_mfma<PREC, RT, CTB>), dim3(n_win), dim3(64), lds, st, d, s.V, s.Tm,
-                       (const uint64_t *)s.TmB, s.bias, s.tnorm, s.tscale, s.win, s.L, s.ctr, s.rq, rq_cap,
+                       (const Frag<PREC> *)s.TmB, s.bias, s.tnorm, s.tscale, s.win, s.L, s.ctr, s.rq, rq_cap,
                        (uint2 *)s.rdesc, cb_max, co, no_rescore);
     return hipGetLastError();
 }

@@ -128,7 +128,7 @@ uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
         uint64_t cap = T * d.K / 8;
         if (cap < (1u << 20)) cap = 1u << 20;
         if (cap > (64u << 20)) cap = 64u << 20;
-        b += r(cap * 8) + r((uint64_t)d.C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64 * 8) +
+        b += r(cap * 8) + r((uint64_t)d.C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64 * 8 * mfma_words(cfg->precision)) +
              r(W * mfma_nchunk(d) * 8);
     }
     return b;
@@ -305,7 +305,7 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         ALLOC(rd, (uint64_t)W * mfma_nchunk(d))
         s.rdesc = rd;
         uint64_t *tmb = nullptr;
-        ALLOC(tmb, (uint64_t)C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64)
+        ALLOC(tmb, (uint64_t)C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64 * mfma_words(cfg->precision))
         s.TmB = tmb;
         // Screen bound coefficients (see the correlation screen in kernels.hip):
         // unit roundoff u and flushed-subnormal floor delta of the format.
