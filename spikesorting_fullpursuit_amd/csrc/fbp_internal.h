@@ -56,6 +56,9 @@ struct DeviceState {
     uint32_t *win_next;  // [W] window list of the next pass
     uint32_t *ov_list;   // [W] overlap windows of the pass
     uint32_t *ov_sorted; // [W] the same, grouped by best label (screened overlap)
+    float *part_v;       // [W] best likelihood within each window (argmax partials)
+    uint32_t *part_t;    // [W] its template
+    uint32_t *part_p;    // [W] its position
     uint8_t *has;        // [W] add decision per entry of the pass list
     uint32_t *cand_idx;  // [W] spike onset per entry of the pass list
     uint32_t *cand_lab;  // [W]

@@ -751,28 +751,34 @@ __global__ void k_zero_rescore(Counters *ctr)
 // from a best of 0: the winner is the smallest (t, position) holding the
 // largest positive value; with no positive value the label and index stay.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_argmax(Dims d, const float *__restrict__ L,
-                                                const uint32_t *__restrict__ win, uint32_t n_win,
-                                                float *best_L, uint32_t *best_lab, uint32_t *best_idx,
-                                                uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *nc, uint8_t *ov)
+// Window partials: one wave per listed window reads its own N positions of
+// every template once and keeps the best (value, t, position) in the
+// reference's scan order (t ascending, then position, strict >, from 0).  A
+// window's likelihood changes only when the window is listed, so the partials
+// of unlisted windows stay current.
+__global__ __launch_bounds__(256) void k_window_partials(Dims d, const float *__restrict__ L,
+                                                         const uint32_t *__restrict__ win, uint32_t n_win,
+                                                         float *part_v, uint32_t *part_t, uint32_t *part_p)
 {
     const uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = lane_of();
     if (k >= n_win) return;
     const uint32_t w = win[k];
-    const uint64_t start_w = (uint64_t)w * d.N, end_w = start_w + d.N;
-    if (end_w >= (uint64_t)(d.K - d.N)) return;   // bp.cl:380-383
-    const uint64_t start = start_w >= d.N ? start_w - d.N : 0;
-    const uint64_t stop = min(end_w + d.N, (uint64_t)d.K);
+    const uint64_t start_w = (uint64_t)w * d.N;
+    const uint64_t stop = min(start_w + d.N, (uint64_t)d.K);
     float bv = 0.0f;
     uint32_t bt = 0xffffffffu, bp = 0xffffffffu;
-    bool chk = false;
-    for (uint32_t t = 0; t < d.T; t++) {
-        const float *Lt = L + (uint64_t)t * d.K;
-        for (uint64_t p = start + lane; p < stop; p += kWave) {
-            const float v = Lt[p];
-            if (v > bv) { bv = v; bt = t; bp = (uint32_t)p; }
-            if (v > 0.0f && p >= start_w && p < end_w) chk = true;
+    for (uint64_t p = start_w + lane; p < stop; p += kWave) {
+        // all templates of this position, loads first
+        for (uint32_t t0 = 0; t0 < d.T; t0 += 8) {
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = t0 + q < d.T ? L[(uint64_t)(t0 + q) * d.K + p] : 0.0f;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const uint32_t t = t0 + q;
+                if (v[q] > bv || (v[q] == bv && v[q] > 0.0f && t < bt)) { bv = v[q]; bt = t; bp = (uint32_t)p; }
+            }
         }
     }
 #pragma unroll
@@ -781,19 +787,47 @@ __global__ __launch_bounds__(256) void k_argmax(Dims d, const float *__restrict_
         const uint32_t ot = __shfl_xor(bt, off), op = __shfl_xor(bp, off);
         if (ov_ > bv || (ov_ == bv && (ot < bt || (ot == bt && op < bp)))) { bv = ov_; bt = ot; bp = op; }
     }
-    const bool any_chk = __any(chk);
     if (lane == 0) {
-        uint32_t lab, idx;
-        if (bv > 0.0f) { lab = bt; idx = bp; }
-        else { lab = best_lab[w]; idx = best_idx[w]; }
-        if (any_chk) nc[w] = 1;
-        ov[w] = (bv > 0.0f && idx >= start_w && idx < end_w) ? 1 : 0;
-        best_L[w] = bv;
-        best_lab[w] = lab;
-        best_idx[w] = idx;
-        ov_idx[w] = idx;
-        ov_lab[w] = lab;
+        part_v[w] = bv;
+        part_t[w] = bt;
+        part_p[w] = bp;
     }
+}
+
+// Windowed argmax over all templates (bp.cl:350-441, with the host's T
+// sequential launches, bpp.py:551-562, fused): the best of the partials of
+// windows w-1, w, w+1, which cover [wN - N, wN + 2N) exactly.  Ties go to the
+// smallest (t, position), as the reference's scan order gives.  With no
+// positive value the label and index stay.
+__global__ __launch_bounds__(256) void k_argmax(Dims d, const uint32_t *__restrict__ win, uint32_t n_win,
+                                                const float *__restrict__ part_v, const uint32_t *__restrict__ part_t,
+                                                const uint32_t *__restrict__ part_p, float *best_L,
+                                                uint32_t *best_lab, uint32_t *best_idx, uint32_t *ov_idx,
+                                                uint32_t *ov_lab, uint8_t *nc, uint8_t *ov)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_win) return;
+    const uint32_t w = win[k];
+    const uint64_t start_w = (uint64_t)w * d.N, end_w = start_w + d.N;
+    if (end_w >= (uint64_t)(d.K - d.N)) return;   // bp.cl:380-383
+    float bv = 0.0f;
+    uint32_t bt = 0xffffffffu, bp = 0xffffffffu;
+    for (uint32_t q = (w > 0 ? w - 1 : 0); q <= w + 1; q++) {
+        const float v = part_v[q];
+        const uint32_t t = part_t[q], p = part_p[q];
+        if (v > bv || (v == bv && v > 0.0f && (t < bt || (t == bt && p < bp)))) { bv = v; bt = t; bp = p; }
+    }
+    const bool chk = part_v[w] > 0.0f;   // a positive value in the own window (bp.cl:396-400)
+    uint32_t lab, idx;
+    if (bv > 0.0f) { lab = bt; idx = bp; }
+    else { lab = best_lab[w]; idx = best_idx[w]; }
+    if (chk) nc[w] = 1;
+    ov[w] = (bv > 0.0f && idx >= start_w && idx < end_w) ? 1 : 0;
+    best_L[w] = bv;
+    best_lab[w] = lab;
+    best_idx[w] = idx;
+    ov_idx[w] = idx;
+    ov_lab[w] = lab;
 }
 
 // ---------------------------------------------------------------------------
@@ -2034,8 +2068,10 @@ hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_wi
 hipError_t launch_argmax(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st)
 {
     if (n_win == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_argmax, dim3((n_win + 3) / 4), dim3(256), 0, st, d, s.L, s.win, n_win, s.best_L,
-                       s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.nc, s.ov);
+    hipLaunchKernelGGL(k_window_partials, dim3((n_win + 3) / 4), dim3(256), 0, st, d, s.L, s.win, n_win, s.part_v,
+                       s.part_t, s.part_p);
+    hipLaunchKernelGGL(k_argmax, dim3((n_win + 255) / 256), dim3(256), 0, st, d, s.win, n_win, s.part_v, s.part_t,
+                       s.part_p, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.nc, s.ov);
     return hipGetLastError();
 }
 

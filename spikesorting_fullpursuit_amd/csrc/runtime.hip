@@ -105,7 +105,7 @@ void free_state(fbp_segment *g)
     void *ptrs[] = {s.V, s.V0, s.Tm, s.TmT, s.bias, s.thr, s.L, s.best_L, s.best_lab, s.best_idx, s.ov_idx,
                     s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
                     s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabB2, s.tnorm,
-                    s.TmB, s.rq, s.tscale, s.rdesc, s.TmQ, s.Tpad, s.ov_sorted};
+                    s.TmB, s.rq, s.tscale, s.rdesc, s.TmQ, s.Tpad, s.ov_sorted, s.part_v, s.part_t, s.part_p};
     for (void *p : ptrs) dfree(p, g->own_stream);
     if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
     s = DeviceState{};
@@ -121,7 +121,7 @@ uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
     uint64_t b = 2 * r(CK * 4) + r(T * d.C * d.N * 4) + r((uint64_t)d.C * d.N * d.Tp * 4) + r(d.Tp * 4) +
                  r(T * 4) + r(T * d.K * 4) + 5 * r(W * 4) + 2 * r(W + 2) + 3 * r(W * 4) + r(W) + 3 * r(W * 4) +
                  r((nb + 1) * 4) + r(sizeof(Counters)) + r(T * 4) + 2 * r(W * 4) + r(T * 4) +
-                 r(T * d.C * (d.N + 2 * d.M) * 4);
+                 r(T * d.C * (d.N + 2 * d.M) * 4) + 3 * r(W * 4);
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED)
         b += r(T * d.S * T * d.S * 4) + r(T * d.S * 4) + r(overlap_qtemplate_floats(d) * 4) + r(W * 4);
     if (cfg->precision != FBP_PREC_FP32) {
@@ -280,6 +280,9 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     ALLOC(s.win, W)
     ALLOC(s.win_next, W)
     ALLOC(s.ov_list, W)
+    ALLOC(s.part_v, W)
+    ALLOC(s.part_t, W)
+    ALLOC(s.part_p, W)
     ALLOC(s.has, W)
     ALLOC(s.cand_idx, W)
     ALLOC(s.cand_lab, W)
