@@ -21,6 +21,7 @@ struct Dims {
     uint32_t reduce_wg;// work-group width of the reference's overlap reduction
     uint64_t items;    // S*S*T overlap combinations per window
     uint64_t nl;       // ceil(items / reduce_wg) reduction groups per window
+    uint32_t ov_batch; // screened overlap: windows per Q-table batch (rows of ovQ)
 };
 
 // Device counters, read back by the host once per pass.
@@ -59,6 +60,7 @@ struct DeviceState {
     float *part_v;       // [W] best likelihood within each window (argmax partials)
     uint32_t *part_t;    // [W] its template
     uint32_t *part_p;    // [W] its position
+    float *ovQ;          // [ov_batch][T][S] screened overlap: Q - B2/2 of a batch of windows
     uint8_t *has;        // [W] add decision per entry of the pass list
     uint32_t *cand_idx;  // [W] spike onset per entry of the pass list
     uint32_t *cand_lab;  // [W]

@@ -1439,6 +1439,58 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
     }
 }
 
+// Q - B2/2 of every unskipped overlap window of a batch, on the matrix cores
+// (q_table_split), into ovQ[k - kbase][T][S] for k_overlap_screened<true>.
+// The window skip rule is the screen kernel's; skipped windows are not written.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_overlap_q(
+    Dims d, const float *__restrict__ V, const float *__restrict__ L, const float *__restrict__ thr,
+    const uint32_t *__restrict__ ov_list, const Counters *__restrict__ ctr, const uint32_t *__restrict__ best_lab,
+    const uint32_t *__restrict__ best_idx, const float *__restrict__ B2, const uint4 *__restrict__ TmQ,
+    float *__restrict__ ovQ, uint32_t kbase)
+{
+    extern __shared__ float dyn[];
+    const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
+    const uint32_t TS = T * S;
+    const QPlan qp = q_plan(d);
+    const uint32_t LPH = qp.LPH;
+    const FastDiv divLPH(LPH);
+    uint16_t *svh = (uint16_t *)dyn;                    // [C][LPH] zero-padded voltage, bf16 hi
+    uint16_t *svl = svh + C * LPH;                      // [C][LPH] bf16 lo
+    float *sQp = (float *)(svl + C * LPH);              // [KS-1][T][S] channel-slice partials
+    float *sQ = sQp + (qp.KS - 1) * TS;                 // [T][S]
+    const uint32_t kend = min(ctr->n_ov, kbase + d.ov_batch);
+    for (uint32_t k = kbase + blockIdx.x; k < kend; k += gridDim.x) {
+        const uint32_t id = ov_list[k];
+        const uint32_t lab = best_lab[id], bidx = best_idx[id];
+        const float Lb = L[(uint64_t)lab * d.K + bidx];
+        if ((Lb < thr[lab]) || ((uint64_t)bidx >= (uint64_t)(d.K - d.N))) continue;   // bp.cl:518-527
+        __syncthreads();   // the previous window's LDS reads are done
+        batched_stage<8>(C * LPH, [&](uint32_t e) {
+            const uint32_t c = divLPH.div(e), y = e - c * LPH;
+            return (y >= M && y - M < N) ? V[(uint64_t)c * d.K + bidx + (y - M)] : 0.0f;
+        }, [&](uint32_t e, float x) {
+            const uint32_t h = to_bf16(x);
+            svh[e] = (uint16_t)h;
+            svl[e] = to_bf16(x - __uint_as_float(h << 16));
+        });
+        __syncthreads();
+        q_table_split(d, qp, TmQ, svh, svl, sQ, sQp);
+        __syncthreads();
+        float *dst = ovQ + (uint64_t)(k - kbase) * TS;
+        batched_stage<8>(TS, [&](uint32_t e) { return B2[e]; }, [&](uint32_t e, float b2) {
+            float q = sQ[e];
+            for (uint32_t j = 1; j < qp.KS; j++) q += sQp[(j - 1) * TS + e];
+            dst[e] = (float)((double)q - 0.5 * (double)b2);
+        });
+    }
+}
+
+size_t overlap_q_lds(const Dims &d)
+{
+    const QPlan qp = q_plan(d);
+    return (uint64_t)d.C * qp.LPH * 4 + (uint64_t)qp.KS * d.T * d.S * 4;
+}
+
 // TmQ[(c*NKQ + kq)*TT + tt][lane] = {hi, lo} fragments (4 bf16 each, sample
 // kq*16 + 4*(lane/16) + i in bits 16i) of template t = tt*16 + lane%16 on
 // channel c; 0 when padded.  The A fragments of q_table_split.
@@ -1475,21 +1527,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw, float *best_L,
     const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov,
     const float *__restrict__ AB, const float *__restrict__ B2, const float *__restrict__ tnorm,
-    const uint4 *__restrict__ TmQ, const float *__restrict__ Tpad, uint32_t debug)
+    const float *__restrict__ Tpad, const float *__restrict__ ovQ, uint32_t kbase, uint32_t debug)
 {
     extern __shared__ float dyn[];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
     const uint32_t TS = T * S;
     const QPlan qp = q_plan(d);
-    const uint32_t LPH = qp.LPH;
-    const FastDiv divN(N), divLPH(LPH), divS(S);
+    const FastDiv divN(N), divS(S);
     double *sE = (double *)dyn;                         // [T] screen bound per template (this window)
     float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset
     float *smain = sv + C * N;                          // [C][N + 2M] padded template of the best label
-    uint16_t *svh = (uint16_t *)(smain + C * (N + 2 * M));   // [C][LPH] zero-padded voltage, bf16 hi (MQ)
-    uint16_t *svl = svh + (MQ ? C * LPH : 0);           // [C][LPH] bf16 lo (MQ)
-    float *sQp = (float *)(svl + (MQ ? C * LPH : 0));   // [KS-1][T][S] channel-slice partials of Q (MQ)
-    float *sQ = sQp + (MQ ? (qp.KS - 1) * TS : 0);      // [T][S] Q - B2/2
+    float *sb = smain + C * (N + 2 * M);                // [C][N + 2M] partner template of an exact score
+    float *sQ = sb + C * (N + 2 * M);                   // [T][S] Q - B2/2
     uint32_t *plist = (uint32_t *)(sQ + TS);            // (t,s) with positive likelihood
     uint32_t *flist = plist + TS;                       // main shifts: positive first, then the rest
     uint32_t *cand = flist + S;                         // [kOvCand] screen candidates (f << 16 | e)
@@ -1519,7 +1568,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const double eabs = MQ ? 2.0 * 3.0 * nq * 0x1p-126 : 0.0;
     // + float32 storage of Q - B2/2 and of the AB table (unit roundoff each)
     const double ecoef = (g + 2.0 * gq + 4.0 * u2) * 1.01;
-    for (uint32_t k = blockIdx.x; k < n_ov; k += gridDim.x) {
+    const uint32_t kend = min(n_ov, kbase + d.ov_batch);
+    for (uint32_t k = kbase + blockIdx.x; k < kend; k += gridDim.x) {
         const uint32_t id = ov_list[k];
         const uint32_t lab = best_lab[id], bidx = best_idx[id];
         const float bl = best_L[id];
@@ -1544,15 +1594,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                              [&](uint32_t e, float x) { sv[e] = x; vv = __builtin_fma((double)x, (double)x, vv); });
             batched_stage<8>(C * R, [&](uint32_t e) { return Tpad[(uint64_t)lab * C * R + e]; },
                              [&](uint32_t e, float x) { smain[e] = x; });
-            if (MQ)
-                batched_stage<8>(C * LPH, [&](uint32_t e) {
-                    const uint32_t c = divLPH.div(e), y = e - c * LPH;
-                    return (y >= M && y - M < N) ? V[(uint64_t)c * d.K + bidx + (y - M)] : 0.0f;
-                }, [&](uint32_t e, float x) {
-                    const uint32_t h = to_bf16(x);
-                    svh[e] = (uint16_t)h;
-                    svl[e] = to_bf16(x - __uint_as_float(h << 16));
-                });
+            if (MQ)   // Q - B2/2 of this window from the matrix-core kernel
+                batched_stage<8>(TS, [&](uint32_t e) { return ovQ[(uint64_t)(k - kbase) * TS + e]; },
+                                 [&](uint32_t e, float q) { sQ[e] = q; });
             for (int off = 32; off > 0; off >>= 1) vv += __shfl_xor(vv, off);
             if (lane_of() == 0) r_vv[threadIdx.x >> 6] = vv;
             __syncthreads();
@@ -1581,24 +1625,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     sQ[e] = (float)q;
                 }
             });
-            if (MQ) {
-                __syncthreads();
-                q_table_split(d, qp, TmQ, svh, svl, sQ, sQp);
-                if (qp.KS > 1) {
-                    __syncthreads();
-                    for (uint32_t e = threadIdx.x; e < TS; e += blockDim.x) {
-                        float q = sQ[e];
-                        for (uint32_t k = 1; k < qp.KS; k++) q += sQp[(k - 1) * TS + e];
-                        sQ[e] = q;
-                    }
-                }
-            }
             __syncthreads();
             const double vn = sqrt(r_vv[0] + r_vv[1] + r_vv[2] + r_vv[3]) * (1.0 + 1e-12);
             const double nlab = tnorm[lab];
-            // fold the voltage-free half norms into Q, and the bound per template
-            batched_stage<8>(TS, [&](uint32_t e) { return B2[e]; },
-                             [&](uint32_t e, float b2) { sQ[e] = (float)((double)sQ[e] - 0.5 * (double)b2); });
+            // fold the voltage-free half norms into Q (done by the matrix-core
+            // kernel for MQ), and the bound per template
+            if (!MQ)
+                batched_stage<8>(TS, [&](uint32_t e) { return B2[e]; },
+                                 [&](uint32_t e, float b2) { sQ[e] = (float)((double)sQ[e] - 0.5 * (double)b2); });
             for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
                 const double nsum = nlab + (double)tnorm[t];
                 sE[t] = ecoef * (nsum * vn + nsum * nsum) + eabs;
@@ -1677,9 +1711,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     push(kv3, kc3);
                     __syncthreads();
                     const uint32_t nc = n_cand;   // <= 4 * blockDim.x = kOvCand
-                    if (MQ && nc <= 16 && !(debug & 4u)) {
+                    if (nc <= 16 && !(debug & 4u)) {
                         // one at a time, the partner template staged in LDS
-                        float *sb = (float *)svh;   // [C][N + 2M] (the Q-table voltage is no longer needed)
                         for (uint32_t k2 = 0; k2 < nc; k2++) {
                             const uint32_t f = cand[k2] >> 16, e = cand[k2] & 0xffffu;
                             const uint32_t t = e / S, s = e - t * S;
@@ -2126,19 +2159,22 @@ hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t m
     return hipGetLastError();
 }
 
-static size_t overlap_screened_lds_mode(const Dims &d, bool mq)
+static size_t overlap_screened_lds_mode(const Dims &d, bool)
 {
     const uint64_t ts = (uint64_t)d.T * d.S;
-    const QPlan qp = q_plan(d);
-    return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + (uint64_t)d.C * (d.N + 2 * d.M) * 4 +
-           (mq ? (uint64_t)d.C * qp.LPH * 4 + (uint64_t)(qp.KS - 1) * ts * 4 : 0) + ts * 4 + ts * 4 +
+    return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + 2 * (uint64_t)d.C * (d.N + 2 * d.M) * 4 + ts * 4 + ts * 4 +
            (uint64_t)d.S * 4 + (uint64_t)kOvCand * 4 + ts + 16;
 }
 
-// The matrix-core Q table is used whenever its padded voltage fits in LDS.
-static bool overlap_uses_mfma_q(const Dims &d) { return overlap_screened_lds_mode(d, true) <= 150 * 1024; }
+// The matrix-core Q table is used whenever its kernel's LDS fits.
+static bool overlap_uses_mfma_q(const Dims &d) { return overlap_q_lds(d) <= 150 * 1024; }
 
-size_t overlap_screened_lds(const Dims &d) { return overlap_screened_lds_mode(d, overlap_uses_mfma_q(d)); }
+size_t overlap_screened_lds(const Dims &d)
+{
+    const size_t a = overlap_screened_lds_mode(d, overlap_uses_mfma_q(d));
+    const size_t b = overlap_uses_mfma_q(d) ? overlap_q_lds(d) : 0;
+    return a > b ? a : b;
+}
 
 uint64_t overlap_qtemplate_floats(const Dims &d)
 {
@@ -2168,27 +2204,40 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
 {
     if (max_ov == 0) return hipSuccess;
     const bool mq = overlap_uses_mfma_q(d);
-    const size_t lds = overlap_screened_lds(d);
+    const size_t lds = overlap_screened_lds_mode(d, mq);
     const void *fn = mq ? (const void *)k_overlap_screened<true> : (const void *)k_overlap_screened<false>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    const uint32_t grid = max_ov < 4096 ? max_ov : 4096;
     const uint32_t *list = s.ov_list;
     if (d.T <= kSortMaxLabels) {
         hipLaunchKernelGGL(k_sort_ov_by_label, dim3(1), dim3(1024), 0, st, d, s.ov_list, s.ctr, s.best_lab,
                            s.ov_sorted);
         list = s.ov_sorted;
     }
-    if (mq)
-        hipLaunchKernelGGL(k_overlap_screened<true>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
-                           list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                           s.tabB2, s.tnorm, (const uint4 *)s.TmQ, s.Tpad, debug);
-    else
-        hipLaunchKernelGGL(k_overlap_screened<false>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
-                           list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                           s.tabB2, s.tnorm, (const uint4 *)s.TmQ, s.Tpad, debug);
+    const size_t qlds = overlap_q_lds(d);
+    if (mq && qlds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_overlap_q, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)qlds);
+        if (e != hipSuccess) return e;
+    }
+    // batches of ov_batch windows: Q tables (matrix cores), then the screen
+    for (uint32_t kb = 0; kb < max_ov; kb += d.ov_batch) {
+        const uint32_t nb = max_ov - kb < d.ov_batch ? max_ov - kb : d.ov_batch;
+        const uint32_t grid = nb < 4096 ? nb : 4096;
+        if (mq) {
+            hipLaunchKernelGGL(k_overlap_q, dim3(grid), dim3(256), qlds, st, d, s.V, s.L, s.thr, list, s.ctr,
+                               s.best_lab, s.best_idx, s.tabB2, (const uint4 *)s.TmQ, s.ovQ, kb);
+            hipLaunchKernelGGL(k_overlap_screened<true>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
+                               list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
+                               s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, debug);
+        } else {
+            hipLaunchKernelGGL(k_overlap_screened<false>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
+                               list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
+                               s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, debug);
+        }
+    }
     return hipGetLastError();
 }
 

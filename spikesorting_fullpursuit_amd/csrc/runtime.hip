@@ -105,7 +105,8 @@ void free_state(fbp_segment *g)
     void *ptrs[] = {s.V, s.V0, s.Tm, s.TmT, s.bias, s.thr, s.L, s.best_L, s.best_lab, s.best_idx, s.ov_idx,
                     s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
                     s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabB2, s.tnorm,
-                    s.TmB, s.rq, s.tscale, s.rdesc, s.TmQ, s.Tpad, s.ov_sorted, s.part_v, s.part_t, s.part_p};
+                    s.TmB, s.rq, s.tscale, s.rdesc, s.TmQ, s.Tpad, s.ov_sorted, s.part_v, s.part_t, s.part_p,
+                    s.ovQ};
     for (void *p : ptrs) dfree(p, g->own_stream);
     if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
     s = DeviceState{};
@@ -123,7 +124,8 @@ uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
                  r((nb + 1) * 4) + r(sizeof(Counters)) + r(T * 4) + 2 * r(W * 4) + r(T * 4) +
                  r(T * d.C * (d.N + 2 * d.M) * 4) + 3 * r(W * 4);
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED)
-        b += r(T * d.S * T * d.S * 4) + r(T * d.S * 4) + r(overlap_qtemplate_floats(d) * 4) + r(W * 4);
+        b += r(T * d.S * T * d.S * 4) + r(T * d.S * 4) + r(overlap_qtemplate_floats(d) * 4) + r(W * 4) +
+             r((uint64_t)d.ov_batch * T * d.S * 4);
     if (cfg->precision != FBP_PREC_FP32) {
         uint64_t cap = T * d.K / 8;
         if (cap < (1u << 20)) cap = 1u << 20;
@@ -233,6 +235,14 @@ static int make_dims(const fbp_config *cfg, Dims &d)
     d.reduce_wg = cfg->reduce_wg;
     d.items = (uint64_t)d.S * d.S * T;
     d.nl = (d.items - 1) / cfg->reduce_wg + 1;
+    {
+        // Q tables of up to 256 MB of windows per screened-overlap batch
+        const uint64_t per = (uint64_t)d.T * d.S * 4;
+        uint64_t nb = (256ull << 20) / per;
+        if (nb < 1024) nb = 1024;
+        if (nb > d.W) nb = d.W;
+        d.ov_batch = (uint32_t)nb;
+    }
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED && overlap_screened_lds(d) > kMaxLds)
         return fail(FBP_EINVAL, "screened overlap needs %zu bytes of LDS (T*S too large)", overlap_screened_lds(d));
     return FBP_OK;
@@ -297,6 +307,7 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         ALLOC(s.tabB2, (uint64_t)T * d.S)
         ALLOC(s.TmQ, overlap_qtemplate_floats(d))
         ALLOC(s.ov_sorted, W)
+        ALLOC(s.ovQ, (uint64_t)d.ov_batch * T * d.S)
     }
     if (cfg->precision != FBP_PREC_FP32) {
         uint64_t cap = (uint64_t)T * K / 8;
