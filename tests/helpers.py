@@ -47,7 +47,7 @@ def screen_bound(rec, precision):
     T = rec['templates'].shape[0]
     u, delta = {"bf16": (2.0 ** -8, 2.0 ** -126), "fp16": (2.0 ** -11, 2.0 ** -14),
                 "fp8": (2.0 ** -4, 2.0 ** -6)}[precision]
-    kstep = 32 if precision == "fp8" else 16
+    kstep = 32
     n = C * kstep * ((N + kstep - 1) // kstep) + 2.0
     gp = n * 2.0 ** -23 / (1 - n * 2.0 ** -23)
     cn = C * N
