@@ -83,8 +83,11 @@ enum {
                                       (no float32 re-score): checks the error bound */
     FBP_DEBUG_OVERLAP_FULL = 2,    /* screened overlap: find the exact-score
                                       candidates by a second full traversal     */
-    FBP_DEBUG_OVERLAP_GLOBAL = 4   /* screened overlap: exact scores read the
+    FBP_DEBUG_OVERLAP_GLOBAL = 4,  /* screened overlap: exact scores read the
                                       partner template from global memory       */
+    FBP_DEBUG_OVERLAP_EXACT = 8    /* screened overlap: score a single candidate
+                                      exactly even when its interval settles the
+                                      parse                                      */
 };
 
 typedef struct fbp_stats {

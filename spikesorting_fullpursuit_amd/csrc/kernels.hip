@@ -989,6 +989,21 @@ __device__ __forceinline__ uint64_t combo_key(const Dims &d, uint32_t wg_bits, u
 
 // parse_overlap_recheck_indices (bp.cl:690-814) for one window, given the best
 // scored combination (score sc, reference work id bp).  Thread 0 only.
+// The branches of the parse after the score comparisons (bp.cl:739-814):
+// returns whether the window keeps its overlap flag.
+__device__ __forceinline__ bool parse_choose(const Dims &d, const float *__restrict__ thr, uint32_t id, uint32_t lab,
+                                             uint32_t bidx, uint32_t t, uint32_t af, uint32_t as, float Lm, float Lo,
+                                             float *best_L, uint32_t *ov_idx, uint32_t *ov_lab)
+{
+    if (Lm <= 0.0f || Lo <= 0.0f) return false;                                          // bp.cl:739-744
+    if (Lm >= Lo && Lm > thr[lab]) { best_L[id] = Lm; ov_lab[id] = lab; ov_idx[id] = af; return true; }
+    if (Lo > Lm && Lo > thr[t]) { best_L[id] = Lo; ov_lab[id] = t; ov_idx[id] = as; return true; }
+    if (Lm > thr[lab]) { best_L[id] = Lm; ov_lab[id] = lab; ov_idx[id] = af; return true; }
+    ov_idx[id] = bidx;                                                                     // bp.cl:806-814
+    ov_lab[id] = lab;
+    return false;
+}
+
 __device__ void parse_overlap(const Dims &d, const float *__restrict__ L, const float *__restrict__ thr, uint32_t id,
                               uint32_t lab, uint32_t bidx, float bl, float sc, uint64_t bp, float *best_L,
                               uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov)
@@ -1005,13 +1020,34 @@ __device__ void parse_overlap(const Dims &d, const float *__restrict__ L, const 
         const float Lm = L[(uint64_t)lab * d.K + af];
         const float Lo = L[(uint64_t)t * d.K + as];
         if (sc < Lm || sc < Lo) keep = false;                  // bp.cl:733-738
-        else if (Lm <= 0.0f || Lo <= 0.0f) keep = false;       // bp.cl:739-744
-        else if (Lm >= Lo && Lm > thr[lab]) { best_L[id] = Lm; ov_lab[id] = lab; ov_idx[id] = af; }
-        else if (Lo > Lm && Lo > thr[t]) { best_L[id] = Lo; ov_lab[id] = t; ov_idx[id] = as; }
-        else if (Lm > thr[lab]) { best_L[id] = Lm; ov_lab[id] = lab; ov_idx[id] = af; }
-        else { ov_idx[id] = bidx; ov_lab[id] = lab; keep = false; }   // bp.cl:806-814
+        else keep = parse_choose(d, thr, id, lab, bidx, t, af, as, Lm, Lo, best_L, ov_idx, ov_lab);
     }
     if (!keep) ov[id] = 0;
+}
+
+// The same parse for a winner whose float32 score is only known to lie in
+// [lo, hi]: returns false, writing nothing, when a comparison of the score is
+// left open by the interval.
+__device__ bool parse_overlap_bounds(const Dims &d, const float *__restrict__ L, const float *__restrict__ thr,
+                                     uint32_t id, uint32_t lab, uint32_t bidx, float bl, double lo, double hi,
+                                     uint64_t bp, float *best_L, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov)
+{
+    const uint32_t S = d.S, T = d.T, M = d.M;
+    const uint64_t span = d.nl * d.reduce_wg;
+    const double g0 = fmax((double)bl, 0.0);                   // keep needs sc > bl and sc > 0
+    if (hi <= g0 || bp % span >= d.items) { ov[id] = 0; return true; }
+    if (!(lo > g0)) return false;
+    const uint32_t t = (uint32_t)(bp % T);
+    const uint32_t f = (uint32_t)((bp / T) % S);
+    const uint32_t s = (uint32_t)((bp / ((uint64_t)S * T)) % S);
+    const uint32_t af = bidx + f - M, as = bidx + s - M;
+    const float Lm = L[(uint64_t)lab * d.K + af];
+    const float Lo = L[(uint64_t)t * d.K + as];
+    const double g1 = fmax((double)Lm, (double)Lo);            // dropped if sc < Lm or sc < Lo
+    if (hi < g1) { ov[id] = 0; return true; }
+    if (!(lo >= g1)) return false;
+    if (!parse_choose(d, thr, id, lab, bidx, t, af, as, Lm, Lo, best_L, ov_idx, ov_lab)) ov[id] = 0;
+    return true;
 }
 
 // Block-wide reduction of (score, key, work id) in winner order; the result is
@@ -1543,7 +1579,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     uint32_t *flist = plist + TS;                       // main shifts: positive first, then the rest
     uint32_t *cand = flist + S;                         // [kOvCand] screen candidates (f << 16 | e)
     uint8_t *spos = (uint8_t *)(cand + kOvCand);        // [T][S] bit1 in bounds, bit0 likelihood > 0
-    __shared__ uint32_t n_cand;
+    __shared__ uint32_t n_cand, s_settled;
     __shared__ float r_sc[4];
     __shared__ uint64_t r_key[4], r_p[4];
     __shared__ double r_d[4][2];
@@ -1584,6 +1620,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 n_fpos = 0;
                 n_fneg = 0;
                 n_cand = 0;
+                s_settled = 0;
             }
             // staging (index quotients by FastDiv: every index is below the
             // LDS size)
@@ -1711,7 +1748,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     push(kv3, kc3);
                     __syncthreads();
                     const uint32_t nc = n_cand;   // <= 4 * blockDim.x = kOvCand
-                    if (nc <= 16 && !(debug & 4u)) {
+                    // A single candidate is the winner (its lower bound is lo_max,
+                    // above every other upper bound); its float32 score is needed
+                    // only when the interval [sh - E, sh + E] leaves a comparison
+                    // of the parse open.
+                    if (nc == 1 && !(debug & 8u)) {
+                        if (threadIdx.x == 0) {
+                            const uint32_t f = cand[0] >> 16, e = cand[0] & 0xffffu;
+                            const uint32_t t = e / S, s = e - t * S;
+                            const double sh = screen(f, sQ[e], ABl[(uint64_t)f * TS + e]);
+                            const double E = sE[t];
+                            uint64_t p;
+                            (void)combo_key(d, wg_bits, span, t, f, s, p);
+                            s_settled = parse_overlap_bounds(d, L, thr, id, lab, bidx, bl, sh - E, sh + E, p, best_L,
+                                                             ov_idx, ov_lab, ov) ? 1u : 0u;
+                        }
+                        __syncthreads();
+                    }
+                    if (s_settled) {
+                        // parsed from the interval
+                    } else if (nc <= 16 && !(debug & 4u)) {
                         // one at a time, the partner template staged in LDS
                         for (uint32_t k2 = 0; k2 < nc; k2++) {
                             const uint32_t f = cand[k2] >> 16, e = cand[k2] & 0xffffu;
@@ -1739,10 +1795,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 }
                 for (int off = 32; off > 0; off >>= 1) scored += __shfl_xor(scored, off);
                 if (lane_of() == 0 && scored) atomicAdd(&ctr_rw->combos, (unsigned long long)scored);
-                block_key_reduce(sc, key, bp, r_sc, r_key, r_p);
+                if (!s_settled) block_key_reduce(sc, key, bp, r_sc, r_key, r_p);
             }
         }
-        if (threadIdx.x == 0) parse_overlap(d, L, thr, id, lab, bidx, bl, sc, bp, best_L, ov_idx, ov_lab, ov);
+        if (threadIdx.x == 0 && (wskip || !s_settled))
+            parse_overlap(d, L, thr, id, lab, bidx, bl, sc, bp, best_L, ov_idx, ov_lab, ov);
     }
 }
 

@@ -461,7 +461,7 @@ int fbp_segment_run(fbp_segment *g, void *stream)
             HIP_TRY(launch_compact_ov(d, s, nw, st), "overlap list");
             HIP_TRY(hipEventRecord(g->ev_stage[2], st), "event");
             if (g->cfg.overlap_mode == FBP_OVERLAP_SCREENED)
-                HIP_TRY(launch_overlap_screened(d, s, nw, (g->cfg.debug_flags >> 1) & 3u, st), "overlap recheck (screened)");
+                HIP_TRY(launch_overlap_screened(d, s, nw, g->cfg.debug_flags & (FBP_DEBUG_OVERLAP_FULL | FBP_DEBUG_OVERLAP_GLOBAL | FBP_DEBUG_OVERLAP_EXACT), st), "overlap recheck (screened)");
             else
                 HIP_TRY(launch_overlap_direct(d, s, nw, st), "overlap recheck");
             HIP_TRY(launch_check_overlap(d, s, nw, st), "overlap reassign");
