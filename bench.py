@@ -103,7 +103,7 @@ def main():
     ms_per_step = 1000.0 * elapsed / args.steps
     value = C * K * args.steps / elapsed
 
-    stages, roofline = roofline_of(args, sums, C, T, N, M, K)
+    stages, roofline = roofline_of(args, sums, C, T, N, M, K, int(last["n_passes"]))
 
     plain = None
     if args.plain_steps > 0 and (args.precision != "fp32" or args.overlap_mode != "direct"):
@@ -257,10 +257,33 @@ def serving_mix_run(args):
             "arrivals_span_s": specs[-1]['arrival_s']}
 
 
-def roofline_of(args, sums, C, T, N, M, K):
-    """Algorithmic work of the two dominant stages and the roofline entry of
-    the slower one (work / HIP-event time summed over the timed steps)."""
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kernels_16ch-300s_bf16-screened.json")
+
+
+def measured_traffic(kernels, passes, workload, precision, overlap_mode):
+    """HBM bytes per pass of a stage (FETCH_SIZE + WRITE_SIZE of its kernels,
+    calibrated against a copy of known size) from the committed rocprofv3
+    summary of this workload, or None."""
+    if workload != "16ch-300s" or precision != "bf16" or overlap_mode != "screened":
+        return None
+    try:
+        tr = json.load(open(TRAFFIC_FILE))["traffic"]
+    except (OSError, ValueError, KeyError):
+        return None
+    total = 0.0
+    for k in kernels:
+        t = tr.get(k)
+        if not t or t["fetch_bytes_per_dispatch"] is None or t["write_bytes_per_dispatch"] is None:
+            return None
+        total += (t["fetch_bytes_per_dispatch"] + t["write_bytes_per_dispatch"]) * t["dispatches"]
+    return total / max(1, passes)
+
+
+def roofline_of(args, sums, C, T, N, M, K, passes_per_step):
+    """Algorithmic work of the two stages (HIP-event time summed over the timed
+    steps) and the roofline entry of the slower one."""
     S = 2 * M + 1
+    R = N + 2 * M
     pos = sums["corr_positions"]
     stages = {}
     t_corr = sums["ms_correlation"] * 1e-3
@@ -272,10 +295,14 @@ def roofline_of(args, sums, C, T, N, M, K):
                                  "work": "2*T*C*N flop per likelihood position (float32 FMA chain)"}
     else:
         nbytes = (4.0 * C + 4.0 * T) * pos
-        stages["correlation"] = {"kernel": "k_corr_mfma + k_rescore", "ms": sums["ms_correlation"],
+        flop = 2.0 * T * C * N * pos
+        stages["correlation"] = {"kernel": "k_corr_mfma + k_rescore_res", "ms": sums["ms_correlation"],
                                  "launches": sums["corr_launches"], "bound": "hbm", "unit": "GB/s",
                                  "achieved": nbytes / t_corr / 1e9 if t_corr else 0.0, "peak": PEAK_HBM_GBS,
-                                 "work": "4*C bytes of voltage read + 4*T bytes of likelihood written per position"}
+                                 "mfma_tflops": flop / t_corr / 1e12 if t_corr else 0.0,
+                                 "mfma_peak": PEAK_TFLOPS[args.precision],
+                                 "work": "4*C bytes of voltage read + 4*T bytes of likelihood written per "
+                                         "position; 2*T*C*N matrix-core flop per position"}
     t_ov = sums["ms_overlap"] * 1e-3
     if args.overlap_mode == "direct":
         flop = (5.0 * C * N + C) * sums["overlap_combos"]
@@ -284,20 +311,34 @@ def roofline_of(args, sums, C, T, N, M, K):
                              "achieved": flop / t_ov / 1e12 if t_ov else 0.0, "peak": PEAK_TFLOPS["fp32"],
                              "work": "5*C*N + C flop per scored (t,f,s) combination"}
     else:
-        # per overlap window: Q table 2*C*sum_s(valid j) per template, plus the
-        # float64 screen (~12 flop) per unskipped combination and the exact
-        # float32 chain of the re-scored ones.
+        # Per overlap window: the window voltage (read by the Q-table kernel and
+        # the screen), the main template, the likelihood signs at every (t, s),
+        # the Q table written and read back, and one 4-byte AB entry per
+        # unskipped combination.  The Q table itself is 3 bf16 products per
+        # multiply-add of the truncated correlations.
+        win = sums.get("overlap_windows", 0)
+        nbytes = 4.0 * sums["screened_combos"] + win * (2.0 * C * N * 4 + C * R * 4 + 3.0 * T * S * 4)
         valid = sum(min(N, N + s - M) - max(0, s - M) for s in range(S))
-        flop = (2.0 * C * T * valid * sums.get("overlap_windows", 0) + 12.0 * sums["screened_combos"]
-                + (5.0 * C * N + C) * sums["overlap_combos"])
-        stages["overlap"] = {"kernel": "k_overlap_screened", "ms": sums["ms_overlap"],
-                             "launches": sums["overlap_launches"], "bound": "mfma", "unit": "TFLOP/s",
-                             "achieved": flop / t_ov / 1e12 if t_ov else 0.0, "peak": PEAK_TFLOPS["fp64"],
-                             "work": "float64 VALU: truncated correlations + screen + exact re-scores"}
+        qflop = 3.0 * 2.0 * C * T * valid * win
+        stages["overlap"] = {"kernel": "k_overlap_q + k_overlap_screened", "ms": sums["ms_overlap"],
+                             "launches": sums["overlap_launches"], "bound": "hbm", "unit": "GB/s",
+                             "achieved": nbytes / t_ov / 1e9 if t_ov else 0.0, "peak": PEAK_HBM_GBS,
+                             "mfma_tflops": qflop / t_ov / 1e12 if t_ov else 0.0, "mfma_peak": PEAK_TFLOPS["bf16"],
+                             "work": "4 B of the AB table per unskipped combination + per window voltage, "
+                                     "main template, likelihood signs and the Q table (bytes); split-bf16 "
+                                     "Q table (matrix-core flop)"}
     dom = max(stages, key=lambda k: stages[k]["ms"])
     st = stages[dom]
+    kernels = {"correlation": ["k_corr_mfma<1, 4, 2>", "k_rescore_res"],
+               "overlap": ["k_overlap_q", "k_overlap_screened<true>", "k_sort_ov_by_label", "k_check_overlap"]}
+    traffic = measured_traffic(kernels.get(dom, []), passes_per_step, args.workload, args.precision,
+                               args.overlap_mode)
+    per_launch = st["achieved"] * (st["ms"] * 1e-3 / max(1, st["launches"])) * (1e9 if st["unit"] == "GB/s" else 1e12)
     roofline = {"bound": st["bound"], "kernel": st["kernel"], "achieved": st["achieved"], "peak": st["peak"],
-                "unit": st["unit"], "frac": st["achieved"] / st["peak"], "traffic": None, "work": st["work"]}
+                "unit": st["unit"], "frac": st["achieved"] / st["peak"],
+                "algorithmic_per_launch": per_launch, "ms_per_launch": st["ms"] / max(1, st["launches"]),
+                "traffic": traffic, "traffic_unit": "bytes per pass (rocprofv3 FETCH_SIZE + WRITE_SIZE)",
+                "work": st["work"]}
     return stages, roofline
 
 

@@ -725,20 +725,6 @@ static size_t rescore_res_lds(const Dims &d)
     return ((uint64_t)d.C * d.N * d.Tp + kRescoreResWaves * span) * sizeof(float);
 }
 
-// Float32 re-score of the queued (template, position) entries.
-__global__ __launch_bounds__(256) void k_rescore(Dims d, const float *__restrict__ V, const float *__restrict__ Tm,
-                                                 const float *__restrict__ bias, const uint64_t *__restrict__ rq,
-                                                 const Counters *__restrict__ ctr, uint32_t rq_cap, float *L)
-{
-    const uint32_t n = min(ctr->n_rescore, rq_cap);
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const uint64_t ent = rq[k];
-        const uint32_t t = (uint32_t)(ent >> 32);
-        const uint64_t i = ent & 0xffffffffull;
-        L[(uint64_t)t * d.K + i] = exact_likelihood(d, V, Tm, bias, t, i);
-    }
-}
-
 __global__ void k_zero_rescore(Counters *ctr)
 {
     if (threadIdx.x == 0) ctr->n_rescore = 0;
