@@ -22,6 +22,7 @@ struct Dims {
     uint64_t items;    // S*S*T overlap combinations per window
     uint64_t nl;       // ceil(items / reduce_wg) reduction groups per window
     uint32_t ov_batch; // screened overlap: windows per Q-table batch (rows of ovQ)
+    uint32_t sup_max;  // widest channel support of a template (C until templates are set)
 };
 
 // Device counters, read back by the host once per pass.
@@ -61,6 +62,8 @@ struct DeviceState {
     uint32_t *part_t;    // [W] its template
     uint32_t *part_p;    // [W] its position
     float *ovQ;          // [ov_batch][T][S] screened overlap: Q - B2/2 of a batch of windows
+    uint32_t *sup_lo;    // [T] first channel on which the template is nonzero
+    uint32_t *sup_hi;    // [T] one past the last such channel (sup_lo == sup_hi: all zero)
     uint8_t *has;        // [W] add decision per entry of the pass list
     uint32_t *cand_idx;  // [W] spike onset per entry of the pass list
     uint32_t *cand_lab;  // [W]

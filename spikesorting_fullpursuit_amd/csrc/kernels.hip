@@ -1094,6 +1094,53 @@ __device__ __forceinline__ float overlap_direct_score(const Dims &d, const float
     return acc;
 }
 
+// The same score over the channels where either template is nonzero: the
+// main template's rows amain[c - lo_m] for c in [lo_m, hi_m), the partner's
+// bt[c - lo_t] for c in [lo_t, hi_t), a zero row elsewhere.  A channel where
+// both are zero adds exactly 0 to the reference chain (every product is +-0,
+// the running sum is never -0 and ss = 0 skips the subtraction), so it is
+// skipped.
+__device__ __forceinline__ float overlap_support_score(const Dims &d, const float *__restrict__ sv,
+                                                       const float *amain, uint32_t lo_m, uint32_t hi_m,
+                                                       const float *bt, uint32_t lo_t, uint32_t hi_t,
+                                                       const float *zero, uint32_t f, uint32_t s)
+{
+    const uint32_t R = d.N + 2 * d.M;
+    const uint32_t c0 = min(lo_m, lo_t), c1 = max(hi_m, hi_t);
+    float acc = 0.0f;
+    for (uint32_t c = c0; c < c1; c++) {
+        const bool ha = c >= lo_m && c < hi_m, hb = c >= lo_t && c < hi_t;
+        if (!ha && !hb) continue;
+        const float *a = (ha ? amain + (c - lo_m) * R : zero) + 2 * d.M - f;
+        const float *b = (hb ? bt + (uint64_t)(c - lo_t) * R : zero) + 2 * d.M - s;
+        const float *vc = sv + c * d.N;
+        float ss = 0.0f;
+        uint32_t j = 0;
+        for (; j + 8 <= d.N; j += 8) {
+            float av[8], bv[8], vv[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                av[q] = a[j + q];
+                bv[q] = b[j + q];
+                vv[q] = vc[j + q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const float sum = av[q] + bv[q];
+                ss = __builtin_fmaf(sum, sum, ss);       // bp.cl:621
+                acc = __builtin_fmaf(sum, vv[q], acc);   // bp.cl:622
+            }
+        }
+        for (; j < d.N; j++) {
+            const float sum = a[j] + b[j];
+            ss = __builtin_fmaf(sum, sum, ss);
+            acc = __builtin_fmaf(sum, vc[j], acc);
+        }
+        if (ss > 0.0f) acc = (float)((double)acc - 0.5 * (double)ss);   // bp.cl:627-630
+    }
+    return acc;
+}
+
 // Tpad[t][c][y] = T[t][c][y - M] for y - M in [0, N), +0.0 otherwise (y < N + 2M).
 __global__ void k_pad_templates(Dims d, const float *__restrict__ Tm, float *__restrict__ Tpad)
 {
@@ -1274,7 +1321,7 @@ constexpr uint32_t kOvCand = 4 * 256;
 // incrementally; no division per element.
 template <int B, class Fn>
 __device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t *spos, const uint32_t *flist,
-                                                   uint32_t n_fpos, uint32_t n_fneg, const uint32_t *plist,
+                                                   uint32_t n_fpos, uint32_t n_fneg, const uint16_t *plist,
                                                    uint32_t n_ppos, const float *__restrict__ ABl,
                                                    const float *sQ, const double *sE, Fn fn)
 {
@@ -1549,22 +1596,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw, float *best_L,
     const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov,
     const float *__restrict__ AB, const float *__restrict__ B2, const float *__restrict__ tnorm,
-    const float *__restrict__ Tpad, const float *__restrict__ ovQ, uint32_t kbase, uint32_t debug)
+    const float *__restrict__ Tpad, const float *__restrict__ ovQ, uint32_t kbase,
+    const uint32_t *__restrict__ sup_lo, const uint32_t *__restrict__ sup_hi, uint32_t debug)
 {
     extern __shared__ float dyn[];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
-    const uint32_t TS = T * S;
+    const uint32_t TS = T * S, R = N + 2 * M, SM = d.sup_max;
     const QPlan qp = q_plan(d);
     const FastDiv divN(N), divS(S);
     double *sE = (double *)dyn;                         // [T] screen bound per template (this window)
     float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset
-    float *smain = sv + C * N;                          // [C][N + 2M] padded template of the best label
-    float *sb = smain + C * (N + 2 * M);                // [C][N + 2M] partner template of an exact score
-    float *sQ = sb + C * (N + 2 * M);                   // [T][S] Q - B2/2
-    uint32_t *plist = (uint32_t *)(sQ + TS);            // (t,s) with positive likelihood
-    uint32_t *flist = plist + TS;                       // main shifts: positive first, then the rest
+    float *smain = sv + C * N;                          // [SM][R] padded best-label template, its support rows
+    float *sb = smain + SM * R;                         // [SM][R] partner template of an exact score
+    float *szero = sb + SM * R;                         // [R] zeros
+    float *sQ = szero + R;                              // [T][S] Q - B2/2
+    uint32_t *flist = (uint32_t *)(sQ + TS);            // main shifts: positive first, then the rest
     uint32_t *cand = flist + S;                         // [kOvCand] screen candidates (f << 16 | e)
-    uint8_t *spos = (uint8_t *)(cand + kOvCand);        // [T][S] bit1 in bounds, bit0 likelihood > 0
+    uint16_t *plist = (uint16_t *)(cand + kOvCand);     // [T*S] (t,s) with positive likelihood (T*S < 2^16)
+    uint8_t *spos = (uint8_t *)(plist + TS);            // [T][S] bit1 in bounds, bit0 likelihood > 0
+    for (uint32_t x = threadIdx.x; x < R; x += blockDim.x) szero[x] = 0.0f;
     __shared__ uint32_t n_cand, s_settled;
     __shared__ float r_sc[4];
     __shared__ uint64_t r_key[4], r_p[4];
@@ -1610,12 +1660,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
             // staging (index quotients by FastDiv: every index is below the
             // LDS size)
-            const uint32_t R = N + 2 * M;
+            const uint32_t lo_m = sup_lo[lab], hi_m = sup_hi[lab];
             double vv = 0.0;
             batched_stage<8>(C * N, [&](uint32_t e) { const uint32_t c = divN.div(e);
                                                        return V[(uint64_t)c * d.K + bidx + (e - c * N)]; },
                              [&](uint32_t e, float x) { sv[e] = x; vv = __builtin_fma((double)x, (double)x, vv); });
-            batched_stage<8>(C * R, [&](uint32_t e) { return Tpad[(uint64_t)lab * C * R + e]; },
+            batched_stage<8>((hi_m - lo_m) * R, [&](uint32_t e) { return Tpad[((uint64_t)lab * C + lo_m) * R + e]; },
                              [&](uint32_t e, float x) { smain[e] = x; });
             if (MQ)   // Q - B2/2 of this window from the matrix-core kernel
                 batched_stage<8>(TS, [&](uint32_t e) { return ovQ[(uint64_t)(k - kbase) * TS + e]; },
@@ -1634,7 +1684,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 const bool inb = !((uint64_t)s + bidx < M || (uint64_t)s + bidx >= (uint64_t)d.K - N + M);
                 const bool pos = inb && lv > 0.0f;
                 spos[e] = (inb ? 2 : 0) | (pos ? 1 : 0);
-                if (pos) plist[atomicAdd(&n_ppos, 1u)] = e;
+                if (pos) plist[atomicAdd(&n_ppos, 1u)] = (uint16_t)e;
                 if (t == lab && pos) flist[atomicAdd(&n_fpos, 1u)] = s;
                 if (!MQ) {
                     const int32_t sa = (int32_t)M - (int32_t)s;
@@ -1714,7 +1764,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 uint32_t scored = 0;
                 const float *tmain = smain;
                 auto exact = [&](uint32_t f, uint32_t t, uint32_t s, const float *bt) {
-                    const float score = overlap_direct_score(d, sv, tmain, bt, f, s);
+                    const float score = overlap_support_score(d, sv, tmain, lo_m, hi_m, bt, sup_lo[t], sup_hi[t],
+                                                              szero, f, s);
                     scored++;
                     uint64_t p;
                     const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
@@ -1758,9 +1809,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         for (uint32_t k2 = 0; k2 < nc; k2++) {
                             const uint32_t f = cand[k2] >> 16, e = cand[k2] & 0xffffu;
                             const uint32_t t = e / S, s = e - t * S;
-                            const float *src = Tpad + (uint64_t)t * C * R;
+                            const float *src = Tpad + ((uint64_t)t * C + sup_lo[t]) * R;
                             __syncthreads();
-                            batched_stage<8>(C * R, [&](uint32_t x) { return src[x]; },
+                            batched_stage<8>((sup_hi[t] - sup_lo[t]) * R, [&](uint32_t x) { return src[x]; },
                                              [&](uint32_t x, float v) { sb[x] = v; });
                             __syncthreads();
                             if (threadIdx.x == 0) exact(f, t, s, sb);
@@ -1769,14 +1820,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         for (uint32_t k2 = threadIdx.x; k2 < nc; k2 += blockDim.x) {
                             const uint32_t f = cand[k2] >> 16, e = cand[k2] & 0xffffu;
                             const uint32_t t = e / S, s = e - t * S;
-                            exact(f, t, s, Tpad + (uint64_t)t * C * R);
+                            exact(f, t, s, Tpad + ((uint64_t)t * C + sup_lo[t]) * R);
                         }
                     }
                 } else {
                     for_each_unskipped<1>(d, spos, flist, nfp, nfn, plist, npp, ABl, sQ, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
                         const double sh = screen(f, qe, ab);
                         if (sh + E < lo_max) return;
-                        exact(f, t, s, Tpad + (uint64_t)t * C * R);
+                        exact(f, t, s, Tpad + ((uint64_t)t * C + sup_lo[t]) * R);
                     });
                 }
                 for (int off = 32; off > 0; off >>= 1) scored += __shfl_xor(scored, off);
@@ -2204,9 +2255,9 @@ hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t m
 
 static size_t overlap_screened_lds_mode(const Dims &d, bool)
 {
-    const uint64_t ts = (uint64_t)d.T * d.S;
-    return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + 2 * (uint64_t)d.C * (d.N + 2 * d.M) * 4 + ts * 4 + ts * 4 +
-           (uint64_t)d.S * 4 + (uint64_t)kOvCand * 4 + ts + 16;
+    const uint64_t ts = (uint64_t)d.T * d.S, R = d.N + 2 * d.M;
+    return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + (2 * (uint64_t)d.sup_max + 1) * R * 4 + ts * 4 +
+           (uint64_t)d.S * 4 + (uint64_t)kOvCand * 4 + ts * 2 + ts + 16;
 }
 
 // The matrix-core Q table is used whenever its kernel's LDS fits.
@@ -2274,11 +2325,11 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
                                s.best_lab, s.best_idx, s.tabB2, (const uint4 *)s.TmQ, s.ovQ, kb);
             hipLaunchKernelGGL(k_overlap_screened<true>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                                list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                               s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, debug);
+                               s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug);
         } else {
             hipLaunchKernelGGL(k_overlap_screened<false>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                                list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                               s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, debug);
+                               s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug);
         }
     }
     return hipGetLastError();

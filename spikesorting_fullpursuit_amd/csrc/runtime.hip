@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/fbp.h"
 #include "fbp_internal.h"
@@ -106,7 +107,7 @@ void free_state(fbp_segment *g)
                     s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
                     s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabB2, s.tnorm,
                     s.TmB, s.rq, s.tscale, s.rdesc, s.TmQ, s.Tpad, s.ov_sorted, s.part_v, s.part_t, s.part_p,
-                    s.ovQ};
+                    s.ovQ, s.sup_lo, s.sup_hi};
     for (void *p : ptrs) dfree(p, g->own_stream);
     if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
     s = DeviceState{};
@@ -122,7 +123,7 @@ uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
     uint64_t b = 2 * r(CK * 4) + r(T * d.C * d.N * 4) + r((uint64_t)d.C * d.N * d.Tp * 4) + r(d.Tp * 4) +
                  r(T * 4) + r(T * d.K * 4) + 5 * r(W * 4) + 2 * r(W + 2) + 3 * r(W * 4) + r(W) + 3 * r(W * 4) +
                  r((nb + 1) * 4) + r(sizeof(Counters)) + r(T * 4) + 2 * r(W * 4) + r(T * 4) +
-                 r(T * d.C * (d.N + 2 * d.M) * 4) + 3 * r(W * 4);
+                 r(T * d.C * (d.N + 2 * d.M) * 4) + 3 * r(W * 4) + 2 * r(T * 4);
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED)
         b += r(T * d.S * T * d.S * 4) + r(T * d.S * 4) + r(overlap_qtemplate_floats(d) * 4) + r(W * 4) +
              r((uint64_t)d.ov_batch * T * d.S * 4);
@@ -243,8 +244,12 @@ static int make_dims(const fbp_config *cfg, Dims &d)
         if (nb > d.W) nb = d.W;
         d.ov_batch = (uint32_t)nb;
     }
+    // the screened overlap's LDS grows with the widest template support, known
+    // when the templates are set; here it must fit for the narrowest one
+    d.sup_max = 1;
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED && overlap_screened_lds(d) > kMaxLds)
         return fail(FBP_EINVAL, "screened overlap needs %zu bytes of LDS (T*S too large)", overlap_screened_lds(d));
+    d.sup_max = C;
     return FBP_OK;
 }
 
@@ -290,6 +295,8 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     ALLOC(s.win, W)
     ALLOC(s.win_next, W)
     ALLOC(s.ov_list, W)
+    ALLOC(s.sup_lo, T)
+    ALLOC(s.sup_hi, T)
     ALLOC(s.part_v, W)
     ALLOC(s.part_t, W)
     ALLOC(s.part_p, W)
@@ -370,11 +377,37 @@ int fbp_segment_set_templates(fbp_segment *g, const float *templates, const floa
 {
     if (!g || !templates || !bias || !thresholds) return fail(FBP_EINVAL, "NULL argument");
     if (int rc = set_device(g)) return rc;
+    // channel support of every template: the channels on which it is nonzero
+    // (the kernels skip the others, whose terms are exactly zero)
+    std::vector<uint32_t> lo(g->d.T), hi(g->d.T);
+    uint32_t sup_max = 1;
+    for (uint32_t t = 0; t < g->d.T; t++) {
+        uint32_t a = g->d.C, b = 0;
+        for (uint32_t c = 0; c < g->d.C; c++) {
+            const float *row = templates + ((uint64_t)t * g->d.C + c) * g->d.N;
+            for (uint32_t j = 0; j < g->d.N; j++)
+                if (row[j] != 0.0f) {
+                    if (c < a) a = c;
+                    b = c + 1;
+                    break;
+                }
+        }
+        if (a >= b) a = b = 0;
+        lo[t] = a;
+        hi[t] = b;
+        if (b - a > sup_max) sup_max = b - a;
+    }
+    g->d.sup_max = sup_max;
+    if (g->cfg.overlap_mode == FBP_OVERLAP_SCREENED && overlap_screened_lds(g->d) > kMaxLds)
+        return fail(FBP_EINVAL, "screened overlap needs %zu bytes of LDS (templates span %u channels)",
+                    overlap_screened_lds(g->d), sup_max);
     const Dims &d = g->d;
     hipStream_t st = g->own_stream;
     float *tmp_bias = nullptr;
     HIP_TRY(dalloc(&tmp_bias, d.T, st), "bias staging");
     hipError_t e = hipMemcpyAsync(g->s.Tm, templates, (uint64_t)d.T * d.C * d.N * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->s.sup_lo, lo.data(), (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->s.sup_hi, hi.data(), (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(tmp_bias, bias, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(g->s.thr, thresholds, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_transpose_templates(d, g->s.Tm, tmp_bias, g->s.TmT, g->s.bias, st);

@@ -153,3 +153,23 @@ def test_low_precision_dense_16ch(oracle_mod, precision):
     np.testing.assert_array_equal(g['residual'].view(np.uint32), o['residual'].view(np.uint32))
     frac = g['stats']['rescored_positions'] / g['stats']['corr_positions'] / 20
     print("re-scored fraction of (t, i):", frac, flush=True)
+
+
+@pytest.mark.parametrize("precision,mode", [("fp32", "screened"), ("bf16", "screened"), ("fp32", "direct")])
+def test_sparse_channel_support(oracle_mod, precision, mode):
+    # Templates nonzero on 6 of 32 channels (a dense probe): the screened
+    # overlap stores and scores only the channels a template is nonzero on.
+    from spikesorting_fullpursuit_amd import synthetic
+    rec = synthetic.make_recording(41, 32, 30000, 0.4, 12, 41, (8.0, 20.0), (5.0, 60.0), gains="gauss",
+                                   spatial_sigma=2.0, support=6, overlap_frac=0.1)
+    sm = rec['separability_metrics']
+    rec['bias'] = np.float32(-0.5 * sm['template_SS'])
+    rec['thr'] = np.float32(sm['neuron_lower_thresholds'])
+    rec['N'] = 41
+    print("case sparse support", precision, mode, flush=True)
+    g = run_gpu(rec, 20, True, 256, precision=precision, overlap_mode=mode)
+    o = run_oracle(oracle_mod, rec, 20, True, 256)
+    assert o['idx'].size > 0
+    np.testing.assert_array_equal(g['idx'], o['idx'])
+    np.testing.assert_array_equal(g['lab'], o['lab'])
+    np.testing.assert_array_equal(g['residual'].view(np.uint32), o['residual'].view(np.uint32))
