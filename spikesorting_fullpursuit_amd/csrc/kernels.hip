@@ -531,7 +531,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
             qbase = __shfl(qbase, 63);
         }
         uint32_t pos = qbase + incl - nflag;
-        // this (window, template chunk)'s slice of the queue, for k_rescore_win
+        // this (window, template chunk)'s slice of the queue, for the re-score kernels
         if (lane == 0) {
             const uint32_t nchunk = (CT + CTB - 1) / CTB;
             const uint32_t kept = qbase >= rq_cap ? 0u : min(tot, rq_cap - qbase);
@@ -576,79 +576,52 @@ __device__ __forceinline__ float fma_chain(float acc, const float *tp, uint32_t 
     return acc;
 }
 
-// Float32 re-score of one (window, template chunk)'s queued entries: one wave,
-// each lane carrying up to kRescoreR chains.  The window's voltage is staged in
-// LDS one channel at a time and template values come from the [C][N][Tp] copy,
-// so for every (c, j) a wave reads one LDS sample per chain and one short run
-// of template values.  Each chain is the reference's: bias, then c = 0..C-1,
-// j = 0..N-1, fused multiply-adds.
-constexpr int kRescorePre = 32;   // per-lane prefetch slots for one channel's templates
-
-__global__ __launch_bounds__(64) void k_rescore_win(Dims d, const float *__restrict__ V,
-                                                    const float *__restrict__ TmT, const float *__restrict__ bias,
-                                                    const uint64_t *__restrict__ rq, const uint2 *__restrict__ rdesc,
-                                                    const uint32_t *__restrict__ win, uint32_t nchunk, float *L)
+// Float32 re-score of one (window, template chunk)'s queued entries when the
+// templates do not fit in LDS: one wave, one chain per lane, over the channels
+// the lane's template is nonzero on, with template rows and voltage read from
+// global memory (L1/L2) eight samples ahead of the chain.  Each chain is the
+// reference's: bias, then c ascending, j = 0..N-1, fused multiply-adds; a
+// channel where the template is zero adds exactly 0 (the running sum starts at
+// the nonzero bias and is never -0), so it is skipped.
+__device__ __forceinline__ float fma_chain_contig(float acc, const float *__restrict__ tp,
+                                                  const float *__restrict__ vp, uint32_t n)
 {
-    extern __shared__ float st_tm[];                  // [N][Tp] templates of channel c
-    __shared__ float sv[2 * kMfmaMaxRT * 16 + 8];     // window voltage of channel c
+    uint32_t j = 0;
+    for (; j + 8 <= n; j += 8) {
+        float a[8], b[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            a[q] = tp[j + q];
+            b[q] = vp[j + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc = __builtin_fmaf(a[q], b[q], acc);
+    }
+    for (; j < n; j++) acc = __builtin_fmaf(tp[j], vp[j], acc);
+    return acc;
+}
+
+__global__ __launch_bounds__(64) void k_rescore_sup(Dims d, const float *__restrict__ V,
+                                                    const float *__restrict__ Tm, const float *__restrict__ bias,
+                                                    const uint32_t *__restrict__ sup_lo,
+                                                    const uint32_t *__restrict__ sup_hi,
+                                                    const uint64_t *__restrict__ rq, const uint2 *__restrict__ rdesc,
+                                                    float *L)
+{
     const uint2 dsc = rdesc[blockIdx.x];
     const uint32_t cnt = dsc.y;
     if (cnt == 0) return;
-    const int lane = threadIdx.x;
-    const uint32_t w = win[blockIdx.x / nchunk];
-    const uint64_t base = (uint64_t)w * d.N;
-    const uint32_t span = ((d.N + 15) / 16) * 16 + d.N;
-    const uint32_t ntm = d.N * d.Tp;
-    const bool pre = ntm <= 64u * kRescorePre && span <= 64u * 4;
+    const uint32_t lane = threadIdx.x;
     for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
         const uint32_t e = e0 + lane;
-        const bool ok = e < cnt;
-        const uint64_t ent = ok ? rq[dsc.x + e] : 0;
-        const uint32_t t = ok ? (uint32_t)(ent >> 32) : 0u;
-        const uint32_t p = ok ? (uint32_t)((ent & 0xffffffffull) - base) : 0u;
+        if (e >= cnt) break;
+        const uint64_t ent = rq[dsc.x + e];
+        const uint32_t t = (uint32_t)(ent >> 32);
+        const uint64_t i = ent & 0xffffffffull;
         float acc = bias[t];
-        // channel c's staging values are loaded into registers while channel
-        // c-1 runs its chains
-        float rt_[kRescorePre], rv[4];
-        auto fetch = [&](uint32_t c) {
-            const float *tc = TmT + (uint64_t)c * ntm;
-            const float *vr = V + (uint64_t)c * d.K;
-#pragma unroll
-            for (int q = 0; q < kRescorePre; q++) {
-                const uint32_t x = lane + 64u * q;
-                rt_[q] = x < ntm ? tc[x] : 0.0f;
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint32_t x = lane + 64u * q;
-                rv[q] = (x < span && base + x < d.K) ? vr[base + x] : 0.0f;
-            }
-        };
-        if (pre) fetch(0);
-        for (uint32_t c = 0; c < d.C; c++) {
-            __syncthreads();
-            if (pre) {
-#pragma unroll
-                for (int q = 0; q < kRescorePre; q++) {
-                    const uint32_t x = lane + 64u * q;
-                    if (x < ntm) st_tm[x] = rt_[q];
-                }
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint32_t x = lane + 64u * q;
-                    if (x < span) sv[x] = rv[q];
-                }
-            } else {
-                const float *vr = V + (uint64_t)c * d.K;
-                for (uint32_t x = lane; x < span; x += 64) sv[x] = base + x < d.K ? vr[base + x] : 0.0f;
-                const float *tc = TmT + (uint64_t)c * ntm;
-                for (uint32_t x = lane; x < ntm; x += 64) st_tm[x] = tc[x];
-            }
-            __syncthreads();
-            if (pre && c + 1 < d.C) fetch(c + 1);
-            acc = fma_chain(acc, st_tm + t, d.Tp, sv + p, d.N);
-        }
-        if (ok) L[(uint64_t)t * d.K + base + p] = acc;
+        for (uint32_t c = sup_lo[t]; c < sup_hi[t]; c++)
+            acc = fma_chain_contig(acc, Tm + ((uint64_t)t * d.C + c) * d.N, V + (uint64_t)c * d.K + i, d.N);
+        L[(uint64_t)t * d.K + i] = acc;
     }
 }
 
@@ -2180,14 +2153,8 @@ hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_wi
                                s.bias, s.rq, (const uint2 *)s.rdesc, s.win, ndesc, nchunk, s.L);
             return hipGetLastError();
         }
-        const size_t lds = (size_t)d.N * d.Tp * sizeof(float);
-        if (lds > 64 * 1024) {
-            hipError_t e2 = hipFuncSetAttribute((const void *)k_rescore_win,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e2 != hipSuccess) return e2;
-        }
-        hipLaunchKernelGGL(k_rescore_win, dim3(n_win * nchunk), dim3(64), lds, st, d, s.V, s.TmT, s.bias, s.rq,
-                           (const uint2 *)s.rdesc, s.win, nchunk, s.L);
+        hipLaunchKernelGGL(k_rescore_sup, dim3(n_win * nchunk), dim3(64), 0, st, d, s.V, s.Tm, s.bias, s.sup_lo,
+                           s.sup_hi, s.rq, (const uint2 *)s.rdesc, s.L);
     }
     return hipGetLastError();
 }

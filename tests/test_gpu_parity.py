@@ -173,3 +173,26 @@ def test_sparse_channel_support(oracle_mod, precision, mode):
     np.testing.assert_array_equal(g['idx'], o['idx'])
     np.testing.assert_array_equal(g['lab'], o['lab'])
     np.testing.assert_array_equal(g['residual'].view(np.uint32), o['residual'].view(np.uint32))
+
+
+def test_large_template_set_rescore(oracle_mod):
+    # 64 channels x 30 templates of 61 samples (0.47 MB) do not fit in LDS: the
+    # float32 re-score reads template rows and voltage from global memory over
+    # each template's 8-channel support.
+    from spikesorting_fullpursuit_amd import synthetic
+    rec = synthetic.make_recording(43, 64, 30000, 0.25, 30, 61, (8.0, 20.0), (5.0, 60.0), gains="gauss",
+                                   spatial_sigma=2.0, support=8, overlap_frac=0.1)
+    sm = rec['separability_metrics']
+    rec['bias'] = np.float32(-0.5 * sm['template_SS'])
+    rec['thr'] = np.float32(sm['neuron_lower_thresholds'])
+    rec['N'] = 61
+    print("case 64ch x 30 templates", flush=True)
+    g = run_gpu(rec, 30, True, 256, precision="bf16", overlap_mode="screened")
+    o = run_oracle(oracle_mod, rec, 30, True, 256)
+    assert o['idx'].size > 0
+    assert g['stats']['rescored_positions'] > 0
+    np.testing.assert_array_equal(g['idx'], o['idx'])
+    np.testing.assert_array_equal(g['lab'], o['lab'])
+    np.testing.assert_array_equal(g['residual'].view(np.uint32), o['residual'].view(np.uint32))
+    pos = o['L'] > 0
+    np.testing.assert_array_equal(g['L'][pos].view(np.uint32), o['L'][pos].view(np.uint32))
