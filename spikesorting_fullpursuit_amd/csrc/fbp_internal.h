@@ -64,6 +64,9 @@ struct DeviceState {
     float *ovQ;          // [ov_batch][T][S] screened overlap: Q - B2/2 of a batch of windows
     uint32_t *sup_lo;    // [T] first channel on which the template is nonzero
     uint32_t *sup_hi;    // [T] one past the last such channel (sup_lo == sup_hi: all zero)
+    uint32_t *tord;      // [CT*16] matrix-core template order: by channel support, then index
+    uint32_t *tile_lo;   // [CT] first channel on which a 16-template tile (in tord) is nonzero
+    uint32_t *tile_hi;   // [CT] one past its last
     uint8_t *has;        // [W] add decision per entry of the pass list
     uint32_t *cand_idx;  // [W] spike onset per entry of the pass list
     uint32_t *cand_lab;  // [W]

@@ -251,7 +251,8 @@ constexpr float kFp8Limit = 440.0f;
 // (0 when padded), scaled by tscale[t] (a power of two; 1 except for fp8).
 template <int PREC>
 __global__ void k_pack_templates(Dims d, const float *__restrict__ Tm, const float *__restrict__ tscale,
-                                 Frag<PREC> *__restrict__ TmB, uint32_t KS, uint32_t CT)
+                                 const uint32_t *__restrict__ tord, Frag<PREC> *__restrict__ TmB, uint32_t KS,
+                                 uint32_t CT)
 {
     using Tr = NarrowTraits<PREC>;
     const uint64_t total = (uint64_t)d.C * KS * CT * 64;
@@ -262,7 +263,8 @@ __global__ void k_pack_templates(Dims d, const float *__restrict__ Tm, const flo
         const uint32_t ct = (uint32_t)(r % CT);
         const uint32_t ks = (uint32_t)((r / CT) % KS);
         const uint32_t c = (uint32_t)(r / ((uint64_t)CT * KS));
-        const uint32_t t = ct * 16 + (lane & 15);
+        const uint32_t ts = ct * 16 + (lane & 15);
+        const uint32_t t = ts < d.T ? tord[ts] : d.T;   // column ts holds template tord[ts]
         const float sc = t < d.T ? tscale[t] : 1.0f;
         Frag<PREC> v;
         for (uint32_t w = 0; w < Tr::kWords; w++) v.w[w] = 0;
@@ -330,7 +332,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
                                                   const uint32_t *__restrict__ win, float *__restrict__ L,
                                                   Counters *ctr, uint64_t *__restrict__ rq, uint32_t rq_cap,
                                                   uint2 *__restrict__ rdesc, uint32_t cb_max, ScreenCoef co,
-                                                  uint32_t no_rescore)
+                                                  uint32_t no_rescore, const uint32_t *__restrict__ tord,
+                                                  const uint32_t *__restrict__ tile_lo,
+                                                  const uint32_t *__restrict__ tile_hi)
 {
     using Tr = NarrowTraits<PREC>;
     constexpr uint32_t KS = (RT * 16 + Tr::kK - 1) / Tr::kK;   // k-steps per channel (N padded to RT*16)
@@ -434,8 +438,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
             for (int ct = 0; ct < CTB; ct++) acc[rt][ct] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+        // channels on which the chunk's template tiles are nonzero (templates
+        // are ordered by channel support, tiles of 16); elsewhere every B
+        // fragment is zero and an MFMA would add exact zeros
+        uint32_t tlo[CTB], thi[CTB], clo = d.C, chi = 0;
+#pragma unroll
+        for (int ct = 0; ct < CTB; ct++) {
+            tlo[ct] = ct < (int)nct ? tile_lo[ct0 + ct] : 0u;
+            thi[ct] = ct < (int)nct ? tile_hi[ct0 + ct] : 0u;
+            if (tlo[ct] < thi[ct]) {
+                clo = min(clo, tlo[ct]);
+                chi = max(chi, thi[ct]);
+            }
+        }
         for (uint32_t c0 = 0; c0 < d.C; c0 += cb_max) {
             const uint32_t cb = min(cb_max, d.C - c0);
+            if (!staged_once && (c0 + cb <= clo || c0 >= chi)) continue;
             if (!(staged_once && have)) {
                 __syncthreads();
                 stage(c0, cb);
@@ -458,14 +476,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
 #pragma unroll
                             for (uint32_t w = 0; w < Tr::kWords; w++) bn[ks][ct].w[w] = 0;
             };
-            fetch_b(0);
-            for (uint32_t c = 0; c < cb; c++) {
+            const uint32_t cs = clo > c0 ? clo - c0 : 0u, ce = chi > c0 ? min(cb, chi - c0) : 0u;
+            if (cs < ce) fetch_b(cs);
+            for (uint32_t c = cs; c < ce; c++) {
                 Frag<PREC> b[KS][CTB];
 #pragma unroll
                 for (uint32_t ks = 0; ks < KS; ks++)
 #pragma unroll
                     for (int ct = 0; ct < CTB; ct++) b[ks][ct] = bn[ks][ct];
-                if (c + 1 < cb) fetch_b(c + 1);
+                if (c + 1 < ce) fetch_b(c + 1);
 #pragma unroll
                 for (uint32_t ks = 0; ks < KS; ks++) {
 #pragma unroll
@@ -478,7 +497,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
 #pragma unroll
                         for (uint32_t w = 0; w < Tr::kWords; w++) a.w[w] = src[w];
 #pragma unroll
-                        for (int ct = 0; ct < CTB; ct++) acc[rt][ct] = mfma_narrow<PREC>(a, b[ks][ct], acc[rt][ct]);
+                        for (int ct = 0; ct < CTB; ct++)
+                            if (c0 + c >= tlo[ct] && c0 + c < thi[ct])
+                                acc[rt][ct] = mfma_narrow<PREC>(a, b[ks][ct], acc[rt][ct]);
                     }
                 }
             }
@@ -486,11 +507,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
         const bool force = PREC == 3 && over;
         // per-template constants of this lane's columns
         float cb_bias[CTB], cb_inv[CTB], cb_tn[CTB];
+        uint32_t cb_t[CTB];
         bool cb_ok[CTB];
 #pragma unroll
         for (int ct = 0; ct < CTB; ct++) {
-            const uint32_t t = (ct0 + ct) * 16 + (lane & 15);
-            cb_ok[ct] = ct < (int)nct && t < d.T;
+            const uint32_t ts = (ct0 + ct) * 16 + (lane & 15);
+            cb_ok[ct] = ct < (int)nct && ts < d.T;
+            const uint32_t t = cb_ok[ct] ? tord[ts] : 0u;   // this lane's column holds template tord[ts]
+            cb_t[ct] = t;
             cb_bias[ct] = cb_ok[ct] ? bias[t] : 0.0f;
             cb_inv[ct] = cb_ok[ct] ? 1.0f / tscale[t] : 1.0f;
             cb_tn[ct] = cb_ok[ct] ? tnorm[t] : 0.0f;
@@ -504,7 +528,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const uint32_t p = rt * 16 + 4 * (lane >> 4) + q;
-                    const uint32_t t = (ct0 + ct) * 16 + (lane & 15);
+                    const uint32_t t = cb_t[ct];
                     if (!cb_ok[ct] || p >= np) continue;
                     const uint64_t i = base + p;
                     if (i + d.N >= d.K) { L[(uint64_t)t * d.K + i] = 0.0f; continue; }   // bp.cl:231-234
@@ -547,7 +571,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
                     if (acc[rt][ct][q] == 0.0f || !cb_ok[ct]) continue;
                     const uint32_t p = rt * 16 + 4 * (lane >> 4) + q;
                     if (p >= np) continue;
-                    const uint32_t t = (ct0 + ct) * 16 + (lane & 15);
+                    const uint32_t t = cb_t[ct];
                     const uint64_t i = base + p;
                     if (i + d.N >= d.K) continue;
                     if (pos < rq_cap) rq[pos] = ((uint64_t)t << 32) | (uint64_t)i;
@@ -1204,7 +1228,9 @@ __global__ __launch_bounds__(256) void k_overlap_direct(
 // in the reference's winner order, so the result equals the direct form's.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_overlap_tables(Dims d, const float *__restrict__ Tm, float *__restrict__ AB,
-                                                       float *__restrict__ B2, float *__restrict__ tnorm)
+                                                       float *__restrict__ B2, float *__restrict__ tnorm,
+                                                       const uint32_t *__restrict__ sup_lo,
+                                                       const uint32_t *__restrict__ sup_hi)
 {
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
     const uint64_t nab = (uint64_t)T * S * T * S;
@@ -1219,8 +1245,8 @@ __global__ __launch_bounds__(256) void k_overlap_tables(Dims d, const float *__r
             const int32_t fa = (int32_t)M - (int32_t)f, sa = (int32_t)M - (int32_t)s;
             const int32_t lo = max(0, max(-fa, -sa));
             const int32_t hi = min((int32_t)N, min((int32_t)N - fa, (int32_t)N - sa));
-            double acc = 0.0;
-            for (uint32_t c = 0; c < C; c++) {
+            double acc = 0.0;   // channels where both templates can be nonzero
+            for (uint32_t c = max(sup_lo[lab], sup_lo[t]); c < min(sup_hi[lab], sup_hi[t]); c++) {
                 const float *a = Tm + ((uint64_t)lab * C + c) * N;
                 const float *b = Tm + ((uint64_t)t * C + c) * N;
                 for (int32_t j = lo; j < hi; j++) acc = __builtin_fma((double)a[j + fa], (double)b[j + sa], acc);
@@ -1232,7 +1258,7 @@ __global__ __launch_bounds__(256) void k_overlap_tables(Dims d, const float *__r
             const int32_t sa = (int32_t)M - (int32_t)s;
             const int32_t lo = max(0, -sa), hi = min((int32_t)N, (int32_t)N - sa);
             double acc = 0.0;
-            for (uint32_t c = 0; c < C; c++) {
+            for (uint32_t c = sup_lo[t]; c < sup_hi[t]; c++) {
                 const float *b = Tm + ((uint64_t)t * C + c) * N;
                 for (int32_t j = lo; j < hi; j++) acc = __builtin_fma((double)b[j + sa], (double)b[j + sa], acc);
             }
@@ -1412,7 +1438,9 @@ __device__ __forceinline__ uint64_t q_bfrag(const uint32_t *row, uint32_t y)
 }
 
 __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__restrict__ TmQ,
-                              const uint16_t *svh, const uint16_t *svl, float *sQ, float *sQp)
+                              const uint16_t *svh, const uint16_t *svl, float *sQ, float *sQp,
+                              const uint32_t *__restrict__ tord, const uint32_t *__restrict__ tile_lo,
+                              const uint32_t *__restrict__ tile_hi)
 {
     const uint32_t S = d.S, T = d.T, TS = T * S;
     const uint32_t lane = lane_of();
@@ -1421,7 +1449,8 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
     for (uint32_t task = wave; task < ntask; task += nwave) {
         const uint32_t ks = task % qp.KS, rest = task / qp.KS;
         const uint32_t g = rest % qp.NSG, tt = rest / qp.NSG;
-        const uint32_t c0 = ks * d.C / qp.KS, c1 = (ks + 1) * d.C / qp.KS;
+        // the slice's channels on which the tile's templates are nonzero
+        const uint32_t c0 = max(ks * d.C / qp.KS, tile_lo[tt]), c1 = max(c0, min((ks + 1) * d.C / qp.KS, tile_hi[tt]));
         const uint32_t st0 = g * qp.SG;
         const uint32_t nst = qp.ST - st0 < qp.SG ? qp.ST - st0 : qp.SG;
         float4v acch[4], accl[4];
@@ -1474,8 +1503,8 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
             const uint32_t scol = (st0 + j) * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const uint32_t t = tt * 16 + 4 * (lane >> 4) + r;
-                if (t < T && scol < S) dst[t * S + scol] = acch[j][r] + accl[j][r];
+                const uint32_t ts = tt * 16 + 4 * (lane >> 4) + r;
+                if (ts < T && scol < S) dst[tord[ts] * S + scol] = acch[j][r] + accl[j][r];
             }
         }
     }
@@ -1488,7 +1517,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     Dims d, const float *__restrict__ V, const float *__restrict__ L, const float *__restrict__ thr,
     const uint32_t *__restrict__ ov_list, const Counters *__restrict__ ctr, const uint32_t *__restrict__ best_lab,
     const uint32_t *__restrict__ best_idx, const float *__restrict__ B2, const uint4 *__restrict__ TmQ,
-    float *__restrict__ ovQ, uint32_t kbase)
+    float *__restrict__ ovQ, uint32_t kbase, const uint32_t *__restrict__ tord, const uint32_t *__restrict__ tile_lo,
+    const uint32_t *__restrict__ tile_hi)
 {
     extern __shared__ float dyn[];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
@@ -1516,7 +1546,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             svl[e] = to_bf16(x - __uint_as_float(h << 16));
         });
         __syncthreads();
-        q_table_split(d, qp, TmQ, svh, svl, sQ, sQp);
+        q_table_split(d, qp, TmQ, svh, svl, sQ, sQp, tord, tile_lo, tile_hi);
         __syncthreads();
         float *dst = ovQ + (uint64_t)(k - kbase) * TS;
         batched_stage<8>(TS, [&](uint32_t e) { return B2[e]; }, [&](uint32_t e, float b2) {
@@ -1536,7 +1566,8 @@ size_t overlap_q_lds(const Dims &d)
 // TmQ[(c*NKQ + kq)*TT + tt][lane] = {hi, lo} fragments (4 bf16 each, sample
 // kq*16 + 4*(lane/16) + i in bits 16i) of template t = tt*16 + lane%16 on
 // channel c; 0 when padded.  The A fragments of q_table_split.
-__global__ void k_pack_qtemplates(Dims d, const float *__restrict__ Tm, uint4 *__restrict__ TmQ)
+__global__ void k_pack_qtemplates(Dims d, const float *__restrict__ Tm, const uint32_t *__restrict__ tord,
+                                  uint4 *__restrict__ TmQ)
 {
     const uint32_t TT = (d.T + 15) / 16, NKQ = (d.N + 15) / 16;
     const uint64_t total = (uint64_t)d.C * NKQ * TT * 64;
@@ -1547,7 +1578,8 @@ __global__ void k_pack_qtemplates(Dims d, const float *__restrict__ Tm, uint4 *_
         const uint32_t tt = (uint32_t)(r % TT);
         const uint32_t kq = (uint32_t)((r / TT) % NKQ);
         const uint32_t c = (uint32_t)(r / ((uint64_t)TT * NKQ));
-        const uint32_t t = tt * 16 + (lane & 15);
+        const uint32_t ts = tt * 16 + (lane & 15);
+        const uint32_t t = ts < d.T ? tord[ts] : d.T;   // row ts holds template tord[ts]
         uint32_t h[2] = {0u, 0u}, l[2] = {0u, 0u};
         for (uint32_t i = 0; i < 4; i++) {
             const uint32_t x = kq * 16 + 4 * (lane >> 4) + i;
@@ -2070,6 +2102,7 @@ static hipError_t launch_pack_p(const Dims &d, const DeviceState &s, hipStream_t
 {
     const uint64_t total = (uint64_t)d.C * mfma_ks(d, PREC) * mfma_ct(d) * 64;
     hipLaunchKernelGGL(k_pack_templates<PREC>, dim3(grid_for(total, 256, 4096)), dim3(256), 0, st, d, s.Tm, s.tscale,
+                       s.tord,
                        (Frag<PREC> *)s.TmB, mfma_ks(d, PREC), mfma_ct(d));
     return hipGetLastError();
 }
@@ -2096,7 +2129,7 @@ static hipError_t launch_corr_mfma_rt(const Dims &d, const DeviceState &s, uint3
     ScreenCoef co{coef[0], coef[1], coef[2], coef[3]};
     hipLaunchKernelGGL((k_corr_mfma<PREC, RT, CTB>), dim3(n_win), dim3(64), lds, st, d, s.V, s.Tm,
                        (const Frag<PREC> *)s.TmB, s.bias, s.tnorm, s.tscale, s.win, s.L, s.ctr, s.rq, rq_cap,
-                       (uint2 *)s.rdesc, cb_max, co, no_rescore);
+                       (uint2 *)s.rdesc, cb_max, co, no_rescore, s.tord, s.tile_lo, s.tile_hi);
     return hipGetLastError();
 }
 
@@ -2253,9 +2286,9 @@ hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_
 {
     const uint64_t total = (uint64_t)d.T * d.S * d.T * d.S + (uint64_t)d.T * d.S + d.T;
     hipLaunchKernelGGL(k_overlap_tables, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st, d, s.Tm, s.tabAB,
-                       s.tabB2, s.tnorm);
+                       s.tabB2, s.tnorm, s.sup_lo, s.sup_hi);
     const uint64_t nq = overlap_qtemplate_floats(d);
-    hipLaunchKernelGGL(k_pack_qtemplates, dim3(grid_for(nq / 4, 256, 4096)), dim3(256), 0, st, d, s.Tm,
+    hipLaunchKernelGGL(k_pack_qtemplates, dim3(grid_for(nq / 4, 256, 4096)), dim3(256), 0, st, d, s.Tm, s.tord,
                        (uint4 *)s.TmQ);
     return hipGetLastError();
 }
@@ -2289,7 +2322,8 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
         const uint32_t grid = nb < 4096 ? nb : 4096;
         if (mq) {
             hipLaunchKernelGGL(k_overlap_q, dim3(grid), dim3(256), qlds, st, d, s.V, s.L, s.thr, list, s.ctr,
-                               s.best_lab, s.best_idx, s.tabB2, (const uint4 *)s.TmQ, s.ovQ, kb);
+                               s.best_lab, s.best_idx, s.tabB2, (const uint4 *)s.TmQ, s.ovQ, kb, s.tord, s.tile_lo,
+                               s.tile_hi);
             hipLaunchKernelGGL(k_overlap_screened<true>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                                list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
                                s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug);

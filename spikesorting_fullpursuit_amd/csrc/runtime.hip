@@ -8,6 +8,7 @@
 // memory, so the host never copies flags, lists or spikes between passes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -107,7 +108,7 @@ void free_state(fbp_segment *g)
                     s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
                     s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabB2, s.tnorm,
                     s.TmB, s.rq, s.tscale, s.rdesc, s.TmQ, s.Tpad, s.ov_sorted, s.part_v, s.part_t, s.part_p,
-                    s.ovQ, s.sup_lo, s.sup_hi};
+                    s.ovQ, s.sup_lo, s.sup_hi, s.tord, s.tile_lo, s.tile_hi};
     for (void *p : ptrs) dfree(p, g->own_stream);
     if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
     s = DeviceState{};
@@ -123,7 +124,7 @@ uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
     uint64_t b = 2 * r(CK * 4) + r(T * d.C * d.N * 4) + r((uint64_t)d.C * d.N * d.Tp * 4) + r(d.Tp * 4) +
                  r(T * 4) + r(T * d.K * 4) + 5 * r(W * 4) + 2 * r(W + 2) + 3 * r(W * 4) + r(W) + 3 * r(W * 4) +
                  r((nb + 1) * 4) + r(sizeof(Counters)) + r(T * 4) + 2 * r(W * 4) + r(T * 4) +
-                 r(T * d.C * (d.N + 2 * d.M) * 4) + 3 * r(W * 4) + 2 * r(T * 4);
+                 r(T * d.C * (d.N + 2 * d.M) * 4) + 3 * r(W * 4) + 2 * r(T * 4) + 3 * r((T + 15) / 16 * 16 * 4);
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED)
         b += r(T * d.S * T * d.S * 4) + r(T * d.S * 4) + r(overlap_qtemplate_floats(d) * 4) + r(W * 4) +
              r((uint64_t)d.ov_batch * T * d.S * 4);
@@ -297,6 +298,9 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     ALLOC(s.ov_list, W)
     ALLOC(s.sup_lo, T)
     ALLOC(s.sup_hi, T)
+    ALLOC(s.tord, (uint64_t)((T + 15) / 16) * 16)
+    ALLOC(s.tile_lo, (T + 15) / 16)
+    ALLOC(s.tile_hi, (T + 15) / 16)
     ALLOC(s.part_v, W)
     ALLOC(s.part_t, W)
     ALLOC(s.part_p, W)
@@ -398,6 +402,24 @@ int fbp_segment_set_templates(fbp_segment *g, const float *templates, const floa
         if (b - a > sup_max) sup_max = b - a;
     }
     g->d.sup_max = sup_max;
+    // matrix-core order: templates by first nonzero channel (then index), so a
+    // tile of 16 covers few channels; tile ranges are the union of supports
+    const uint32_t CT = (g->d.T + 15) / 16;
+    std::vector<uint32_t> ord(CT * 16), tlo(CT), thi(CT);
+    for (uint32_t t = 0; t < g->d.T; t++) ord[t] = t;
+    std::stable_sort(ord.begin(), ord.begin() + g->d.T, [&](uint32_t x, uint32_t y) { return lo[x] < lo[y]; });
+    for (uint32_t t = g->d.T; t < CT * 16; t++) ord[t] = g->d.T;
+    for (uint32_t ct = 0; ct < CT; ct++) {
+        uint32_t a = g->d.C, b = 0;
+        for (uint32_t q = ct * 16; q < ct * 16 + 16 && q < g->d.T; q++)
+            if (lo[ord[q]] < hi[ord[q]]) {
+                a = std::min(a, lo[ord[q]]);
+                b = std::max(b, hi[ord[q]]);
+            }
+        if (a >= b) a = b = 0;
+        tlo[ct] = a;
+        thi[ct] = b;
+    }
     if (g->cfg.overlap_mode == FBP_OVERLAP_SCREENED && overlap_screened_lds(g->d) > kMaxLds)
         return fail(FBP_EINVAL, "screened overlap needs %zu bytes of LDS (templates span %u channels)",
                     overlap_screened_lds(g->d), sup_max);
@@ -408,6 +430,9 @@ int fbp_segment_set_templates(fbp_segment *g, const float *templates, const floa
     hipError_t e = hipMemcpyAsync(g->s.Tm, templates, (uint64_t)d.T * d.C * d.N * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(g->s.sup_lo, lo.data(), (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(g->s.sup_hi, hi.data(), (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->s.tord, ord.data(), (uint64_t)CT * 16 * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->s.tile_lo, tlo.data(), (uint64_t)CT * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->s.tile_hi, thi.data(), (uint64_t)CT * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(tmp_bias, bias, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(g->s.thr, thresholds, (uint64_t)d.T * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_transpose_templates(d, g->s.Tm, tmp_bias, g->s.TmT, g->s.bias, st);
