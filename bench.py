@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--serve-inflight", type=int, default=8)
     ap.add_argument("--no-precision-report", action="store_true",
                     help="skip the likelihood error report against the float64 oracle (4 ch x 60 s)")
+    ap.add_argument("--segments", type=int, default=0,
+                    help="also run the recording as this many equal time segments concurrently (one HIP stream "
+                         "each) and report their throughput")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -130,6 +133,10 @@ def main():
     if args.serve_requests > 0:
         serve = serving_mix_run(args)
 
+    concurrent = None
+    if args.segments > 1:
+        concurrent = concurrent_segments_run(args, V, sm, bias, thr, N, M, do_overlap)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(rec, bias, thr, N, M, do_overlap, args.cpu_target_s, seg_cls=_lib.Segment,
@@ -153,6 +160,7 @@ def main():
         "stages": stages,
         "fp32_plain": plain,
         "serving": serve,
+        "concurrent_segments": concurrent,
         "precision_errors": prec,
     }
     print(json.dumps(out), flush=True)
@@ -215,6 +223,42 @@ def precision_report(args):
         log("%s screen: max |L - L64| = %.4g, events equal after re-score: %s" % (prec, e.max(), same))
     out["spikes_fp32"] = int(ev32[0].size)
     return out
+
+
+def concurrent_segments_run(args, V, sm, bias, thr, N, M, do_overlap):
+    """The recording cut into args.segments equal time segments, each its own
+    fbp_segment (own HIP stream) driven from its own host thread, all at once.
+    Segment seams are not merged: this measures throughput of concurrent
+    segments of the same total size."""
+    import concurrent.futures
+    from spikesorting_fullpursuit_amd import _lib
+    C, K = V.shape
+    T = sm['templates'].shape[0]
+    nseg = args.segments
+    bounds = np.linspace(0, K, nseg + 1).astype(np.int64)
+    segs = []
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        sg = _lib.Segment(C, int(b - a), T, N, M, do_overlap, 256, args.precision, args.overlap_mode)
+        sg.set_templates(sm['templates'], bias, thr)
+        sg.set_voltage(np.ascontiguousarray(V[:, a:b]))
+        segs.append(sg)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=nseg) as ex:
+        def run_all():
+            list(ex.map(lambda sg: sg.run(), segs))
+        for _ in range(args.warmup):
+            run_all()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            run_all()
+        dt = (time.perf_counter() - t) / args.steps
+    spikes = sum(sg.num_spikes() for sg in segs)
+    passes = max(sg.stats()['n_passes'] for sg in segs)
+    for sg in segs:
+        sg.close()
+    log("concurrent segments: %d x %d samples in %.3f s per step, %d spikes" % (nseg, K // nseg, dt, spikes))
+    return {"segments": nseg, "samples_per_segment": int(bounds[1] - bounds[0]), "ms_per_step": 1000.0 * dt,
+            "value": C * K / dt, "unit": "ch·samples/s", "spikes": int(spikes), "max_passes": int(passes),
+            "note": "equal time segments run concurrently, one HIP stream each; seams not merged"}
 
 
 def serving_mix_run(args):

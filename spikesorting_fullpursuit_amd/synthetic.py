@@ -132,14 +132,16 @@ def make_recording(seed, n_chans, fs, duration_s, n_templates, n_samples_per_cha
         on_all = np.concatenate(onsets)
         lab_all = np.concatenate(labels)
         # Sum every spike's contribution per channel in float64 (bincount is
-        # deterministic), then add it to the noise once.
+        # deterministic), then add it to the noise once.  Only the spikes of
+        # templates that are nonzero on the channel are gathered.
+        nz = np.any(tm3 != 0.0, axis=2)   # [T][C]
         for c in range(C):
-            vals = tm3[lab_all, c, :].astype(np.float64)
-            keep = np.any(vals != 0.0, axis=1)
+            keep = nz[lab_all, c]
             if not np.any(keep):
                 continue
+            vals = tm3[lab_all[keep], c, :].astype(np.float64)
             idx = (on_all[keep, None] + np.arange(N)[None, :]).ravel()
-            add = np.bincount(idx, weights=vals[keep].ravel(), minlength=K)
+            add = np.bincount(idx, weights=vals.ravel(), minlength=K)
             voltage[c] += add.astype(np.float32)
     voltage = voltage.astype(dtype, copy=False)
 
