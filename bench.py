@@ -396,7 +396,9 @@ def cpu_baseline(rec, bias, thr, N, M, do_overlap, target_s, seg_cls, precision,
     tm = rec['separability_metrics']['templates']
     C, K = V.shape
     fs = rec['sort_info']['sampling_rate']
-    n = min(K, int(fs * 0.5))
+    # first slice: 0.5 s for the 16-channel, 20-template workload, shorter in
+    # proportion for larger C*T (the oracle's overlap recheck is dense)
+    n = min(K, max(20 * N, int(fs * 0.5 * min(1.0, 320.0 / (C * tm.shape[0])))))
     while True:
         t = time.perf_counter()
         o = oracle.pursuit_chunk(V[:, :n], tm, bias, thr, N, M, do_overlap=do_overlap)

@@ -312,10 +312,11 @@ struct ScreenCoef {
 
 __device__ __forceinline__ float exact_likelihood(const Dims &d, const float *__restrict__ V,
                                                   const float *__restrict__ Tm, const float *__restrict__ bias,
-                                                  uint32_t t, uint64_t i)
+                                                  uint32_t c_lo, uint32_t c_hi, uint32_t t, uint64_t i)
 {
+    // channels outside [c_lo, c_hi) hold a zero template and add exactly 0
     float acc = bias[t];
-    for (uint32_t c = 0; c < d.C; c++) {
+    for (uint32_t c = c_lo; c < c_hi; c++) {
         const float *tc = Tm + ((uint64_t)t * d.C + c) * d.N;
         const float *vc = V + (uint64_t)c * d.K + i;
         for (uint32_t j = 0; j < d.N; j++) acc = __builtin_fmaf(tc[j], vc[j], acc);
@@ -334,7 +335,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
                                                   uint2 *__restrict__ rdesc, uint32_t cb_max, ScreenCoef co,
                                                   uint32_t no_rescore, const uint32_t *__restrict__ tord,
                                                   const uint32_t *__restrict__ tile_lo,
-                                                  const uint32_t *__restrict__ tile_hi)
+                                                  const uint32_t *__restrict__ tile_hi,
+                                                  const uint32_t *__restrict__ sup_lo,
+                                                  const uint32_t *__restrict__ sup_hi)
 {
     using Tr = NarrowTraits<PREC>;
     constexpr uint32_t KS = (RT * 16 + Tr::kK - 1) / Tr::kK;   // k-steps per channel (N padded to RT*16)
@@ -575,7 +578,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
                     const uint64_t i = base + p;
                     if (i + d.N >= d.K) continue;
                     if (pos < rq_cap) rq[pos] = ((uint64_t)t << 32) | (uint64_t)i;
-                    else L[(uint64_t)t * d.K + i] = exact_likelihood(d, V, Tm, bias, t, i);   // queue full
+                    else L[(uint64_t)t * d.K + i] = exact_likelihood(d, V, Tm, bias, sup_lo[t], sup_hi[t], t, i);   // queue full
                     pos++;
                 }
     }
@@ -2129,7 +2132,7 @@ static hipError_t launch_corr_mfma_rt(const Dims &d, const DeviceState &s, uint3
     ScreenCoef co{coef[0], coef[1], coef[2], coef[3]};
     hipLaunchKernelGGL((k_corr_mfma<PREC, RT, CTB>), dim3(n_win), dim3(64), lds, st, d, s.V, s.Tm,
                        (const Frag<PREC> *)s.TmB, s.bias, s.tnorm, s.tscale, s.win, s.L, s.ctr, s.rq, rq_cap,
-                       (uint2 *)s.rdesc, cb_max, co, no_rescore, s.tord, s.tile_lo, s.tile_hi);
+                       (uint2 *)s.rdesc, cb_max, co, no_rescore, s.tord, s.tile_lo, s.tile_hi, s.sup_lo, s.sup_hi);
     return hipGetLastError();
 }
 

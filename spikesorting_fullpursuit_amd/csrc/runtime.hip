@@ -131,7 +131,7 @@ uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
     if (cfg->precision != FBP_PREC_FP32) {
         uint64_t cap = T * d.K / 8;
         if (cap < (1u << 20)) cap = 1u << 20;
-        if (cap > (64u << 20)) cap = 64u << 20;
+        if (cap > (512u << 20)) cap = 512u << 20;
         b += r(cap * 8) + r((uint64_t)d.C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64 * 8 * mfma_words(cfg->precision)) +
              r(W * mfma_nchunk(d) * 8);
     }
@@ -323,7 +323,7 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     if (cfg->precision != FBP_PREC_FP32) {
         uint64_t cap = (uint64_t)T * K / 8;
         if (cap < (1u << 20)) cap = 1u << 20;
-        if (cap > (64u << 20)) cap = 64u << 20;
+        if (cap > (512u << 20)) cap = 512u << 20;
         g->rq_cap = (uint32_t)cap;
         ALLOC(s.rq, cap)
         uint64_t *rd = nullptr;
