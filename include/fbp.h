@@ -85,9 +85,11 @@ enum {
                                       candidates by a second full traversal     */
     FBP_DEBUG_OVERLAP_GLOBAL = 4,  /* screened overlap: exact scores read the
                                       partner template from global memory       */
-    FBP_DEBUG_OVERLAP_EXACT = 8    /* screened overlap: score a single candidate
+    FBP_DEBUG_OVERLAP_EXACT = 8,   /* screened overlap: score a single candidate
                                       exactly even when its interval settles the
                                       parse                                      */
+    FBP_DEBUG_OVERLAP_ALL_ROWS = 16 /* screened overlap: traverse every row, none
+                                      skipped by its upper bound                 */
 };
 
 typedef struct fbp_stats {

@@ -77,6 +77,7 @@ struct DeviceState {
     uint32_t *sp_lab;    // [cap]
     float *clips;        // [n][C*N] adjusted clips (allocated on demand)
     float *tabAB;        // [T][S][T][S] screened overlap: shifted-template cross products
+    float *tabABmin;     // [T][S][T]    min over the last shift of tabAB (row bounds of the screen)
     float *tabB2;        // [T][S]       squared norms of shifted, truncated templates
     float *tnorm;        // [T]          template norms (upper bounds)
     float *TmQ;          // screened overlap: split-bf16 A fragments of the matrix-core Q table (uint4 each)

@@ -1325,7 +1325,7 @@ template <int B, class Fn>
 __device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t *spos, const uint32_t *flist,
                                                    uint32_t n_fpos, uint32_t n_fneg, const uint16_t *plist,
                                                    uint32_t n_ppos, const float *__restrict__ ABl,
-                                                   const float *sQ, const double *sE, Fn fn)
+                                                   const float *sQ, const double *sE, const uint8_t *rowok, Fn fn)
 {
     const uint32_t S = d.S, N = d.N, T = d.T, TS = d.T * d.S;
     const uint32_t step = blockDim.x;
@@ -1345,7 +1345,7 @@ __device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t 
 #pragma unroll
             for (int q = 0; q < B; q++) {
                 const uint32_t e = e0 + q * step;
-                const bool ok = e < TS;
+                const bool ok = e < TS && rowok[a * T + t];   // rows (f, t) that can reach lo_max
                 tq[q] = t;
                 sq[q] = s;
                 ab[q] = ok ? abf[e] : 0.0f;
@@ -1373,10 +1373,12 @@ __device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t 
         uint32_t ff[B], ee[B], tq[B];
 #pragma unroll
         for (int q = 0; q < B; q++) {
-            const bool ok = r0 + q * step < total;
+            bool ok = r0 + q * step < total;
             ff[q] = ok ? flist[n_fpos + ra] : 0u;
             ee[q] = ok ? plist[rb] : 0u;
             tq[q] = (uint32_t)(((uint64_t)ee[q] * mS) >> 32);
+            ok = ok && rowok[(n_fpos + ra) * T + tq[q]];
+            ff[q] = ok ? ff[q] : 0xffffu;   // skipped
             ab[q] = ok ? ABl[(uint64_t)ff[q] * TS + ee[q]] : 0.0f;
             qe[q] = ok ? sQ[ee[q]] : 0.0f;
             Et[q] = ok ? sE[tq[q]] : 0.0;
@@ -1388,7 +1390,7 @@ __device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t 
         for (int q = 0; q < B; q++) {
             if (r0 + q * step >= total) break;
             const uint32_t f = ff[q], e = ee[q], t = tq[q], s = e - t * S;
-            if ((f > s ? f - s : s - f) <= N) fn(f, t, s, e, ab[q], qe[q], Et[q]);
+            if (f != 0xffffu && (f > s ? f - s : s - f) <= N) fn(f, t, s, e, ab[q], qe[q], Et[q]);
         }
     }
 }
@@ -1603,8 +1605,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
     const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw, float *best_L,
     const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov,
-    const float *__restrict__ AB, const float *__restrict__ B2, const float *__restrict__ tnorm,
-    const float *__restrict__ Tpad, const float *__restrict__ ovQ, uint32_t kbase,
+    const float *__restrict__ AB, const float *__restrict__ ABmin, const float *__restrict__ B2,
+    const float *__restrict__ tnorm, const float *__restrict__ Tpad, const float *__restrict__ ovQ, uint32_t kbase,
     const uint32_t *__restrict__ sup_lo, const uint32_t *__restrict__ sup_hi, uint32_t debug)
 {
     extern __shared__ float dyn[];
@@ -1622,6 +1624,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     uint32_t *cand = flist + S;                         // [kOvCand] screen candidates (f << 16 | e)
     uint16_t *plist = (uint16_t *)(cand + kOvCand);     // [T*S] (t,s) with positive likelihood (T*S < 2^16)
     uint8_t *spos = (uint8_t *)(plist + TS);            // [T][S] bit1 in bounds, bit0 likelihood > 0
+    uint8_t *srow = spos + TS;                          // [rows of flist][T] row (f, t) may reach lo_max
+    float *sqmax = (float *)(((uintptr_t)(srow + TS) + 3) & ~(uintptr_t)3);   // [2][T] max over (positive) s of Q'
+    __shared__ double r_ub[kOvWaves];
+    __shared__ uint32_t r_ur[kOvWaves];
     for (uint32_t x = threadIdx.x; x < R; x += blockDim.x) szero[x] = 0.0f;
     __shared__ uint32_t n_cand, s_settled;
     __shared__ float r_sc[4];
@@ -1732,6 +1738,69 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             auto screen = [&](uint32_t f, float qe, float ab) {
                 return (double)sQl[f] + (double)qe - (double)ab;
             };
+            // Row bounds.  For the main shift f of flist row a and template t,
+            // every combination (f, t, s) of the row has sh + E <= U(a, t) =
+            // Q'(lab, f) + max_s Q'(t, s) - min_s AB(lab, f, t, s) + E_t, plus
+            // the float64 rounding of both sides (the max over s runs over the
+            // positive-likelihood s for rows of non-positive main shifts).  The
+            // largest lower bound sh - E over the row with the largest U is at
+            // most lo_max; rows with U below it hold no combination that reaches
+            // lo_max, so neither the bounds nor the candidates change when the
+            // traversal skips them.
+            for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+                float m = -INFINITY, mp = -INFINITY;
+                for (uint32_t s = 0; s < S; s++) {
+                    const float q = sQ[t * S + s];
+                    m = fmaxf(m, q);
+                    if (spos[t * S + s] & 1) mp = fmaxf(mp, q);
+                }
+                sqmax[t] = m;
+                sqmax[T + t] = mp;
+            }
+            __syncthreads();
+            const uint32_t nrows = (nfp + nfn) * T;
+            const float *mABl = ABmin + (uint64_t)lab * S * T;   // [f][t]
+            auto row_bound = [&](uint32_t r) {
+                const uint32_t a = r / T, t = r - a * T, f = flist[a];
+                const double x = sQl[f], y = sqmax[(a < nfp ? 0 : T) + t], z = mABl[f * T + t], E = sE[t];
+                if (y == -INFINITY) return -(double)INFINITY;   // no combination in the row
+                return x + y - z + E + 0x1p-50 * (fabs(x) + fabs(y) + fabs(z) + E);
+            };
+            double ub = -INFINITY;
+            uint32_t ur = 0;
+            for (uint32_t r = threadIdx.x; r < nrows; r += blockDim.x) {
+                const double u = row_bound(r);
+                if (u > ub) { ub = u; ur = r; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ou = __shfl_xor(ub, off);
+                const uint32_t orr = __shfl_xor(ur, off);
+                if (ou > ub || (ou == ub && orr < ur)) { ub = ou; ur = orr; }
+            }
+            if (lane_of() == 0) { r_ub[threadIdx.x >> 6] = ub; r_ur[threadIdx.x >> 6] = ur; }
+            __syncthreads();
+            ub = r_ub[0];
+            ur = r_ur[0];
+            for (int q = 1; q < (int)kOvWaves; q++)
+                if (r_ub[q] > ub || (r_ub[q] == ub && r_ur[q] < ur)) { ub = r_ub[q]; ur = r_ur[q]; }
+            double lo_est = -INFINITY;
+            if (ub > -INFINITY) {
+                const uint32_t a = ur / T, t = ur - a * T, f = flist[a];
+                for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+                    const uint8_t sp = spos[t * S + s];
+                    const bool allowed = (a < nfp ? (sp & 2) : (sp & 1)) && (f > s ? f - s : s - f) <= N;
+                    if (allowed)
+                        lo_est = fmax(lo_est, screen(f, sQ[t * S + s], ABl[(uint64_t)f * TS + t * S + s]) - sE[t]);
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) lo_est = fmax(lo_est, __shfl_xor(lo_est, off));
+            __syncthreads();
+            if (lane_of() == 0) r_ub[threadIdx.x >> 6] = lo_est;
+            __syncthreads();
+            lo_est = fmax(fmax(r_ub[0], r_ub[1]), fmax(r_ub[2], r_ub[3]));
+            for (uint32_t r = threadIdx.x; r < nrows; r += blockDim.x)
+                srow[r] = (debug & 16u) || row_bound(r) >= lo_est ? 1 : 0;
+            __syncthreads();
             // pass A: largest lower and upper bounds of the score.  Each thread
             // also keeps its four combinations with the largest upper bounds
             // (kv0 >= kv1 >= kv2 >= kv3, packed f << 16 | e in kc*) and the
@@ -1740,7 +1809,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             double kv0 = -INFINITY, kv1 = -INFINITY, kv2 = -INFINITY, kv3 = -INFINITY;
             uint32_t kc0 = 0, kc1 = 0, kc2 = 0, kc3 = 0;
             uint32_t screened = 0;
-            for_each_unskipped<4>(d, spos, flist, nfp, nfn, plist, npp, ABl, sQ, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
+            for_each_unskipped<4>(d, spos, flist, nfp, nfn, plist, npp, ABl, sQ, sE, srow, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
                 const double sh = screen(f, qe, ab);
                 const double hi = sh + E;
                 lo_max = fmax(lo_max, sh - E);
@@ -1832,7 +1901,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         }
                     }
                 } else {
-                    for_each_unskipped<1>(d, spos, flist, nfp, nfn, plist, npp, ABl, sQ, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
+                    for_each_unskipped<1>(d, spos, flist, nfp, nfn, plist, npp, ABl, sQ, sE, srow, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
                         const double sh = screen(f, qe, ab);
                         if (sh + E < lo_max) return;
                         exact(f, t, s, Tpad + ((uint64_t)t * C + sup_lo[t]) * R);
@@ -2260,7 +2329,7 @@ static size_t overlap_screened_lds_mode(const Dims &d, bool)
 {
     const uint64_t ts = (uint64_t)d.T * d.S, R = d.N + 2 * d.M;
     return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + (2 * (uint64_t)d.sup_max + 1) * R * 4 + ts * 4 +
-           (uint64_t)d.S * 4 + (uint64_t)kOvCand * 4 + ts * 2 + ts + 16;
+           (uint64_t)d.S * 4 + (uint64_t)kOvCand * 4 + ts * 2 + 2 * ts + 4 + (uint64_t)d.T * 8 + 16;
 }
 
 // The matrix-core Q table is used whenever its kernel's LDS fits.
@@ -2285,11 +2354,26 @@ hipError_t launch_pad_templates(const Dims &d, const DeviceState &s, hipStream_t
     return hipGetLastError();
 }
 
+// ABmin[lab][f][t] = min over s of AB[lab][f][t][s] (the row bounds of the screen).
+__global__ void k_overlap_abmin(Dims d, const float *__restrict__ AB, float *__restrict__ ABmin)
+{
+    const uint64_t total = (uint64_t)d.T * d.S * d.T;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const float *row = AB + e * d.S;   // [lab][f][t] rows of S
+        float m = INFINITY;
+        for (uint32_t s = 0; s < d.S; s++) m = fminf(m, row[s]);
+        ABmin[e] = m;
+    }
+}
+
 hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_t st)
 {
     const uint64_t total = (uint64_t)d.T * d.S * d.T * d.S + (uint64_t)d.T * d.S + d.T;
     hipLaunchKernelGGL(k_overlap_tables, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st, d, s.Tm, s.tabAB,
                        s.tabB2, s.tnorm, s.sup_lo, s.sup_hi);
+    const uint64_t nmin = (uint64_t)d.T * d.S * d.T;
+    hipLaunchKernelGGL(k_overlap_abmin, dim3(grid_for(nmin, 256, 65536)), dim3(256), 0, st, d, s.tabAB, s.tabABmin);
     const uint64_t nq = overlap_qtemplate_floats(d);
     hipLaunchKernelGGL(k_pack_qtemplates, dim3(grid_for(nq / 4, 256, 4096)), dim3(256), 0, st, d, s.Tm, s.tord,
                        (uint4 *)s.TmQ);
@@ -2329,11 +2413,11 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
                                s.tile_hi);
             hipLaunchKernelGGL(k_overlap_screened<true>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                                list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                               s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug);
+                               s.tabABmin, s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug);
         } else {
             hipLaunchKernelGGL(k_overlap_screened<false>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                                list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                               s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug);
+                               s.tabABmin, s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug);
         }
     }
     return hipGetLastError();

@@ -106,9 +106,9 @@ void free_state(fbp_segment *g)
     DeviceState &s = g->s;
     void *ptrs[] = {s.V, s.V0, s.Tm, s.TmT, s.bias, s.thr, s.L, s.best_L, s.best_lab, s.best_idx, s.ov_idx,
                     s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
-                    s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabB2, s.tnorm,
-                    s.TmB, s.rq, s.tscale, s.rdesc, s.TmQ, s.Tpad, s.ov_sorted, s.part_v, s.part_t, s.part_p,
-                    s.ovQ, s.sup_lo, s.sup_hi, s.tord, s.tile_lo, s.tile_hi};
+                    s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabABmin, s.tabB2,
+                    s.tnorm, s.TmB, s.rq, s.tscale, s.rdesc, s.TmQ, s.Tpad, s.ov_sorted, s.part_v, s.part_t,
+                    s.part_p, s.ovQ, s.sup_lo, s.sup_hi, s.tord, s.tile_lo, s.tile_hi};
     for (void *p : ptrs) dfree(p, g->own_stream);
     if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
     s = DeviceState{};
@@ -315,6 +315,7 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     ALLOC(s.Tpad, (uint64_t)T * C * (N + 2 * d.M))
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED) {
         ALLOC(s.tabAB, (uint64_t)T * d.S * T * d.S)
+        ALLOC(s.tabABmin, (uint64_t)T * d.S * T)
         ALLOC(s.tabB2, (uint64_t)T * d.S)
         ALLOC(s.TmQ, overlap_qtemplate_floats(d))
         ALLOC(s.ov_sorted, W)
@@ -518,10 +519,13 @@ int fbp_segment_run(fbp_segment *g, void *stream)
         if (g->cfg.do_overlap) {                                                   // bpp.py:565-661
             HIP_TRY(launch_compact_ov(d, s, nw, st), "overlap list");
             HIP_TRY(hipEventRecord(g->ev_stage[2], st), "event");
-            if (g->cfg.overlap_mode == FBP_OVERLAP_SCREENED)
-                HIP_TRY(launch_overlap_screened(d, s, nw, g->cfg.debug_flags & (FBP_DEBUG_OVERLAP_FULL | FBP_DEBUG_OVERLAP_GLOBAL | FBP_DEBUG_OVERLAP_EXACT), st), "overlap recheck (screened)");
-            else
+            if (g->cfg.overlap_mode == FBP_OVERLAP_SCREENED) {
+                const uint32_t dbg = g->cfg.debug_flags & (FBP_DEBUG_OVERLAP_FULL | FBP_DEBUG_OVERLAP_GLOBAL |
+                                                           FBP_DEBUG_OVERLAP_EXACT | FBP_DEBUG_OVERLAP_ALL_ROWS);
+                HIP_TRY(launch_overlap_screened(d, s, nw, dbg, st), "overlap recheck (screened)");
+            } else {
                 HIP_TRY(launch_overlap_direct(d, s, nw, st), "overlap recheck");
+            }
             HIP_TRY(launch_check_overlap(d, s, nw, st), "overlap reassign");
             HIP_TRY(hipEventRecord(g->ev_stage[3], st), "event");
         }
