@@ -1302,6 +1302,51 @@ __device__ __forceinline__ void batched_stage(uint32_t n, Ld ld, St st)
     }
 }
 
+// One array of a joint staging: U elements per thread and round.
+template <int U, class Ld, class St>
+struct StageArr {
+    uint32_t n;
+    Ld ld;
+    St st;
+    float v[U];
+    __device__ __forceinline__ uint32_t rounds() const { return (n + U * blockDim.x - 1) / (U * blockDim.x); }
+    __device__ __forceinline__ void load(uint32_t r)
+    {
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const uint32_t x = threadIdx.x + (r * U + q) * blockDim.x;
+            v[q] = x < n ? ld(x) : 0.0f;
+        }
+    }
+    __device__ __forceinline__ void store(uint32_t r)
+    {
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const uint32_t x = threadIdx.x + (r * U + q) * blockDim.x;
+            if (x < n) st(x, v[q]);
+        }
+    }
+};
+
+template <int U, class Ld, class St>
+__device__ __forceinline__ StageArr<U, Ld, St> stage_arr(uint32_t n, Ld ld, St st)
+{
+    return StageArr<U, Ld, St>{n, ld, st, {}};
+}
+
+// Stages several arrays together: each round issues the loads of every array
+// before it stores any, so their latencies overlap.
+template <class... A>
+__device__ __forceinline__ void batched_stage_all(A &...a)
+{
+    uint32_t nr = 0;
+    ((nr = a.rounds() > nr ? a.rounds() : nr), ...);
+    for (uint32_t r = 0; r < nr; r++) {
+        (a.load(r), ...);
+        (a.store(r), ...);
+    }
+}
+
 // x / d as floor(x * ceil(2^32 / d) / 2^32): exact while x * d < 2^32.
 struct FastDiv {
     uint64_t m;
@@ -1314,84 +1359,87 @@ struct FastDiv {
 constexpr uint32_t kOvCand = 4 * 256;
 
 // Calls fn(f, t, s, e = t*S + s, AB[lab][f][t][s], sQ[e], sE[t]) for every
-// combination of one window that the reference does not skip (bp.cl:534-576),
-// in no particular order: the main shifts f whose likelihood is positive
-// against every in-bounds (t, s) within N of f, then the other in-bounds main
-// shifts against the (t, s) whose likelihood is positive.  Each thread loads
-// the operands of B combinations (global AB, LDS sQ, sE, spos) before it
-// evaluates them, so their latencies overlap.  Quotients advance
+// combination of one window that the reference does not skip (bp.cl:534-576)
+// and whose row (main shift f, template t) is listed in krow, in no
+// particular order.  Rows r = a*T + t (f = flist[a]) of positive main shifts
+// are krow[0, n_kp) and pair with every in-bounds s within N of f; rows of the
+// other in-bounds main shifts are krow[TS-1], krow[TS-2], ... (n_kn of them)
+// and pair with the s of template t whose likelihood is positive, listed at
+// plist[pofs[t], pofs[t] + pcnt[t]) (at most maxcnt).  Each thread loads the
+// operands of B combinations (global AB, LDS sQ, sE, spos) before it
+// evaluates them, so their latencies overlap.  Row and column indices advance
 // incrementally; no division per element.
 template <int B, class Fn>
-__device__ __forceinline__ void for_each_unskipped(const Dims &d, const uint8_t *spos, const uint32_t *flist,
-                                                   uint32_t n_fpos, uint32_t n_fneg, const uint16_t *plist,
-                                                   uint32_t n_ppos, const float *__restrict__ ABl,
-                                                   const float *sQ, const double *sE, const uint8_t *rowok, Fn fn)
+__device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT, const uint8_t *spos,
+                                              const uint32_t *flist, const uint16_t *krow, uint32_t n_kp,
+                                              uint32_t n_kn, const uint16_t *plist, const uint16_t *pofs,
+                                              const uint16_t *pcnt, uint32_t maxcnt, const float *__restrict__ ABl,
+                                              const float *sQ, const double *sE, Fn fn)
 {
     const uint32_t S = d.S, N = d.N, T = d.T, TS = d.T * d.S;
     const uint32_t step = blockDim.x;
-    const uint32_t stT = step / S, stS = step - stT * S;
-    const uint32_t t00 = threadIdx.x / S, s00 = threadIdx.x - t00 * S;
-    // e / S for e < T*S: e * ceil(2^32 / S) / 2^32 is exact while e * S < 2^32
-    const uint64_t mS = ((1ull << 32) + S - 1) / S;
-    for (uint32_t a = 0; a < n_fpos; a++) {
-        const uint32_t f = flist[a];
-        const float *abf = ABl + (uint64_t)f * TS;
-        uint32_t t = t00, s = s00;
-        for (uint32_t e0 = threadIdx.x; e0 < TS; e0 += B * step) {
+    {
+        const uint32_t total = n_kp * S;
+        const uint32_t stK = step / S, stS = step - stK * S;
+        uint32_t k = threadIdx.x / S, s = threadIdx.x - k * S;
+        for (uint32_t x0 = threadIdx.x; x0 < total; x0 += B * step) {
             float ab[B], qe[B];
             double Et[B];
-            uint32_t tq[B], sq[B];
-            uint8_t sp[B];
+            uint32_t fq[B], tq[B], sq[B];
+            bool okq[B];
 #pragma unroll
             for (int q = 0; q < B; q++) {
-                const uint32_t e = e0 + q * step;
-                const bool ok = e < TS && rowok[a * T + t];   // rows (f, t) that can reach lo_max
+                const bool in = x0 + q * step < total;
+                const uint32_t r = in ? krow[k] : 0u;
+                const uint32_t a = divT.div(r), t = r - a * T, f = flist[a], e = t * S + s;
+                const bool ok = in && (spos[e] & 2) && (f > s ? f - s : s - f) <= N;
+                ab[q] = ok ? ABl[(uint64_t)f * TS + e] : 0.0f;
+                qe[q] = ok ? sQ[e] : 0.0f;
+                Et[q] = ok ? sE[t] : 0.0;
+                fq[q] = f;
                 tq[q] = t;
                 sq[q] = s;
-                ab[q] = ok ? abf[e] : 0.0f;
-                sp[q] = ok ? spos[e] : (uint8_t)0;
-                qe[q] = ok ? sQ[e] : 0.0f;
-                Et[q] = (ok && t < T) ? sE[t] : 0.0;
+                okq[q] = ok;
                 s += stS;
-                t += stT;
-                if (s >= S) { s -= S; t++; }
+                k += stK;
+                if (s >= S) { s -= S; k++; }
             }
 #pragma unroll
-            for (int q = 0; q < B; q++) {
-                const uint32_t e = e0 + q * step, sv = sq[q];
-                if ((sp[q] & 2) && (f > sv ? f - sv : sv - f) <= N) fn(f, tq[q], sv, e, ab[q], qe[q], Et[q]);
-            }
+            for (int q = 0; q < B; q++)
+                if (okq[q]) fn(fq[q], tq[q], sq[q], tq[q] * S + sq[q], ab[q], qe[q], Et[q]);
         }
     }
-    const uint32_t total = n_fneg * n_ppos;
+    const uint32_t total = n_kn * maxcnt;
     if (total == 0) return;
-    const uint32_t stA = step / n_ppos, stB = step - stA * n_ppos;
-    uint32_t ra = threadIdx.x / n_ppos, rb = threadIdx.x - ra * n_ppos;
-    for (uint32_t r0 = threadIdx.x; r0 < total; r0 += B * step) {
+    const uint32_t stK = step / maxcnt, stJ = step - stK * maxcnt;
+    uint32_t k = threadIdx.x / maxcnt, j = threadIdx.x - k * maxcnt;
+    for (uint32_t x0 = threadIdx.x; x0 < total; x0 += B * step) {
         float ab[B], qe[B];
         double Et[B];
-        uint32_t ff[B], ee[B], tq[B];
+        uint32_t fq[B], eq[B], tq[B];
+        bool okq[B];
 #pragma unroll
         for (int q = 0; q < B; q++) {
-            bool ok = r0 + q * step < total;
-            ff[q] = ok ? flist[n_fpos + ra] : 0u;
-            ee[q] = ok ? plist[rb] : 0u;
-            tq[q] = (uint32_t)(((uint64_t)ee[q] * mS) >> 32);
-            ok = ok && rowok[(n_fpos + ra) * T + tq[q]];
-            ff[q] = ok ? ff[q] : 0xffffu;   // skipped
-            ab[q] = ok ? ABl[(uint64_t)ff[q] * TS + ee[q]] : 0.0f;
-            qe[q] = ok ? sQ[ee[q]] : 0.0f;
-            Et[q] = ok ? sE[tq[q]] : 0.0;
-            rb += stB;
-            ra += stA;
-            if (rb >= n_ppos) { rb -= n_ppos; ra++; }
+            const bool in = x0 + q * step < total;
+            const uint32_t r = in ? krow[TS - 1 - k] : 0u;
+            const uint32_t a = divT.div(r), t = r - a * T, f = flist[a];
+            const bool has = in && j < pcnt[t];
+            const uint32_t e = has ? plist[pofs[t] + j] : 0u, s = e - t * S;
+            const bool ok = has && (f > s ? f - s : s - f) <= N;
+            ab[q] = ok ? ABl[(uint64_t)f * TS + e] : 0.0f;
+            qe[q] = ok ? sQ[e] : 0.0f;
+            Et[q] = ok ? sE[t] : 0.0;
+            fq[q] = f;
+            eq[q] = e;
+            tq[q] = t;
+            okq[q] = ok;
+            j += stJ;
+            k += stK;
+            if (j >= maxcnt) { j -= maxcnt; k++; }
         }
 #pragma unroll
-        for (int q = 0; q < B; q++) {
-            if (r0 + q * step >= total) break;
-            const uint32_t f = ff[q], e = ee[q], t = tq[q], s = e - t * S;
-            if (f != 0xffffu && (f > s ? f - s : s - f) <= N) fn(f, t, s, e, ab[q], qe[q], Et[q]);
-        }
+        for (int q = 0; q < B; q++)
+            if (okq[q]) fn(fq[q], tq[q], eq[q] - tq[q] * S, eq[q], ab[q], qe[q], Et[q]);
     }
 }
 
@@ -1598,10 +1646,11 @@ __global__ void k_pack_qtemplates(Dims d, const float *__restrict__ Tm, const ui
     }
 }
 
-// Register budget of three waves per SIMD (no spills); LDS allows about four
-// blocks per CU at the 16-channel shapes.
+// Register budget of four waves per SIMD (measured faster than three despite a
+// few spilled registers); LDS allows four blocks per CU at the 16-channel
+// shapes.
 template <bool MQ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_overlap_screened(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_overlap_screened(
     Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
     const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw, float *best_L,
     const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov,
@@ -1613,28 +1662,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
     const uint32_t TS = T * S, R = N + 2 * M, SM = d.sup_max;
     const QPlan qp = q_plan(d);
-    const FastDiv divN(N), divS(S);
+    const FastDiv divN(N), divS(S), divT(T);
     double *sE = (double *)dyn;                         // [T] screen bound per template (this window)
     float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset
     float *smain = sv + C * N;                          // [SM][R] padded best-label template, its support rows
-    float *sb = smain + SM * R;                         // [SM][R] partner template of an exact score
-    float *szero = sb + SM * R;                         // [R] zeros
+    float *szero = smain + SM * R;                      // [R] zeros
     float *sQ = szero + R;                              // [T][S] Q - B2/2
-    uint32_t *flist = (uint32_t *)(sQ + TS);            // main shifts: positive first, then the rest
-    uint32_t *cand = flist + S;                         // [kOvCand] screen candidates (f << 16 | e)
-    uint16_t *plist = (uint16_t *)(cand + kOvCand);     // [T*S] (t,s) with positive likelihood (T*S < 2^16)
-    uint8_t *spos = (uint8_t *)(plist + TS);            // [T][S] bit1 in bounds, bit0 likelihood > 0
-    uint8_t *srow = spos + TS;                          // [rows of flist][T] row (f, t) may reach lo_max
-    float *sqmax = (float *)(((uintptr_t)(srow + TS) + 3) & ~(uintptr_t)3);   // [2][T] max over (positive) s of Q'
+    float *stn = sQ + TS;                               // [T] template norms
+    uint32_t *flist = (uint32_t *)(stn + T);            // main shifts: positive first, then the rest
+    float *sb = (float *)(flist + S);                   // [SM][R] partner template of an exact score
+    uint32_t *cand = (uint32_t *)(sb + SM * R);         // [kOvCand] screen candidates (f << 16 | e)
+    uint16_t *krow = (uint16_t *)sb;                    // [T*S] rows kept by their bounds, until the candidates
+    uint16_t *plist = (uint16_t *)(sb + max(SM * R + kOvCand, (TS + 1) / 2));   // e = t*S + s with positive
+                                                        // likelihood, grouped by t
+    uint16_t *pofs = plist + TS, *pcnt = pofs + T;      // [T] start and length of each template's group
+    uint8_t *spos = (uint8_t *)(pcnt + T);              // [T][S] bit1 in bounds, bit0 likelihood > 0
+    float *sqmax = (float *)(((uintptr_t)(spos + TS) + 3) & ~(uintptr_t)3);   // [2][T] max over (positive) s of Q'
     __shared__ double r_ub[kOvWaves];
-    __shared__ uint32_t r_ur[kOvWaves];
     for (uint32_t x = threadIdx.x; x < R; x += blockDim.x) szero[x] = 0.0f;
+    for (uint32_t x = threadIdx.x; x < T; x += blockDim.x) stn[x] = tnorm[x];
     __shared__ uint32_t n_cand, s_settled;
     __shared__ float r_sc[4];
     __shared__ uint64_t r_key[4], r_p[4];
     __shared__ double r_d[4][2];
     __shared__ double r_vv[4];
-    __shared__ uint32_t n_ppos, n_fpos, n_fneg;
+    __shared__ uint32_t n_ppos, n_fpos, n_fneg, n_kp, n_kn, s_maxcnt;
     const Counters *ctr = ctr_rw;
     const uint32_t n_ov = ctr->n_ov;
     const uint32_t wg_bits = 31 - __builtin_clz(d.reduce_wg);
@@ -1655,41 +1707,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // + float32 storage of Q - B2/2 and of the AB table (unit roundoff each)
     const double ecoef = (g + 2.0 * gq + 4.0 * u2) * 1.01;
     const uint32_t kend = min(n_ov, kbase + d.ov_batch);
+    // Window header (id, label, onset, best likelihood, label support) of the
+    // block's next window, loaded in three dependent steps spread over the
+    // current window so their latencies hide behind its work.  No block
+    // writes another block's window entries.
+    uint32_t id = 0, lab = 0, bidx = 0, lo_m = 0, hi_m = 0;
+    float bl = 0.0f, Lb = 0.0f, thl = 0.0f;
+    uint32_t nid = 0, nlab = 0, nbidx = 0, nlo = 0, nhi = 0;
+    float nbl = 0.0f, nLb = 0.0f, nthl = 0.0f;
+    int nstep = 0;
+    uint32_t kn = kbase + blockIdx.x;   // window whose header is loading
+    auto head_step = [&](int upto) {
+        if (kn >= kend) return;
+        if (nstep < 1 && upto >= 1) { nid = ov_list[kn]; nstep = 1; }
+        if (nstep < 2 && upto >= 2) { nlab = best_lab[nid]; nbidx = best_idx[nid]; nbl = best_L[nid]; nstep = 2; }
+        if (nstep < 3 && upto >= 3) {
+            nLb = L[(uint64_t)nlab * d.K + nbidx];
+            nthl = thr[nlab];
+            nlo = sup_lo[nlab];
+            nhi = sup_hi[nlab];
+            nstep = 3;
+        }
+    };
+    head_step(3);
     for (uint32_t k = kbase + blockIdx.x; k < kend; k += gridDim.x) {
-        const uint32_t id = ov_list[k];
-        const uint32_t lab = best_lab[id], bidx = best_idx[id];
-        const float bl = best_L[id];
-        const float Lb = L[(uint64_t)lab * d.K + bidx];
-        const bool wskip = (Lb < thr[lab]) || ((uint64_t)bidx >= (uint64_t)(d.K - d.N));   // bp.cl:518-527
+        id = nid; lab = nlab; bidx = nbidx; bl = nbl; Lb = nLb; thl = nthl; lo_m = nlo; hi_m = nhi;
+        kn = k + gridDim.x;
+        nstep = 0;
+        head_step(1);
+        const bool wskip = (Lb < thl) || ((uint64_t)bidx >= (uint64_t)(d.K - d.N));   // bp.cl:518-527
         float sc = -INFINITY;
         uint64_t key = ~0ull, bp = 0;
         if (!wskip) {
             __syncthreads();
             if (threadIdx.x == 0) {
                 n_ppos = 0;
-                n_fpos = 0;
-                n_fneg = 0;
+                n_kp = 0;
+                n_kn = 0;
+                s_maxcnt = 0;
                 n_cand = 0;
                 s_settled = 0;
             }
             // staging (index quotients by FastDiv: every index is below the
-            // LDS size)
-            const uint32_t lo_m = sup_lo[lab], hi_m = sup_hi[lab];
+            // LDS size); the likelihood signs at every in-bounds (t, s), where
+            // the loaded value is -1 if (t, s) is out of bounds
             double vv = 0.0;
-            batched_stage<8>(C * N, [&](uint32_t e) { const uint32_t c = divN.div(e);
-                                                       return V[(uint64_t)c * d.K + bidx + (e - c * N)]; },
-                             [&](uint32_t e, float x) { sv[e] = x; vv = __builtin_fma((double)x, (double)x, vv); });
-            batched_stage<8>((hi_m - lo_m) * R, [&](uint32_t e) { return Tpad[((uint64_t)lab * C + lo_m) * R + e]; },
-                             [&](uint32_t e, float x) { smain[e] = x; });
-            if (MQ)   // Q - B2/2 of this window from the matrix-core kernel
-                batched_stage<8>(TS, [&](uint32_t e) { return ovQ[(uint64_t)(k - kbase) * TS + e]; },
-                                 [&](uint32_t e, float q) { sQ[e] = q; });
-            for (int off = 32; off > 0; off >>= 1) vv += __shfl_xor(vv, off);
-            if (lane_of() == 0) r_vv[threadIdx.x >> 6] = vv;
-            __syncthreads();
-            // likelihood signs at every in-bounds (t, s); the loaded value is
-            // -1 where (t, s) is out of bounds
-            batched_stage<8>(TS, [&](uint32_t e) {
+            auto a_v = stage_arr<4>(C * N, [&](uint32_t e) { const uint32_t c = divN.div(e);
+                                                              return V[(uint64_t)c * d.K + bidx + (e - c * N)]; },
+                                    [&](uint32_t e, float x) { sv[e] = x; vv = __builtin_fma((double)x, (double)x, vv); });
+            auto a_main = stage_arr<8>((hi_m - lo_m) * R, [&](uint32_t e) { return Tpad[((uint64_t)lab * C + lo_m) * R + e]; },
+                                       [&](uint32_t e, float x) { smain[e] = x; });
+            auto a_q = stage_arr<10>(MQ ? TS : 0, [&](uint32_t e) { return ovQ[(uint64_t)(k - kbase) * TS + e]; },
+                                     [&](uint32_t e, float q) { sQ[e] = q; });
+            auto a_l = stage_arr<10>(TS, [&](uint32_t e) {
                 const uint32_t t = divS.div(e), s = e - t * S;
                 const bool inb = !((uint64_t)s + bidx < M || (uint64_t)s + bidx >= (uint64_t)d.K - N + M);   // bp.cl:534,538
                 return inb ? L[(uint64_t)t * d.K + bidx + s - M] : -1.0f;
@@ -1698,8 +1768,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 const bool inb = !((uint64_t)s + bidx < M || (uint64_t)s + bidx >= (uint64_t)d.K - N + M);
                 const bool pos = inb && lv > 0.0f;
                 spos[e] = (inb ? 2 : 0) | (pos ? 1 : 0);
-                if (pos) plist[atomicAdd(&n_ppos, 1u)] = (uint16_t)e;
-                if (t == lab && pos) flist[atomicAdd(&n_fpos, 1u)] = s;
                 if (!MQ) {
                     const int32_t sa = (int32_t)M - (int32_t)s;
                     const int32_t lo = max(0, -sa), hi = min((int32_t)N, (int32_t)N - sa);
@@ -1712,29 +1780,85 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     sQ[e] = (float)q;
                 }
             });
+            if (MQ) {
+                batched_stage_all(a_v, a_main, a_q, a_l);
+            } else {   // the float64 Q table reads the staged voltage
+                batched_stage_all(a_v, a_main);
+                __syncthreads();
+                batched_stage_all(a_l);
+            }
+            for (int off = 32; off > 0; off >>= 1) vv += __shfl_xor(vv, off);
+            if (lane_of() == 0) r_vv[threadIdx.x >> 6] = vv;
             __syncthreads();
+            head_step(2);
             const double vn = sqrt(r_vv[0] + r_vv[1] + r_vv[2] + r_vv[3]) * (1.0 + 1e-12);
-            const double nlab = tnorm[lab];
+            const double norm_lab = stn[lab];
             // fold the voltage-free half norms into Q (done by the matrix-core
             // kernel for MQ), and the bound per template
             if (!MQ)
                 batched_stage<8>(TS, [&](uint32_t e) { return B2[e]; },
                                  [&](uint32_t e, float b2) { sQ[e] = (float)((double)sQ[e] - 0.5 * (double)b2); });
             for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
-                const double nsum = nlab + (double)tnorm[t];
+                const double nsum = norm_lab + (double)stn[t];
                 sE[t] = ecoef * (nsum * vn + nsum * nsum) + eabs;
             }
-            // in-bounds main shifts without a positive likelihood go after the positive ones
-            if (threadIdx.x == 0) {
-                uint32_t nn = 0;
-                for (uint32_t f = 0; f < S; f++)
-                    if ((spos[lab * S + f] & 3) == 2) flist[n_fpos + nn++] = f;
-                n_fneg = nn;
+            if (!MQ) __syncthreads();   // Q - B2/2 complete
+            // One wave per template t, by ballot over s: the group of
+            // positive-likelihood (t, s) in plist, max over s of Q'(t, s) and
+            // over the positive s; for t = lab also the main shifts: the
+            // positive ones, then the other in-bounds ones, each ascending.
+            {
+                const uint32_t l = lane_of();
+                const uint64_t below = (1ull << l) - 1;
+                for (uint32_t t = threadIdx.x >> 6; t < T; t += kOvWaves) {
+                    float m = -INFINITY, mp = -INFINITY;
+                    uint32_t cnt = 0, cneg = 0;
+                    for (uint32_t s0 = 0; s0 < S; s0 += kWave) {
+                        const uint32_t s = s0 + l;
+                        const uint8_t sp = s < S ? spos[t * S + s] : (uint8_t)0;
+                        const float q = s < S ? sQ[t * S + s] : -INFINITY;
+                        m = fmaxf(m, q);
+                        if (sp & 1) mp = fmaxf(mp, q);
+                        cnt += __popcll(__ballot(sp & 1));
+                        cneg += __popcll(__ballot((sp & 3) == 2));
+                    }
+                    for (int off = 32; off > 0; off >>= 1) {
+                        m = fmaxf(m, __shfl_xor(m, off));
+                        mp = fmaxf(mp, __shfl_xor(mp, off));
+                    }
+                    uint32_t base = 0;
+                    if (l == 0) {
+                        base = atomicAdd(&n_ppos, cnt);
+                        atomicMax(&s_maxcnt, cnt);
+                        pofs[t] = (uint16_t)base;
+                        pcnt[t] = (uint16_t)cnt;
+                        sqmax[t] = m;
+                        sqmax[T + t] = mp;
+                        if (t == lab) {
+                            n_fpos = cnt;
+                            n_fneg = cneg;
+                        }
+                    }
+                    base = __shfl(base, 0);
+                    uint32_t op = 0, on = 0;
+                    for (uint32_t s0 = 0; s0 < S; s0 += kWave) {
+                        const uint32_t s = s0 + l;
+                        const uint8_t sp = s < S ? spos[t * S + s] : (uint8_t)0;
+                        const uint64_t mpos = __ballot(sp & 1), mneg = __ballot((sp & 3) == 2);
+                        if (sp & 1) plist[base + op + __popcll(mpos & below)] = (uint16_t)(t * S + s);
+                        if (t == lab) {
+                            if (sp & 1) flist[op + __popcll(mpos & below)] = s;
+                            if ((sp & 3) == 2) flist[cnt + on + __popcll(mneg & below)] = s;
+                        }
+                        op += __popcll(mpos);
+                        on += __popcll(mneg);
+                    }
+                }
             }
             __syncthreads();
             const float *ABl = AB + (uint64_t)lab * S * TS;
             const float *sQl = sQ + lab * S;
-            const uint32_t nfp = n_fpos, nfn = n_fneg, npp = n_ppos;
+            const uint32_t nfp = n_fpos, nfn = n_fneg, maxcnt = s_maxcnt;
             auto screen = [&](uint32_t f, float qe, float ab) {
                 return (double)sQl[f] + (double)qe - (double)ab;
             };
@@ -1742,51 +1866,57 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             // every combination (f, t, s) of the row has sh + E <= U(a, t) =
             // Q'(lab, f) + max_s Q'(t, s) - min_s AB(lab, f, t, s) + E_t, plus
             // the float64 rounding of both sides (the max over s runs over the
-            // positive-likelihood s for rows of non-positive main shifts).  The
-            // largest lower bound sh - E over the row with the largest U is at
-            // most lo_max; rows with U below it hold no combination that reaches
-            // lo_max, so neither the bounds nor the candidates change when the
-            // traversal skips them.
-            for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
-                float m = -INFINITY, mp = -INFINITY;
-                for (uint32_t s = 0; s < S; s++) {
-                    const float q = sQ[t * S + s];
-                    m = fmaxf(m, q);
-                    if (spos[t * S + s] & 1) mp = fmaxf(mp, q);
-                }
-                sqmax[t] = m;
-                sqmax[T + t] = mp;
-            }
-            __syncthreads();
+            // positive-likelihood s for rows of non-positive main shifts).  Each
+            // wave evaluates the row with its largest U; the largest lower bound
+            // sh - E found there, lo_est, is at most lo_max.  Rows with U below
+            // lo_est hold no combination that reaches lo_max, so neither the
+            // bounds nor the candidates change when the traversal skips them.
             const uint32_t nrows = (nfp + nfn) * T;
             const float *mABl = ABmin + (uint64_t)lab * S * T;   // [f][t]
-            auto row_bound = [&](uint32_t r) {
-                const uint32_t a = r / T, t = r - a * T, f = flist[a];
-                const double x = sQl[f], y = sqmax[(a < nfp ? 0 : T) + t], z = mABl[f * T + t], E = sE[t];
-                if (y == -INFINITY) return -(double)INFINITY;   // no combination in the row
-                return x + y - z + E + 0x1p-50 * (fabs(x) + fabs(y) + fabs(z) + E);
+            constexpr int RB = 4;   // rows per thread whose loads are in flight together
+            auto row_bounds = [&](auto use) {
+                const uint32_t step = blockDim.x, stA = step / T, stT = step - stA * T;
+                uint32_t a = threadIdx.x / T, t = threadIdx.x - a * T;
+                for (uint32_t r0 = threadIdx.x; r0 < nrows; r0 += RB * step) {
+                    float z[RB];
+                    uint32_t aq[RB], tq[RB];
+#pragma unroll
+                    for (int q = 0; q < RB; q++) {
+                        const bool in = r0 + q * step < nrows;
+                        aq[q] = a;
+                        tq[q] = t;
+                        z[q] = in ? mABl[flist[a] * T + t] : 0.0f;
+                        t += stT;
+                        a += stA;
+                        if (t >= T) { t -= T; a++; }
+                    }
+#pragma unroll
+                    for (int q = 0; q < RB; q++) {
+                        if (r0 + q * step < nrows) {
+                            const uint32_t f = flist[aq[q]];
+                            const double x = sQl[f], y = sqmax[(aq[q] < nfp ? 0 : T) + tq[q]], zz = z[q];
+                            const double E = sE[tq[q]];
+                            const double u = y == -INFINITY ? -(double)INFINITY   // no combination in the row
+                                             : x + y - zz + E + 0x1p-50 * (fabs(x) + fabs(y) + fabs(zz) + E);
+                            use(aq[q] * T + tq[q], aq[q], tq[q], u);
+                        }
+                    }
+                }
             };
             double ub = -INFINITY;
             uint32_t ur = 0;
-            for (uint32_t r = threadIdx.x; r < nrows; r += blockDim.x) {
-                const double u = row_bound(r);
+            row_bounds([&](uint32_t r, uint32_t, uint32_t, double u) {
                 if (u > ub) { ub = u; ur = r; }
-            }
+            });
             for (int off = 32; off > 0; off >>= 1) {
                 const double ou = __shfl_xor(ub, off);
                 const uint32_t orr = __shfl_xor(ur, off);
                 if (ou > ub || (ou == ub && orr < ur)) { ub = ou; ur = orr; }
             }
-            if (lane_of() == 0) { r_ub[threadIdx.x >> 6] = ub; r_ur[threadIdx.x >> 6] = ur; }
-            __syncthreads();
-            ub = r_ub[0];
-            ur = r_ur[0];
-            for (int q = 1; q < (int)kOvWaves; q++)
-                if (r_ub[q] > ub || (r_ub[q] == ub && r_ur[q] < ur)) { ub = r_ub[q]; ur = r_ur[q]; }
             double lo_est = -INFINITY;
-            if (ub > -INFINITY) {
+            if (ub > -INFINITY) {   // this wave's best row
                 const uint32_t a = ur / T, t = ur - a * T, f = flist[a];
-                for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+                for (uint32_t s = lane_of(); s < S; s += kWave) {
                     const uint8_t sp = spos[t * S + s];
                     const bool allowed = (a < nfp ? (sp & 2) : (sp & 1)) && (f > s ? f - s : s - f) <= N;
                     if (allowed)
@@ -1794,13 +1924,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 }
             }
             for (int off = 32; off > 0; off >>= 1) lo_est = fmax(lo_est, __shfl_xor(lo_est, off));
-            __syncthreads();
             if (lane_of() == 0) r_ub[threadIdx.x >> 6] = lo_est;
             __syncthreads();
             lo_est = fmax(fmax(r_ub[0], r_ub[1]), fmax(r_ub[2], r_ub[3]));
-            for (uint32_t r = threadIdx.x; r < nrows; r += blockDim.x)
-                srow[r] = (debug & 16u) || row_bound(r) >= lo_est ? 1 : 0;
+            if (debug & 16u) lo_est = -INFINITY;
+            // kept rows: positive main shifts from the front, the others from the back
+            row_bounds([&](uint32_t r, uint32_t a, uint32_t t, double u) {
+                if (u < lo_est) return;
+                if (a < nfp) krow[atomicAdd(&n_kp, 1u)] = (uint16_t)r;
+                else if (pcnt[t]) krow[TS - 1 - atomicAdd(&n_kn, 1u)] = (uint16_t)r;
+            });
             __syncthreads();
+            const uint32_t nkp = n_kp, nkn = n_kn;
             // pass A: largest lower and upper bounds of the score.  Each thread
             // also keeps its four combinations with the largest upper bounds
             // (kv0 >= kv1 >= kv2 >= kv3, packed f << 16 | e in kc*) and the
@@ -1809,7 +1944,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             double kv0 = -INFINITY, kv1 = -INFINITY, kv2 = -INFINITY, kv3 = -INFINITY;
             uint32_t kc0 = 0, kc1 = 0, kc2 = 0, kc3 = 0;
             uint32_t screened = 0;
-            for_each_unskipped<4>(d, spos, flist, nfp, nfn, plist, npp, ABl, sQ, sE, srow, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
+            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, sQ, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
                 const double sh = screen(f, qe, ab);
                 const double hi = sh + E;
                 lo_max = fmax(lo_max, sh - E);
@@ -1836,6 +1971,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             __syncthreads();
             lo_max = fmax(fmax(r_d[0][0], r_d[1][0]), fmax(r_d[2][0], r_d[3][0]));
             hi_max = fmax(fmax(r_d[0][1], r_d[1][1]), fmax(r_d[2][1], r_d[3][1]));
+            head_step(3);
             // Nothing can beat best_L: parse rejects without any exact score.
             if (hi_max > (double)bl) {
                 uint32_t scored = 0;
@@ -1901,7 +2037,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         }
                     }
                 } else {
-                    for_each_unskipped<1>(d, spos, flist, nfp, nfn, plist, npp, ABl, sQ, sE, srow, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
+                    for_each_kept<1>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, sQ, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
                         const double sh = screen(f, qe, ab);
                         if (sh + E < lo_max) return;
                         exact(f, t, s, Tpad + ((uint64_t)t * C + sup_lo[t]) * R);
@@ -1914,6 +2050,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
         if (threadIdx.x == 0 && (wskip || !s_settled))
             parse_overlap(d, L, thr, id, lab, bidx, bl, sc, bp, best_L, ov_idx, ov_lab, ov);
+        head_step(3);
     }
 }
 
@@ -2328,8 +2465,11 @@ hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t m
 static size_t overlap_screened_lds_mode(const Dims &d, bool)
 {
     const uint64_t ts = (uint64_t)d.T * d.S, R = d.N + 2 * d.M;
-    return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + (2 * (uint64_t)d.sup_max + 1) * R * 4 + ts * 4 +
-           (uint64_t)d.S * 4 + (uint64_t)kOvCand * 4 + ts * 2 + 2 * ts + 4 + (uint64_t)d.T * 8 + 16;
+    const uint64_t sbc = (uint64_t)d.sup_max * R + kOvCand;   // partner template + candidates, or kept rows
+    const uint64_t krw = (ts + 1) / 2;
+    return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + ((uint64_t)d.sup_max + 1) * R * 4 + ts * 4 +
+           (uint64_t)d.T * 4 + (uint64_t)d.S * 4 + (sbc > krw ? sbc : krw) * 4 + ts * 2 + (uint64_t)d.T * 4 +
+           ts + 4 + (uint64_t)d.T * 8 + 16;
 }
 
 // The matrix-core Q table is used whenever its kernel's LDS fits.
