@@ -30,9 +30,9 @@ struct Counters {
     uint32_t n_ov;       // overlap windows of the pass
     uint32_t n_new;      // spikes added by the pass
     uint32_t n_next;     // windows of the next pass
-    uint32_t n_rescore;  // low-precision path: positions re-scored in float32
+    uint32_t n_rescore;  // low-precision path: re-score queue entries (groups of 4 positions)
     uint32_t n_combo;    // screened overlap: combinations re-scored directly
-    uint32_t n_rescore_overflow;
+    uint32_t n_rescore_pos;  // low-precision path: positions re-scored in float32
     uint32_t pad[2];
     unsigned long long combos;   // overlap combinations scored exactly since the run started
     unsigned long long screened; // combinations screened in float64 (screened overlap)

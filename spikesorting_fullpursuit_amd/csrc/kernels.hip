@@ -532,19 +532,40 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
                 for (int q = 0; q < 4; q++) {
                     const uint32_t p = rt * 16 + 4 * (lane >> 4) + q;
                     const uint32_t t = cb_t[ct];
-                    if (!cb_ok[ct] || p >= np) continue;
+                    if (!cb_ok[ct] || p >= np) { acc[rt][ct][q] = 0.0f; continue; }
                     const uint64_t i = base + p;
-                    if (i + d.N >= d.K) { L[(uint64_t)t * d.K + i] = 0.0f; continue; }   // bp.cl:231-234
+                    if (i + d.N >= d.K) {   // bp.cl:231-234
+                        L[(uint64_t)t * d.K + i] = 0.0f;
+                        acc[rt][ct][q] = 0.0f;
+                        continue;
+                    }
                     const float lh = cb_bias[ct] + acc[rt][ct][q] * cb_inv[ct];
                     const float vn = vnb[p];
                     const float e = co.k1 * cb_tn[ct] * vn + co.k2 * (cb_tn[ct] + vn * cb_inv[ct]) + co.k3 * cb_inv[ct]
                                     + co.k4 * fabsf(cb_bias[ct]);
                     L[(uint64_t)t * d.K + i] = lh;
                     const bool fl = force || !(lh <= -e);
-                    nflag += fl ? 1u : 0u;
                     acc[rt][ct][q] = fl ? 1.0f : 0.0f;   // keep the decision for the queue pass
                 }
         if (no_rescore) continue;
+        // Queue entries are groups: a lane's four consecutive positions of one
+        // template, with a mask of those that need float32.
+        auto group_mask = [&](int rt, int ct) {
+            uint32_t m = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) m |= acc[rt][ct][q] != 0.0f ? 1u << q : 0u;
+            return m;
+        };
+        uint32_t npos = 0;
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+            for (int ct = 0; ct < CTB; ct++) {
+                const uint32_t m = group_mask(rt, ct);
+                nflag += m ? 1u : 0u;
+                npos += __popc(m);
+            }
+        for (int off = 32; off > 0; off >>= 1) npos += __shfl_xor(npos, off);
         // wave-wide reservation in the re-score queue
         uint32_t incl = nflag;
         for (int off = 1; off < 64; off <<= 1) {
@@ -554,7 +575,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
         const uint32_t tot = __shfl(incl, 63);
         uint32_t qbase = 0;
         if (tot) {
-            if (lane == 63) qbase = atomicAdd(&ctr->n_rescore, tot);
+            if (lane == 63) {
+                qbase = atomicAdd(&ctr->n_rescore, tot);
+                atomicAdd(&ctr->n_rescore_pos, npos);
+            }
             qbase = __shfl(qbase, 63);
         }
         uint32_t pos = qbase + incl - nflag;
@@ -568,19 +592,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
-            for (int ct = 0; ct < CTB; ct++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    if (acc[rt][ct][q] == 0.0f || !cb_ok[ct]) continue;
-                    const uint32_t p = rt * 16 + 4 * (lane >> 4) + q;
-                    if (p >= np) continue;
-                    const uint32_t t = cb_t[ct];
-                    const uint64_t i = base + p;
-                    if (i + d.N >= d.K) continue;
-                    if (pos < rq_cap) rq[pos] = ((uint64_t)t << 32) | (uint64_t)i;
-                    else L[(uint64_t)t * d.K + i] = exact_likelihood(d, V, Tm, bias, sup_lo[t], sup_hi[t], t, i);   // queue full
-                    pos++;
+            for (int ct = 0; ct < CTB; ct++) {
+                const uint32_t m = group_mask(rt, ct);
+                if (!m) continue;
+                const uint32_t t = cb_t[ct];
+                const uint64_t i0 = base + rt * 16 + 4 * (lane >> 4);
+                if (pos < rq_cap) {
+                    rq[pos] = ((uint64_t)m << 48) | ((uint64_t)t << 32) | i0;
+                } else {   // queue full
+                    for (int q = 0; q < 4; q++)
+                        if (m >> q & 1)
+                            L[(uint64_t)t * d.K + i0 + q] = exact_likelihood(d, V, Tm, bias, sup_lo[t], sup_hi[t], t, i0 + q);
                 }
+                pos++;
+            }
     }
 }
 
@@ -601,6 +626,45 @@ __device__ __forceinline__ float fma_chain(float acc, const float *tp, uint32_t 
     }
     for (; j < n; j++) acc = __builtin_fmaf(tp[j * ts], vp[j], acc);
     return acc;
+}
+
+// Four chains of one template at consecutive positions p, p+1, p+2, p+3
+// (vp = the voltage at p): each template sample is read once for the four,
+// and the voltage slides through registers, one new sample per step.  Every
+// chain is the reference's fused multiply-add order.  vp[0, n + 3) must be
+// readable.
+__device__ __forceinline__ void fma_chain4(float (&acc)[4], const float *tp, uint32_t ts, const float *vp, uint32_t n)
+{
+    float v0 = vp[0], v1 = vp[1], v2 = vp[2];
+    uint32_t j = 0;
+    for (; j + 8 <= n; j += 8) {
+        float a[8], b[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            a[q] = tp[(j + q) * ts];
+            b[q] = vp[j + q + 3];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            acc[0] = __builtin_fmaf(a[q], v0, acc[0]);
+            acc[1] = __builtin_fmaf(a[q], v1, acc[1]);
+            acc[2] = __builtin_fmaf(a[q], v2, acc[2]);
+            acc[3] = __builtin_fmaf(a[q], b[q], acc[3]);
+            v0 = v1;
+            v1 = v2;
+            v2 = b[q];
+        }
+    }
+    for (; j < n; j++) {
+        const float a = tp[j * ts], b = vp[j + 3];
+        acc[0] = __builtin_fmaf(a, v0, acc[0]);
+        acc[1] = __builtin_fmaf(a, v1, acc[1]);
+        acc[2] = __builtin_fmaf(a, v2, acc[2]);
+        acc[3] = __builtin_fmaf(a, b, acc[3]);
+        v0 = v1;
+        v1 = v2;
+        v2 = b;
+    }
 }
 
 // Float32 re-score of one (window, template chunk)'s queued entries when the
@@ -643,12 +707,15 @@ __global__ __launch_bounds__(64) void k_rescore_sup(Dims d, const float *__restr
         const uint32_t e = e0 + lane;
         if (e >= cnt) break;
         const uint64_t ent = rq[dsc.x + e];
-        const uint32_t t = (uint32_t)(ent >> 32);
-        const uint64_t i = ent & 0xffffffffull;
-        float acc = bias[t];
-        for (uint32_t c = sup_lo[t]; c < sup_hi[t]; c++)
-            acc = fma_chain_contig(acc, Tm + ((uint64_t)t * d.C + c) * d.N, V + (uint64_t)c * d.K + i, d.N);
-        L[(uint64_t)t * d.K + i] = acc;
+        const uint32_t t = (uint32_t)(ent >> 32) & 0xffffu, m = (uint32_t)(ent >> 48) & 15u;
+        const uint64_t i0 = ent & 0xffffffffull;
+        for (int q = 0; q < 4; q++) {
+            if (!(m >> q & 1)) continue;
+            float acc = bias[t];
+            for (uint32_t c = sup_lo[t]; c < sup_hi[t]; c++)
+                acc = fma_chain_contig(acc, Tm + ((uint64_t)t * d.C + c) * d.N, V + (uint64_t)c * d.K + i0 + q, d.N);
+            L[(uint64_t)t * d.K + i0 + q] = acc;
+        }
     }
 }
 
@@ -690,9 +757,10 @@ __global__ __launch_bounds__(1024) void k_rescore_res(Dims d, const float *__res
             const uint32_t e = e0 + lane;
             const bool ok = e < cnt;
             const uint64_t ent = ok ? rq[dsc.x + e] : 0;
-            const uint32_t t = ok ? (uint32_t)(ent >> 32) : 0u;
-            const uint32_t p = ok ? (uint32_t)((ent & 0xffffffffull) - base) : 0u;
-            float acc = bias[t];
+            const uint32_t t = (uint32_t)(ent >> 32) & 0xffffu, m = (uint32_t)(ent >> 48) & 15u;
+            const uint32_t p = ok ? (uint32_t)((ent & 0xffffffffull) - base) : 0u;   // multiple of 4, below 64
+            const float b0 = bias[t];
+            float acc[4] = {b0, b0, b0, b0};
             float rv[4];
             auto fetch = [&](uint32_t c) {
                 const float *vr = V + (uint64_t)c * d.K;
@@ -712,9 +780,12 @@ __global__ __launch_bounds__(1024) void k_rescore_res(Dims d, const float *__res
                 }
                 wave_sync();
                 if (c + 1 < C) fetch(c + 1);
-                acc = fma_chain(acc, lds + (uint64_t)c * N * Tp + t, Tp, vb + p, N);
+                fma_chain4(acc, lds + (uint64_t)c * N * Tp + t, Tp, vb + p, N);
             }
-            if (ok) L[(uint64_t)t * d.K + base + p] = acc;
+            if (ok)
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (m >> q & 1) L[(uint64_t)t * d.K + base + p + q] = acc[q];
         }
     }
 }
@@ -727,7 +798,10 @@ static size_t rescore_res_lds(const Dims &d)
 
 __global__ void k_zero_rescore(Counters *ctr)
 {
-    if (threadIdx.x == 0) ctr->n_rescore = 0;
+    if (threadIdx.x == 0) {
+        ctr->n_rescore = 0;
+        ctr->n_rescore_pos = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------

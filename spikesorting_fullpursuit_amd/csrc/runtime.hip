@@ -539,7 +539,7 @@ int fbp_segment_run(fbp_segment *g, void *stream)
         g->stats.windows_evaluated += nw;
         g->stats.corr_launches++;
         g->stats.corr_positions += (uint64_t)nw * d.N;
-        if (g->cfg.precision != FBP_PREC_FP32) g->stats.rescored_positions += g->h_ctr->n_rescore;
+        if (g->cfg.precision != FBP_PREC_FP32) g->stats.rescored_positions += g->h_ctr->n_rescore_pos;
         float ms_c = 0.0f, ms_o = 0.0f;
         (void)hipEventElapsedTime(&ms_c, g->ev_stage[0], g->ev_stage[1]);
         g->stats.ms_correlation += ms_c;
