@@ -720,12 +720,14 @@ __global__ __launch_bounds__(64) void k_rescore_sup(Dims d, const float *__restr
 }
 
 // Same re-score with every template resident in LDS: one block per CU holds
-// the [C][N][Tp] copy for its whole life, and each of its waves walks the
-// (window, chunk) descriptors with a stride, staging the window's voltage one
-// channel at a time in a region of its own (wave-level ordering only; the
-// next channel's samples are fetched while the current one runs).  Used when
-// the templates and sixteen voltage rows fit in LDS.
+// the [C][N][Tp] copy for its whole life.  Each wave takes a contiguous slice
+// of the queue and walks it in batches of up to 64 entries from at most
+// kRescoreBatchWins windows (entries of a window are contiguous in the queue),
+// staging those windows' voltage one channel at a time in a region of its own
+// (wave-level ordering only; the next channel's samples are fetched while the
+// current one runs).  Used when the templates fit in LDS.
 constexpr uint32_t kRescoreResWaves = 16;
+constexpr uint32_t kRescoreStage = 512;   // voltage floats a wave stages per channel
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -736,64 +738,89 @@ __device__ __forceinline__ void wave_sync()
 
 __global__ __launch_bounds__(1024) void k_rescore_res(Dims d, const float *__restrict__ V,
                                                       const float *__restrict__ TmT, const float *__restrict__ bias,
-                                                      const uint64_t *__restrict__ rq, const uint2 *__restrict__ rdesc,
-                                                      const uint32_t *__restrict__ win, uint32_t ndesc,
-                                                      uint32_t nchunk, float *L)
+                                                      const uint64_t *__restrict__ rq, const Counters *__restrict__ ctr,
+                                                      uint32_t rq_cap, float *L)
 {
-    extern __shared__ float lds[];   // [C][N][Tp] templates, then [waves][span] voltage rows
+    extern __shared__ float lds[];   // [C][N][Tp] templates, then [waves][kRescoreStage] voltage rows
     const uint32_t N = d.N, Tp = d.Tp, C = d.C;
     const uint32_t ntm = C * N * Tp;
     const uint32_t span = ((N + 15) / 16) * 16 + N;   // window positions (padded) + template length
+    const uint32_t nwb = kRescoreStage / span < 4 ? kRescoreStage / span : 4u;   // windows per batch
+    if (nwb == 0) return;   // the host launches this kernel only for span <= kRescoreStage
     for (uint32_t x = threadIdx.x; x < ntm; x += blockDim.x) lds[x] = TmT[x];
     __syncthreads();
     const uint32_t lane = lane_of(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    float *vb = lds + ntm + wave * span;
-    for (uint32_t di = blockIdx.x * nw + wave; di < ndesc; di += gridDim.x * nw) {
-        const uint2 dsc = rdesc[di];
-        const uint32_t cnt = dsc.y;
-        if (cnt == 0) continue;
-        const uint64_t base = (uint64_t)win[di / nchunk] * N;
-        for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
-            const uint32_t e = e0 + lane;
-            const bool ok = e < cnt;
-            const uint64_t ent = ok ? rq[dsc.x + e] : 0;
-            const uint32_t t = (uint32_t)(ent >> 32) & 0xffffu, m = (uint32_t)(ent >> 48) & 15u;
-            const uint32_t p = ok ? (uint32_t)((ent & 0xffffffffull) - base) : 0u;   // multiple of 4, below 64
-            const float b0 = bias[t];
-            float acc[4] = {b0, b0, b0, b0};
-            float rv[4];
-            auto fetch = [&](uint32_t c) {
-                const float *vr = V + (uint64_t)c * d.K;
+    float *vb = lds + ntm + wave * kRescoreStage;
+    const uint32_t n = min(ctr->n_rescore, rq_cap);
+    const uint32_t gw = blockIdx.x * nw + wave, nwave = gridDim.x * nw;
+    const uint32_t per = (n + nwave - 1) / nwave;
+    const uint32_t e_end = min(n, (gw + 1) * per);
+    for (uint32_t b0 = gw * per; b0 < e_end;) {
+        const uint32_t e = b0 + lane;
+        const bool in = e < e_end;
+        const uint64_t ent = in ? rq[e] : 0;
+        const uint32_t t = (uint32_t)(ent >> 32) & 0xffffu, m = (uint32_t)(ent >> 48) & 15u;
+        const uint32_t i0 = (uint32_t)ent;
+        const uint32_t wid = i0 / N;
+        // windows of the batch: ranks of the first entry of each window
+        const uint32_t wprev = __shfl_up(wid, 1);
+        const bool first = in && (lane == 0 || wprev != wid);
+        const uint64_t fm = __ballot(first);
+        const uint32_t rank = __popcll(fm & ((2ull << lane) - 1)) - 1;   // in: lane 0 is a first
+        const bool ok = in && rank < nwb;
+        const uint64_t okm = __ballot(ok);
+        uint32_t wb[4];
+        uint32_t nwin = 0;
+        {
+            uint64_t f = fm;
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint32_t x = lane + 64u * q;
-                    rv[q] = (x < span && base + x < d.K) ? vr[base + x] : 0.0f;
-                }
-            };
-            fetch(0);
-            for (uint32_t c = 0; c < C; c++) {
-                wave_sync();   // every lane is done with channel c - 1
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint32_t x = lane + 64u * q;
-                    if (x < span) vb[x] = rv[q];
-                }
-                wave_sync();
-                if (c + 1 < C) fetch(c + 1);
-                fma_chain4(acc, lds + (uint64_t)c * N * Tp + t, Tp, vb + p, N);
+            for (int r = 0; r < 4; r++) {
+                const int l = f ? __builtin_ctzll(f) : 0;
+                wb[r] = __shfl(wid, l) * N;
+                if (f && (uint32_t)r < nwb) nwin = r + 1;
+                f &= f - 1;
             }
-            if (ok)
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-                    if (m >> q & 1) L[(uint64_t)t * d.K + base + p + q] = acc[q];
         }
+        const uint32_t p = ok ? i0 - wid * N + rank * span : 0u;   // offset in the staged rows
+        const float b0v = bias[t];
+        float acc[4] = {b0v, b0v, b0v, b0v};
+        // staged sample x of a channel: window x / span, offset x % span
+        constexpr uint32_t kQ = kRescoreStage / 64;
+        uint32_t src[kQ];
+        bool has[kQ];
+#pragma unroll
+        for (uint32_t q = 0; q < kQ; q++) {
+            const uint32_t x = lane + 64u * q, r = x / span, o = x - r * span;
+            const uint32_t rr = r < 4 ? r : 3u;
+            has[q] = r < nwin && (uint64_t)wb[rr] + o < d.K;
+            src[q] = has[q] ? wb[rr] + o : 0u;
+        }
+        float rv[kQ];
+        auto fetch = [&](uint32_t c) {
+            const float *vr = V + (uint64_t)c * d.K;
+#pragma unroll
+            for (uint32_t q = 0; q < kQ; q++) rv[q] = has[q] ? vr[src[q]] : 0.0f;
+        };
+        fetch(0);
+        for (uint32_t c = 0; c < C; c++) {
+            wave_sync();   // every lane is done with channel c - 1
+#pragma unroll
+            for (uint32_t q = 0; q < kQ; q++) vb[lane + 64u * q] = rv[q];
+            wave_sync();
+            if (c + 1 < C) fetch(c + 1);
+            fma_chain4(acc, lds + (uint64_t)c * N * Tp + t, Tp, vb + p, N);
+        }
+        if (ok)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (m >> q & 1) L[(uint64_t)t * d.K + i0 + q] = acc[q];
+        b0 += __popcll(okm);
     }
 }
 
 static size_t rescore_res_lds(const Dims &d)
 {
-    const uint64_t span = ((d.N + 15) / 16) * 16 + d.N;
-    return ((uint64_t)d.C * d.N * d.Tp + kRescoreResWaves * span) * sizeof(float);
+    return ((uint64_t)d.C * d.N * d.Tp + kRescoreResWaves * kRescoreStage) * sizeof(float);
 }
 
 __global__ void k_zero_rescore(Counters *ctr)
@@ -2454,7 +2481,7 @@ hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_wi
     if (!no_rescore) {
         const uint32_t nchunk = mfma_nchunk(d);
         const size_t rlds = rescore_res_lds(d);
-        if (rlds <= 150 * 1024) {
+        if (rlds <= 150 * 1024 && ((d.N + 15) / 16) * 16 + d.N <= kRescoreStage) {
             hipError_t e2 = hipFuncSetAttribute((const void *)k_rescore_res,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds);
             if (e2 != hipSuccess) return e2;
@@ -2462,11 +2489,8 @@ hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_wi
             if (hipGetDevice(&dev) != hipSuccess ||
                 hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
                 ncu = 256;
-            const uint32_t ndesc = n_win * nchunk;
-            const uint32_t need = (ndesc + kRescoreResWaves - 1) / kRescoreResWaves;
-            const uint32_t grid = need < (uint32_t)ncu ? need : (uint32_t)ncu;
-            hipLaunchKernelGGL(k_rescore_res, dim3(grid), dim3(64 * kRescoreResWaves), rlds, st, d, s.V, s.TmT,
-                               s.bias, s.rq, (const uint2 *)s.rdesc, s.win, ndesc, nchunk, s.L);
+            hipLaunchKernelGGL(k_rescore_res, dim3(ncu), dim3(64 * kRescoreResWaves), rlds, st, d, s.V, s.TmT,
+                               s.bias, s.rq, s.ctr, rq_cap, s.L);
             return hipGetLastError();
         }
         hipLaunchKernelGGL(k_rescore_sup, dim3(n_win * nchunk), dim3(64), 0, st, d, s.V, s.Tm, s.bias, s.sup_lo,
