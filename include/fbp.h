@@ -167,6 +167,35 @@ int fbp_segment_get_residual(fbp_segment *seg, float *residual);
 int fbp_segment_get_likelihood(fbp_segment *seg, float *likelihood);
 int fbp_segment_get_stats(const fbp_segment *seg, fbp_stats *stats);
 
+/* Device address of the segment's input voltage [C][row_stride] (restored
+ * before every run), for fbp_get_clips on resident data. */
+int fbp_segment_voltage_device(const fbp_segment *seg, const float **d_voltage, uint64_t *row_stride);
+
+/* get_clips (parallel/segment_parallel.py:440-509), as binary pursuit's
+ * caller uses it on the input voltage (full_binary_pursuit.py:357-359).
+ * d_voltage: device float32 [n_chans][row_stride], n_samples valid per row.
+ * event_indices: host int64 [n_events], ascending as the reference requires.
+ * channels: host uint32 [n_sel] (the reference's `neighbors`).  The clip of
+ * event k is samples [index + w0, index + w1) of each selected channel, in
+ * `channels` order (w0, w1 from time_window_to_samples).  With check_valid,
+ * events are trimmed from the front while index + w0 < 0 and from the back
+ * while index + w1 >= n_samples (segment_parallel.py:462-484); an event left
+ * in between whose clip leaves the data is FBP_EINVAL.  valid: host uint8
+ * [n_events] (1 = clip returned); clips: host float32
+ * [n_valid][n_sel * (w1 - w0)] in event order. */
+int fbp_get_clips(int device, const float *d_voltage, uint64_t row_stride, uint32_t n_chans, uint64_t n_samples,
+                  const int64_t *event_indices, uint64_t n_events, const uint32_t *channels, uint32_t n_sel,
+                  int32_t w0, int32_t w1, int check_valid, float *clips, uint8_t *valid, uint64_t *n_valid);
+
+/* Mean clip of every unit: np.mean(clips[labels == u], axis=0) on float32
+ * clips (full_binary_pursuit.py:366-368), bit for bit: rows added in input
+ * order in float32, the sum divided by the row count.  clips: host float32
+ * [n][width]; labels: host uint32 [n], each < n_units; means: host float32
+ * [n_units][width] (NaN for a unit without rows); counts: optional host
+ * uint64 [n_units]. */
+int fbp_unit_mean_clips(int device, const float *clips, uint64_t n, uint32_t width, const uint32_t *labels,
+                        uint32_t n_units, float *means, uint64_t *counts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -226,3 +226,61 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
     else:
         ev, lab, bpb, clips = [], [], [], []
     return ev, lab, bpb, clips, residual
+
+
+def get_clips(probe_dict, voltage, neighbors, event_indices, clip_width, check_valid=True):
+    """parallel/segment_parallel.py:451-509 (restated, without memmap): the
+    front/back trimming scans and the per-channel slices, on the host."""
+    window, _ = time_window_to_samples(clip_width, probe_dict['sampling_rate'])
+    W = window[1] - window[0]
+    if len(event_indices) == 0:
+        return np.zeros((0, W * len(neighbors)), dtype=probe_dict['v_dtype']), np.ones(0, dtype=bool)
+    valid = np.ones(event_indices.shape[0], dtype=bool)
+    start = 0
+    stop = event_indices.shape[0] - 1
+    if check_valid:
+        while event_indices[start] + window[0] < 0:
+            valid[start] = False
+            start += 1
+            if start == event_indices.size:
+                valid[:] = False
+                return None, valid
+        while event_indices[stop] + window[1] >= probe_dict['n_samples']:
+            valid[stop] = False
+            stop -= 1
+            if stop < 0:
+                valid[:] = False
+                return None, valid
+    out = np.empty((np.count_nonzero(valid), W * len(neighbors)), dtype=probe_dict['v_dtype'])
+    for row, k in enumerate(range(start, stop + 1)):
+        for j, chan in enumerate(neighbors):
+            out[row, j * W:(j + 1) * W] = voltage[chan, event_indices[k] + window[0]:event_indices[k] + window[1]]
+    return out, valid
+
+
+def unit_mean_clips(clips, labels, n_units):
+    """full_binary_pursuit.py:368: np.mean(clips[labels == unit], axis=0) of
+    every unit (NaN rows for units without clips)."""
+    out = np.full((n_units, clips.shape[1]), np.nan, dtype=np.float32)
+    for u in range(n_units):
+        sel = labels == u
+        if np.any(sel):
+            out[u] = np.mean(clips[sel, :], axis=0)
+    return out
+
+
+def unit_channels(clips, labels, n_channels, n_samples_per_chan, thresholds, sigma):
+    """full_binary_pursuit.py:365-383 (restated): {channel: [units]}, each
+    unit on the first channel with the best range / (3 * noise) of its mean
+    clip."""
+    out = {c: [] for c in range(n_channels)}
+    for unit in np.unique(labels):
+        tmpl = np.mean(clips[labels == unit, :], axis=0)
+        best, best_c = -1.0, None
+        for c in range(n_channels):
+            seg = tmpl[c * n_samples_per_chan:(c + 1) * n_samples_per_chan]
+            snr = (np.amax(seg) - np.amin(seg)) / (3 * (thresholds[c] / sigma))
+            if snr > best:
+                best, best_c = snr, c
+        out[best_c].append(unit)
+    return out

@@ -34,6 +34,7 @@ EXPORTS = (
     "fbp_segment_set_voltage", "fbp_segment_set_voltage_device", "fbp_segment_run",
     "fbp_segment_num_spikes", "fbp_segment_get_spikes", "fbp_segment_get_clips",
     "fbp_segment_get_residual", "fbp_segment_get_likelihood", "fbp_segment_get_stats",
+    "fbp_segment_voltage_device", "fbp_get_clips", "fbp_unit_mean_clips",
 )
 
 
@@ -109,6 +110,14 @@ def _declare(lib):
     lib.fbp_segment_get_residual.argtypes = [vp, f32p]
     lib.fbp_segment_get_likelihood.argtypes = [vp, f32p]
     lib.fbp_segment_get_stats.argtypes = [vp, P(FbpStats)]
+    lib.fbp_segment_voltage_device.argtypes = [vp, P(vp), u64p]
+    i64p = P(ctypes.c_int64)
+    u8p = P(ctypes.c_uint8)
+    lib.fbp_get_clips.argtypes = [ctypes.c_int, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, i64p,
+                                  ctypes.c_uint64, u32p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32,
+                                  ctypes.c_int, f32p, u8p, u64p]
+    lib.fbp_unit_mean_clips.argtypes = [ctypes.c_int, f32p, ctypes.c_uint64, ctypes.c_uint32, u32p,
+                                        ctypes.c_uint32, f32p, u64p]
     for name in EXPORTS:
         if name not in ("fbp_last_error", "fbp_segment_destroy"):
             getattr(lib, name).restype = ctypes.c_int
@@ -190,6 +199,7 @@ class Segment:
         check(self._lib.fbp_segment_create(device, ctypes.byref(self.cfg), ctypes.byref(self._h)),
               "fbp_segment_create")
         self.C, self.K, self.T, self.N = n_chans, n_samples, n_templates, template_len
+        self.device = device
 
     def close(self):
         if self._h is not None and self._h.value:
@@ -267,7 +277,50 @@ class Segment:
         check(self._lib.fbp_segment_get_likelihood(self._h, _f32p(out)), "fbp_segment_get_likelihood")
         return out
 
+    def voltage_device(self):
+        """(device address, row stride) of the segment's input voltage."""
+        ptr, stride = ctypes.c_void_p(None), ctypes.c_uint64(0)
+        check(self._lib.fbp_segment_voltage_device(self._h, ctypes.byref(ptr), ctypes.byref(stride)),
+              "fbp_segment_voltage_device")
+        return ptr.value, stride.value
+
     def stats(self):
         st = FbpStats()
         check(self._lib.fbp_segment_get_stats(self._h, ctypes.byref(st)), "fbp_segment_get_stats")
         return st.as_dict()
+
+
+def get_clips_device(d_voltage, row_stride, n_chans, n_samples, event_indices, channels, w0, w1,
+                     check_valid=True, device=0):
+    """fbp_get_clips on device-resident float32 voltage: (clips [n_valid][len(channels)*(w1-w0)],
+    valid mask [n_events])."""
+    lib = load()
+    idx = np.ascontiguousarray(event_indices, dtype=np.int64)
+    ch = np.ascontiguousarray(channels, dtype=np.uint32)
+    width = ch.size * (w1 - w0)
+    valid = np.zeros(idx.size, dtype=np.uint8)
+    n_valid = ctypes.c_uint64(0)
+    # the valid events are a contiguous run of at most n_events
+    out = np.empty((idx.size, width), dtype=np.float32)
+    check(lib.fbp_get_clips(device, ctypes.c_void_p(d_voltage), row_stride, n_chans, n_samples,
+                            idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), idx.size, _u32p(ch), ch.size,
+                            w0, w1, 1 if check_valid else 0, _f32p(out),
+                            valid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(n_valid)),
+          "fbp_get_clips")
+    return out[:n_valid.value], valid.astype(bool)
+
+
+def unit_mean_clips(clips, labels, n_units, device=0):
+    """fbp_unit_mean_clips: (means [n_units][width] float32, counts [n_units])."""
+    lib = load()
+    c = np.ascontiguousarray(clips, dtype=np.float32)
+    if c.ndim != 2:
+        raise ValueError("clips must be two-dimensional")
+    lab = np.ascontiguousarray(labels, dtype=np.uint32)
+    if lab.size != c.shape[0]:
+        raise ValueError("one label per clip is required")
+    means = np.empty((n_units, c.shape[1]), dtype=np.float32)
+    counts = np.zeros(n_units, dtype=np.uint64)
+    check(lib.fbp_unit_mean_clips(device, _f32p(c), c.shape[0], c.shape[1], _u32p(lab), n_units, _f32p(means),
+                                  counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))), "fbp_unit_mean_clips")
+    return means, counts

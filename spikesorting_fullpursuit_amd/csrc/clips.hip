@@ -1,0 +1,87 @@
+// Clip gathers around the binary pursuit call: get_clips on the input voltage
+// (parallel/segment_parallel.py:440-509, called at full_binary_pursuit.py:357-359)
+// and the per-unit mean clip the caller reduces them to
+// (full_binary_pursuit.py:366-368).  Both are data movement (HBM-bound); the
+// mean keeps numpy's float32 order so it is bit-exact.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fbp_internal.h"
+
+namespace fbp {
+namespace {
+
+// clips[k][j * W + x] = V[chan[j]][idx[k] + w0 + x] for x < W = w1 - w0.  One
+// block per event; threads stride over the n_sel * W samples of the clip, so
+// each channel's W samples are read and written contiguously.
+__global__ __launch_bounds__(256) void k_get_clips(const float *__restrict__ V, uint64_t row_stride,
+                                                   const int64_t *__restrict__ idx, uint64_t n_events,
+                                                   const uint32_t *__restrict__ chan, uint32_t n_sel, int32_t w0,
+                                                   uint32_t W, float *__restrict__ clips)
+{
+    const uint32_t width = n_sel * W;
+    for (uint64_t k = blockIdx.x; k < n_events; k += gridDim.x) {
+        const int64_t start = idx[k] + w0;   // >= 0 and start + W <= n_samples (checked on the host)
+        float *dst = clips + k * width;
+        for (uint32_t e = threadIdx.x; e < width; e += blockDim.x) {
+            const uint32_t j = e / W, x = e - j * W;
+            dst[e] = V[(uint64_t)chan[j] * row_stride + (uint64_t)start + x];
+        }
+    }
+}
+
+// means[u][x] = (((c[r0][x] + c[r1][x]) + c[r2][x]) + ...) / n_u in float32,
+// rows r0, r1, ... of unit u in input order (rows[off[u] .. off[u + 1])): the
+// order numpy's axis-0 float32 mean adds them.  One thread per (unit,
+// column); the row loads of eight steps are issued before their adds.
+__global__ __launch_bounds__(256) void k_unit_means(const float *__restrict__ clips, uint32_t width,
+                                                    const uint32_t *__restrict__ rows,
+                                                    const uint64_t *__restrict__ off, uint32_t n_units,
+                                                    float *__restrict__ means)
+{
+    const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (g >= (uint64_t)n_units * width) return;
+    const uint32_t u = (uint32_t)(g / width), x = (uint32_t)(g - (uint64_t)u * width);
+    const uint64_t b = off[u], e = off[u + 1];
+    if (b == e) {
+        means[g] = __builtin_nanf("");   // numpy: mean of no rows
+        return;
+    }
+    float acc = clips[(uint64_t)rows[b] * width + x];
+    uint64_t r = b + 1;
+    for (; r + 8 <= e; r += 8) {
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = clips[(uint64_t)rows[r + q] * width + x];
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc = acc + v[q];
+    }
+    for (; r < e; r++) acc = acc + clips[(uint64_t)rows[r] * width + x];
+    means[g] = acc / (float)(e - b);
+}
+
+}  // namespace
+
+hipError_t launch_get_clips(const float *V, uint64_t row_stride, const int64_t *idx, uint64_t n_events,
+                            const uint32_t *chan, uint32_t n_sel, int32_t w0, uint32_t W, float *clips,
+                            hipStream_t st)
+{
+    if (n_events == 0) return hipSuccess;
+    const uint64_t grid = n_events < 65536 ? n_events : 65536;
+    hipLaunchKernelGGL(k_get_clips, dim3((uint32_t)grid), dim3(256), 0, st, V, row_stride, idx, n_events, chan,
+                       n_sel, w0, W, clips);
+    return hipGetLastError();
+}
+
+hipError_t launch_unit_means(const float *clips, uint32_t width, const uint32_t *rows, const uint64_t *off,
+                             uint32_t n_units, float *means, hipStream_t st)
+{
+    const uint64_t total = (uint64_t)n_units * width;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unit_means, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, clips, width, rows,
+                       off, n_units, means);
+    return hipGetLastError();
+}
+
+}  // namespace fbp

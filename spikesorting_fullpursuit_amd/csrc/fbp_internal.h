@@ -128,6 +128,11 @@ hipError_t launch_compact_next(const Dims &d, const DeviceState &s, hipStream_t 
 hipError_t launch_residual(const Dims &d, const DeviceState &s, uint64_t spike_base,
                            uint32_t n_new, hipStream_t st);
 hipError_t launch_clear_win_spike(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
+hipError_t launch_get_clips(const float *V, uint64_t row_stride, const int64_t *idx, uint64_t n_events,
+                            const uint32_t *chan, uint32_t n_sel, int32_t w0, uint32_t W, float *clips,
+                            hipStream_t st);
+hipError_t launch_unit_means(const float *clips, uint32_t width, const uint32_t *rows, const uint64_t *off,
+                             uint32_t n_units, float *means, hipStream_t st);
 hipError_t launch_clips(const Dims &d, const DeviceState &s, uint64_t n_spikes, hipStream_t st);
 
 }  // namespace fbp

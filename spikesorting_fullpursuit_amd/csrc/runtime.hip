@@ -637,4 +637,117 @@ int fbp_segment_get_stats(const fbp_segment *g, fbp_stats *stats)
     return FBP_OK;
 }
 
+int fbp_segment_voltage_device(const fbp_segment *g, const float **d_voltage, uint64_t *row_stride)
+{
+    if (!g || !d_voltage || !row_stride) return fail(FBP_EINVAL, "NULL argument");
+    if (!g->have_voltage) return fail(FBP_ESTATE, "segment has no voltage");
+    *d_voltage = g->s.V0;
+    *row_stride = g->d.K;
+    return FBP_OK;
+}
+
+int fbp_get_clips(int device, const float *d_voltage, uint64_t row_stride, uint32_t n_chans, uint64_t n_samples,
+                  const int64_t *event_indices, uint64_t n_events, const uint32_t *channels, uint32_t n_sel,
+                  int32_t w0, int32_t w1, int check_valid, float *clips, uint8_t *valid, uint64_t *n_valid)
+{
+    if (!n_valid || (n_events && (!event_indices || !valid)) || (n_sel && !channels))
+        return fail(FBP_EINVAL, "NULL argument");
+    if (w1 <= w0) return fail(FBP_EINVAL, "empty clip window [%d, %d)", w0, w1);
+    if (row_stride < n_samples) return fail(FBP_EINVAL, "row stride %llu below the sample count %llu",
+                                            (unsigned long long)row_stride, (unsigned long long)n_samples);
+    for (uint32_t j = 0; j < n_sel; j++)
+        if (channels[j] >= n_chans) return fail(FBP_EINVAL, "channel %u out of range (%u channels)", channels[j], n_chans);
+    *n_valid = 0;
+    if (n_events == 0) return FBP_OK;
+    // segment_parallel.py:462-484: invalid events are trimmed from the front
+    // and the back only; the events between must lie inside the data.
+    uint64_t lo = 0, hi = n_events;   // valid events are [lo, hi)
+    if (check_valid) {
+        while (lo < n_events && event_indices[lo] + w0 < 0) lo++;
+        while (hi > lo && event_indices[hi - 1] + w1 >= (int64_t)n_samples) hi--;
+    }
+    for (uint64_t k = 0; k < n_events; k++) valid[k] = (k >= lo && k < hi) ? 1 : 0;
+    for (uint64_t k = lo; k < hi; k++)
+        if (event_indices[k] + w0 < 0 || event_indices[k] + w1 > (int64_t)n_samples)
+            return fail(FBP_EINVAL, "event %llu (index %lld) has a clip outside the data; event indices must be "
+                        "ascending", (unsigned long long)k, (long long)event_indices[k]);
+    const uint64_t n = hi - lo;
+    *n_valid = n;
+    const uint32_t W = (uint32_t)(w1 - w0);
+    const uint64_t width = (uint64_t)n_sel * W;
+    if (n == 0 || width == 0) return FBP_OK;
+    if (!clips) return fail(FBP_EINVAL, "NULL argument");
+    if (!d_voltage) return fail(FBP_EINVAL, "NULL voltage");
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+    int64_t *d_idx = nullptr;
+    uint32_t *d_chan = nullptr;
+    float *d_clips = nullptr;
+    hipError_t e = dalloc(&d_idx, n, st);
+    if (e == hipSuccess) e = dalloc(&d_chan, n_sel, st);
+    if (e == hipSuccess) e = dalloc(&d_clips, n * width, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_idx, event_indices + lo, n * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_chan, channels, (uint64_t)n_sel * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_get_clips(d_voltage, row_stride, d_idx, n, d_chan, n_sel, w0, W, d_clips, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(clips, d_clips, n * width * 4, hipMemcpyDeviceToHost, st);
+    dfree(d_idx, st);
+    dfree(d_chan, st);
+    dfree(d_clips, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    if (e == hipSuccess) e = es;
+    HIP_TRY(e, "get_clips");
+    return FBP_OK;
+}
+
+int fbp_unit_mean_clips(int device, const float *clips, uint64_t n, uint32_t width, const uint32_t *labels,
+                        uint32_t n_units, float *means, uint64_t *counts)
+{
+    if ((n && (!clips || !labels)) || (n_units && width && !means)) return fail(FBP_EINVAL, "NULL argument");
+    // rows of each unit in input order (counting sort)
+    std::vector<uint64_t> off(n_units + 1, 0);
+    for (uint64_t k = 0; k < n; k++) {
+        if (labels[k] >= n_units) return fail(FBP_EINVAL, "label %u out of range (%u units)", labels[k], n_units);
+        off[labels[k] + 1]++;
+    }
+    for (uint32_t u = 0; u < n_units; u++) off[u + 1] += off[u];
+    if (counts)
+        for (uint32_t u = 0; u < n_units; u++) counts[u] = off[u + 1] - off[u];
+    if (n_units == 0 || width == 0) return FBP_OK;
+    std::vector<uint32_t> rows(n ? n : 1);
+    {
+        std::vector<uint64_t> pos(off.begin(), off.end() - 1);
+        for (uint64_t k = 0; k < n; k++) {
+            if (k > 0xffffffffull) return fail(FBP_EINVAL, "more than 2^32 clips");
+            rows[pos[labels[k]]++] = (uint32_t)k;
+        }
+    }
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+    float *d_clips = nullptr, *d_means = nullptr;
+    uint32_t *d_rows = nullptr;
+    uint64_t *d_off = nullptr;
+    hipError_t e = dalloc(&d_clips, n * width, st);
+    if (e == hipSuccess) e = dalloc(&d_rows, n, st);
+    if (e == hipSuccess) e = dalloc(&d_off, (uint64_t)n_units + 1, st);
+    if (e == hipSuccess) e = dalloc(&d_means, (uint64_t)n_units * width, st);
+    if (e == hipSuccess && n) e = hipMemcpyAsync(d_clips, clips, n * width * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && n) e = hipMemcpyAsync(d_rows, rows.data(), n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), ((uint64_t)n_units + 1) * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_unit_means(d_clips, width, d_rows, d_off, n_units, d_means, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(means, d_means, (uint64_t)n_units * width * 4, hipMemcpyDeviceToHost, st);
+    dfree(d_clips, st);
+    dfree(d_rows, st);
+    dfree(d_off, st);
+    dfree(d_means, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    if (e == hipSuccess) e = es;
+    HIP_TRY(e, "unit means");
+    return FBP_OK;
+}
+
 }  // extern "C"

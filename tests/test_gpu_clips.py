@@ -1,0 +1,119 @@
+"""Clip gathers and unit means on the GPU (fbp_get_clips, fbp_unit_mean_clips)
+against the oracle, bit for bit, and the post-pursuit output of a segment."""
+import numpy as np
+import pytest
+
+from helpers import small_case
+from spikesorting_fullpursuit_amd import _lib, binary_pursuit_parallel as bpp
+from spikesorting_fullpursuit_amd.full_binary_pursuit import binary_pursuit_segment
+from spikesorting_fullpursuit_amd.segment import get_clips
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.int16])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_get_clips_matches_oracle(oracle_mod, seed, dtype):
+    rng = np.random.default_rng(seed)
+    C, K = 7, 50000
+    V = (rng.standard_normal((C, K)) * 300).astype(dtype)
+    pd = {'sampling_rate': 30000, 'n_samples': K, 'v_dtype': dtype}
+    cw = [-1e-3, 1e-3]   # window [-30, 31)
+    ev = np.sort(rng.integers(0, K, 3000)).astype(np.int64)
+    ev[:3] = [0, 5, 29]              # trimmed at the front
+    ev[-2:] = [K - 31, K - 1]        # trimmed at the back
+    ev.sort()
+    nb = rng.permutation(C)[:5]
+    print("case get_clips", seed, dtype.__name__, flush=True)
+    got, gv = get_clips(pd, V, nb, ev, cw)
+    ref, rv = oracle_mod.get_clips(pd, V, nb, ev, cw)
+    np.testing.assert_array_equal(gv, rv)
+    assert got.dtype == ref.dtype and got.shape == ref.shape
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_get_clips_edge_cases(oracle_mod):
+    V = np.arange(4 * 300, dtype=np.float32).reshape(4, 300)
+    pd = {'sampling_rate': 40000, 'n_samples': 300, 'v_dtype': np.float32}
+    cw = [-7.5e-4, 7.5e-4]
+    print("case get_clips edges", flush=True)
+    for ev in (np.array([1, 2]), np.array([280, 290]), np.array([], dtype=np.int64), np.array([30, 268])):
+        got, gv = get_clips(pd, V, [2, 1], ev, cw)
+        ref, rv = oracle_mod.get_clips(pd, V, [2, 1], ev, cw)
+        np.testing.assert_array_equal(gv, rv)
+        if ref is None:
+            assert got is None
+        else:
+            np.testing.assert_array_equal(got, ref)
+    with pytest.raises(ValueError):   # an unordered event in the middle whose clip leaves the data
+        get_clips(pd, V, [0], np.array([40, 2, 100]), cw)
+
+
+def test_unit_means_bitwise(oracle_mod):
+    rng = np.random.default_rng(5)
+    n, width, units = 20000, 4 * 61, 9
+    clips = (rng.standard_normal((n, width)) * np.exp(rng.uniform(-8, 8, (n, 1)))).astype(np.float32)
+    labels = rng.integers(0, units, n)
+    labels[labels == 4] = 5   # unit 4 has no clips
+    print("case unit means", flush=True)
+    got, counts = _lib.unit_mean_clips(clips, labels, units)
+    ref = oracle_mod.unit_mean_clips(clips, labels, units)
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+    np.testing.assert_array_equal(counts, np.bincount(labels, minlength=units))
+
+
+def test_resident_segment_clips(oracle_mod):
+    rec = small_case(61, C=4, T=5, N=61, duration_s=0.3)
+    V = rec['voltage']
+    C, K = V.shape
+    pd = {'sampling_rate': 40000, 'n_samples': K, 'v_dtype': np.float32}
+    ev = np.arange(40, K - 40, 97, dtype=np.int64)
+    print("case resident segment clips", flush=True)
+    with _lib.Segment(C, K, 5, 61, 30) as seg:
+        seg.set_templates(rec['templates'], rec['bias'], rec['thr'])
+        seg.set_voltage(V)
+        seg.run()   # the gather reads the input voltage, not the residual
+        got, gv = get_clips(pd, seg, np.arange(C), ev, [-7.5e-4, 7.5e-4])
+    ref, rv = oracle_mod.get_clips(pd, V, np.arange(C), ev, [-7.5e-4, 7.5e-4])
+    np.testing.assert_array_equal(gv, rv)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("adjusted", [False, True])
+def test_binary_pursuit_segment_output(oracle_mod, adjusted):
+    rec = small_case(62, C=4, T=5, N=61, duration_s=0.5, overlap_frac=0.1)
+    V, sm = rec['voltage'], dict(rec['separability_metrics'])
+    C, K = V.shape
+    si = dict(rec['sort_info'], get_adjusted_clips=adjusted, n_channels=C, sigma=4.0,
+              output_separability_metrics=False)
+    thr = np.full(C, 4.0)
+    items = [{'channel': c, 'ID': 100 + c, 'index_window': [5000, 5000 + K], 'n_samples': K, 'seg_number': 0,
+              'thresholds': thr, 'neighbors': np.arange(C)} for c in range(C)]
+    starts = [c * 61 for c in range(C)]
+    stops = [(c + 1) * 61 for c in range(C)]
+    print("case segment output", adjusted, flush=True)
+    out = binary_pursuit_segment(V, np.float32, si, sm, items, starts, stops, [-7.5e-4, 7.5e-4], 61,
+                                 output_clips=True)
+    # oracle: pursuit with one chunk, clips of the input voltage (or adjusted
+    # clips), unit channels from numpy means
+    chunk_len, _ = bpp.plan_chunks(C, K, 40000, 5, 61, 30, _lib.device_memory(0)[0])
+    ev, lab, bpb, clips, _ = oracle_mod.binary_pursuit(V, np.float32, dict(si, clip_width=[-7.5e-4, 7.5e-4],
+                                                                           n_samples_per_chan=61),
+                                                       sm, None, chunk_len)
+    if not adjusted:
+        clips, _ = oracle_mod.get_clips({'sampling_rate': 40000, 'n_samples': K, 'v_dtype': np.float32}, V,
+                                        np.arange(C), ev, [-7.5e-4, 7.5e-4])
+    assign = oracle_mod.unit_channels(clips, lab, C, 61, thr, 4.0)
+    assert len(ev) > 0
+    for c in range(C):
+        got = out[c]
+        assert got[4] == 100 + c
+        units = assign[c]
+        if not units:
+            assert got[:4] == [[], [], [], []]
+            continue
+        sel = np.concatenate([np.flatnonzero(lab == u) for u in units])
+        np.testing.assert_array_equal(got[0], ev[sel] + 5000)
+        np.testing.assert_array_equal(got[1], lab[sel])
+        np.testing.assert_array_equal(got[2], clips[sel])
+        assert got[3].all()
