@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--serve-inflight", type=int, default=8)
     ap.add_argument("--no-precision-report", action="store_true",
                     help="skip the likelihood error report against the float64 oracle (4 ch x 60 s)")
+    ap.add_argument("--no-post", action="store_true",
+                    help="skip the post-pursuit clip gather and unit means (get_clips on the input voltage)")
     ap.add_argument("--segments", type=int, default=0,
                     help="also run the recording as this many equal time segments concurrently (one HIP stream "
                          "each) and report their throughput")
@@ -137,6 +139,10 @@ def main():
     if args.segments > 1:
         concurrent = concurrent_segments_run(args, V, sm, bias, thr, N, M, do_overlap)
 
+    post = None
+    if not args.no_post:
+        post = post_pursuit_run(seg, V, rec['sort_info'], T)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(rec, bias, thr, N, M, do_overlap, args.cpu_target_s, seg_cls=_lib.Segment,
@@ -161,10 +167,58 @@ def main():
         "fp32_plain": plain,
         "serving": serve,
         "concurrent_segments": concurrent,
+        "post_pursuit": post,
         "precision_errors": prec,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def post_pursuit_run(seg, V, sort_info, T, repeats=3):
+    """The caller's clips after binary pursuit (full_binary_pursuit.py:357-368):
+    get_clips of every found spike on all channels of the resident input
+    voltage (fbp_get_clips, clips copied to the host) and the per-unit mean
+    clips (fbp_unit_mean_clips, clips uploaded).  Times are whole calls,
+    PCIe copies included; the numpy gather of the same clips is timed beside
+    them on the host."""
+    from spikesorting_fullpursuit_amd import _lib
+    from spikesorting_fullpursuit_amd.segment import get_clips, time_window_to_samples
+    idx, lab = seg.spikes()
+    C, K = V.shape
+    window, _ = time_window_to_samples(sort_info['clip_width'], sort_info['sampling_rate'])
+    ev = idx.astype(np.int64) - window[0]   # event indices = onsets + clip_init_samples
+    pd = {'sampling_rate': sort_info['sampling_rate'], 'n_samples': K, 'v_dtype': np.float32}
+    chans = np.arange(C)
+    t_clip, t_mean = [], []
+    for _ in range(repeats):
+        t = time.perf_counter()
+        clips, valid = get_clips(pd, seg, chans, ev, sort_info['clip_width'])
+        t_clip.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        means, counts = _lib.unit_mean_clips(clips, lab, T)
+        t_mean.append(time.perf_counter() - t)
+    W = window[1] - window[0]
+    rows = np.arange(0, ev.size, max(1, ev.size // 2000))
+    sample_ok = all(np.array_equal(clips[r], V[:, ev[r] + window[0]:ev[r] + window[1]].ravel()) for r in rows)
+    unit = int(np.argmax(counts))
+    mean_ok = bool(np.array_equal(means[unit].view(np.uint32),
+                                  np.mean(clips[lab == unit], axis=0).view(np.uint32)))
+    t = time.perf_counter()
+    host = np.empty_like(clips)
+    for r in range(ev.size):
+        host[r] = V[:, ev[r] + window[0]:ev[r] + window[1]].ravel()
+    t_host = time.perf_counter() - t
+    nbytes = clips.nbytes
+    tc, tm = float(np.median(t_clip)), float(np.median(t_mean))
+    log("post-pursuit: %d clips x %d floats, get_clips %.1f ms, unit means %.1f ms, host gather %.1f ms"
+        % (ev.size, C * W, 1e3 * tc, 1e3 * tm, 1e3 * t_host))
+    return {"n_clips": int(ev.size), "clip_floats": int(C * W), "clip_bytes": int(nbytes),
+            "get_clips_ms": 1e3 * tc, "get_clips_GBps": nbytes / tc / 1e9,
+            "unit_means_ms": 1e3 * tm, "n_units": int(np.count_nonzero(counts)),
+            "host_numpy_gather_ms": 1e3 * t_host, "all_valid": bool(valid.all()),
+            "sampled_clips_equal_voltage": bool(sample_ok), "largest_unit_mean_equals_numpy": mean_ok,
+            "note": "whole calls: get_clips includes the device-to-host copy of the clips, unit means the "
+                    "host-to-device copy; the gather and mean kernels are in the rocprofv3 kernel table"}
 
 
 def precision_report(args):

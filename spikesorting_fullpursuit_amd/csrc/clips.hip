@@ -13,20 +13,22 @@ namespace fbp {
 namespace {
 
 // clips[k][j * W + x] = V[chan[j]][idx[k] + w0 + x] for x < W = w1 - w0.  One
-// block per event; threads stride over the n_sel * W samples of the clip, so
-// each channel's W samples are read and written contiguously.
+// block per event, one wave per channel at a time: lane x copies sample x
+// (and x + 64, ...), so each channel's W samples are read and written
+// contiguously.
 __global__ __launch_bounds__(256) void k_get_clips(const float *__restrict__ V, uint64_t row_stride,
                                                    const int64_t *__restrict__ idx, uint64_t n_events,
                                                    const uint32_t *__restrict__ chan, uint32_t n_sel, int32_t w0,
                                                    uint32_t W, float *__restrict__ clips)
 {
     const uint32_t width = n_sel * W;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     for (uint64_t k = blockIdx.x; k < n_events; k += gridDim.x) {
         const int64_t start = idx[k] + w0;   // >= 0 and start + W <= n_samples (checked on the host)
         float *dst = clips + k * width;
-        for (uint32_t e = threadIdx.x; e < width; e += blockDim.x) {
-            const uint32_t j = e / W, x = e - j * W;
-            dst[e] = V[(uint64_t)chan[j] * row_stride + (uint64_t)start + x];
+        for (uint32_t j = wave; j < n_sel; j += nwave) {
+            const float *src = V + (uint64_t)chan[j] * row_stride + (uint64_t)start;
+            for (uint32_t x = lane; x < W; x += 64) dst[j * W + x] = src[x];
         }
     }
 }
