@@ -196,6 +196,22 @@ int fbp_get_clips(int device, const float *d_voltage, uint64_t row_stride, uint3
 int fbp_unit_mean_clips(int device, const float *clips, uint64_t n, uint32_t width, const uint32_t *labels,
                         uint32_t n_units, float *means, uint64_t *counts);
 
+/* Overlap-template search residuals (find_overlap_templates,
+ * c_cython/sort_cython.pyx:333-456, before binary pursuit at
+ * full_binary_pursuit.py:257-262).  templates: host float32 [n_templates]
+ * [n_chans * n_samples_per_chan].  For each triple (test, n1, n2) of
+ * `triples` (host uint32 [n_triples][3]) and every shift pair (s1, s2) in
+ * [-n_pre, n_post]^2, the residual np.sum((test - (shift(n1, s1) +
+ * shift(n2, s2)))**2) in float64 with numpy's summation order (bit-exact);
+ * best_residual / best_combo (host [n_triples]) receive the smallest
+ * residual and the first (s1 + n_pre) * (n_pre + n_post + 1) + (s2 + n_pre)
+ * holding it (0xffffffff when no residual is below infinity), the choice of
+ * the reference's strict < in (s1, s2) loop order. */
+int fbp_overlap_template_residuals(int device, const float *templates, uint32_t n_templates, uint32_t n_chans,
+                                   uint32_t n_samples_per_chan, int32_t n_pre, int32_t n_post,
+                                   const uint32_t *triples, uint64_t n_triples, double *best_residual,
+                                   uint32_t *best_combo);
+
 #ifdef __cplusplus
 }
 #endif

@@ -284,3 +284,58 @@ def unit_channels(clips, labels, n_channels, n_samples_per_chan, thresholds, sig
                 best, best_c = snr, c
         out[best_c].append(unit)
     return out
+
+
+def find_overlap_templates(templates, n_samples_per_chan, n_chans, n_pre_inds, n_post_inds, n_template_spikes):
+    """c_cython/sort_cython.pyx:333-456 (restated with numpy, loops as in the
+    reference).  Returns (templates_to_check, smallest residual of each)."""
+    N = n_samples_per_chan
+
+    def shifted(t, shift):
+        out = np.zeros_like(t)
+        for c in range(n_chans):
+            ct = t[c * N:(c + 1) * N]
+            if shift > 0:
+                out[c * N + shift:(c + 1) * N] = ct[:-shift]
+            else:
+                out[c * N:(c + 1) * N + shift] = ct[-shift:]
+        return out
+
+    to_check, residuals = [], []
+    T = templates.shape[0]
+    if T < 3:
+        return to_check, residuals
+    ss = np.sum(templates ** 2, axis=1)
+    acc = np.zeros(templates.shape[1])
+    all_shifts = []
+    for t in range(T):
+        a = np.zeros((n_post_inds + 1 + n_pre_inds, templates.shape[1]))
+        for i, s in enumerate(range(-n_pre_inds, n_post_inds + 1)):
+            a[i, :] = shifted(templates[t, :], s)
+        all_shifts.append(a)
+    for test_unit in np.argsort(ss)[-1::-1]:
+        test = templates[test_unit, :]
+        min_c, max_c = np.argmin(test) // N, np.argmax(test) // N
+        best_val, best = np.inf, None
+        for n1 in range(T):
+            if n_template_spikes[n1] < 4 * n_template_spikes[test_unit] or n1 == test_unit:
+                continue
+            for n2 in range(n1 + 1, T):
+                if n_template_spikes[n2] < 4 * n_template_spikes[test_unit] or n2 == test_unit:
+                    continue
+                lo, hi = slice(min_c * N, (min_c + 1) * N), slice(max_c * N, (max_c + 1) * N)
+                if np.amin(templates[n1, lo]) + np.amin(templates[n2, lo]) > 0.75 * np.amin(test[lo]):
+                    continue
+                if np.amax(templates[n1, hi]) + np.amax(templates[n2, hi]) < 0.75 * np.amax(test[hi]):
+                    continue
+                for i1 in range(n_post_inds + 1 + n_pre_inds):
+                    for i2 in range(n_post_inds + 1 + n_pre_inds):
+                        acc[:] = all_shifts[n1][i1, :]
+                        acc += all_shifts[n2][i2, :]
+                        r = np.sum((test - acc) ** 2)
+                        if r < best_val:
+                            best_val, best = r, ([n1, n2], np.copy(acc))
+        if best is not None:
+            to_check.append([test_unit, best[1], best[0]])
+            residuals.append(best_val)
+    return to_check, residuals

@@ -34,7 +34,7 @@ EXPORTS = (
     "fbp_segment_set_voltage", "fbp_segment_set_voltage_device", "fbp_segment_run",
     "fbp_segment_num_spikes", "fbp_segment_get_spikes", "fbp_segment_get_clips",
     "fbp_segment_get_residual", "fbp_segment_get_likelihood", "fbp_segment_get_stats",
-    "fbp_segment_voltage_device", "fbp_get_clips", "fbp_unit_mean_clips",
+    "fbp_segment_voltage_device", "fbp_get_clips", "fbp_unit_mean_clips", "fbp_overlap_template_residuals",
 )
 
 
@@ -118,6 +118,9 @@ def _declare(lib):
                                   ctypes.c_int, f32p, u8p, u64p]
     lib.fbp_unit_mean_clips.argtypes = [ctypes.c_int, f32p, ctypes.c_uint64, ctypes.c_uint32, u32p,
                                         ctypes.c_uint32, f32p, u64p]
+    lib.fbp_overlap_template_residuals.argtypes = [ctypes.c_int, f32p, ctypes.c_uint32, ctypes.c_uint32,
+                                                   ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32, u32p,
+                                                   ctypes.c_uint64, P(ctypes.c_double), u32p]
     for name in EXPORTS:
         if name not in ("fbp_last_error", "fbp_segment_destroy"):
             getattr(lib, name).restype = ctypes.c_int
@@ -324,3 +327,17 @@ def unit_mean_clips(clips, labels, n_units, device=0):
     check(lib.fbp_unit_mean_clips(device, _f32p(c), c.shape[0], c.shape[1], _u32p(lab), n_units, _f32p(means),
                                   counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))), "fbp_unit_mean_clips")
     return means, counts
+
+
+def overlap_template_residuals(templates, n_chans, n_samples_per_chan, n_pre, n_post, triples, device=0):
+    """fbp_overlap_template_residuals: (best residual float64 [n], best combination uint32 [n])."""
+    lib = load()
+    tm = np.ascontiguousarray(templates, dtype=np.float32)
+    tri = np.ascontiguousarray(triples, dtype=np.uint32).reshape(-1, 3)
+    res = np.zeros(tri.shape[0], dtype=np.float64)
+    combo = np.zeros(tri.shape[0], dtype=np.uint32)
+    check(lib.fbp_overlap_template_residuals(device, _f32p(tm), tm.shape[0], n_chans, n_samples_per_chan, n_pre,
+                                             n_post, _u32p(tri), tri.shape[0],
+                                             res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _u32p(combo)),
+          "fbp_overlap_template_residuals")
+    return res, combo

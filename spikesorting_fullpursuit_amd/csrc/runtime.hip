@@ -166,6 +166,35 @@ int ensure_spike_capacity(fbp_segment *g, uint64_t need, uint64_t keep, hipStrea
     return FBP_OK;
 }
 
+// numpy's float64 sum of n contiguous elements as a program for the device:
+// blocks of 8192 added in order, each summed pairwise (halves rounded down to
+// a multiple of 8, leaves of at most 128 elements).  A leaf op is
+// start << 8 | length; 0xffffffff adds the top two stack entries.
+void pairwise_leaves(std::vector<uint32_t> &prog, uint64_t start, uint64_t n, int depth, int &max_depth)
+{
+    if (n <= 128) {
+        prog.push_back((uint32_t)(start << 8) | (uint32_t)n);
+        max_depth = std::max(max_depth, depth + 1);
+        return;
+    }
+    uint64_t n2 = n / 2;
+    n2 -= n2 % 8;
+    pairwise_leaves(prog, start, n2, depth, max_depth);
+    pairwise_leaves(prog, start + n2, n - n2, depth + 1, max_depth);
+    prog.push_back(0xffffffffu);
+}
+
+int numpy_sum_program(uint64_t n, std::vector<uint32_t> &prog)
+{
+    int max_depth = 0;
+    for (uint64_t b = 0; b < n; b += 8192) {
+        pairwise_leaves(prog, b, std::min<uint64_t>(8192, n - b), b ? 1 : 0, max_depth);
+        if (b) prog.push_back(0xffffffffu);
+    }
+    return max_depth;
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -698,6 +727,56 @@ int fbp_get_clips(int device, const float *d_voltage, uint64_t row_stride, uint3
     (void)hipStreamDestroy(st);
     if (e == hipSuccess) e = es;
     HIP_TRY(e, "get_clips");
+    return FBP_OK;
+}
+
+int fbp_overlap_template_residuals(int device, const float *templates, uint32_t n_templates, uint32_t n_chans,
+                                   uint32_t n_samples_per_chan, int32_t n_pre, int32_t n_post,
+                                   const uint32_t *triples, uint64_t n_triples, double *best_residual,
+                                   uint32_t *best_combo)
+{
+    if (n_triples && (!templates || !triples || !best_residual || !best_combo)) return fail(FBP_EINVAL, "NULL argument");
+    if (n_pre < 0 || n_post < 0) return fail(FBP_EINVAL, "negative shift range (%d, %d)", n_pre, n_post);
+    const uint64_t CN = (uint64_t)n_chans * n_samples_per_chan;
+    if (CN == 0 || CN >= (1ull << 24)) return fail(FBP_EINVAL, "template length %llu out of range", (unsigned long long)CN);
+    if (3 * CN * sizeof(float) > kMaxLds)
+        return fail(FBP_EINVAL, "three templates of %llu samples do not fit in LDS", (unsigned long long)CN);
+    for (uint64_t k = 0; k < 3 * n_triples; k++)
+        if (triples[k] >= n_templates) return fail(FBP_EINVAL, "template %u out of range", triples[k]);
+    const uint64_t S = (uint64_t)n_pre + n_post + 1;
+    if (S * S >= (1ull << 32)) return fail(FBP_EINVAL, "shift range too large");
+    if (n_triples == 0) return FBP_OK;
+    std::vector<uint32_t> prog;
+    if (numpy_sum_program(CN, prog) > 16) return fail(FBP_EINVAL, "summation tree too deep");
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+    float *d_tm = nullptr;
+    uint32_t *d_tri = nullptr, *d_prog = nullptr, *d_combo = nullptr;
+    double *d_res = nullptr;
+    hipError_t e = dalloc(&d_tm, (uint64_t)n_templates * CN, st);
+    if (e == hipSuccess) e = dalloc(&d_tri, 3 * n_triples, st);
+    if (e == hipSuccess) e = dalloc(&d_prog, prog.size(), st);
+    if (e == hipSuccess) e = dalloc(&d_res, n_triples, st);
+    if (e == hipSuccess) e = dalloc(&d_combo, n_triples, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_tm, templates, (uint64_t)n_templates * CN * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_tri, triples, 3 * n_triples * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_prog, prog.data(), prog.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = launch_overlap_template_residuals(d_tm, n_chans, n_samples_per_chan, n_pre, (uint32_t)S, d_tri, n_triples,
+                                              d_prog, (uint32_t)prog.size(), d_res, d_combo, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(best_residual, d_res, n_triples * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(best_combo, d_combo, n_triples * 4, hipMemcpyDeviceToHost, st);
+    dfree(d_tm, st);
+    dfree(d_tri, st);
+    dfree(d_prog, st);
+    dfree(d_res, st);
+    dfree(d_combo, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    if (e == hipSuccess) e = es;
+    HIP_TRY(e, "overlap template residuals");
     return FBP_OK;
 }
 
