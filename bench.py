@@ -142,6 +142,8 @@ def main():
     post = None
     if not args.no_post:
         post = post_pursuit_run(seg, V, rec['sort_info'], T)
+        post["overlap_templates"] = overlap_templates_run(sm['templates'], C, N, rec['sort_info'], seg,
+                                                          cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -219,6 +221,51 @@ def post_pursuit_run(seg, V, sort_info, T, repeats=3):
             "sampled_clips_equal_voltage": bool(sample_ok), "largest_unit_mean_equals_numpy": mean_ok,
             "note": "whole calls: get_clips includes the device-to-host copy of the clips, unit means the "
                     "host-to-device copy; the gather and mean kernels are in the rocprofv3 kernel table"}
+
+
+def overlap_templates_run(templates, C, N, sort_info, seg, cpu=True, repeats=3):
+    """find_overlap_templates (sort_cython.pyx:333-456) on the workload's
+    templates with the shift range full_binary_pursuit uses
+    (full_binary_pursuit.py:257-262: |chan_win| - 1 on each side) and the
+    found spike counts per unit.  The GPU call is timed whole; the oracle
+    (the reference's loops in numpy) is timed on the first test unit's
+    candidate pairs and scaled by the number of pairs."""
+    from spikesorting_fullpursuit_amd.overlap_templates import _shifted as _shift_f64
+    from spikesorting_fullpursuit_amd.overlap_templates import candidate_pairs, find_overlap_templates
+    from spikesorting_fullpursuit_amd.segment import time_window_to_samples
+    chan_win, _ = time_window_to_samples(sort_info['clip_width'], sort_info['sampling_rate'])
+    n_pre, n_post = abs(chan_win[0]) - 1, abs(chan_win[1]) - 1
+    tm = np.ascontiguousarray(templates, dtype=np.float32)
+    T = tm.shape[0]
+    _, lab = seg.spikes()
+    spikes = np.bincount(lab, minlength=T).astype(np.int64)
+    times = []
+    for _ in range(repeats):
+        t = time.perf_counter()
+        found, res = find_overlap_templates(tm, N, C, n_pre, n_post, spikes, return_residuals=True)
+        times.append(time.perf_counter() - t)
+    order = np.argsort(np.sum(tm ** 2, axis=1))[-1::-1]
+    pairs = [(u, candidate_pairs(tm, u, N, spikes)) for u in order]
+    n_triples = sum(len(p) for _, p in pairs)
+    S = n_pre + n_post + 1
+    out = {"n_templates": int(T), "shift_range": [-int(n_pre), int(n_post)], "candidate_triples": int(n_triples),
+           "residuals": int(n_triples * S * S), "gpu_ms": 1e3 * float(np.median(times)), "found": len(found),
+           "work": "per residual: C*N float64 squared differences summed in numpy's pairwise order"}
+    if cpu and n_triples:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        # the oracle's shift-pair loops (the reference's, in numpy) for the
+        # first candidate triple, scaled by the number of triples
+        u, (n1, n2) = next((u, p[0]) for u, p in pairs if p)
+        sh = [np.stack([_shift_f64(tm[n], s, C, N) for s in range(-n_pre, n_post + 1)]) for n in (n1, n2)]
+        t = time.perf_counter()
+        oracle._triple_scan(tm[u], sh[0], sh[1], np.zeros(C * N), np.inf, None, n1, n2)
+        dt = time.perf_counter() - t
+        out["cpu_oracle"] = {"sample": "one candidate triple (%d shift pairs)" % (S * S), "seconds": dt,
+                             "estimated_seconds_all": dt * n_triples, "cores": 1}
+    log("overlap templates: %d candidate triples x %d shift pairs, GPU %.2f ms, %d found"
+        % (n_triples, S * S, out["gpu_ms"], len(found)))
+    return out
 
 
 def precision_report(args):

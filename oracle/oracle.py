@@ -286,6 +286,18 @@ def unit_channels(clips, labels, n_channels, n_samples_per_chan, thresholds, sig
     return out
 
 
+def _triple_scan(test, shifts1, shifts2, acc, best_val, best, n1, n2):
+    """The shift-pair loops of sort_cython.pyx:428-450 for one pair."""
+    for i1 in range(shifts1.shape[0]):
+        for i2 in range(shifts2.shape[0]):
+            acc[:] = shifts1[i1, :]
+            acc += shifts2[i2, :]
+            r = np.sum((test - acc) ** 2)
+            if r < best_val:
+                best_val, best = r, ([n1, n2], np.copy(acc))
+    return best_val, best
+
+
 def find_overlap_templates(templates, n_samples_per_chan, n_chans, n_pre_inds, n_post_inds, n_template_spikes):
     """c_cython/sort_cython.pyx:333-456 (restated with numpy, loops as in the
     reference).  Returns (templates_to_check, smallest residual of each)."""
@@ -328,13 +340,7 @@ def find_overlap_templates(templates, n_samples_per_chan, n_chans, n_pre_inds, n
                     continue
                 if np.amax(templates[n1, hi]) + np.amax(templates[n2, hi]) < 0.75 * np.amax(test[hi]):
                     continue
-                for i1 in range(n_post_inds + 1 + n_pre_inds):
-                    for i2 in range(n_post_inds + 1 + n_pre_inds):
-                        acc[:] = all_shifts[n1][i1, :]
-                        acc += all_shifts[n2][i2, :]
-                        r = np.sum((test - acc) ** 2)
-                        if r < best_val:
-                            best_val, best = r, ([n1, n2], np.copy(acc))
+                best_val, best = _triple_scan(test, all_shifts[n1], all_shifts[n2], acc, best_val, best, n1, n2)
         if best is not None:
             to_check.append([test_unit, best[1], best[0]])
             residuals.append(best_val)
