@@ -667,13 +667,8 @@ __device__ __forceinline__ void fma_chain4(float (&acc)[4], const float *tp, uin
     }
 }
 
-// Float32 re-score of one (window, template chunk)'s queued entries when the
-// templates do not fit in LDS: one wave, one chain per lane, over the channels
-// the lane's template is nonzero on, with template rows and voltage read from
-// global memory (L1/L2) eight samples ahead of the chain.  Each chain is the
-// reference's: bias, then c ascending, j = 0..N-1, fused multiply-adds; a
-// channel where the template is zero adds exactly 0 (the running sum starts at
-// the nonzero bias and is never -0), so it is skipped.
+// acc += tp[j] * vp[j] for j = 0..n-1, in order, fused; operands read from
+// global memory eight ahead of their multiply-adds.
 __device__ __forceinline__ float fma_chain_contig(float acc, const float *__restrict__ tp,
                                                   const float *__restrict__ vp, uint32_t n)
 {
@@ -692,6 +687,14 @@ __device__ __forceinline__ float fma_chain_contig(float acc, const float *__rest
     return acc;
 }
 
+// Float32 re-score of one (window, template chunk)'s queued groups when the
+// templates do not fit in LDS: one wave; each batch of 64 groups is expanded
+// in LDS into its flagged positions, one chain per lane, over the channels
+// the lane's template is nonzero on, with template rows and voltage read from
+// global memory (L1/L2) eight samples ahead of the chain.  Each chain is the
+// reference's: bias, then c ascending, j = 0..N-1, fused multiply-adds; a
+// channel where the template is zero adds exactly 0 (the running sum starts at
+// the nonzero bias and is never -0), so it is skipped.
 __global__ __launch_bounds__(64) void k_rescore_sup(Dims d, const float *__restrict__ V,
                                                     const float *__restrict__ Tm, const float *__restrict__ bias,
                                                     const uint32_t *__restrict__ sup_lo,
@@ -699,23 +702,35 @@ __global__ __launch_bounds__(64) void k_rescore_sup(Dims d, const float *__restr
                                                     const uint64_t *__restrict__ rq, const uint2 *__restrict__ rdesc,
                                                     float *L)
 {
+    __shared__ uint64_t pos[4 * 64];   // (template << 32) | position of a batch's flagged positions
     const uint2 dsc = rdesc[blockIdx.x];
     const uint32_t cnt = dsc.y;
     if (cnt == 0) return;
     const uint32_t lane = threadIdx.x;
     for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
         const uint32_t e = e0 + lane;
-        if (e >= cnt) break;
-        const uint64_t ent = rq[dsc.x + e];
-        const uint32_t t = (uint32_t)(ent >> 32) & 0xffffu, m = (uint32_t)(ent >> 48) & 15u;
-        const uint64_t i0 = ent & 0xffffffffull;
-        for (int q = 0; q < 4; q++) {
-            if (!(m >> q & 1)) continue;
+        const uint64_t ent = e < cnt ? rq[dsc.x + e] : 0;
+        const uint32_t m = (uint32_t)(ent >> 48) & 15u;
+        uint32_t incl = __popc(m);
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if (lane >= (uint32_t)off) incl += y;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        uint32_t o = incl - __popc(m);
+        const uint64_t base = ent & 0x0000ffffffffffffull;   // template << 32 | first position
+        for (int q = 0; q < 4; q++)
+            if (m >> q & 1) pos[o++] = base + q;
+        __syncthreads();
+        for (uint32_t p = lane; p < total; p += 64) {
+            const uint32_t t = (uint32_t)(pos[p] >> 32);
+            const uint64_t i = pos[p] & 0xffffffffull;
             float acc = bias[t];
             for (uint32_t c = sup_lo[t]; c < sup_hi[t]; c++)
-                acc = fma_chain_contig(acc, Tm + ((uint64_t)t * d.C + c) * d.N, V + (uint64_t)c * d.K + i0 + q, d.N);
-            L[(uint64_t)t * d.K + i0 + q] = acc;
+                acc = fma_chain_contig(acc, Tm + ((uint64_t)t * d.C + c) * d.N, V + (uint64_t)c * d.K + i, d.N);
+            L[(uint64_t)t * d.K + i] = acc;
         }
+        __syncthreads();
     }
 }
 
@@ -1777,7 +1792,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     uint16_t *plist = (uint16_t *)(sb + max(SM * R + kOvCand, (TS + 1) / 2));   // e = t*S + s with positive
                                                         // likelihood, grouped by t
     uint16_t *pofs = plist + TS, *pcnt = pofs + T;      // [T] start and length of each template's group
-    uint8_t *spos = (uint8_t *)(pcnt + T);              // [T][S] bit1 in bounds, bit0 likelihood > 0
+    uint16_t *tlist = pcnt + T;                         // [T] templates with a positive likelihood, ascending
+    uint8_t *spos = (uint8_t *)(tlist + T);             // [T][S] bit1 in bounds, bit0 likelihood > 0
     float *sqmax = (float *)(((uintptr_t)(spos + TS) + 3) & ~(uintptr_t)3);   // [2][T] max over (positive) s of Q'
     __shared__ double r_ub[kOvWaves];
     for (uint32_t x = threadIdx.x; x < R; x += blockDim.x) szero[x] = 0.0f;
@@ -1787,7 +1803,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ uint64_t r_key[4], r_p[4];
     __shared__ double r_d[4][2];
     __shared__ double r_vv[4];
-    __shared__ uint32_t n_ppos, n_fpos, n_fneg, n_kp, n_kn, s_maxcnt;
+    __shared__ uint32_t n_ppos, n_fpos, n_fneg, n_kp, n_kn, s_maxcnt, n_tp;
     const Counters *ctr = ctr_rw;
     const uint32_t n_ov = ctr->n_ov;
     const uint32_t wg_bits = 31 - __builtin_clz(d.reduce_wg);
@@ -1957,6 +1973,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 }
             }
             __syncthreads();
+            if (threadIdx.x < kWave) {   // templates with a positive likelihood, ascending (one wave, by ballot)
+                const uint32_t l = threadIdx.x;
+                uint32_t nt = 0;
+                for (uint32_t t0 = 0; t0 < T; t0 += kWave) {
+                    const bool has = t0 + l < T && pcnt[t0 + l] > 0;
+                    const uint64_t m = __ballot(has);
+                    if (has) tlist[nt + __popcll(m & ((1ull << l) - 1))] = (uint16_t)(t0 + l);
+                    nt += __popcll(m);
+                }
+                if (l == 0) n_tp = nt;
+            }
+            __syncthreads();
             const float *ABl = AB + (uint64_t)lab * S * TS;
             const float *sQl = sQ + lab * S;
             const uint32_t nfp = n_fpos, nfn = n_fneg, maxcnt = s_maxcnt;
@@ -1972,28 +2000,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             // sh - E found there, lo_est, is at most lo_max.  Rows with U below
             // lo_est hold no combination that reaches lo_max, so neither the
             // bounds nor the candidates change when the traversal skips them.
-            const uint32_t nrows = (nfp + nfn) * T;
+            // Rows: every (positive main shift, template), then every (other
+            // main shift, template with a positive likelihood); rows of
+            // templates without one hold no combination.
+            const uint32_t ntp = n_tp, nfpr = nfp * T;
+            const uint32_t nrows = nfpr + nfn * ntp;
+            const FastDiv divTP(ntp ? ntp : 1u);
             const float *mABl = ABmin + (uint64_t)lab * S * T;   // [f][t]
             constexpr int RB = 4;   // rows per thread whose loads are in flight together
             auto row_bounds = [&](auto use) {
-                const uint32_t step = blockDim.x, stA = step / T, stT = step - stA * T;
-                uint32_t a = threadIdx.x / T, t = threadIdx.x - a * T;
-                for (uint32_t r0 = threadIdx.x; r0 < nrows; r0 += RB * step) {
+                for (uint32_t r0 = threadIdx.x; r0 < nrows; r0 += RB * blockDim.x) {
                     float z[RB];
                     uint32_t aq[RB], tq[RB];
 #pragma unroll
                     for (int q = 0; q < RB; q++) {
-                        const bool in = r0 + q * step < nrows;
-                        aq[q] = a;
+                        const uint32_t x = r0 + q * blockDim.x;
+                        const bool in = x < nrows;
+                        uint32_t a, t;
+                        if (x < nfpr) {
+                            a = divT.div(x);
+                            t = x - a * T;
+                        } else {
+                            const uint32_t y = x - nfpr, b = divTP.div(y);
+                            a = nfp + b;
+                            t = in ? tlist[y - b * ntp] : 0u;
+                        }
+                        aq[q] = in ? a : 0u;
                         tq[q] = t;
                         z[q] = in ? mABl[flist[a] * T + t] : 0.0f;
-                        t += stT;
-                        a += stA;
-                        if (t >= T) { t -= T; a++; }
                     }
 #pragma unroll
                     for (int q = 0; q < RB; q++) {
-                        if (r0 + q * step < nrows) {
+                        if (r0 + q * blockDim.x < nrows) {
                             const uint32_t f = flist[aq[q]];
                             const double x = sQl[f], y = sqmax[(aq[q] < nfp ? 0 : T) + tq[q]], zz = z[q];
                             const double E = sE[tq[q]];
@@ -2566,7 +2604,7 @@ static size_t overlap_screened_lds_mode(const Dims &d, bool)
     const uint64_t sbc = (uint64_t)d.sup_max * R + kOvCand;   // partner template + candidates, or kept rows
     const uint64_t krw = (ts + 1) / 2;
     return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + ((uint64_t)d.sup_max + 1) * R * 4 + ts * 4 +
-           (uint64_t)d.T * 4 + (uint64_t)d.S * 4 + (sbc > krw ? sbc : krw) * 4 + ts * 2 + (uint64_t)d.T * 4 +
+           (uint64_t)d.T * 4 + (uint64_t)d.S * 4 + (sbc > krw ? sbc : krw) * 4 + ts * 2 + (uint64_t)d.T * 6 +
            ts + 4 + (uint64_t)d.T * 8 + 16;
 }
 
