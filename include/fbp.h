@@ -196,6 +196,15 @@ int fbp_get_clips(int device, const float *d_voltage, uint64_t row_stride, uint3
 int fbp_unit_mean_clips(int device, const float *clips, uint64_t n, uint32_t width, const uint32_t *labels,
                         uint32_t n_units, float *means, uint64_t *counts);
 
+/* Squared distance of every clip to a template, as the duplicate removal
+ * after binary pursuit computes it (np.sum((clip - template) ** 2) on float32
+ * clips, analyze_spike_timing.py:129-130, with numpy's pairwise summation
+ * order: bit-exact).  clips: host float32 [n][width]; template_index: host
+ * uint32 [n] (< n_templates); templates: host float32 [n_templates][width];
+ * sse: host float32 [n]. */
+int fbp_clip_template_sse(int device, const float *clips, uint64_t n, uint32_t width, const uint32_t *template_index,
+                          const float *templates, uint32_t n_templates, float *sse);
+
 /* Overlap-template search residuals (find_overlap_templates,
  * c_cython/sort_cython.pyx:333-456, before binary pursuit at
  * full_binary_pursuit.py:257-262).  templates: host float32 [n_templates]

@@ -345,3 +345,47 @@ def find_overlap_templates(templates, n_samples_per_chan, n_chans, n_pre_inds, n
             to_check.append([test_unit, best[1], best[0]])
             residuals.append(best_val)
     return to_check, residuals
+
+
+def remove_binary_pursuit_duplicates(event_indices, clips, unit_template, binary_pursuit_bool, tol_inds):
+    """analyze_spike_timing.py:116-156 (restated)."""
+    keep = np.ones(event_indices.size, dtype=bool)
+    sse = np.zeros(2)
+    cur, nxt = 0, 1
+    while nxt < event_indices.size:
+        if event_indices[nxt] - event_indices[cur] <= tol_inds:
+            if binary_pursuit_bool[cur] and binary_pursuit_bool[nxt]:
+                sse[0] = np.sum((clips[cur, :] - unit_template) ** 2)
+                sse[1] = np.sum((clips[nxt, :] - unit_template) ** 2)
+                if sse[0] <= sse[1]:
+                    keep[nxt] = False
+                    nxt += 1
+                else:
+                    keep[cur] = False
+                    cur = nxt
+                    nxt += 1
+            elif binary_pursuit_bool[cur] and ~binary_pursuit_bool[nxt]:
+                nxt += 1
+            else:
+                cur = nxt
+                nxt += 1
+        else:
+            cur = nxt
+            nxt += 1
+    return keep
+
+
+def remove_segment_binary_pursuit_duplicates(sort_data, n_chans, n_segments, half_clip_inds):
+    """postprocessing.py:521-546 (restated) on sort_data[chan][seg]."""
+    for chan in range(n_chans):
+        for seg in range(n_segments):
+            d = sort_data[chan][seg]
+            if len(d[0]) == 0:
+                continue
+            keep = np.ones(d[0].shape[0], dtype=bool)
+            for lab in np.unique(d[1]):
+                sel = d[1] == lab
+                keep[sel] = remove_binary_pursuit_duplicates(d[0][sel], d[2][sel, :], np.mean(d[2][sel, :], axis=0),
+                                                             d[3][sel], 2 * half_clip_inds)
+            d[0], d[1], d[2], d[3] = d[0][keep], d[1][keep], d[2][keep, :], d[3][keep]
+    return sort_data

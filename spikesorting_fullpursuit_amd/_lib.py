@@ -35,6 +35,7 @@ EXPORTS = (
     "fbp_segment_num_spikes", "fbp_segment_get_spikes", "fbp_segment_get_clips",
     "fbp_segment_get_residual", "fbp_segment_get_likelihood", "fbp_segment_get_stats",
     "fbp_segment_voltage_device", "fbp_get_clips", "fbp_unit_mean_clips", "fbp_overlap_template_residuals",
+    "fbp_clip_template_sse",
 )
 
 
@@ -118,6 +119,8 @@ def _declare(lib):
                                   ctypes.c_int, f32p, u8p, u64p]
     lib.fbp_unit_mean_clips.argtypes = [ctypes.c_int, f32p, ctypes.c_uint64, ctypes.c_uint32, u32p,
                                         ctypes.c_uint32, f32p, u64p]
+    lib.fbp_clip_template_sse.argtypes = [ctypes.c_int, f32p, ctypes.c_uint64, ctypes.c_uint32, u32p, f32p,
+                                          ctypes.c_uint32, f32p]
     lib.fbp_overlap_template_residuals.argtypes = [ctypes.c_int, f32p, ctypes.c_uint32, ctypes.c_uint32,
                                                    ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32, u32p,
                                                    ctypes.c_uint64, P(ctypes.c_double), u32p]
@@ -341,3 +344,17 @@ def overlap_template_residuals(templates, n_chans, n_samples_per_chan, n_pre, n_
                                              res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _u32p(combo)),
           "fbp_overlap_template_residuals")
     return res, combo
+
+
+def clip_template_sse(clips, template_index, templates, device=0):
+    """fbp_clip_template_sse: float32 [n], np.sum((clips[k] - templates[template_index[k]]) ** 2)."""
+    lib = load()
+    c = np.ascontiguousarray(clips, dtype=np.float32)
+    tm = np.ascontiguousarray(templates, dtype=np.float32).reshape(-1, c.shape[1] if c.ndim == 2 else 0)
+    idx = np.ascontiguousarray(template_index, dtype=np.uint32)
+    if c.ndim != 2 or idx.size != c.shape[0]:
+        raise ValueError("clips must be [n][width] with one template index per clip")
+    out = np.zeros(c.shape[0], dtype=np.float32)
+    check(lib.fbp_clip_template_sse(device, _f32p(c), c.shape[0], c.shape[1], _u32p(idx), _f32p(tm), tm.shape[0],
+                                    _f32p(out)), "fbp_clip_template_sse")
+    return out

@@ -1,8 +1,11 @@
 // Clip gathers around the binary pursuit call: get_clips on the input voltage
-// (parallel/segment_parallel.py:440-509, called at full_binary_pursuit.py:357-359)
-// and the per-unit mean clip the caller reduces them to
-// (full_binary_pursuit.py:366-368).  Both are data movement (HBM-bound); the
-// mean keeps numpy's float32 order so it is bit-exact.
+// (parallel/segment_parallel.py:440-509, called at full_binary_pursuit.py:357-359),
+// the per-unit mean clip the caller reduces them to
+// (full_binary_pursuit.py:366-368), and the squared distance of every clip to
+// its unit's template that the duplicate removal after binary pursuit
+// compares (analyze_spike_timing.py:129-130).  All are data movement
+// (HBM-bound); the reductions keep numpy's float32 order so they are
+// bit-exact.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -63,7 +66,62 @@ __global__ __launch_bounds__(256) void k_unit_means(const float *__restrict__ cl
     means[g] = acc / (float)(e - b);
 }
 
+// sse[k] = np.sum((clips[k] - templates[tidx[k]]) ** 2) in float32 with
+// numpy's pairwise summation order, given as a program (leaf op start << 8 |
+// length, 0xffffffff adds the top two stack entries).  One thread per clip.
+__global__ __launch_bounds__(256) void k_clip_sse(const float *__restrict__ clips, uint64_t n, uint32_t width,
+                                                  const uint32_t *__restrict__ tidx,
+                                                  const float *__restrict__ templates,
+                                                  const uint32_t *__restrict__ prog, uint32_t n_prog,
+                                                  float *__restrict__ sse)
+{
+    const uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float *c = clips + k * width, *t = templates + (uint64_t)tidx[k] * width;
+    auto val = [&](uint32_t e) {
+        const float d = c[e] - t[e];
+        return d * d;
+    };
+    float stk[16];
+    int sp = 0;
+    for (uint32_t p = 0; p < n_prog; p++) {
+        const uint32_t op = prog[p];
+        if (op == 0xffffffffu) {
+            stk[sp - 2] = stk[sp - 2] + stk[sp - 1];
+            sp--;
+            continue;
+        }
+        const uint32_t start = op >> 8, len = op & 0xffu;
+        float res;
+        if (len < 8) {
+            res = 0.0f;
+            for (uint32_t i = 0; i < len; i++) res += val(start + i);
+        } else {
+            float r[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) r[q] = val(start + q);
+            uint32_t i = 8;
+            for (; i < len - len % 8; i += 8)
+#pragma unroll
+                for (int q = 0; q < 8; q++) r[q] += val(start + i + q);
+            res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+            for (; i < len; i++) res += val(start + i);
+        }
+        stk[sp++] = res;
+    }
+    sse[k] = stk[0];
+}
+
 }  // namespace
+
+hipError_t launch_clip_sse(const float *clips, uint64_t n, uint32_t width, const uint32_t *tidx,
+                           const float *templates, const uint32_t *prog, uint32_t n_prog, float *sse, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clip_sse, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, clips, n, width, tidx,
+                       templates, prog, n_prog, sse);
+    return hipGetLastError();
+}
 
 hipError_t launch_get_clips(const float *V, uint64_t row_stride, const int64_t *idx, uint64_t n_events,
                             const uint32_t *chan, uint32_t n_sel, int32_t w0, uint32_t W, float *clips,

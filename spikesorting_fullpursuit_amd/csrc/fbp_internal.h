@@ -131,6 +131,8 @@ hipError_t launch_clear_win_spike(const Dims &d, const DeviceState &s, uint32_t 
 hipError_t launch_get_clips(const float *V, uint64_t row_stride, const int64_t *idx, uint64_t n_events,
                             const uint32_t *chan, uint32_t n_sel, int32_t w0, uint32_t W, float *clips,
                             hipStream_t st);
+hipError_t launch_clip_sse(const float *clips, uint64_t n, uint32_t width, const uint32_t *tidx,
+                           const float *templates, const uint32_t *prog, uint32_t n_prog, float *sse, hipStream_t st);
 hipError_t launch_unit_means(const float *clips, uint32_t width, const uint32_t *rows, const uint64_t *off,
                              uint32_t n_units, float *means, hipStream_t st);
 hipError_t launch_overlap_template_residuals(const float *Tm, uint32_t C, uint32_t N, int32_t n_pre, uint32_t S,

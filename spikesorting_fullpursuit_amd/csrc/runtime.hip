@@ -780,6 +780,46 @@ int fbp_overlap_template_residuals(int device, const float *templates, uint32_t 
     return FBP_OK;
 }
 
+int fbp_clip_template_sse(int device, const float *clips, uint64_t n, uint32_t width, const uint32_t *template_index,
+                          const float *templates, uint32_t n_templates, float *sse)
+{
+    if (n && (!clips || !template_index || !templates || !sse)) return fail(FBP_EINVAL, "NULL argument");
+    if (width == 0 || width >= (1u << 24)) return fail(FBP_EINVAL, "clip width %u out of range", width);
+    for (uint64_t k = 0; k < n; k++)
+        if (template_index[k] >= n_templates)
+            return fail(FBP_EINVAL, "template index %u out of range (%u templates)", template_index[k], n_templates);
+    if (n == 0) return FBP_OK;
+    std::vector<uint32_t> prog;
+    if (numpy_sum_program(width, prog) > 16) return fail(FBP_EINVAL, "summation tree too deep");
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+    float *d_clips = nullptr, *d_tm = nullptr, *d_sse = nullptr;
+    uint32_t *d_idx = nullptr, *d_prog = nullptr;
+    hipError_t e = dalloc(&d_clips, n * width, st);
+    if (e == hipSuccess) e = dalloc(&d_tm, (uint64_t)n_templates * width, st);
+    if (e == hipSuccess) e = dalloc(&d_idx, n, st);
+    if (e == hipSuccess) e = dalloc(&d_prog, prog.size(), st);
+    if (e == hipSuccess) e = dalloc(&d_sse, n, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_clips, clips, n * width * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_tm, templates, (uint64_t)n_templates * width * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_idx, template_index, n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_prog, prog.data(), prog.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_clip_sse(d_clips, n, width, d_idx, d_tm, d_prog, (uint32_t)prog.size(), d_sse, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(sse, d_sse, n * 4, hipMemcpyDeviceToHost, st);
+    dfree(d_clips, st);
+    dfree(d_tm, st);
+    dfree(d_idx, st);
+    dfree(d_prog, st);
+    dfree(d_sse, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    if (e == hipSuccess) e = es;
+    HIP_TRY(e, "clip template sse");
+    return FBP_OK;
+}
+
 int fbp_unit_mean_clips(int device, const float *clips, uint64_t n, uint32_t width, const uint32_t *labels,
                         uint32_t n_units, float *means, uint64_t *counts)
 {
