@@ -205,6 +205,15 @@ def post_pursuit_run(seg, V, sort_info, T, repeats=3):
     unit = int(np.argmax(counts))
     mean_ok = bool(np.array_equal(means[unit].view(np.uint32),
                                   np.mean(clips[lab == unit], axis=0).view(np.uint32)))
+    # duplicate removal after binary pursuit (postprocessing.py:521-546) on
+    # the found spikes as one channel and segment, events ascending
+    from spikesorting_fullpursuit_amd.postprocessing import remove_segment_binary_pursuit_duplicates
+    order = np.argsort(ev, kind="stable")
+    t = time.perf_counter()
+    sd = [[[ev[order], lab[order].astype(np.int64), clips[order], np.ones(ev.size, dtype=bool), 0]]]
+    remove_segment_binary_pursuit_duplicates(sd, 1, 1, -window[0])
+    t_dup = time.perf_counter() - t
+    n_dup = ev.size - sd[0][0][0].size
     t = time.perf_counter()
     host = np.empty_like(clips)
     for r in range(ev.size):
@@ -218,6 +227,7 @@ def post_pursuit_run(seg, V, sort_info, T, repeats=3):
             "get_clips_ms": 1e3 * tc, "get_clips_GBps": nbytes / tc / 1e9,
             "unit_means_ms": 1e3 * tm, "n_units": int(np.count_nonzero(counts)),
             "host_numpy_gather_ms": 1e3 * t_host, "all_valid": bool(valid.all()),
+            "duplicate_removal_ms": 1e3 * t_dup, "duplicates_removed": int(n_dup),
             "sampled_clips_equal_voltage": bool(sample_ok), "largest_unit_mean_equals_numpy": mean_ok,
             "note": "whole calls: get_clips includes the device-to-host copy of the clips, unit means the "
                     "host-to-device copy; the gather and mean kernels are in the rocprofv3 kernel table"}
