@@ -187,6 +187,12 @@ int fbp_get_clips(int device, const float *d_voltage, uint64_t row_stride, uint3
                   const int64_t *event_indices, uint64_t n_events, const uint32_t *channels, uint32_t n_sel,
                   int32_t w0, int32_t w1, int check_valid, float *clips, uint8_t *valid, uint64_t *n_valid);
 
+/* fbp_get_clips on a host voltage float32 [n_chans][row_stride]: the
+ * selected channels' rows are uploaded for the gather. */
+int fbp_get_clips_host(int device, const float *voltage, uint64_t row_stride, uint32_t n_chans, uint64_t n_samples,
+                       const int64_t *event_indices, uint64_t n_events, const uint32_t *channels, uint32_t n_sel,
+                       int32_t w0, int32_t w1, int check_valid, float *clips, uint8_t *valid, uint64_t *n_valid);
+
 /* Mean clip of every unit: np.mean(clips[labels == u], axis=0) on float32
  * clips (full_binary_pursuit.py:366-368), bit for bit: rows added in input
  * order in float32, the sum divided by the row count.  clips: host float32

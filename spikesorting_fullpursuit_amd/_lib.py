@@ -35,7 +35,7 @@ EXPORTS = (
     "fbp_segment_num_spikes", "fbp_segment_get_spikes", "fbp_segment_get_clips",
     "fbp_segment_get_residual", "fbp_segment_get_likelihood", "fbp_segment_get_stats",
     "fbp_segment_voltage_device", "fbp_get_clips", "fbp_unit_mean_clips", "fbp_overlap_template_residuals",
-    "fbp_clip_template_sse",
+    "fbp_clip_template_sse", "fbp_get_clips_host",
 )
 
 
@@ -117,6 +117,9 @@ def _declare(lib):
     lib.fbp_get_clips.argtypes = [ctypes.c_int, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, i64p,
                                   ctypes.c_uint64, u32p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32,
                                   ctypes.c_int, f32p, u8p, u64p]
+    lib.fbp_get_clips_host.argtypes = [ctypes.c_int, f32p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
+                                       i64p, ctypes.c_uint64, u32p, ctypes.c_uint32, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.c_int, f32p, u8p, u64p]
     lib.fbp_unit_mean_clips.argtypes = [ctypes.c_int, f32p, ctypes.c_uint64, ctypes.c_uint32, u32p,
                                         ctypes.c_uint32, f32p, u64p]
     lib.fbp_clip_template_sse.argtypes = [ctypes.c_int, f32p, ctypes.c_uint64, ctypes.c_uint32, u32p, f32p,
@@ -313,6 +316,23 @@ def get_clips_device(d_voltage, row_stride, n_chans, n_samples, event_indices, c
                             w0, w1, 1 if check_valid else 0, _f32p(out),
                             valid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(n_valid)),
           "fbp_get_clips")
+    return out[:n_valid.value], valid.astype(bool)
+
+
+def get_clips_host(voltage, n_samples, event_indices, channels, w0, w1, check_valid=True, device=0):
+    """fbp_get_clips_host on a host float32 voltage [C][row length]."""
+    lib = load()
+    v = np.ascontiguousarray(voltage, dtype=np.float32)
+    idx = np.ascontiguousarray(event_indices, dtype=np.int64)
+    ch = np.ascontiguousarray(channels, dtype=np.uint32)
+    valid = np.zeros(idx.size, dtype=np.uint8)
+    n_valid = ctypes.c_uint64(0)
+    out = np.empty((idx.size, ch.size * (w1 - w0)), dtype=np.float32)
+    check(lib.fbp_get_clips_host(device, _f32p(v), v.shape[1], v.shape[0], n_samples,
+                                 idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), idx.size, _u32p(ch), ch.size,
+                                 w0, w1, 1 if check_valid else 0, _f32p(out),
+                                 valid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(n_valid)),
+          "fbp_get_clips_host")
     return out[:n_valid.value], valid.astype(bool)
 
 

@@ -675,9 +675,11 @@ int fbp_segment_voltage_device(const fbp_segment *g, const float **d_voltage, ui
     return FBP_OK;
 }
 
-int fbp_get_clips(int device, const float *d_voltage, uint64_t row_stride, uint32_t n_chans, uint64_t n_samples,
-                  const int64_t *event_indices, uint64_t n_events, const uint32_t *channels, uint32_t n_sel,
-                  int32_t w0, int32_t w1, int check_valid, float *clips, uint8_t *valid, uint64_t *n_valid)
+// get_clips from device (d_voltage) or host (h_voltage) float32 rows.
+static int get_clips_impl(int device, const float *d_voltage, const float *h_voltage, uint64_t row_stride,
+                          uint32_t n_chans, uint64_t n_samples, const int64_t *event_indices, uint64_t n_events,
+                          const uint32_t *channels, uint32_t n_sel, int32_t w0, int32_t w1, int check_valid,
+                          float *clips, uint8_t *valid, uint64_t *n_valid)
 {
     if (!n_valid || (n_events && (!event_indices || !valid)) || (n_sel && !channels))
         return fail(FBP_EINVAL, "NULL argument");
@@ -706,28 +708,59 @@ int fbp_get_clips(int device, const float *d_voltage, uint64_t row_stride, uint3
     const uint64_t width = (uint64_t)n_sel * W;
     if (n == 0 || width == 0) return FBP_OK;
     if (!clips) return fail(FBP_EINVAL, "NULL argument");
-    if (!d_voltage) return fail(FBP_EINVAL, "NULL voltage");
+    if (!d_voltage && !h_voltage) return fail(FBP_EINVAL, "NULL voltage");
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
     hipStream_t st = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
     int64_t *d_idx = nullptr;
     uint32_t *d_chan = nullptr;
-    float *d_clips = nullptr;
-    hipError_t e = dalloc(&d_idx, n, st);
+    float *d_clips = nullptr, *d_rows = nullptr;
+    std::vector<uint32_t> chan(channels, channels + n_sel);
+    hipError_t e = hipSuccess;
+    if (h_voltage) {   // upload the selected channels' rows; they become rows 0..n_sel-1
+        e = dalloc(&d_rows, (uint64_t)n_sel * n_samples, st);
+        for (uint32_t j = 0; j < n_sel && e == hipSuccess; j++) {
+            e = hipMemcpyAsync(d_rows + (uint64_t)j * n_samples, h_voltage + (uint64_t)channels[j] * row_stride,
+                               n_samples * 4, hipMemcpyHostToDevice, st);
+            chan[j] = j;
+        }
+        d_voltage = d_rows;
+        row_stride = n_samples;
+    }
+    if (e == hipSuccess) e = dalloc(&d_idx, n, st);
     if (e == hipSuccess) e = dalloc(&d_chan, n_sel, st);
     if (e == hipSuccess) e = dalloc(&d_clips, n * width, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_idx, event_indices + lo, n * 8, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_chan, channels, (uint64_t)n_sel * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_chan, chan.data(), (uint64_t)n_sel * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_get_clips(d_voltage, row_stride, d_idx, n, d_chan, n_sel, w0, W, d_clips, st);
     if (e == hipSuccess) e = hipMemcpyAsync(clips, d_clips, n * width * 4, hipMemcpyDeviceToHost, st);
     dfree(d_idx, st);
     dfree(d_chan, st);
     dfree(d_clips, st);
+    dfree(d_rows, st);
     const hipError_t es = hipStreamSynchronize(st);
     (void)hipStreamDestroy(st);
     if (e == hipSuccess) e = es;
     HIP_TRY(e, "get_clips");
     return FBP_OK;
+}
+
+int fbp_get_clips(int device, const float *d_voltage, uint64_t row_stride, uint32_t n_chans, uint64_t n_samples,
+                  const int64_t *event_indices, uint64_t n_events, const uint32_t *channels, uint32_t n_sel,
+                  int32_t w0, int32_t w1, int check_valid, float *clips, uint8_t *valid, uint64_t *n_valid)
+{
+    if (n_events && !d_voltage) return fail(FBP_EINVAL, "NULL voltage");
+    return get_clips_impl(device, d_voltage, nullptr, row_stride, n_chans, n_samples, event_indices, n_events,
+                          channels, n_sel, w0, w1, check_valid, clips, valid, n_valid);
+}
+
+int fbp_get_clips_host(int device, const float *voltage, uint64_t row_stride, uint32_t n_chans, uint64_t n_samples,
+                       const int64_t *event_indices, uint64_t n_events, const uint32_t *channels, uint32_t n_sel,
+                       int32_t w0, int32_t w1, int check_valid, float *clips, uint8_t *valid, uint64_t *n_valid)
+{
+    if (n_events && !voltage) return fail(FBP_EINVAL, "NULL voltage");
+    return get_clips_impl(device, nullptr, voltage, row_stride, n_chans, n_samples, event_indices, n_events,
+                          channels, n_sel, w0, w1, check_valid, clips, valid, n_valid);
 }
 
 int fbp_overlap_template_residuals(int device, const float *templates, uint32_t n_templates, uint32_t n_chans,

@@ -36,8 +36,8 @@ def get_clips(probe_dict, voltage, neighbors, event_indices, clip_width, use_mem
     (clips, valid mask), (None, all-False mask) when no event is valid, and a
     (0, width) array for no events.
 
-    `voltage` is a [C][K] numpy array (uploaded once) or a Segment, whose
-    resident input voltage is used without a copy.  The gather runs on the
+    `voltage` is a [C][K] numpy array (the selected channels are uploaded)
+    or a Segment, whose resident input voltage is used without a copy.  The gather runs on the
     device (fbp_get_clips); float32 and 8/16-bit integer voltages are exact,
     other dtypes are refused.  use_memmap (clips written to a file) is not
     supported.
@@ -54,8 +54,12 @@ def get_clips(probe_dict, voltage, neighbors, event_indices, clip_width, use_mem
     if len(event_indices) == 0:
         return np.zeros((0, width), dtype=probe_dict['v_dtype']), np.ones(0, dtype="bool")
     if isinstance(voltage, _lib.Segment):
+        if np.any(neighbors < 0) or np.any(neighbors >= voltage.C):
+            raise IndexError("neighbors index a channel outside the voltage")
         ptr, stride = voltage.voltage_device()
-        n_chans, keep, device = voltage.C, None, voltage.device
+        clips, valid = _lib.get_clips_device(ptr, stride, voltage.C, int(probe_dict['n_samples']),
+                                             event_indices.astype(np.int64), neighbors.astype(np.uint32),
+                                             window[0], window[1], check_valid=check_valid, device=voltage.device)
     else:
         voltage = np.asarray(voltage)
         if voltage.ndim != 2:
@@ -63,15 +67,11 @@ def get_clips(probe_dict, voltage, neighbors, event_indices, clip_width, use_mem
         if voltage.dtype.type not in _EXACT_IN_F32:
             raise TypeError("get_clips on the device takes float32 or 8/16-bit integer voltage, not %s"
                             % voltage.dtype)
-        import torch
-        keep = torch.from_numpy(np.ascontiguousarray(voltage, dtype=np.float32)).to("cuda:%d" % device)
-        ptr, stride, n_chans = keep.data_ptr(), voltage.shape[1], voltage.shape[0]
-    if np.any(neighbors < 0) or np.any(neighbors >= n_chans):
-        raise IndexError("neighbors index a channel outside the voltage")
-    clips, valid = _lib.get_clips_device(ptr, stride, n_chans, int(probe_dict['n_samples']),
-                                         event_indices.astype(np.int64), neighbors.astype(np.uint32),
-                                         window[0], window[1], check_valid=check_valid, device=device)
-    del keep
+        if np.any(neighbors < 0) or np.any(neighbors >= voltage.shape[0]):
+            raise IndexError("neighbors index a channel outside the voltage")
+        clips, valid = _lib.get_clips_host(voltage, int(probe_dict['n_samples']), event_indices.astype(np.int64),
+                                           neighbors.astype(np.uint32), window[0], window[1],
+                                           check_valid=check_valid, device=device)
     if check_valid:
         # the reference returns None when a trimming scan runs off the end:
         # every event starts before the data, or every event ends after it
