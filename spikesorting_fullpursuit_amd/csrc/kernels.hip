@@ -1700,23 +1700,52 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     float *sQp = (float *)(svl + C * LPH);              // [KS-1][T][S] channel-slice partials
     float *sQ = sQp + (qp.KS - 1) * TS;                 // [T][S]
     const uint32_t kend = min(ctr->n_ov, kbase + d.ov_batch);
+    // The zero padding of the staged rows is written once; a window stages
+    // only its C*N samples.  The next window's header is loaded in three
+    // dependent steps during the current one (no block writes what another
+    // block reads).
+    for (uint32_t e = threadIdx.x; e < C * LPH; e += blockDim.x) {
+        svh[e] = 0;
+        svl[e] = 0;
+    }
+    const uint32_t CN = C * N;
+    const FastDiv divN(N);
+    uint32_t nid = 0, nlab = 0, nbidx = 0;
+    float nLb = 0.0f, nthl = 0.0f;
+    int nstep = 0;
+    uint32_t kn = kbase + blockIdx.x;
+    auto head_step = [&](int upto) {
+        if (kn >= kend) return;
+        if (nstep < 1 && upto >= 1) { nid = ov_list[kn]; nstep = 1; }
+        if (nstep < 2 && upto >= 2) { nlab = best_lab[nid]; nbidx = best_idx[nid]; nstep = 2; }
+        if (nstep < 3 && upto >= 3) { nLb = L[(uint64_t)nlab * d.K + nbidx]; nthl = thr[nlab]; nstep = 3; }
+    };
+    auto put_v = [&](uint32_t e, float x) {
+        const uint32_t c = divN.div(e), y = c * LPH + M + (e - c * N);
+        const uint32_t h = to_bf16(x);
+        svh[y] = (uint16_t)h;
+        svl[y] = to_bf16(x - __uint_as_float(h << 16));
+    };
+    head_step(3);
     for (uint32_t k = kbase + blockIdx.x; k < kend; k += gridDim.x) {
-        const uint32_t id = ov_list[k];
-        const uint32_t lab = best_lab[id], bidx = best_idx[id];
-        const float Lb = L[(uint64_t)lab * d.K + bidx];
-        if ((Lb < thr[lab]) || ((uint64_t)bidx >= (uint64_t)(d.K - d.N))) continue;   // bp.cl:518-527
+        const uint32_t bidx = nbidx;
+        const bool skip = (nLb < nthl) || ((uint64_t)bidx >= (uint64_t)(d.K - d.N));   // bp.cl:518-527
+        kn = k + gridDim.x;
+        nstep = 0;
+        head_step(1);
+        if (skip) {
+            head_step(3);
+            continue;
+        }
         __syncthreads();   // the previous window's LDS reads are done
-        batched_stage<8>(C * LPH, [&](uint32_t e) {
-            const uint32_t c = divLPH.div(e), y = e - c * LPH;
-            return (y >= M && y - M < N) ? V[(uint64_t)c * d.K + bidx + (y - M)] : 0.0f;
-        }, [&](uint32_t e, float x) {
-            const uint32_t h = to_bf16(x);
-            svh[e] = (uint16_t)h;
-            svl[e] = to_bf16(x - __uint_as_float(h << 16));
-        });
+        batched_stage<8>(CN, [&](uint32_t e) { const uint32_t c = divN.div(e);
+                                               return V[(uint64_t)c * d.K + bidx + (e - c * N)]; },
+                         put_v);
+        head_step(2);
         __syncthreads();
         q_table_split(d, qp, TmQ, svh, svl, sQ, sQp, tord, tile_lo, tile_hi);
         __syncthreads();
+        head_step(3);
         float *dst = ovQ + (uint64_t)(k - kbase) * TS;
         batched_stage<8>(TS, [&](uint32_t e) { return B2[e]; }, [&](uint32_t e, float b2) {
             float q = sQ[e];
