@@ -2066,15 +2066,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                             const double E = sE[tq[q]];
                             const double u = y == -INFINITY ? -(double)INFINITY   // no combination in the row
                                              : x + y - zz + E + 0x1p-50 * (fabs(x) + fabs(y) + fabs(zz) + E);
-                            use(aq[q] * T + tq[q], aq[q], tq[q], u);
+                            use(aq[q] * T + tq[q], aq[q], tq[q], u, q);
                         }
                     }
                 }
             };
+            // With at most RB rows per thread, the second use of the bounds
+            // reads them from registers.
+            const bool one_batch = nrows <= RB * blockDim.x;
+            double uc[RB];
+            uint32_t rc[RB];
             double ub = -INFINITY;
             uint32_t ur = 0;
-            row_bounds([&](uint32_t r, uint32_t, uint32_t, double u) {
+            row_bounds([&](uint32_t r, uint32_t, uint32_t, double u, int q) {
                 if (u > ub) { ub = u; ur = r; }
+                uc[q] = u;
+                rc[q] = r;
             });
             for (int off = 32; off > 0; off >>= 1) {
                 const double ou = __shfl_xor(ub, off);
@@ -2097,11 +2104,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             lo_est = fmax(fmax(r_ub[0], r_ub[1]), fmax(r_ub[2], r_ub[3]));
             if (debug & 16u) lo_est = -INFINITY;
             // kept rows: positive main shifts from the front, the others from the back
-            row_bounds([&](uint32_t r, uint32_t a, uint32_t t, double u) {
+            auto keep_row = [&](uint32_t r, uint32_t a, uint32_t t, double u, int) {
                 if (u < lo_est) return;
                 if (a < nfp) krow[atomicAdd(&n_kp, 1u)] = (uint16_t)r;
                 else if (pcnt[t]) krow[TS - 1 - atomicAdd(&n_kn, 1u)] = (uint16_t)r;
-            });
+            };
+            if (one_batch) {
+#pragma unroll
+                for (int q = 0; q < RB; q++)
+                    if (threadIdx.x + q * blockDim.x < nrows) {
+                        const uint32_t a = divT.div(rc[q]);
+                        keep_row(rc[q], a, rc[q] - a * T, uc[q], q);
+                    }
+            } else {
+                row_bounds(keep_row);
+            }
             __syncthreads();
             const uint32_t nkp = n_kp, nkn = n_kn;
             // pass A: largest lower and upper bounds of the score.  Each thread
