@@ -25,6 +25,13 @@ struct Dims {
     uint32_t sup_max;  // widest channel support of a template (C until templates are set)
 };
 
+// The re-score queue is split into kQueueShards equal shards; a correlation
+// block reserves entries in shard blockIdx % kQueueShards, whose counters
+// (entries, positions) sit kShardStride words apart, a cache line each, so
+// reservations do not serialise on one address.
+constexpr uint32_t kQueueShards = 64;
+constexpr uint32_t kShardStride = 32;
+
 // Device counters, read back by the host once per pass.
 struct Counters {
     uint32_t n_ov;       // overlap windows of the pass
@@ -85,6 +92,7 @@ struct DeviceState {
     float *tscale;       // [T]          fp8 screen: per-template power-of-two scale (1 otherwise)
     void *TmB;           // packed bf16/fp16/fp8 template fragments for the matrix cores
     uint64_t *rq;        // re-score queue: (template << 32) | position
+    uint32_t *rq_shard;  // per-shard reservation counters of the queue (entries, positions)
     void *rdesc;         // [W * chunks] (queue start, count) per (window, template chunk)
 };
 

@@ -188,18 +188,18 @@ __device__ __forceinline__ uint16_t to_f16(float x)
 }
 
 // Element traits of the three screens: bf16 and fp16 use
-// v_mfma_f32_16x16x32{bf16,f16} (k = 32, 8 two-byte elements per lane: two
-// aligned 8-byte LDS reads from four shifted voltage copies), fp8 e4m3 uses
-// v_mfma_f32_16x16x32_fp8_fp8 (k = 32, 8 one-byte elements per lane: one
-// aligned 8-byte read from eight copies).  Lane l holds row (A) or column (B)
-// l % 16 at k = 8 * (l / 16) + 0..7.
+// v_mfma_f32_16x16x32{bf16,f16} (k = 32, 8 two-byte elements per lane: four
+// aligned 4-byte LDS words, read in pairs, from two shifted voltage copies),
+// fp8 e4m3 uses v_mfma_f32_16x16x32_fp8_fp8 (k = 32, 8 one-byte elements per
+// lane: two aligned 4-byte words from four copies).  Lane l holds row (A) or
+// column (B) l % 16 at k = 8 * (l / 16) + 0..7.
 template <int PREC>
 struct NarrowTraits {
     static constexpr uint32_t kBytes = PREC == 3 ? 1 : 2;   // bytes per element
     static constexpr uint32_t kK = 32;                        // k per MFMA
     static constexpr uint32_t kPer = 8;                       // elements per lane fragment
     static constexpr uint32_t kWords = kPer * kBytes / 8;     // 8-byte words per fragment
-    static constexpr uint32_t kCopies = 8 / kBytes;           // shifted voltage copies in LDS
+    static constexpr uint32_t kCopies = 4 / kBytes;           // shifted voltage copies in LDS
 };
 
 template <int PREC>
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
                                                   const uint32_t *__restrict__ tile_lo,
                                                   const uint32_t *__restrict__ tile_hi,
                                                   const uint32_t *__restrict__ sup_lo,
-                                                  const uint32_t *__restrict__ sup_hi)
+                                                  const uint32_t *__restrict__ sup_hi, uint32_t *rq_shard)
 {
     using Tr = NarrowTraits<PREC>;
     constexpr uint32_t KS = (RT * 16 + Tr::kK - 1) / Tr::kK;   // k-steps per channel (N padded to RT*16)
@@ -494,11 +494,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
                     for (int rt = 0; rt < RT; rt++) {
                         const uint32_t o = rt * 16 + ks * Tr::kK + Tr::kPer * (lane >> 4) + (lane & 15);
                         const uint32_t r = o % Tr::kCopies;
-                        const uint64_t *src =
-                            (const uint64_t *)(vb + (((uint64_t)r * cb + c) * LA + (o - r)) * Tr::kBytes);
+                        const uint32_t *src =
+                            (const uint32_t *)(vb + (((uint64_t)r * cb + c) * LA + (o - r)) * Tr::kBytes);
                         Frag<PREC> a;
 #pragma unroll
-                        for (uint32_t w = 0; w < Tr::kWords; w++) a.w[w] = src[w];
+                        for (uint32_t w = 0; w < Tr::kWords; w++)
+                            a.w[w] = (uint64_t)src[2 * w] | ((uint64_t)src[2 * w + 1] << 32);
 #pragma unroll
                         for (int ct = 0; ct < CTB; ct++)
                             if (c0 + c >= tlo[ct] && c0 + c < thi[ct])
@@ -575,18 +576,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
         const uint32_t tot = __shfl(incl, 63);
         uint32_t qbase = 0;
         if (tot) {
-            if (lane == 63) {
-                qbase = atomicAdd(&ctr->n_rescore, tot);
-                atomicAdd(&ctr->n_rescore_pos, npos);
+            if (lane == 63) {   // one counter pair per shard, each on a cache line of its own
+                uint32_t *sh = rq_shard + (blockIdx.x % kQueueShards) * kShardStride;
+                qbase = atomicAdd(sh, tot);
+                atomicAdd(sh + 1, npos);
             }
             qbase = __shfl(qbase, 63);
         }
+        // qbase is within the block's shard [shard * cap, (shard + 1) * cap)
+        const uint32_t cap = rq_cap / kQueueShards, sbase = (blockIdx.x % kQueueShards) * cap;
         uint32_t pos = qbase + incl - nflag;
         // this (window, template chunk)'s slice of the queue, for the re-score kernels
         if (lane == 0) {
             const uint32_t nchunk = (CT + CTB - 1) / CTB;
-            const uint32_t kept = qbase >= rq_cap ? 0u : min(tot, rq_cap - qbase);
-            rdesc[(uint64_t)blockIdx.x * nchunk + ct0 / CTB] = make_uint2(qbase, kept);
+            const uint32_t kept = qbase >= cap ? 0u : min(tot, cap - qbase);
+            rdesc[(uint64_t)blockIdx.x * nchunk + ct0 / CTB] = make_uint2(sbase + qbase, kept);
         }
         if (!nflag) continue;
 #pragma unroll
@@ -597,9 +601,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
                 if (!m) continue;
                 const uint32_t t = cb_t[ct];
                 const uint64_t i0 = base + rt * 16 + 4 * (lane >> 4);
-                if (pos < rq_cap) {
-                    rq[pos] = ((uint64_t)m << 48) | ((uint64_t)t << 32) | i0;
-                } else {   // queue full
+                if (pos < cap) {
+                    rq[sbase + pos] = ((uint64_t)m << 48) | ((uint64_t)t << 32) | i0;
+                } else {   // shard full
                     for (int q = 0; q < 4; q++)
                         if (m >> q & 1)
                             L[(uint64_t)t * d.K + i0 + q] = exact_likelihood(d, V, Tm, bias, sup_lo[t], sup_hi[t], t, i0 + q);
@@ -753,8 +757,8 @@ __device__ __forceinline__ void wave_sync()
 
 __global__ __launch_bounds__(1024) void k_rescore_res(Dims d, const float *__restrict__ V,
                                                       const float *__restrict__ TmT, const float *__restrict__ bias,
-                                                      const uint64_t *__restrict__ rq, const Counters *__restrict__ ctr,
-                                                      uint32_t rq_cap, float *L)
+                                                      const uint64_t *__restrict__ rq,
+                                                      const uint32_t *__restrict__ rq_shard, uint32_t rq_cap, float *L)
 {
     extern __shared__ float lds[];   // [C][N][Tp] templates, then [waves][kRescoreStage] voltage rows
     const uint32_t N = d.N, Tp = d.Tp, C = d.C;
@@ -766,14 +770,31 @@ __global__ __launch_bounds__(1024) void k_rescore_res(Dims d, const float *__res
     __syncthreads();
     const uint32_t lane = lane_of(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     float *vb = lds + ntm + wave * kRescoreStage;
-    const uint32_t n = min(ctr->n_rescore, rq_cap);
+    // the shards' kept entries, concatenated: shard j holds virtual entries
+    // [spre[j], spre[j + 1]) at rq[j * cap + (v - spre[j])]
+    __shared__ uint32_t spre[kQueueShards + 1];
+    const uint32_t cap = rq_cap / kQueueShards;
+    if (threadIdx.x < kWave) {
+        uint32_t v = threadIdx.x < kQueueShards ? min(rq_shard[threadIdx.x * kShardStride], cap) : 0u;
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t y = __shfl_up(v, off);
+            if (lane >= (uint32_t)off) v += y;
+        }
+        if (threadIdx.x < kQueueShards) spre[threadIdx.x + 1] = v;
+        if (threadIdx.x == 0) spre[0] = 0;
+    }
+    __syncthreads();
+    const uint32_t n = spre[kQueueShards];
     const uint32_t gw = blockIdx.x * nw + wave, nwave = gridDim.x * nw;
     const uint32_t per = (n + nwave - 1) / nwave;
     const uint32_t e_end = min(n, (gw + 1) * per);
     for (uint32_t b0 = gw * per; b0 < e_end;) {
         const uint32_t e = b0 + lane;
         const bool in = e < e_end;
-        const uint64_t ent = in ? rq[e] : 0;
+        uint32_t sj = 0;   // shard of virtual entry e: the last j with spre[j] <= e
+        for (uint32_t step = kQueueShards / 2; step > 0; step >>= 1)
+            if (spre[sj + step] <= e) sj += step;
+        const uint64_t ent = in ? rq[(uint64_t)sj * cap + (e - spre[sj])] : 0;
         const uint32_t t = (uint32_t)(ent >> 32) & 0xffffu, m = (uint32_t)(ent >> 48) & 15u;
         const uint32_t i0 = (uint32_t)ent;
         const uint32_t wid = i0 / N;
@@ -838,11 +859,34 @@ static size_t rescore_res_lds(const Dims &d)
     return ((uint64_t)d.C * d.N * d.Tp + kRescoreResWaves * kRescoreStage) * sizeof(float);
 }
 
-__global__ void k_zero_rescore(Counters *ctr)
+__global__ void k_zero_rescore(Counters *ctr, uint32_t *rq_shard)
 {
     if (threadIdx.x == 0) {
         ctr->n_rescore = 0;
         ctr->n_rescore_pos = 0;
+    }
+    if (threadIdx.x < kQueueShards) {
+        rq_shard[threadIdx.x * kShardStride] = 0;
+        rq_shard[threadIdx.x * kShardStride + 1] = 0;
+    }
+}
+
+// Totals of the shards (queued entries kept, positions flagged) for the host.
+__global__ void k_sum_rescore(Counters *ctr, const uint32_t *rq_shard, uint32_t rq_cap)
+{
+    const uint32_t cap = rq_cap / kQueueShards;
+    uint32_t e = 0, p = 0;
+    if (threadIdx.x < kQueueShards) {
+        e = min(rq_shard[threadIdx.x * kShardStride], cap);
+        p = rq_shard[threadIdx.x * kShardStride + 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        e += __shfl_xor(e, off);
+        p += __shfl_xor(p, off);
+    }
+    if (threadIdx.x == 0) {
+        ctr->n_rescore = e;
+        ctr->n_rescore_pos = p;
     }
 }
 
@@ -1876,6 +1920,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
     };
     head_step(3);
+    unsigned long long acc_screened = 0, acc_scored = 0;   // combinations screened / scored exactly
     for (uint32_t k = kbase + blockIdx.x; k < kend; k += gridDim.x) {
         id = nid; lab = nlab; bidx = nbidx; bl = nbl; Lb = nLb; thl = nthl; lo_m = nlo; hi_m = nhi;
         kn = k + gridDim.x;
@@ -2152,7 +2197,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 screened += __shfl_xor(screened, off);
             }
             if (lane_of() == 0) { r_d[threadIdx.x >> 6][0] = lo_max; r_d[threadIdx.x >> 6][1] = hi_max; }
-            if (lane_of() == 0 && screened) atomicAdd(&ctr_rw->screened, (unsigned long long)screened);
+            if (lane_of() == 0) acc_screened += screened;
             __syncthreads();
             lo_max = fmax(fmax(r_d[0][0], r_d[1][0]), fmax(r_d[2][0], r_d[3][0]));
             hi_max = fmax(fmax(r_d[0][1], r_d[1][1]), fmax(r_d[2][1], r_d[3][1]));
@@ -2229,7 +2274,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     });
                 }
                 for (int off = 32; off > 0; off >>= 1) scored += __shfl_xor(scored, off);
-                if (lane_of() == 0 && scored) atomicAdd(&ctr_rw->combos, (unsigned long long)scored);
+                if (lane_of() == 0) acc_scored += scored;
                 if (!s_settled) block_key_reduce(sc, key, bp, r_sc, r_key, r_p);
             }
         }
@@ -2237,6 +2282,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             parse_overlap(d, L, thr, id, lab, bidx, bl, sc, bp, best_L, ov_idx, ov_lab, ov);
         head_step(3);
     }
+    // statistics: one update per wave for all of its windows
+    if (lane_of() == 0 && acc_screened) atomicAdd(&ctr_rw->screened, acc_screened);
+    if (lane_of() == 0 && acc_scored) atomicAdd(&ctr_rw->combos, acc_scored);
 }
 
 // check_overlap_reassignments (bp.cl:822-875).  ov bit 1 marks a window that
@@ -2523,7 +2571,8 @@ static hipError_t launch_corr_mfma_rt(const Dims &d, const DeviceState &s, uint3
     ScreenCoef co{coef[0], coef[1], coef[2], coef[3]};
     hipLaunchKernelGGL((k_corr_mfma<PREC, RT, CTB>), dim3(n_win), dim3(64), lds, st, d, s.V, s.Tm,
                        (const Frag<PREC> *)s.TmB, s.bias, s.tnorm, s.tscale, s.win, s.L, s.ctr, s.rq, rq_cap,
-                       (uint2 *)s.rdesc, cb_max, co, no_rescore, s.tord, s.tile_lo, s.tile_hi, s.sup_lo, s.sup_hi);
+                       (uint2 *)s.rdesc, cb_max, co, no_rescore, s.tord, s.tile_lo, s.tile_hi, s.sup_lo, s.sup_hi,
+                       s.rq_shard);
     return hipGetLastError();
 }
 
@@ -2556,12 +2605,13 @@ hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_wi
                               const float *coef, uint32_t rq_cap, uint32_t no_rescore, hipStream_t st)
 {
     if (n_win == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_zero_rescore, dim3(1), dim3(64), 0, st, s.ctr);
+    hipLaunchKernelGGL(k_zero_rescore, dim3(1), dim3(64), 0, st, s.ctr, s.rq_shard);
     hipError_t e;
     if (precision == 1) e = launch_corr_mfma_p<1>(d, s, n_win, coef, rq_cap, no_rescore, st);
     else if (precision == 2) e = launch_corr_mfma_p<2>(d, s, n_win, coef, rq_cap, no_rescore, st);
     else e = launch_corr_mfma_p<3>(d, s, n_win, coef, rq_cap, no_rescore, st);
     if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sum_rescore, dim3(1), dim3(64), 0, st, s.ctr, s.rq_shard, rq_cap);
     if (!no_rescore) {
         const uint32_t nchunk = mfma_nchunk(d);
         const size_t rlds = rescore_res_lds(d);
@@ -2574,7 +2624,7 @@ hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_wi
                 hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
                 ncu = 256;
             hipLaunchKernelGGL(k_rescore_res, dim3(ncu), dim3(64 * kRescoreResWaves), rlds, st, d, s.V, s.TmT,
-                               s.bias, s.rq, s.ctr, rq_cap, s.L);
+                               s.bias, s.rq, s.rq_shard, rq_cap, s.L);
             return hipGetLastError();
         }
         hipLaunchKernelGGL(k_rescore_sup, dim3(n_win * nchunk), dim3(64), 0, st, d, s.V, s.Tm, s.bias, s.sup_lo,

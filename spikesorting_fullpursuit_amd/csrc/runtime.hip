@@ -107,7 +107,7 @@ void free_state(fbp_segment *g)
     void *ptrs[] = {s.V, s.V0, s.Tm, s.TmT, s.bias, s.thr, s.L, s.best_L, s.best_lab, s.best_idx, s.ov_idx,
                     s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
                     s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabABmin, s.tabB2,
-                    s.tnorm, s.TmB, s.rq, s.tscale, s.rdesc, s.TmQ, s.Tpad, s.ov_sorted, s.part_v, s.part_t,
+                    s.tnorm, s.TmB, s.rq, s.rq_shard, s.tscale, s.rdesc, s.TmQ, s.Tpad, s.ov_sorted, s.part_v, s.part_t,
                     s.part_p, s.ovQ, s.sup_lo, s.sup_hi, s.tord, s.tile_lo, s.tile_hi};
     for (void *p : ptrs) dfree(p, g->own_stream);
     if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
@@ -356,6 +356,7 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         if (cap > (512u << 20)) cap = 512u << 20;
         g->rq_cap = (uint32_t)cap;
         ALLOC(s.rq, cap)
+        ALLOC(s.rq_shard, kQueueShards * kShardStride)
         uint64_t *rd = nullptr;
         ALLOC(rd, (uint64_t)W * mfma_nchunk(d))
         s.rdesc = rd;
