@@ -326,8 +326,11 @@ __device__ __forceinline__ float exact_likelihood(const Dims &d, const float *__
 
 constexpr int kMfmaMaxRT = 6;   // row tiles per window: N <= 96
 
+// Register budget: four waves per SIMD for up to 64 positions and 32 template
+// columns (measured faster than three, despite a few spilled registers), fewer
+// for the larger tiles.
 template <int PREC, int RT, int CTB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 && RT <= 4 && CTB <= 2 ? 3 : (RT * CTB > 16 ? 1 : 2)))) void k_corr_mfma(Dims d, const float *__restrict__ V, const float *__restrict__ Tm,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 && RT <= 4 && CTB <= 2 ? 4 : (RT * CTB > 16 ? 1 : 2)))) void k_corr_mfma(Dims d, const float *__restrict__ V, const float *__restrict__ Tm,
                                                   const Frag<PREC> *__restrict__ TmB, const float *__restrict__ bias,
                                                   const float *__restrict__ tnorm, const float *__restrict__ tscale,
                                                   const uint32_t *__restrict__ win, float *__restrict__ L,
