@@ -2626,7 +2626,11 @@ hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_wi
             if (hipGetDevice(&dev) != hipSuccess ||
                 hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
                 ncu = 256;
-            hipLaunchKernelGGL(k_rescore_res, dim3(ncu), dim3(64 * kRescoreResWaves), rlds, st, d, s.V, s.TmT,
+            // every block copies the templates into LDS first: no more blocks
+            // than about one wave per listed window
+            const uint32_t need = (n_win + kRescoreResWaves - 1) / kRescoreResWaves;
+            const uint32_t grid = need < (uint32_t)ncu ? need : (uint32_t)ncu;
+            hipLaunchKernelGGL(k_rescore_res, dim3(grid), dim3(64 * kRescoreResWaves), rlds, st, d, s.V, s.TmT,
                                s.bias, s.rq, s.rq_shard, rq_cap, s.L);
             return hipGetLastError();
         }

@@ -126,8 +126,8 @@ uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
                  r((nb + 1) * 4) + r(sizeof(Counters)) + r(T * 4) + 2 * r(W * 4) + r(T * 4) +
                  r(T * d.C * (d.N + 2 * d.M) * 4) + 3 * r(W * 4) + 2 * r(T * 4) + 3 * r((T + 15) / 16 * 16 * 4);
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED)
-        b += r(T * d.S * T * d.S * 4) + r(T * d.S * 4) + r(overlap_qtemplate_floats(d) * 4) + r(W * 4) +
-             r((uint64_t)d.ov_batch * T * d.S * 4);
+        b += r(T * d.S * T * d.S * 4) + r(T * d.S * T * 4) + r(T * d.S * 4) + r(overlap_qtemplate_floats(d) * 4) +
+             r(W * 4) + r((uint64_t)d.ov_batch * T * d.S * 4);
     if (cfg->precision != FBP_PREC_FP32) {
         uint64_t cap = T * d.K / 8;
         if (cap < (1u << 20)) cap = 1u << 20;
@@ -267,9 +267,10 @@ static int make_dims(const fbp_config *cfg, Dims &d)
     d.items = (uint64_t)d.S * d.S * T;
     d.nl = (d.items - 1) / cfg->reduce_wg + 1;
     {
-        // Q tables of up to 256 MB of windows per screened-overlap batch
+        // Q tables of up to 1 GiB of windows per screened-overlap batch (a
+        // single batch for most recordings: no launches beyond the windows)
         const uint64_t per = (uint64_t)d.T * d.S * 4;
-        uint64_t nb = (256ull << 20) / per;
+        uint64_t nb = (1ull << 30) / per;
         if (nb < 1024) nb = 1024;
         if (nb > d.W) nb = d.W;
         d.ov_batch = (uint32_t)nb;
