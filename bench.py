@@ -176,18 +176,25 @@ def main():
     return 0
 
 
-def post_pursuit_run(seg, V, sort_info, T, repeats=3):
+def post_pursuit_run(seg, V, sort_info, T, repeats=3, max_clip_bytes=4 << 30):
     """The caller's clips after binary pursuit (full_binary_pursuit.py:357-368):
     get_clips of every found spike on all channels of the resident input
     voltage (fbp_get_clips, clips copied to the host) and the per-unit mean
     clips (fbp_unit_mean_clips, clips uploaded).  Times are whole calls,
     PCIe copies included; the numpy gather of the same clips is timed beside
-    them on the host."""
+    them on the host.  Host memory holds a few copies of the clips, so a
+    recording whose clips exceed max_clip_bytes uses its earliest spikes up to
+    that size."""
     from spikesorting_fullpursuit_amd import _lib
     from spikesorting_fullpursuit_amd.segment import get_clips, time_window_to_samples
     idx, lab = seg.spikes()
     C, K = V.shape
     window, _ = time_window_to_samples(sort_info['clip_width'], sort_info['sampling_rate'])
+    n_found = idx.size
+    n_keep = min(n_found, max(1, max_clip_bytes // (4 * C * (window[1] - window[0]))))
+    if n_keep < n_found:
+        first = np.argsort(idx, kind="stable")[:n_keep]
+        idx, lab = idx[first], lab[first]
     ev = idx.astype(np.int64) - window[0]   # event indices = onsets + clip_init_samples
     pd = {'sampling_rate': sort_info['sampling_rate'], 'n_samples': K, 'v_dtype': np.float32}
     chans = np.arange(C)
@@ -223,7 +230,8 @@ def post_pursuit_run(seg, V, sort_info, T, repeats=3):
     tc, tm = float(np.median(t_clip)), float(np.median(t_mean))
     log("post-pursuit: %d clips x %d floats, get_clips %.1f ms, unit means %.1f ms, host gather %.1f ms"
         % (ev.size, C * W, 1e3 * tc, 1e3 * tm, 1e3 * t_host))
-    return {"n_clips": int(ev.size), "clip_floats": int(C * W), "clip_bytes": int(nbytes),
+    return {"n_clips": int(ev.size), "n_spikes_found": int(n_found), "clip_floats": int(C * W),
+            "clip_bytes": int(nbytes),
             "get_clips_ms": 1e3 * tc, "get_clips_GBps": nbytes / tc / 1e9,
             "unit_means_ms": 1e3 * tm, "n_units": int(np.count_nonzero(counts)),
             "host_numpy_gather_ms": 1e3 * t_host, "all_valid": bool(valid.all()),
