@@ -88,8 +88,10 @@ enum {
     FBP_DEBUG_OVERLAP_EXACT = 8,   /* screened overlap: score a single candidate
                                       exactly even when its interval settles the
                                       parse                                      */
-    FBP_DEBUG_OVERLAP_ALL_ROWS = 16 /* screened overlap: traverse every row, none
+    FBP_DEBUG_OVERLAP_ALL_ROWS = 16, /* screened overlap: traverse every row, none
                                       skipped by its upper bound                 */
+    FBP_DEBUG_OVERLAP_SMALL_BATCH = 32 /* screened overlap: batches of 37 windows
+                                      (Q tables, then the screen, per batch)     */
 };
 
 typedef struct fbp_stats {

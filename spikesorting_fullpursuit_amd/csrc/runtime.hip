@@ -272,6 +272,7 @@ static int make_dims(const fbp_config *cfg, Dims &d)
         const uint64_t per = (uint64_t)d.T * d.S * 4;
         uint64_t nb = (1ull << 30) / per;
         if (nb < 1024) nb = 1024;
+        if (cfg->debug_flags & FBP_DEBUG_OVERLAP_SMALL_BATCH) nb = 37;
         if (nb > d.W) nb = d.W;
         d.ov_batch = (uint32_t)nb;
     }

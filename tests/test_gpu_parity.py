@@ -76,14 +76,15 @@ def test_dense_16ch_spike_list_grows(oracle_mod, mode):
         assert g['stats']['screened_combos'] > 20 * g['stats']['overlap_combos']
 
 
-@pytest.mark.parametrize("debug", [0, 2, 4, 8, 12, 16, 18])
+@pytest.mark.parametrize("debug", [0, 2, 4, 8, 12, 16, 18, 32, 50])
 def test_screened_overlap_candidate_paths(oracle_mod, debug):
     # The screened overlap finds its exact-score candidates from per-thread
     # lists (partner templates staged in LDS) and settles single candidates
     # from their score interval, unless forced onto the second traversal (2),
     # global partner reads (4) or exact scores for single candidates (8); rows
-    # of the traversal are skipped by their upper bound unless 16 is set.
-    # Every route gives the oracle's events.
+    # of the traversal are skipped by their upper bound unless 16 is set; 32
+    # splits each pass's overlap windows into batches of 37 (the Q tables and
+    # the screen run per batch).  Every route gives the oracle's events.
     from spikesorting_fullpursuit_amd import synthetic
     rec = synthetic.config("16ch-300s", seed=7, duration_s=0.5)
     sm = rec['separability_metrics']
@@ -98,6 +99,8 @@ def test_screened_overlap_candidate_paths(oracle_mod, debug):
     np.testing.assert_array_equal(g['residual'].view(np.uint32), o['residual'].view(np.uint32))
     if debug & 8:
         assert g['stats']['overlap_combos'] > 0
+    if debug & 32:
+        assert g['stats']['overlap_windows'] > 2 * 37   # several batches
     if debug == 0:
         full = run_gpu(rec, 30, True, 256, precision="bf16", overlap_mode="screened", debug_flags=16)
         assert g['stats']['screened_combos'] < full['stats']['screened_combos']
