@@ -213,6 +213,23 @@ int fbp_unit_mean_clips(int device, const float *clips, uint64_t n, uint32_t wid
 int fbp_clip_template_sse(int device, const float *clips, uint64_t n, uint32_t width, const uint32_t *template_index,
                           const float *templates, uint32_t n_templates, float *sse);
 
+/* Noise covariance of every channel over a sample of residual clip rows,
+ * cov[c] = np.cov(clips[rows, c*N:(c+1)*N], rowvar=False, ddof=0) in float64
+ * (full_binary_pursuit.py:245-252; the reference draws the rows with
+ * np.random.randint per channel, the caller passes the ones it drew).
+ * clips: host float32 [n_clips][n_chans*N]; rows: host uint64 [n_chans][n_rows]
+ * (each < n_clips), or NULL for every clip (n_rows = n_clips); cov: host
+ * float64 [n_chans][N][N].  N <= 128, n_rows >= 1. */
+int fbp_channel_covariance(int device, const float *clips, uint64_t n_clips, uint32_t n_chans, uint32_t N,
+                           const uint64_t *rows, uint64_t n_rows, double *cov);
+
+/* Likelihood variance of every template, var[t] = sum_c t_c^T cov_c t_c in
+ * float64 (neuron_separability.py:126-135, the input of the binary-pursuit
+ * thresholds, :137-142).  templates: host float64 [T][n_chans*N]; cov: host
+ * float64 [n_chans][N][N]; var: host float64 [T].  N <= 128. */
+int fbp_template_variances(int device, const double *templates, uint32_t n_templates, uint32_t n_chans, uint32_t N,
+                           const double *cov, double *var);
+
 /* Overlap-template search residuals (find_overlap_templates,
  * c_cython/sort_cython.pyx:333-456, before binary pursuit at
  * full_binary_pursuit.py:257-262).  templates: host float32 [n_templates]

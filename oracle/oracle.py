@@ -389,3 +389,42 @@ def remove_segment_binary_pursuit_duplicates(sort_data, n_chans, n_segments, hal
                                                              d[3][sel], 2 * half_clip_inds)
             d[0], d[1], d[2], d[3] = d[0][keep], d[1][keep], d[2][keep, :], d[3][keep]
     return sort_data
+
+
+def channel_covariance(clips, n_chans, n_samples_per_chan, rows=None):
+    """Per-channel noise covariance of clip rows, np.cov(..., rowvar=False,
+    ddof=0) in float64 (full_binary_pursuit.py:245-252); rows [n_chans][n_rows]
+    or None for every clip."""
+    N = n_samples_per_chan
+    out = []
+    for c in range(n_chans):
+        sel = clips if rows is None else clips[np.asarray(rows[c], dtype=np.int64)]
+        out.append(np.cov(sel[:, c * N:(c + 1) * N], rowvar=False, ddof=0))
+    return np.stack(out)
+
+
+def separability_metrics(templates, channel_covariance_mats, sort_info):
+    """neuron_separability.py:88-152 restated: per template the likelihood
+    variance sum_c (t_c[None, :] @ cov_c) @ t_c[:, None] (:126-135), the lower
+    threshold and confidence bound (:137-142), peak channel (:144-146) and
+    contamination (:147-150); channel_p_noise (:114-121)."""
+    from scipy.stats import norm
+    C, N = sort_info['n_channels'], sort_info['n_samples_per_chan']
+    tm = np.vstack(templates)
+    T = tm.shape[0]
+    ss = np.sum(tm ** 2, axis=1)
+    var = np.zeros(T)
+    thr, ci, peak, cont = np.zeros(T), np.zeros(T), np.zeros(T, dtype=np.int64), np.zeros(T)
+    for n in range(T):
+        for c in range(C):
+            t = tm[n, c * N:(c + 1) * N].astype(np.float64)
+            var[n] += ((t[None, :] @ channel_covariance_mats[c]) @ t[:, None])[0, 0]
+        e = 0.5 * ss[n]
+        ci[n] = e - sort_info['sigma_bp_CI'] * np.sqrt(var[n])
+        thr[n] = -1 * e + sort_info['sigma_bp_noise'] * np.sqrt(var[n])
+        peak[n] = np.argmax(np.abs(tm[n])) // N
+        cont[n] = norm.sf(thr[n], -e, np.sqrt(var[n]))
+    p_noise = np.array([(sort_info['n_samples'] - sort_info['n_threshold_crossings'][c]) / sort_info['n_samples']
+                        for c in range(C)], dtype=np.float64)
+    return {'template_SS': ss, 'neuron_variances': var, 'neuron_lower_thresholds': thr, 'neuron_lower_CI': ci,
+            'peak_channel': peak, 'contamination': cont, 'channel_p_noise': p_noise}

@@ -35,7 +35,7 @@ EXPORTS = (
     "fbp_segment_num_spikes", "fbp_segment_get_spikes", "fbp_segment_get_clips",
     "fbp_segment_get_residual", "fbp_segment_get_likelihood", "fbp_segment_get_stats",
     "fbp_segment_voltage_device", "fbp_get_clips", "fbp_unit_mean_clips", "fbp_overlap_template_residuals",
-    "fbp_clip_template_sse", "fbp_get_clips_host",
+    "fbp_clip_template_sse", "fbp_get_clips_host", "fbp_channel_covariance", "fbp_template_variances",
 )
 
 
@@ -127,6 +127,11 @@ def _declare(lib):
     lib.fbp_overlap_template_residuals.argtypes = [ctypes.c_int, f32p, ctypes.c_uint32, ctypes.c_uint32,
                                                    ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32, u32p,
                                                    ctypes.c_uint64, P(ctypes.c_double), u32p]
+    f64p = P(ctypes.c_double)
+    lib.fbp_channel_covariance.argtypes = [ctypes.c_int, f32p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                           u64p, ctypes.c_uint64, f64p]
+    lib.fbp_template_variances.argtypes = [ctypes.c_int, f64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                           f64p, f64p]
     for name in EXPORTS:
         if name not in ("fbp_last_error", "fbp_segment_destroy"):
             getattr(lib, name).restype = ctypes.c_int
@@ -165,6 +170,10 @@ def _f32p(a):
 
 def _u32p(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def _f64p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
 def device_count():
@@ -377,4 +386,41 @@ def clip_template_sse(clips, template_index, templates, device=0):
     out = np.zeros(c.shape[0], dtype=np.float32)
     check(lib.fbp_clip_template_sse(device, _f32p(c), c.shape[0], c.shape[1], _u32p(idx), _f32p(tm), tm.shape[0],
                                     _f32p(out)), "fbp_clip_template_sse")
+    return out
+
+
+def channel_covariance(clips, n_chans, n_samples_per_chan, rows=None, device=0):
+    """fbp_channel_covariance: float64 [n_chans][N][N], per channel
+    np.cov(clips[rows[c], c*N:(c+1)*N], rowvar=False, ddof=0); rows is
+    [n_chans][n_rows] (row indices drawn per channel) or None for every clip."""
+    lib = load()
+    c = np.ascontiguousarray(clips, dtype=np.float32)
+    N = int(n_samples_per_chan)
+    if c.ndim != 2 or c.shape[1] != n_chans * N:
+        raise ValueError("clips must be [n_clips][n_chans * n_samples_per_chan]")
+    out = np.zeros((n_chans, N, N), dtype=np.float64)
+    if rows is None:
+        r, n_rows, rp = None, c.shape[0], None
+    else:
+        r = np.ascontiguousarray(rows, dtype=np.uint64)
+        if r.ndim != 2 or r.shape[0] != n_chans:
+            raise ValueError("rows must be [n_chans][n_rows]")
+        n_rows = r.shape[1]
+        rp = r.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    check(lib.fbp_channel_covariance(device, _f32p(c), c.shape[0], n_chans, N, rp, n_rows, _f64p(out)),
+          "fbp_channel_covariance")
+    return out
+
+
+def template_variances(templates, channel_covariance_mats, device=0):
+    """fbp_template_variances: float64 [T], sum over channels of t_c^T cov_c t_c."""
+    lib = load()
+    cov = np.ascontiguousarray(np.stack([np.asarray(m, dtype=np.float64) for m in channel_covariance_mats]))
+    n_chans, N = cov.shape[0], cov.shape[1]
+    if cov.shape != (n_chans, N, N):
+        raise ValueError("covariance matrices must be square, one per channel")
+    tm = np.ascontiguousarray(templates, dtype=np.float64).reshape(-1, n_chans * N)
+    out = np.zeros(tm.shape[0], dtype=np.float64)
+    check(lib.fbp_template_variances(device, _f64p(tm), tm.shape[0], n_chans, N, _f64p(cov), _f64p(out)),
+          "fbp_template_variances")
     return out

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfbp.so")
-SOURCES = ["kernels.hip", "runtime.hip", "clips.hip", "overlap_templates.hip"]
+SOURCES = ["kernels.hip", "runtime.hip", "clips.hip", "overlap_templates.hip", "separability.hip"]
 ARCH = os.environ.get("FBP_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: only explicit fused multiply-adds, in the reference's order.

@@ -856,6 +856,75 @@ int fbp_clip_template_sse(int device, const float *clips, uint64_t n, uint32_t w
     return FBP_OK;
 }
 
+int fbp_channel_covariance(int device, const float *clips, uint64_t n_clips, uint32_t n_chans, uint32_t N,
+                           const uint64_t *rows, uint64_t n_rows, double *cov)
+{
+    if (!clips || !cov) return fail(FBP_EINVAL, "NULL argument");
+    if (N == 0 || N > kMaxCovSamples) return fail(FBP_EINVAL, "samples per channel %u out of range (1..%u)", N,
+                                                  kMaxCovSamples);
+    if (!rows) n_rows = n_clips;
+    if (n_rows == 0) return fail(FBP_EINVAL, "no clip rows to compute a covariance from");
+    if (rows)
+        for (uint64_t k = 0; k < (uint64_t)n_chans * n_rows; k++)
+            if (rows[k] >= n_clips)
+                return fail(FBP_EINVAL, "row %llu out of range (%llu clips)", (unsigned long long)rows[k],
+                            (unsigned long long)n_clips);
+    if (n_chans == 0) return FBP_OK;
+    const uint64_t width = (uint64_t)n_chans * N;
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+    float *d_clips = nullptr;
+    uint64_t *d_rows = nullptr;
+    double *d_cov = nullptr;
+    const uint64_t ncov = (uint64_t)n_chans * N * N;
+    hipError_t e = dalloc(&d_clips, n_clips * width, st);
+    if (e == hipSuccess && rows) e = dalloc(&d_rows, (uint64_t)n_chans * n_rows, st);
+    if (e == hipSuccess) e = dalloc(&d_cov, ncov, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_clips, clips, n_clips * width * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && rows)
+        e = hipMemcpyAsync(d_rows, rows, (uint64_t)n_chans * n_rows * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_channel_covariance(d_clips, width, d_rows, n_rows, n_chans, N, d_cov, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(cov, d_cov, ncov * 8, hipMemcpyDeviceToHost, st);
+    dfree(d_clips, st);
+    dfree(d_rows, st);
+    dfree(d_cov, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    if (e == hipSuccess) e = es;
+    HIP_TRY(e, "channel covariance");
+    return FBP_OK;
+}
+
+int fbp_template_variances(int device, const double *templates, uint32_t n_templates, uint32_t n_chans, uint32_t N,
+                           const double *cov, double *var)
+{
+    if (n_templates && (!templates || !cov || !var)) return fail(FBP_EINVAL, "NULL argument");
+    if (N == 0 || N > kMaxCovSamples) return fail(FBP_EINVAL, "samples per channel %u out of range (1..%u)", N,
+                                                  kMaxCovSamples);
+    if (n_templates == 0) return FBP_OK;
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+    double *d_tm = nullptr, *d_cov = nullptr, *d_var = nullptr;
+    const uint64_t ntm = (uint64_t)n_templates * n_chans * N, ncov = (uint64_t)n_chans * N * N;
+    hipError_t e = dalloc(&d_tm, ntm ? ntm : 1, st);
+    if (e == hipSuccess) e = dalloc(&d_cov, ncov ? ncov : 1, st);
+    if (e == hipSuccess) e = dalloc(&d_var, n_templates, st);
+    if (e == hipSuccess && ntm) e = hipMemcpyAsync(d_tm, templates, ntm * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && ncov) e = hipMemcpyAsync(d_cov, cov, ncov * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_template_variance(d_tm, n_templates, n_chans, N, d_cov, d_var, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(var, d_var, (uint64_t)n_templates * 8, hipMemcpyDeviceToHost, st);
+    dfree(d_tm, st);
+    dfree(d_cov, st);
+    dfree(d_var, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    if (e == hipSuccess) e = es;
+    HIP_TRY(e, "template variances");
+    return FBP_OK;
+}
+
 int fbp_unit_mean_clips(int device, const float *clips, uint64_t n, uint32_t width, const uint32_t *labels,
                         uint32_t n_units, float *means, uint64_t *counts)
 {
