@@ -226,6 +226,8 @@ def post_pursuit_run(seg, V, sort_info, T, repeats=3, max_clip_bytes=4 << 30):
     for r in range(ev.size):
         host[r] = V[:, ev[r] + window[0]:ev[r] + window[1]].ravel()
     t_host = time.perf_counter() - t
+    del host
+    thresholds = threshold_inputs_run(clips, lab, means, counts, C, W, K)
     nbytes = clips.nbytes
     tc, tm = float(np.median(t_clip)), float(np.median(t_mean))
     log("post-pursuit: %d clips x %d floats, get_clips %.1f ms, unit means %.1f ms, host gather %.1f ms"
@@ -234,11 +236,51 @@ def post_pursuit_run(seg, V, sort_info, T, repeats=3, max_clip_bytes=4 << 30):
             "clip_bytes": int(nbytes),
             "get_clips_ms": 1e3 * tc, "get_clips_GBps": nbytes / tc / 1e9,
             "unit_means_ms": 1e3 * tm, "n_units": int(np.count_nonzero(counts)),
+            "threshold_inputs": thresholds,
             "host_numpy_gather_ms": 1e3 * t_host, "all_valid": bool(valid.all()),
             "duplicate_removal_ms": 1e3 * t_dup, "duplicates_removed": int(n_dup),
             "sampled_clips_equal_voltage": bool(sample_ok), "largest_unit_mean_equals_numpy": mean_ok,
             "note": "whole calls: get_clips includes the device-to-host copy of the clips, unit means the "
                     "host-to-device copy; the gather and mean kernels are in the rocprofv3 kernel table"}
+
+
+def threshold_inputs_run(clips, lab, means, counts, C, W, n_samples, n_cov_samples=100000):
+    """The inputs of the binary-pursuit thresholds from the found spikes:
+    residual clips (clip - unit mean), per-channel noise covariance over
+    n_cov_samples rows drawn per channel (full_binary_pursuit.py:243-252;
+    fbp_channel_covariance) and the separability metrics of the unit means
+    (neuron_separability.py:88-152; likelihood variances in
+    fbp_template_variances).  numpy's np.cov over the same rows is timed on
+    the host beside it."""
+    from spikesorting_fullpursuit_amd.neuron_separability import (clip_noise_covariance,
+                                                                  compute_separability_metrics)
+    units = np.flatnonzero(counts)
+    resid = clips - means[lab]
+    si = {'n_channels': C, 'n_samples_per_chan': W, 'n_samples': n_samples,
+          'n_threshold_crossings': np.zeros(C), 'sigma_bp_noise': 3.719, 'sigma_bp_CI': np.inf}
+    t_cov, t_sep = [], []
+    for _ in range(3):
+        t = time.perf_counter()
+        covs = clip_noise_covariance(resid, C, W, n_cov_samples, rng=np.random.RandomState(0))
+        t_cov.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        sm = compute_separability_metrics(list(means[units].astype(np.float64)), covs, si)
+        t_sep.append(time.perf_counter() - t)
+    draw = np.random.RandomState(0)
+    n_rows = min(n_cov_samples, resid.shape[0])
+    t = time.perf_counter()
+    host = []
+    for c in range(C):
+        rows = draw.randint(0, resid.shape[0], n_cov_samples) if n_cov_samples < resid.shape[0] else slice(None)
+        host.append(np.cov(resid[rows, c * W:(c + 1) * W], rowvar=False, ddof=0))
+    t_host = time.perf_counter() - t
+    err = max(float(np.max(np.abs(covs[c] - host[c])) / np.max(np.abs(host[c]))) for c in range(C))
+    log("threshold inputs: covariance of %d channels x %d rows %.1f ms (numpy %.1f ms), metrics of %d units %.1f ms"
+        % (C, n_rows, 1e3 * np.median(t_cov), 1e3 * t_host, units.size, 1e3 * np.median(t_sep)))
+    return {"covariance_ms": 1e3 * float(np.median(t_cov)), "rows_per_channel": int(n_rows),
+            "numpy_cov_ms": 1e3 * t_host, "max_rel_diff_vs_numpy": err,
+            "separability_metrics_ms": 1e3 * float(np.median(t_sep)), "units": int(units.size),
+            "note": "whole calls, residual clips uploaded per call; np.cov of the same rows on the host"}
 
 
 def overlap_templates_run(templates, C, N, sort_info, seg, cpu=True, repeats=3):
