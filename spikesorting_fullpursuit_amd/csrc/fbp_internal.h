@@ -150,8 +150,12 @@ hipError_t launch_overlap_template_residuals(const float *Tm, uint32_t C, uint32
 hipError_t launch_clips(const Dims &d, const DeviceState &s, uint64_t n_spikes, hipStream_t st);
 // samples per channel the covariance and template-variance kernels hold in LDS
 constexpr uint32_t kMaxCovSamples = 128;
+// row slices of the covariance (returns their count R, sets their length);
+// scratch: colpart [C][R][N], part [C][R][N][N] doubles
+uint32_t channel_covariance_slices(uint64_t n_rows, uint64_t *slice);
 hipError_t launch_channel_covariance(const float *clips, uint64_t width, const uint64_t *rows, uint64_t n_rows,
-                                     uint32_t C, uint32_t N, double *cov, hipStream_t st);
+                                     uint32_t C, uint32_t N, double *colpart, double *part, double *cov,
+                                     hipStream_t st);
 hipError_t launch_template_variance(const double *Tm, uint32_t T, uint32_t C, uint32_t N, const double *cov,
                                     double *var, hipStream_t st);
 

@@ -876,19 +876,26 @@ int fbp_channel_covariance(int device, const float *clips, uint64_t n_clips, uin
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
     float *d_clips = nullptr;
     uint64_t *d_rows = nullptr;
-    double *d_cov = nullptr;
+    double *d_cov = nullptr, *d_colpart = nullptr, *d_part = nullptr;
     const uint64_t ncov = (uint64_t)n_chans * N * N;
+    uint64_t slice = 0;
+    const uint32_t R = channel_covariance_slices(n_rows, &slice);
     hipError_t e = dalloc(&d_clips, n_clips * width, st);
     if (e == hipSuccess && rows) e = dalloc(&d_rows, (uint64_t)n_chans * n_rows, st);
     if (e == hipSuccess) e = dalloc(&d_cov, ncov, st);
+    if (e == hipSuccess) e = dalloc(&d_colpart, (uint64_t)n_chans * R * N, st);
+    if (e == hipSuccess) e = dalloc(&d_part, ncov * R, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_clips, clips, n_clips * width * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && rows)
         e = hipMemcpyAsync(d_rows, rows, (uint64_t)n_chans * n_rows * 8, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = launch_channel_covariance(d_clips, width, d_rows, n_rows, n_chans, N, d_cov, st);
+    if (e == hipSuccess)
+        e = launch_channel_covariance(d_clips, width, d_rows, n_rows, n_chans, N, d_colpart, d_part, d_cov, st);
     if (e == hipSuccess) e = hipMemcpyAsync(cov, d_cov, ncov * 8, hipMemcpyDeviceToHost, st);
     dfree(d_clips, st);
     dfree(d_rows, st);
     dfree(d_cov, st);
+    dfree(d_colpart, st);
+    dfree(d_part, st);
     const hipError_t es = hipStreamSynchronize(st);
     (void)hipStreamDestroy(st);
     if (e == hipSuccess) e = es;
