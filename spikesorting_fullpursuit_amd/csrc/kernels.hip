@@ -2785,7 +2785,10 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
     // batches of ov_batch windows: Q tables (matrix cores), then the screen
     for (uint32_t kb = 0; kb < max_ov; kb += d.ov_batch) {
         const uint32_t nb = max_ov - kb < d.ov_batch ? max_ov - kb : d.ov_batch;
-        const uint32_t grid = nb < 4096 ? nb : 4096;
+        // up to 8192 blocks (eight per resident slot at four per CU): windows
+        // differ in work, and smaller grids leave a longer tail (measured:
+        // 2048 and 4096 slower, 16384 no faster, one block per window slower)
+        const uint32_t grid = nb < 8192 ? nb : 8192;
         if (mq) {
             hipLaunchKernelGGL(k_overlap_q, dim3(grid), dim3(256), qlds, st, d, s.V, s.L, s.thr, list, s.ctr,
                                s.best_lab, s.best_idx, s.tabB2, (const uint4 *)s.TmQ, s.ovQ, kb, s.tord, s.tile_lo,
