@@ -1622,8 +1622,15 @@ __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT
 // block's waves; slice partials are added in slice order afterwards.
 constexpr uint32_t kOvWaves = 4;   // waves of a k_overlap_screened block
 
+// Templates past the last full 16-row tile (TR = T % 16 of them, TR <= 8)
+// share one tile as ND shift offsets: row g*TR + i holds template
+// TTF*16 + i moved right by 16*g samples, so one 16-column shift tile yields
+// shifts 16*(st0 + g) + col of all TR templates over NKQ + ND - 1 K-steps.
+// Every output still meets nonzero products in exactly NKQ K-steps (the other
+// steps add exact zeros).
 struct QPlan {
     uint32_t TT, ST, NKQ, SG, NSG, KS, LPH;
+    uint32_t TTF, TR, ND, NTG, NKT;   // full tiles; tail rows, offsets, groups, K-steps
 };
 
 __host__ __device__ inline QPlan q_plan(const Dims &d)
@@ -1634,7 +1641,18 @@ __host__ __device__ inline QPlan q_plan(const Dims &d)
     p.NKQ = (d.N + 15) / 16;
     p.SG = p.ST < 4 ? p.ST : 4;
     p.NSG = (p.ST + p.SG - 1) / p.SG;
-    const uint32_t base = p.TT * p.NSG;
+    const uint32_t rem = d.T % 16;
+    if (rem > 0 && rem <= 8 && p.ST >= 2) {
+        p.TTF = p.TT - 1;
+        p.TR = rem;
+        p.ND = 16 / rem < p.ST ? 16 / rem : p.ST;
+        p.NTG = (p.ST + p.ND - 1) / p.ND;
+        p.NKT = p.NKQ + p.ND - 1;
+    } else {
+        p.TTF = p.TT;
+        p.TR = p.ND = p.NTG = p.NKT = 0;
+    }
+    const uint32_t base = p.TTF * p.NSG + p.NTG;
     uint32_t ks = base < kOvWaves ? kOvWaves / base : 1;
     p.KS = ks < d.C ? ks : d.C;
     // bf16 elements per channel row of the split voltage: every lane reads the
@@ -1653,17 +1671,83 @@ __device__ __forceinline__ uint64_t q_bfrag(const uint32_t *row, uint32_t y)
     return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
+// The tail tile's tasks (see QPlan): A fragments at
+// TmQ[(C*NKQ*TT) + (c*NKT + kq)][lane], one shift tile per task.
+__device__ __forceinline__ void q_tail_task(const Dims &d, const QPlan &qp, const uint4 *__restrict__ TmQ,
+                                            const uint16_t *svh, const uint16_t *svl, float *dst,
+                                            const uint32_t *__restrict__ tord, uint32_t c0, uint32_t c1,
+                                            uint32_t tg)
+{
+    const uint32_t S = d.S, T = d.T;
+    const uint32_t lane = lane_of();
+    const uint32_t st0 = tg * qp.ND;
+    const uint32_t nd = qp.ST - st0 < qp.ND ? qp.ST - st0 : qp.ND;   // offsets inside the shift range
+    const uint32_t nk = qp.NKQ + nd - 1;
+    const uint4 *ap = TmQ + (uint64_t)d.C * qp.NKQ * qp.TT * 64 + lane;
+    const uint32_t ybase = 4 * (lane >> 4) + st0 * 16 + (lane & 15);
+    // six independent accumulator chains (hi*hi, hi*lo, lo*hi for even and
+    // odd K-steps) so consecutive matrix-core operations do not wait on each
+    // other; every term still passes through fewer roundings than the chain
+    // bound of the screen counts
+    const float4v z = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+    float4v h0 = z, h1 = z, x0 = z, x1 = z, y0s = z, y1s = z;
+    for (uint32_t c = c0; c < c1; c++) {
+        const uint32_t *rh = (const uint32_t *)(svh + c * qp.LPH);
+        const uint32_t *rl = (const uint32_t *)(svl + c * qp.LPH);
+        uint4 an = ap[(uint64_t)c * qp.NKT * 64];
+        for (uint32_t kq = 0; kq < nk; kq++) {
+            const uint4 a = an;
+            if (kq + 1 < nk) an = ap[((uint64_t)c * qp.NKT + kq + 1) * 64];
+            const uint64_t ah = (uint64_t)a.x | ((uint64_t)a.y << 32);
+            const uint64_t al = (uint64_t)a.z | ((uint64_t)a.w << 32);
+            const uint32_t y0 = kq * 16 + ybase;
+            const uint64_t bh = q_bfrag(rh, y0), bl = q_bfrag(rl, y0);
+            if (kq & 1) {
+                h1 = mfma_bf16_k16(ah, bh, h1);
+                x1 = mfma_bf16_k16(ah, bl, x1);
+                y1s = mfma_bf16_k16(al, bh, y1s);
+            } else {
+                h0 = mfma_bf16_k16(ah, bh, h0);
+                x0 = mfma_bf16_k16(ah, bl, x0);
+                y0s = mfma_bf16_k16(al, bh, y0s);
+            }
+        }
+    }
+    float4v acch, accl;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        acch[r] = h0[r] + h1[r];
+        accl[r] = (x0[r] + y0s[r]) + (x1[r] + y1s[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const uint32_t row = 4 * (lane >> 4) + r, g = row / qp.TR, i = row - g * qp.TR;
+        const uint32_t ts = qp.TTF * 16 + i, s = (st0 + g) * 16 + (lane & 15);
+        if (g < nd && ts < T && s < S) dst[tord[ts] * S + s] = acch[r] + accl[r];
+    }
+}
+
 __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__restrict__ TmQ,
                               const uint16_t *svh, const uint16_t *svl, float *sQ, float *sQp,
                               const uint32_t *__restrict__ tord, const uint32_t *__restrict__ tile_lo,
-                              const uint32_t *__restrict__ tile_hi)
+                              const uint32_t *__restrict__ tile_hi, uint32_t rot)
 {
     const uint32_t S = d.S, T = d.T, TS = T * S;
     const uint32_t lane = lane_of();
     const uint32_t wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const uint32_t ntask = qp.TT * qp.NSG * qp.KS;
-    for (uint32_t task = wave; task < ntask; task += nwave) {
+    const uint32_t ntask = (qp.TTF * qp.NSG + qp.NTG) * qp.KS;
+    for (uint32_t it = wave; it < ntask; it += nwave) {
+        // with one task per wave, the waves take turns at the (lighter) tail
+        // tasks from window to window, so every SIMD gets the same work
+        const uint32_t task = ntask == nwave ? (it + rot) % nwave : it;
         const uint32_t ks = task % qp.KS, rest = task / qp.KS;
+        if (rest >= qp.TTF * qp.NSG) {
+            const uint32_t tl = qp.TT - 1;
+            const uint32_t c0 = max(ks * d.C / qp.KS, tile_lo[tl]), c1 = max(c0, min((ks + 1) * d.C / qp.KS, tile_hi[tl]));
+            q_tail_task(d, qp, TmQ, svh, svl, ks == 0 ? sQ : sQp + (uint64_t)(ks - 1) * TS, tord, c0, c1,
+                        rest - qp.TTF * qp.NSG);
+            continue;
+        }
         const uint32_t g = rest % qp.NSG, tt = rest / qp.NSG;
         // the slice's channels on which the tile's templates are nonzero
         const uint32_t c0 = max(ks * d.C / qp.KS, tile_lo[tt]), c1 = max(c0, min((ks + 1) * d.C / qp.KS, tile_hi[tt]));
@@ -1790,7 +1874,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                          put_v);
         head_step(2);
         __syncthreads();
-        q_table_split(d, qp, TmQ, svh, svl, sQ, sQp, tord, tile_lo, tile_hi);
+        q_table_split(d, qp, TmQ, svh, svl, sQ, sQp, tord, tile_lo, tile_hi, k & 3u);
         __syncthreads();
         head_step(3);
         float *dst = ovQ + (uint64_t)(k - kbase) * TS;
@@ -1835,6 +1919,29 @@ __global__ void k_pack_qtemplates(Dims d, const float *__restrict__ Tm, const ui
             l[i >> 1] |= vl << (16 * (i & 1));
         }
         TmQ[e] = uint4{h[0], h[1], l[0], l[1]};
+    }
+    // tail tile: TmQ[total + (c*NKT + kq)][lane], row g*TR + i = template
+    // TTF*16 + i moved right by 16*g samples
+    const QPlan qp = q_plan(d);
+    const uint64_t tail = (uint64_t)d.C * qp.NKT * 64;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < tail;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t lane = (uint32_t)(e % 64);
+        const uint64_t r = e / 64;
+        const uint32_t kq = (uint32_t)(r % qp.NKT), c = (uint32_t)(r / qp.NKT);
+        const uint32_t row = lane & 15, g = row / qp.TR, i = row - g * qp.TR;
+        const uint32_t ts = qp.TTF * 16 + i;
+        const uint32_t t = (g < qp.ND && ts < d.T) ? tord[ts] : d.T;
+        uint32_t h[2] = {0u, 0u}, l[2] = {0u, 0u};
+        for (uint32_t q = 0; q < 4; q++) {
+            const int32_t x = (int32_t)(kq * 16 + 4 * (lane >> 4) + q) - (int32_t)(16 * g);
+            const float v = (t < d.T && x >= 0 && x < (int32_t)d.N) ? Tm[((uint64_t)t * d.C + c) * d.N + x] : 0.0f;
+            const uint32_t vh = to_bf16(v);
+            const uint32_t vl = to_bf16(v - __uint_as_float(vh << 16));
+            h[q >> 1] |= vh << (16 * (q & 1));
+            l[q >> 1] |= vl << (16 * (q & 1));
+        }
+        TmQ[total + e] = uint4{h[0], h[1], l[0], l[1]};
     }
 }
 
@@ -2723,7 +2830,9 @@ size_t overlap_screened_lds(const Dims &d)
 
 uint64_t overlap_qtemplate_floats(const Dims &d)
 {
-    return (uint64_t)d.C * ((d.N + 15) / 16) * ((d.T + 15) / 16) * 64 * 4;   // one uint4 per entry
+    // one uint4 per entry: the full tiles' fragments, then the tail tile's
+    const QPlan qp = q_plan(d);
+    return ((uint64_t)d.C * qp.NKQ * qp.TT + (uint64_t)d.C * qp.NKT) * 64 * 4;
 }
 
 hipError_t launch_pad_templates(const Dims &d, const DeviceState &s, hipStream_t st)
