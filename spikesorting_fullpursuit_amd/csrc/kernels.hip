@@ -1622,8 +1622,8 @@ __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT
 // block's waves; slice partials are added in slice order afterwards.
 constexpr uint32_t kOvWaves = 4;   // waves of a k_overlap_screened block
 
-// Templates past the last full 16-row tile (TR = T % 16 of them, TR <= 8)
-// share one tile as ND shift offsets: row g*TR + i holds template
+// With at most four tiles, the templates past the last full 16-row tile
+// (TR = T % 16 of them, TR <= 8) share one tile as ND shift offsets: row g*TR + i holds template
 // TTF*16 + i moved right by 16*g samples, so one 16-column shift tile yields
 // shifts 16*(st0 + g) + col of all TR templates over NKQ + ND - 1 K-steps.
 // Every output still meets nonzero products in exactly NKQ K-steps (the other
@@ -1642,7 +1642,7 @@ __host__ __device__ inline QPlan q_plan(const Dims &d)
     p.SG = p.ST < 4 ? p.ST : 4;
     p.NSG = (p.ST + p.SG - 1) / p.SG;
     const uint32_t rem = d.T % 16;
-    if (rem > 0 && rem <= 8 && p.ST >= 2) {
+    if (rem > 0 && rem <= 8 && p.ST >= 2 && p.TT <= 4) {   // where the last tile's padding is a large share
         p.TTF = p.TT - 1;
         p.TR = rem;
         p.ND = 16 / rem < p.ST ? 16 / rem : p.ST;
