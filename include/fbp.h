@@ -168,6 +168,11 @@ int fbp_segment_get_residual(fbp_segment *seg, float *residual);
 /* Likelihood buffer [T][K] as it stands after the run (parity checks). */
 int fbp_segment_get_likelihood(fbp_segment *seg, float *likelihood);
 int fbp_segment_get_stats(const fbp_segment *seg, fbp_stats *stats);
+/* Window list of the pass that would come next when the run was stopped by
+ * cfg.max_passes (the reference's window_indices = nonzero(
+ * check_window_on_next_pass), binary_pursuit_parallel.py:683-689), ascending;
+ * *n = 0 after a run that converged.  For parity checks of the flag rules. */
+int fbp_segment_get_window_list(fbp_segment *seg, uint32_t *windows, uint64_t capacity, uint64_t *n);
 
 /* Device address of the segment's input voltage [C][row_stride] (restored
  * before every run), for fbp_get_clips on resident data. */

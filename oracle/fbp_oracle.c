@@ -31,6 +31,12 @@
  *   - compute_residual subtracts, at every sample, the templates of the pass's
  *     spikes in ascending (onset, label, record position) order, with no lost
  *     update.
+ *
+ * Channels on which a template is zero are skipped in the likelihood and the
+ * overlap score: their terms are fmaf(0, v, acc) = acc and a zero channel sum
+ * of squares (no -0.5*ss step, bp.cl:627), so the result is the same value
+ * (the one exception, a running sum of exactly -0.0 turned into +0.0 by a
+ * zero term, compares equal and decides every branch the same way).
  */
 #include <math.h>
 #include <stdint.h>
@@ -72,7 +78,7 @@ int fbo_num_threads(void)
  * and adds Tm[t,c,j]*V[c,i+j] channel-major, sample-minor.
  * ------------------------------------------------------------------------- */
 static void likelihood_window(const fbo_params *p, const float *V, const float *Tm,
-                              const float *bias, uint32_t w, float *L)
+                              const float *bias, const uint8_t *nz, uint32_t w, float *L)
 {
     const uint32_t C = p->n_chans, K = p->n_samples, T = p->n_templates, N = p->template_len;
     /* bp.cl:286-288: window w covers [wN, min((w+1)N, K)). */
@@ -94,6 +100,7 @@ static void likelihood_window(const fbo_params *p, const float *V, const float *
             }
             for (uint32_t r = 0; r < nvalid; r++) acc[r] = bias[t];
             for (uint32_t c = 0; c < C; c++) {
+                if (!nz[(uint64_t)t * C + c]) continue;
                 const float *vc = V + (uint64_t)c * K + start + base;
                 const float *tc = tt + (uint64_t)c * N;
                 for (uint32_t j = 0; j < N; j++) {
@@ -108,14 +115,35 @@ static void likelihood_window(const fbo_params *p, const float *V, const float *
     }
 }
 
+/* nz[t*C + c] = 1 when template t has a nonzero sample on channel c. */
+static uint8_t *channel_nonzero(const fbo_params *p, const float *Tm)
+{
+    const uint32_t C = p->n_chans, T = p->n_templates, N = p->template_len;
+    uint8_t *nz = (uint8_t *)calloc((uint64_t)T * C + 1, 1);
+    if (!nz) return NULL;
+    for (uint64_t tc = 0; tc < (uint64_t)T * C; tc++)
+        for (uint32_t j = 0; j < N; j++)
+            if (Tm[tc * N + j] != 0.0f) { nz[tc] = 1; break; }
+    return nz;
+}
+
+static void likelihood_list(const fbo_params *p, const float *V, const float *Tm, const float *bias,
+                            const uint8_t *nz, const uint32_t *windows, uint64_t n_windows, float *L)
+{
+    int64_t nw = (int64_t)n_windows;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < nw; k++) likelihood_window(p, V, Tm, bias, nz, windows[k], L);
+}
+
 /* bp.cl:255-300 (compute_full_likelihood) over a list of windows. */
 int fbo_likelihood(const fbo_params *p, const float *V, const float *Tm, const float *bias,
                    const uint32_t *windows, uint64_t n_windows, float *L)
 {
     if (!p || !V || !Tm || !bias || !L || p->template_len == 0 || p->n_chans == 0) return FBO_EINVAL;
-    int64_t nw = (int64_t)n_windows;
-#pragma omp parallel for schedule(dynamic, 64)
-    for (int64_t k = 0; k < nw; k++) likelihood_window(p, V, Tm, bias, windows[k], L);
+    uint8_t *nz = channel_nonzero(p, Tm);
+    if (!nz) return FBO_ENOMEM;
+    likelihood_list(p, V, Tm, bias, nz, windows, n_windows, L);
+    free(nz);
     return FBO_OK;
 }
 
@@ -198,8 +226,8 @@ static void argmax_window(const fbo_params *p, fbo_state *s, uint32_t id)
  * window whose best spike is (lab, bidx); bp.cl:578-632.  Returns 0 when the
  * work item is skipped (bp.cl:516-577). */
 static float overlap_score(const fbo_params *p, const float *V, const float *Tm, const float *thr,
-                           const float *L, uint32_t lab, uint32_t bidx, uint32_t t, uint32_t f,
-                           uint32_t sh)
+                           const float *L, const uint8_t *nz, uint32_t lab, uint32_t bidx, uint32_t t,
+                           uint32_t f, uint32_t sh)
 {
     const uint32_t C = p->n_chans, K = p->n_samples, N = p->template_len, M = p->max_shift;
     const float Lb = L[(uint64_t)lab * K + bidx];
@@ -220,6 +248,7 @@ static float overlap_score(const fbo_params *p, const float *V, const float *Tm,
     const uint32_t sfj = sa < 0 ? (uint32_t)(-sa) : 0;
     float acc = 0.0f;
     for (uint32_t c = 0; c < C; c++) {
+        if (!nz[(uint64_t)lab * C + c] && !nz[(uint64_t)t * C + c]) continue;
         const float *tf = Tm + ((uint64_t)lab * C + c) * N;
         const float *ts = Tm + ((uint64_t)t * C + c) * N;
         const float *vc = V + (uint64_t)c * K + bidx;
@@ -241,7 +270,7 @@ static float overlap_score(const fbo_params *p, const float *V, const float *Tm,
  * the max_local_reduction tree of bp.cl:167-186) followed by bp.cl:657-815
  * (parse_overlap_recheck_indices) for that window. */
 static void overlap_window(const fbo_params *p, fbo_state *s, const float *V, const float *Tm,
-                           const float *thr, uint32_t id, float *vals, uint32_t *ids)
+                           const float *thr, const uint8_t *nz, uint32_t id, float *vals, uint32_t *ids)
 {
     const uint32_t T = p->n_templates, M = p->max_shift, WG = p->reduce_wg, K = p->n_samples;
     const uint64_t S = 2ull * M + 1;
@@ -259,7 +288,7 @@ static void overlap_window(const fbo_params *p, fbo_state *s, const float *V, co
             const uint32_t t = (uint32_t)(q % T);
             const uint32_t f = (uint32_t)((q / T) % S);
             const uint32_t sh = (uint32_t)((q / (S * T)) % S);
-            vals[l] = overlap_score(p, V, Tm, thr, s->L, lab, bidx, t, f, sh);
+            vals[l] = overlap_score(p, V, Tm, thr, s->L, nz, lab, bidx, t, f, sh);
             ids[l] = l;
         }
         for (uint32_t na = WG >> 1; na > 0; na >>= 1)                        /* bp.cl:172-185 */
@@ -419,11 +448,15 @@ int fbo_adjusted_clips(const fbo_params *p, const float *V, const float *Tm, con
  * order (pass, then ascending window) to out_idx/out_lab (capacity cap).
  * If pass_counts is given it receives the spikes added in each pass.
  * If L_out is given it receives the final likelihood buffer [T][K].
+ * If next_windows is given and the pass cap stopped the loop, it receives the
+ * window list of the pass that would have come next (bpp.py:683-689) and
+ * *n_next its length (0 otherwise).
  * ------------------------------------------------------------------------- */
 int fbo_pursuit_chunk(const fbo_params *p, float *V, const float *Tm, const float *bias,
                       const float *thr, uint32_t *out_idx, uint32_t *out_lab, uint64_t cap,
                       uint64_t *n_out, uint32_t *n_passes, uint32_t *pass_counts,
-                      uint32_t pass_counts_cap, float *L_out)
+                      uint32_t pass_counts_cap, float *L_out, uint32_t *next_windows,
+                      uint64_t *n_next)
 {
     if (!p || !V || !Tm || !bias || !thr || !n_out) return FBO_EINVAL;
     const uint32_t C = p->n_chans, K = p->n_samples, T = p->n_templates, N = p->template_len;
@@ -445,21 +478,28 @@ int fbo_pursuit_chunk(const fbo_params *p, float *V, const float *Tm, const floa
     s.ov_list = (uint32_t *)malloc(W * 4);
     s.has_spike = (uint8_t *)calloc(W, 1);
     uint32_t *pidx = (uint32_t *)malloc(W * 4), *plab = (uint32_t *)malloc(W * 4);
+    uint8_t *nz = channel_nonzero(p, Tm);
     const uint32_t WG = p->reduce_wg;
     float *vals = NULL; uint32_t *ids = NULL;
     int rc = FBO_OK;
     if (!s.L || !s.best_L || !s.best_lab || !s.best_idx || !s.ov_idx || !s.ov_lab || !s.next_check ||
-        !s.ov || !s.snap || !s.windows || !s.ov_list || !s.has_spike || !pidx || !plab) {
+        !s.ov || !s.snap || !s.windows || !s.ov_list || !s.has_spike || !pidx || !plab || !nz) {
         rc = FBO_ENOMEM;
         goto done;
     }
+    if (n_next) *n_next = 0;
     for (uint64_t w = 0; w < W; w++) s.windows[w] = (uint32_t)w;              /* bpp.py:418 */
     uint64_t nw = W, total = 0;
     uint32_t passes = 0;
     for (;;) {
-        if (p->max_passes && passes >= p->max_passes) { rc = FBO_EPASSES; break; }
+        if (p->max_passes && passes >= p->max_passes) {
+            rc = FBO_EPASSES;
+            if (next_windows) memcpy(next_windows, s.windows, nw * 4);
+            if (n_next) *n_next = nw;
+            break;
+        }
         passes++;
-        fbo_likelihood(p, V, Tm, bias, s.windows, nw, s.L);                    /* bpp.py:538-549 */
+        likelihood_list(p, V, Tm, bias, nz, s.windows, nw, s.L);               /* bpp.py:538-549 */
         for (uint64_t k = 0; k < nw; k++) argmax_window(p, &s, s.windows[k]);  /* bpp.py:551-562 */
         uint64_t n_ov = 0;                                                     /* bpp.py:566-569 */
         for (uint64_t w = 0; w < W; w++) if (s.ov[w]) s.ov_list[n_ov++] = (uint32_t)w;
@@ -475,7 +515,8 @@ int fbo_pursuit_chunk(const fbo_params *p, float *V, const float *Tm, const floa
 #ifdef _OPENMP
                 th = omp_get_thread_num();
 #endif
-                overlap_window(p, &s, V, Tm, thr, s.ov_list[k], vals + (uint64_t)th * WG, ids + (uint64_t)th * WG);
+                overlap_window(p, &s, V, Tm, thr, nz, s.ov_list[k], vals + (uint64_t)th * WG,
+                               ids + (uint64_t)th * WG);
             }
             free(vals); free(ids); vals = NULL; ids = NULL;
             check_reassignments(p, &s, n_ov);
@@ -502,6 +543,6 @@ int fbo_pursuit_chunk(const fbo_params *p, float *V, const float *Tm, const floa
 done:
     free(s.L); free(s.best_L); free(s.best_lab); free(s.best_idx); free(s.ov_idx); free(s.ov_lab);
     free(s.next_check); free(s.ov); free(s.snap); free(s.windows); free(s.ov_list); free(s.has_spike);
-    free(pidx); free(plab); free(vals); free(ids);
+    free(pidx); free(plab); free(vals); free(ids); free(nz);
     return rc;
 }

@@ -47,7 +47,7 @@ def load():
         P = ctypes.POINTER
         f32p, u32p, u64p, f64p = P(ctypes.c_float), P(ctypes.c_uint32), P(ctypes.c_uint64), P(ctypes.c_double)
         lib.fbo_pursuit_chunk.argtypes = [P(Params), f32p, f32p, f32p, f32p, u32p, u32p, ctypes.c_uint64,
-                                          u64p, u32p, u32p, ctypes.c_uint32, f32p]
+                                          u64p, u32p, u32p, ctypes.c_uint32, f32p, u32p, u64p]
         lib.fbo_pursuit_chunk.restype = ctypes.c_int
         lib.fbo_likelihood.argtypes = [P(Params), f32p, f32p, f32p, u32p, ctypes.c_uint64, f32p]
         lib.fbo_likelihood.restype = ctypes.c_int
@@ -133,7 +133,9 @@ def pursuit_chunk(V, templates, bias, thr, N, M, do_overlap=True, reduce_wg=256,
     """Run the pass loop on one chunk.
 
     Returns dict(idx, lab (uint32, record order), residual [C][K], n_passes,
-    pass_counts, L (final likelihood buffer, if want_L)).
+    pass_counts, L (final likelihood buffer, if want_L), next_windows (the
+    window list of the next pass when max_passes stopped the loop, else
+    empty)).
     """
     V = np.array(V, dtype=np.float32, order="C", copy=True)
     C, K = V.shape
@@ -151,13 +153,15 @@ def pursuit_chunk(V, templates, bias, thr, N, M, do_overlap=True, reduce_wg=256,
         n_passes = ctypes.c_uint32(0)
         pass_counts = np.zeros(pc_cap, dtype=np.uint32)
         L = np.zeros((T, K), dtype=np.float32) if want_L else None
+        nxt = np.zeros(W, dtype=np.uint32)
+        n_next = ctypes.c_uint64(0)
         Vw = V.copy()
         p = make_params(C, K, T, N, M, do_overlap, reduce_wg, max_passes)
         rc = load().fbo_pursuit_chunk(
             ctypes.byref(p), _ptr(Vw, ctypes.c_float), _ptr(tm, ctypes.c_float), _ptr(bias, ctypes.c_float),
             _ptr(thr, ctypes.c_float), _ptr(out_idx, ctypes.c_uint32), _ptr(out_lab, ctypes.c_uint32), cap,
             ctypes.byref(n_out), ctypes.byref(n_passes), _ptr(pass_counts, ctypes.c_uint32), pc_cap,
-            _ptr(L, ctypes.c_float) if want_L else None)
+            _ptr(L, ctypes.c_float) if want_L else None, _ptr(nxt, ctypes.c_uint32), ctypes.byref(n_next))
         if rc == -3:
             cap *= 4
             continue
@@ -167,7 +171,8 @@ def pursuit_chunk(V, templates, bias, thr, N, M, do_overlap=True, reduce_wg=256,
     n = int(n_out.value)
     npass = int(n_passes.value)
     return dict(idx=out_idx[:n].copy(), lab=out_lab[:n].copy(), residual=Vw, n_passes=npass,
-                pass_counts=pass_counts[:min(npass, pc_cap)].copy(), L=L, capped=(rc == -4))
+                pass_counts=pass_counts[:min(npass, pc_cap)].copy(), L=L, capped=(rc == -4),
+                next_windows=nxt[:int(n_next.value)].copy())
 
 
 def time_window_to_samples(time_window, sampling_rate):
@@ -207,8 +212,11 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
         stop = min(n_samples, start + chunk_len)
         V = np.float32(voltage[:, start:stop])
         r = pursuit_chunk(V, templates, bias, thr, N, M, sort_info['do_overlap_recheck'], reduce_wg)
-        nxt = onsets[k + 1] if k + 1 < len(onsets) else n_samples
-        residual[:, start:nxt] = r['residual'][:, :nxt - start]
+        # chunk k supplies [onset_k + N, onset_{k+1} + N): the samples whose
+        # covering spikes all start inside chunk k
+        lo = 0 if k == 0 else min(start + N, n_samples)
+        hi = min(onsets[k + 1] + N, n_samples) if k + 1 < len(onsets) else n_samples
+        residual[:, lo:hi] = r['residual'][:, lo - start:hi - start]
         if r['idx'].size > 0:
             if sort_info['get_adjusted_clips']:
                 clips = adjusted_clips(r['residual'], templates, N, r['idx'], r['lab'])

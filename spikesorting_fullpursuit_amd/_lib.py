@@ -36,6 +36,7 @@ EXPORTS = (
     "fbp_segment_get_residual", "fbp_segment_get_likelihood", "fbp_segment_get_stats",
     "fbp_segment_voltage_device", "fbp_get_clips", "fbp_unit_mean_clips", "fbp_overlap_template_residuals",
     "fbp_clip_template_sse", "fbp_get_clips_host", "fbp_channel_covariance", "fbp_template_variances",
+    "fbp_segment_get_window_list",
 )
 
 
@@ -111,6 +112,7 @@ def _declare(lib):
     lib.fbp_segment_get_residual.argtypes = [vp, f32p]
     lib.fbp_segment_get_likelihood.argtypes = [vp, f32p]
     lib.fbp_segment_get_stats.argtypes = [vp, P(FbpStats)]
+    lib.fbp_segment_get_window_list.argtypes = [vp, u32p, ctypes.c_uint64, u64p]
     lib.fbp_segment_voltage_device.argtypes = [vp, P(vp), u64p]
     i64p = P(ctypes.c_int64)
     u8p = P(ctypes.c_uint8)
@@ -294,6 +296,16 @@ class Segment:
         out = np.zeros((self.T, self.K), dtype=np.float32)
         check(self._lib.fbp_segment_get_likelihood(self._h, _f32p(out)), "fbp_segment_get_likelihood")
         return out
+
+    def window_list(self):
+        """Windows the next pass would evaluate after a run stopped by
+        max_passes (empty after a run that converged)."""
+        n = ctypes.c_uint64(0)
+        W = -(-self.K // self.N)
+        out = np.zeros(W, dtype=np.uint32)
+        check(self._lib.fbp_segment_get_window_list(self._h, _u32p(out), W, ctypes.byref(n)),
+              "fbp_segment_get_window_list")
+        return out[:n.value].copy()
 
     def voltage_device(self):
         """(device address, row stride) of the segment's input voltage."""

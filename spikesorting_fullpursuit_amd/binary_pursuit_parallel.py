@@ -53,6 +53,23 @@ def plan_chunks(n_chans, n_samples, sampling_rate, n_templates, template_samples
     return num_indices_per_chunk, onsets
 
 
+def residual_span(chunk_onsets, k, template_samples_per_chan, n_samples):
+    """Samples [lo, hi) of the returned residual taken from chunk k.
+
+    Chunks overlap by 3N samples (binary_pursuit_parallel.py:363-367).  A
+    chunk sees only spikes whose onsets lie inside it, and a spike changes
+    the N samples from its onset on, so the first N samples of chunk k + 1
+    can still carry tails of chunk k's spikes that chunk k + 1 never
+    subtracts.  The switch from chunk k to chunk k + 1 is therefore N samples
+    after chunk k + 1's onset: every spike touching the samples a chunk
+    supplies starts inside that chunk.
+    """
+    N = int(template_samples_per_chan)
+    lo = 0 if k == 0 else min(chunk_onsets[k] + N, n_samples)
+    hi = min(chunk_onsets[k + 1] + N, n_samples) if k + 1 < len(chunk_onsets) else n_samples
+    return lo, hi
+
+
 def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shift_inds=None,
                    kernels_path=None, max_gpu_memory=None, *, return_residual=False, precision="fp32",
                    overlap_mode="direct", reduce_work_group=256, device=0, max_passes=0, stats=None,
@@ -141,8 +158,8 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
                 stats.append(seg.stats())
             idx, lab = seg.spikes()
             if return_residual:
-                nxt = chunk_onsets[k + 1] if k + 1 < len(chunk_onsets) else n_samples
-                residual[:, start_index:nxt] = seg.residual()[:, :nxt - start_index]
+                lo, hi = residual_span(chunk_onsets, k, N, n_samples)
+                residual[:, lo:hi] = seg.residual()[:, lo - start_index:hi - start_index]
             if verbose:
                 print("Found", idx.size, "secret spikes in chunk", k + 1, flush=True)
             if idx.size > 0:

@@ -65,6 +65,7 @@ struct fbp_segment {
     Counters *h_ctr = nullptr;   // pinned host copy of the device counters
     uint64_t cap = 0;            // spike capacity of sp_idx / sp_lab
     uint64_t n_spikes = 0;
+    uint64_t n_pending = 0;      // window list length of the next pass after a capped run
     uint64_t clips_cap = 0;
     bool have_templates = false, have_voltage = false, ran = false;
     fbp_stats stats{};
@@ -516,6 +517,7 @@ int fbp_segment_run(fbp_segment *g, void *stream)
     DeviceState &s = g->s;
     g->ran = false;
     g->n_spikes = 0;
+    g->n_pending = 0;
     g->stats = fbp_stats{};
     HIP_TRY(hipEventRecord(g->ev_start, st), "event");
     // Fresh state for the chunk (bpp.py:386-423); L needs no clearing because
@@ -537,7 +539,11 @@ int fbp_segment_run(fbp_segment *g, void *stream)
     uint32_t passes = 0;
     int rc = FBP_OK;
     for (;;) {
-        if (g->cfg.max_passes && passes >= g->cfg.max_passes) { rc = fail(FBP_EPASSES, "pass cap %u reached", passes); break; }
+        if (g->cfg.max_passes && passes >= g->cfg.max_passes) {
+            g->n_pending = nw;   // s.win holds the list the next pass would evaluate
+            rc = fail(FBP_EPASSES, "pass cap %u reached", passes);
+            break;
+        }
         passes++;
         if ((rc = ensure_spike_capacity(g, total + nw, total, st))) break;
         HIP_TRY(hipEventRecord(g->ev_stage[0], st), "event");
@@ -659,6 +665,20 @@ int fbp_segment_get_likelihood(fbp_segment *g, float *likelihood)
     if (!g->ran) return fail(FBP_ESTATE, "segment has not run");
     if (int rc = set_device(g)) return rc;
     HIP_TRY(hipMemcpy(likelihood, g->s.L, (uint64_t)g->d.T * g->d.K * 4, hipMemcpyDeviceToHost), "likelihood");
+    return FBP_OK;
+}
+
+int fbp_segment_get_window_list(fbp_segment *g, uint32_t *windows, uint64_t capacity, uint64_t *n)
+{
+    if (!g || !n) return fail(FBP_EINVAL, "NULL argument");
+    if (!g->ran) return fail(FBP_ESTATE, "segment has not run");
+    *n = g->n_pending;
+    if (g->n_pending == 0) return FBP_OK;
+    if (!windows || capacity < g->n_pending)
+        return fail(FBP_EINVAL, "window list of %llu entries does not fit in %llu",
+                    (unsigned long long)g->n_pending, (unsigned long long)capacity);
+    if (int rc = set_device(g)) return rc;
+    HIP_TRY(hipMemcpy(windows, g->s.win, g->n_pending * 4, hipMemcpyDeviceToHost), "window list");
     return FBP_OK;
 }
 

@@ -170,3 +170,45 @@ def test_config_validation_messages():
     cfg = _lib.FbpConfig(4, 5000, 2, 61, 30, 1, 100, 0, 0, 0)
     assert lib.fbp_segment_create(0, ctypes.byref(cfg), ctypes.byref(h)) == _lib.FBP_EINVAL
     assert b"power of two" in lib.fbp_last_error()
+
+
+def test_multichunk_residual_is_each_chunks_own_subtraction(monkeypatch, oracle_mod):
+    # Several chunks with a shorter tail: the residual the drop-in returns
+    # equals, on the samples each chunk supplies, the voltage minus exactly
+    # the templates of that chunk's spikes (float64, float32 rounding bound);
+    # the supplied spans tile the recording, and every sample a chunk
+    # supplies lies at least N samples after its onset, so no spike covering
+    # it starts before the chunk (the residual of the first N samples of a
+    # chunk lacks the tails of the previous chunk's spikes).
+    monkeypatch.setattr(_lib, "Segment", _OracleSegment)
+    monkeypatch.setattr(_lib, "device_count", lambda: 1)
+    monkeypatch.setattr(_lib, "device_memory", lambda device=0: (288 * 2 ** 30, 288 * 2 ** 30))
+    rec = small_case(24, C=4, T=5, N=61, duration_s=0.5, overlap_frac=0.2, rate=(40.0, 80.0))
+    V = rec['voltage']
+    C, K = V.shape
+    N = 61
+    budget = 0.3e6
+    chunk_len, onsets = bpp.plan_chunks(C, K, 40000, 5, N, 30, 288 * 2 ** 30, budget)
+    assert len(onsets) >= 3 and onsets[-1] + chunk_len > K   # a shorter tail chunk
+    sm = dict(rec['separability_metrics'])
+    out = bpp.binary_pursuit(V, np.float32, rec['sort_info'], sm, None, max_gpu_memory=budget,
+                             return_residual=True)
+    res = out[4]
+    tm = rec['templates'].reshape(5, C, N).astype(np.float64)
+    covered = np.zeros(K, dtype=int)
+    for k, start in enumerate(onsets):
+        stop = min(K, start + chunk_len)
+        lo, hi = bpp.residual_span(onsets, k, N, K)
+        assert start <= lo < hi <= stop
+        if k > 0:
+            assert lo >= start + N
+        covered[lo:hi] += 1
+        r = oracle_mod.pursuit_chunk(V[:, start:stop], rec['templates'], rec['bias'], rec['thr'], N, 30)
+        assert r['idx'].size > 0
+        expect = V[:, start:stop].astype(np.float64)
+        for i, l in zip(r['idx'], r['lab']):
+            n = min(N, stop - start - i)
+            expect[:, i:i + n] -= tm[l][:, :n]
+        err = np.abs(res[:, lo:hi] - expect[:, lo - start:hi - start])
+        assert err.max() < 1e-3, (k, err.max())
+    assert np.all(covered == 1)
