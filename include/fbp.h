@@ -228,6 +228,23 @@ int fbp_clip_template_sse(int device, const float *clips, uint64_t n, uint32_t w
 int fbp_channel_covariance(int device, const float *clips, uint64_t n_clips, uint32_t n_chans, uint32_t N,
                            const uint64_t *rows, uint64_t n_rows, double *cov);
 
+/* fbp_channel_covariance for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Inputs and outputs are the
+ * caller's DEVICE memory: d_clips float32 [n_clips][n_chans*N], d_rows uint64
+ * [n_chans][n_rows] (each < n_clips; a larger index is not read and makes
+ * that channel's covariance NaN) or NULL for every clip, d_cov float64
+ * [n_chans][N][N]; results are bit-identical to fbp_channel_covariance.  The
+ * call allocates nothing: the partial sums live in a fixed 32 MiB scratch
+ * array that every GPU the library runs on holds until the process ends, and
+ * requests whose partial sums exceed it (n_chans * R * N * (N + 1) doubles,
+ * R = row slices: ceil(n_rows / max(256, ceil(n_rows / 64)))) fail with
+ * FBP_EINVAL.  The kernels are enqueued on `stream` (a hipStream_t, NULL for
+ * the null stream) and the call returns without waiting.  Every call on one
+ * GPU shares its scratch array: calls that overlap on one GPU race. */
+int fbp_channel_covariance_noalloc(int device, const float *d_clips, uint64_t n_clips, uint32_t n_chans, uint32_t N,
+                                   const uint64_t *d_rows, uint64_t n_rows, double *d_cov, void *stream);
+
 /* Likelihood variance of every template, var[t] = sum_c t_c^T cov_c t_c in
  * float64 (neuron_separability.py:126-135, the input of the binary-pursuit
  * thresholds, :137-142).  templates: host float64 [T][n_chans*N]; cov: host

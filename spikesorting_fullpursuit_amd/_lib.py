@@ -36,7 +36,7 @@ EXPORTS = (
     "fbp_segment_get_residual", "fbp_segment_get_likelihood", "fbp_segment_get_stats",
     "fbp_segment_voltage_device", "fbp_get_clips", "fbp_unit_mean_clips", "fbp_overlap_template_residuals",
     "fbp_clip_template_sse", "fbp_get_clips_host", "fbp_channel_covariance", "fbp_template_variances",
-    "fbp_segment_get_window_list",
+    "fbp_segment_get_window_list", "fbp_channel_covariance_noalloc",
 )
 
 
@@ -132,6 +132,8 @@ def _declare(lib):
     f64p = P(ctypes.c_double)
     lib.fbp_channel_covariance.argtypes = [ctypes.c_int, f32p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                            u64p, ctypes.c_uint64, f64p]
+    lib.fbp_channel_covariance_noalloc.argtypes = [ctypes.c_int, vp, ctypes.c_uint64, ctypes.c_uint32,
+                                                   ctypes.c_uint32, vp, ctypes.c_uint64, vp, vp]
     lib.fbp_template_variances.argtypes = [ctypes.c_int, f64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                            f64p, f64p]
     for name in EXPORTS:
@@ -422,6 +424,17 @@ def channel_covariance(clips, n_chans, n_samples_per_chan, rows=None, device=0):
     check(lib.fbp_channel_covariance(device, _f32p(c), c.shape[0], n_chans, N, rp, n_rows, _f64p(out)),
           "fbp_channel_covariance")
     return out
+
+
+def channel_covariance_noalloc(d_clips, n_clips, n_chans, n_samples_per_chan, d_rows, n_rows, d_cov, stream=None,
+                               device=0):
+    """fbp_channel_covariance_noalloc on device addresses (ints): enqueues the
+    covariance on `stream` and returns without waiting; allocates nothing."""
+    lib = load()
+    check(lib.fbp_channel_covariance_noalloc(device, ctypes.c_void_p(d_clips), n_clips, n_chans, n_samples_per_chan,
+                                             ctypes.c_void_p(d_rows) if d_rows else None, n_rows,
+                                             ctypes.c_void_p(d_cov), ctypes.c_void_p(stream) if stream else None),
+          "fbp_channel_covariance_noalloc")
 
 
 def template_variances(templates, channel_covariance_mats, device=0):

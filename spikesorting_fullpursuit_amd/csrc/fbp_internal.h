@@ -98,6 +98,10 @@ struct DeviceState {
 
 constexpr int kCompactItems = 1024;   // items per block of the compaction kernels
 
+// Record the message of a failed C-ABI call for fbp_last_error() on this
+// thread (runtime.hip) and return `code`.
+int set_error(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+
 // Launch wrappers (kernels.hip).  All enqueue on `st` and return hipError_t.
 hipError_t launch_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
@@ -153,9 +157,9 @@ constexpr uint32_t kMaxCovSamples = 128;
 // row slices of the covariance (returns their count R, sets their length);
 // scratch: colpart [C][R][N], part [C][R][N][N] doubles
 uint32_t channel_covariance_slices(uint64_t n_rows, uint64_t *slice);
-hipError_t launch_channel_covariance(const float *clips, uint64_t width, const uint64_t *rows, uint64_t n_rows,
-                                     uint32_t C, uint32_t N, double *colpart, double *part, double *cov,
-                                     hipStream_t st);
+hipError_t launch_channel_covariance(const float *clips, uint64_t width, uint64_t n_clips, const uint64_t *rows,
+                                     uint64_t n_rows, uint32_t C, uint32_t N, double *colpart, double *part,
+                                     double *cov, hipStream_t st);
 hipError_t launch_template_variance(const double *Tm, uint32_t T, uint32_t C, uint32_t N, const double *cov,
                                     double *var, hipStream_t st);
 

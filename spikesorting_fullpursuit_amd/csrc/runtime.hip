@@ -198,6 +198,17 @@ int numpy_sum_program(uint64_t n, std::vector<uint32_t> &prog)
 
 }  // namespace
 
+int fbp::set_error(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
 extern "C" {
 
 int fbp_version(void) { return 100; }
@@ -909,7 +920,8 @@ int fbp_channel_covariance(int device, const float *clips, uint64_t n_clips, uin
     if (e == hipSuccess && rows)
         e = hipMemcpyAsync(d_rows, rows, (uint64_t)n_chans * n_rows * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = launch_channel_covariance(d_clips, width, d_rows, n_rows, n_chans, N, d_colpart, d_part, d_cov, st);
+        e = launch_channel_covariance(d_clips, width, n_clips, d_rows, n_rows, n_chans, N, d_colpart, d_part, d_cov,
+                                      st);
     if (e == hipSuccess) e = hipMemcpyAsync(cov, d_cov, ncov * 8, hipMemcpyDeviceToHost, st);
     dfree(d_clips, st);
     dfree(d_rows, st);

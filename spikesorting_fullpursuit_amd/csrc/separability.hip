@@ -7,7 +7,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
+#include "../../include/fbp.h"
 #include "fbp_internal.h"
 
 namespace fbp {
@@ -26,10 +28,13 @@ constexpr uint32_t kCovRows = 32;                           // clip rows staged 
 // products over the slice's rows, then per entry the slices' sums in slice
 // order.  Every sum has one fixed order, so repeated calls agree bit for bit;
 // entries with i > j are written from their mirror, so every matrix is
-// exactly symmetric.
+// exactly symmetric.  A row index >= n_clips (possible only when the rows
+// come from device memory the host has not checked) is not read: it turns
+// its channel's sums into NaN.
 __global__ __launch_bounds__(kMaxCovSamples) void k_cov_colsum(const float *__restrict__ clips, uint64_t width,
-                                                               const uint64_t *__restrict__ rows, uint64_t n_rows,
-                                                               uint32_t N, uint64_t slice, double *__restrict__ part)
+                                                               uint64_t n_clips, const uint64_t *__restrict__ rows,
+                                                               uint64_t n_rows, uint32_t N, uint64_t slice,
+                                                               double *__restrict__ part)
 {
     const uint32_t c = blockIdx.x, r = blockIdx.y, R = gridDim.y, j = threadIdx.x;
     if (j >= N) return;
@@ -37,12 +42,16 @@ __global__ __launch_bounds__(kMaxCovSamples) void k_cov_colsum(const float *__re
     const uint64_t b = (uint64_t)r * slice, e = b + slice < n_rows ? b + slice : n_rows;
     const float *col = clips + (uint64_t)c * N + j;
     double s = 0.0;
-    for (uint64_t k = b; k < e; k++) s += (double)col[(rc ? rc[k] : k) * width];
+    for (uint64_t k = b; k < e; k++) {
+        const uint64_t row = rc ? rc[k] : k;
+        s += row < n_clips ? (double)col[row * width] : __builtin_nan("");
+    }
     part[((uint64_t)c * R + r) * N + j] = s;
 }
 
 __global__ __launch_bounds__(kCovThreads) void k_cov_products(const float *__restrict__ clips, uint64_t width,
-                                                              const uint64_t *__restrict__ rows, uint64_t n_rows,
+                                                              uint64_t n_clips, const uint64_t *__restrict__ rows,
+                                                              uint64_t n_rows,
                                                               uint32_t N, uint64_t slice,
                                                               const double *__restrict__ colpart,
                                                               double *__restrict__ part)
@@ -68,7 +77,7 @@ __global__ __launch_bounds__(kCovThreads) void k_cov_products(const float *__res
         for (uint32_t x = threadIdx.x; x < nr * N; x += blockDim.x) {
             const uint32_t rr = x / N, j = x - rr * N;
             const uint64_t row = rc ? rc[r0 + rr] : r0 + rr;
-            xs[rr][j] = (double)clips[row * width + col0 + j] - mean[j];
+            xs[rr][j] = (row < n_clips ? (double)clips[row * width + col0 + j] : __builtin_nan("")) - mean[j];
         }
         __syncthreads();
 #pragma unroll
@@ -150,19 +159,19 @@ uint32_t channel_covariance_slices(uint64_t n_rows, uint64_t *slice)
     return (uint32_t)((n_rows + len - 1) / len);
 }
 
-hipError_t launch_channel_covariance(const float *clips, uint64_t width, const uint64_t *rows, uint64_t n_rows,
-                                     uint32_t C, uint32_t N, double *colpart, double *part, double *cov,
-                                     hipStream_t st)
+hipError_t launch_channel_covariance(const float *clips, uint64_t width, uint64_t n_clips, const uint64_t *rows,
+                                     uint64_t n_rows, uint32_t C, uint32_t N, double *colpart, double *part,
+                                     double *cov, hipStream_t st)
 {
     if (C == 0 || N == 0 || n_rows == 0) return hipSuccess;
     if (N > kMaxCovSamples) return hipErrorInvalidValue;
     uint64_t slice = 0;
     const uint32_t R = channel_covariance_slices(n_rows, &slice);
     const uint32_t chunks = (N * N + kCovChunk - 1) / kCovChunk;
-    hipLaunchKernelGGL(k_cov_colsum, dim3(C, R), dim3(kMaxCovSamples), 0, st, clips, width, rows, n_rows, N, slice,
-                       colpart);
-    hipLaunchKernelGGL(k_cov_products, dim3(C, chunks, R), dim3(kCovThreads), 0, st, clips, width, rows, n_rows, N,
-                       slice, (const double *)colpart, part);
+    hipLaunchKernelGGL(k_cov_colsum, dim3(C, R), dim3(kMaxCovSamples), 0, st, clips, width, n_clips, rows, n_rows, N,
+                       slice, colpart);
+    hipLaunchKernelGGL(k_cov_products, dim3(C, chunks, R), dim3(kCovThreads), 0, st, clips, width, n_clips, rows,
+                       n_rows, N, slice, (const double *)colpart, part);
     const uint64_t total = (uint64_t)C * N * N;
     const uint64_t blocks = (total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096;
     hipLaunchKernelGGL(k_cov_final, dim3((uint32_t)blocks), dim3(256), 0, st, C, N, R, n_rows, (const double *)part,
@@ -180,3 +189,75 @@ hipError_t launch_template_variance(const double *Tm, uint32_t T, uint32_t C, ui
 }
 
 }  // namespace fbp
+
+// ---------------------------------------------------------------------------
+// fbp_channel_covariance_noalloc: the covariance entry point for callers that
+// cannot allocate device memory (code under graph capture, serving loops).
+// Inputs and outputs are the caller's device memory; the per-slice partial
+// sums live in `scratch`, a fixed array of this library's code object that
+// every GPU holds from the moment the library is loaded on it until the
+// process ends.  Its host-side name is not a device address: the address is
+// looked up once per device (hipGetSymbolAddress) and only that is used.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint64_t kScratchBytes = 32ull << 20;   // 32 MiB per GPU
+constexpr int kMaxScratchDevices = 64;
+
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on `device` (current); looked up on the first
+// call for that device.
+hipError_t scratch_address(int device, double **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = (double *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int fbp_channel_covariance_noalloc(int device, const float *d_clips, uint64_t n_clips, uint32_t n_chans,
+                                              uint32_t N, const uint64_t *d_rows, uint64_t n_rows, double *d_cov,
+                                              void *stream)
+{
+    using namespace fbp;
+    if (!d_clips || !d_cov) return set_error(FBP_EINVAL, "NULL argument");
+    if (N == 0 || N > kMaxCovSamples)
+        return set_error(FBP_EINVAL, "samples per channel %u out of range (1..%u)", N, kMaxCovSamples);
+    if (!d_rows) n_rows = n_clips;
+    if (n_rows == 0) return set_error(FBP_EINVAL, "no clip rows to compute a covariance from");
+    if (n_chans == 0) return FBP_OK;
+    uint64_t slice = 0;
+    const uint64_t R = channel_covariance_slices(n_rows, &slice);
+    const uint64_t colpart = ((uint64_t)n_chans * R * N + 31) / 32 * 32;   // doubles, 256-byte aligned
+    const uint64_t need = (colpart + (uint64_t)n_chans * R * N * N) * sizeof(double);
+    if (need > kScratchBytes)
+        return set_error(FBP_EINVAL,
+                         "covariance of %u channels x %u samples over %llu row slices needs %llu bytes of scratch; "
+                         "the allocation-free entry point has %llu (use fbp_channel_covariance)",
+                         n_chans, N, (unsigned long long)R, (unsigned long long)need,
+                         (unsigned long long)kScratchBytes);
+    int n_dev = 0;
+    if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev <= 0) return set_error(FBP_ENODEV, "No GPU available");
+    if (device < 0 || device >= n_dev || device >= kMaxScratchDevices)
+        return set_error(FBP_EINVAL, "device %d out of range", device);
+    hipError_t e = hipSetDevice(device);
+    double *base = nullptr;
+    if (e == hipSuccess) e = scratch_address(device, &base);
+    if (e == hipSuccess)
+        e = launch_channel_covariance(d_clips, (uint64_t)n_chans * N, n_clips, d_rows, n_rows, n_chans, N, base,
+                                      base + colpart, d_cov, (hipStream_t)stream);
+    if (e != hipSuccess) return set_error(FBP_EHIP, "channel covariance (no allocation): %s", hipGetErrorString(e));
+    return FBP_OK;
+}
+

@@ -66,3 +66,50 @@ def screen_bound(rec, precision):
         inv = np.where(mx > 0, 1.0 / 2.0 ** np.floor(np.log2(224.0 / np.where(mx > 0, mx, 1.0))), 1.0)
     return (k1 * tn[:, None] * vn[None, :] + k2 * (tn[:, None] + vn[None, :] * inv[:, None]) + k3 * inv[:, None]
             + k4 * bias[:, None])
+
+
+class HipMemory:
+    """Device buffers for tests that call device-pointer entry points: the
+    HIP runtime libfbp.so itself links (libamdhip64.so.7), bound with ctypes,
+    so the buffers live in the same runtime instance as the library's."""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self):
+        import ctypes
+        from spikesorting_fullpursuit_amd import _lib
+        _lib.load()
+        self.ct = ctypes
+        self.hip = ctypes.CDLL("libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.hip.hipFree.argtypes = [ctypes.c_void_p]
+        self.hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.live = []
+
+    def _ok(self, rc, what):
+        assert rc == 0, "%s failed with HIP error %d" % (what, rc)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.ct.c_void_p()
+        self._ok(self.hip.hipMalloc(self.ct.byref(p), max(a.nbytes, 1)), "hipMalloc")
+        self.live.append(p.value)
+        self._ok(self.hip.hipMemcpy(p.value, a.ctypes.data, a.nbytes, self.H2D), "hipMemcpy")
+        return p.value
+
+    def empty(self, nbytes):
+        p = self.ct.c_void_p()
+        self._ok(self.hip.hipMalloc(self.ct.byref(p), max(nbytes, 1)), "hipMalloc")
+        self.live.append(p.value)
+        return p.value
+
+    def download(self, ptr, shape, dtype):
+        out = np.empty(shape, dtype=dtype)
+        self._ok(self.hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+        self._ok(self.hip.hipMemcpy(out.ctypes.data, ptr, out.nbytes, self.D2H), "hipMemcpy")
+        return out
+
+    def close(self):
+        for p in self.live:
+            self.hip.hipFree(p)
+        self.live = []
