@@ -209,6 +209,13 @@ int fbp_get_clips_host(int device, const float *voltage, uint64_t row_stride, ui
 int fbp_unit_mean_clips(int device, const float *clips, uint64_t n, uint32_t width, const uint32_t *labels,
                         uint32_t n_units, float *means, uint64_t *counts);
 
+/* The same mean in float64, as np.mean computes it for clips of integer or
+ * float64 voltage (full_binary_pursuit.py:368 with v_dtype int16, float64):
+ * clips host float64 [n][width] (integer clips converted exactly), rows added
+ * in input order in float64; means host float64 [n_units][width]. */
+int fbp_unit_mean_clips_f64(int device, const double *clips, uint64_t n, uint32_t width, const uint32_t *labels,
+                            uint32_t n_units, double *means, uint64_t *counts);
+
 /* Squared distance of every clip to a template, as the duplicate removal
  * after binary pursuit computes it (np.sum((clip - template) ** 2) on float32
  * clips, analyze_spike_timing.py:129-130, with numpy's pairwise summation

@@ -36,7 +36,7 @@ EXPORTS = (
     "fbp_segment_get_residual", "fbp_segment_get_likelihood", "fbp_segment_get_stats",
     "fbp_segment_voltage_device", "fbp_get_clips", "fbp_unit_mean_clips", "fbp_overlap_template_residuals",
     "fbp_clip_template_sse", "fbp_get_clips_host", "fbp_channel_covariance", "fbp_template_variances",
-    "fbp_segment_get_window_list", "fbp_channel_covariance_noalloc",
+    "fbp_segment_get_window_list", "fbp_channel_covariance_noalloc", "fbp_unit_mean_clips_f64",
 )
 
 
@@ -124,6 +124,8 @@ def _declare(lib):
                                        ctypes.c_int32, ctypes.c_int, f32p, u8p, u64p]
     lib.fbp_unit_mean_clips.argtypes = [ctypes.c_int, f32p, ctypes.c_uint64, ctypes.c_uint32, u32p,
                                         ctypes.c_uint32, f32p, u64p]
+    lib.fbp_unit_mean_clips_f64.argtypes = [ctypes.c_int, P(ctypes.c_double), ctypes.c_uint64, ctypes.c_uint32,
+                                            u32p, ctypes.c_uint32, P(ctypes.c_double), u64p]
     lib.fbp_clip_template_sse.argtypes = [ctypes.c_int, f32p, ctypes.c_uint64, ctypes.c_uint32, u32p, f32p,
                                           ctypes.c_uint32, f32p]
     lib.fbp_overlap_template_residuals.argtypes = [ctypes.c_int, f32p, ctypes.c_uint32, ctypes.c_uint32,
@@ -360,18 +362,29 @@ def get_clips_host(voltage, n_samples, event_indices, channels, w0, w1, check_va
 
 
 def unit_mean_clips(clips, labels, n_units, device=0):
-    """fbp_unit_mean_clips: (means [n_units][width] float32, counts [n_units])."""
+    """Mean clip of every unit, as np.mean(clips[labels == u], axis=0) gives it:
+    float32 clips are averaged in float32 (fbp_unit_mean_clips), clips of any
+    other real dtype in float64 (fbp_unit_mean_clips_f64), numpy's result type
+    for integer and float64 input.  Returns (means [n_units][width], counts)."""
     lib = load()
-    c = np.ascontiguousarray(clips, dtype=np.float32)
+    c = np.asarray(clips)
     if c.ndim != 2:
         raise ValueError("clips must be two-dimensional")
     lab = np.ascontiguousarray(labels, dtype=np.uint32)
     if lab.size != c.shape[0]:
         raise ValueError("one label per clip is required")
-    means = np.empty((n_units, c.shape[1]), dtype=np.float32)
     counts = np.zeros(n_units, dtype=np.uint64)
-    check(lib.fbp_unit_mean_clips(device, _f32p(c), c.shape[0], c.shape[1], _u32p(lab), n_units, _f32p(means),
-                                  counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))), "fbp_unit_mean_clips")
+    cp = counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    if c.dtype == np.float32:
+        c = np.ascontiguousarray(c)
+        means = np.empty((n_units, c.shape[1]), dtype=np.float32)
+        check(lib.fbp_unit_mean_clips(device, _f32p(c), c.shape[0], c.shape[1], _u32p(lab), n_units, _f32p(means),
+                                      cp), "fbp_unit_mean_clips")
+    else:
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        means = np.empty((n_units, c.shape[1]), dtype=np.float64)
+        check(lib.fbp_unit_mean_clips_f64(device, _f64p(c), c.shape[0], c.shape[1], _u32p(lab), n_units,
+                                          _f64p(means), cp), "fbp_unit_mean_clips_f64")
     return means, counts
 
 

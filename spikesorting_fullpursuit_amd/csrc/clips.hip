@@ -40,30 +40,31 @@ __global__ __launch_bounds__(256) void k_get_clips(const float *__restrict__ V, 
 // rows r0, r1, ... of unit u in input order (rows[off[u] .. off[u + 1])): the
 // order numpy's axis-0 float32 mean adds them.  One thread per (unit,
 // column); the row loads of eight steps are issued before their adds.
-__global__ __launch_bounds__(256) void k_unit_means(const float *__restrict__ clips, uint32_t width,
+template <class F>
+__global__ __launch_bounds__(256) void k_unit_means(const F *__restrict__ clips, uint32_t width,
                                                     const uint32_t *__restrict__ rows,
                                                     const uint64_t *__restrict__ off, uint32_t n_units,
-                                                    float *__restrict__ means)
+                                                    F *__restrict__ means)
 {
     const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (g >= (uint64_t)n_units * width) return;
     const uint32_t u = (uint32_t)(g / width), x = (uint32_t)(g - (uint64_t)u * width);
     const uint64_t b = off[u], e = off[u + 1];
     if (b == e) {
-        means[g] = __builtin_nanf("");   // numpy: mean of no rows
+        means[g] = (F)__builtin_nan("");   // numpy: mean of no rows
         return;
     }
-    float acc = clips[(uint64_t)rows[b] * width + x];
+    F acc = clips[(uint64_t)rows[b] * width + x];
     uint64_t r = b + 1;
     for (; r + 8 <= e; r += 8) {
-        float v[8];
+        F v[8];
 #pragma unroll
         for (int q = 0; q < 8; q++) v[q] = clips[(uint64_t)rows[r + q] * width + x];
 #pragma unroll
         for (int q = 0; q < 8; q++) acc = acc + v[q];
     }
     for (; r < e; r++) acc = acc + clips[(uint64_t)rows[r] * width + x];
-    means[g] = acc / (float)(e - b);
+    means[g] = acc / (F)(e - b);
 }
 
 // sse[k] = np.sum((clips[k] - templates[tidx[k]]) ** 2) in float32 with
@@ -134,14 +135,20 @@ hipError_t launch_get_clips(const float *V, uint64_t row_stride, const int64_t *
     return hipGetLastError();
 }
 
-hipError_t launch_unit_means(const float *clips, uint32_t width, const uint32_t *rows, const uint64_t *off,
-                             uint32_t n_units, float *means, hipStream_t st)
+template <class F>
+hipError_t launch_unit_means(const F *clips, uint32_t width, const uint32_t *rows, const uint64_t *off,
+                             uint32_t n_units, F *means, hipStream_t st)
 {
     const uint64_t total = (uint64_t)n_units * width;
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_unit_means, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, clips, width, rows,
+    hipLaunchKernelGGL(k_unit_means<F>, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, clips, width, rows,
                        off, n_units, means);
     return hipGetLastError();
 }
+
+template hipError_t launch_unit_means<float>(const float *, uint32_t, const uint32_t *, const uint64_t *, uint32_t,
+                                             float *, hipStream_t);
+template hipError_t launch_unit_means<double>(const double *, uint32_t, const uint32_t *, const uint64_t *,
+                                              uint32_t, double *, hipStream_t);
 
 }  // namespace fbp

@@ -145,8 +145,9 @@ hipError_t launch_get_clips(const float *V, uint64_t row_stride, const int64_t *
                             hipStream_t st);
 hipError_t launch_clip_sse(const float *clips, uint64_t n, uint32_t width, const uint32_t *tidx,
                            const float *templates, const uint32_t *prog, uint32_t n_prog, float *sse, hipStream_t st);
-hipError_t launch_unit_means(const float *clips, uint32_t width, const uint32_t *rows, const uint64_t *off,
-                             uint32_t n_units, float *means, hipStream_t st);
+template <class F>
+hipError_t launch_unit_means(const F *clips, uint32_t width, const uint32_t *rows, const uint64_t *off,
+                             uint32_t n_units, F *means, hipStream_t st);
 hipError_t launch_overlap_template_residuals(const float *Tm, uint32_t C, uint32_t N, int32_t n_pre, uint32_t S,
                                              const uint32_t *triples, uint64_t n_triples, const uint32_t *prog,
                                              uint32_t n_prog, double *best_res, uint32_t *best_combo,

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -79,16 +80,18 @@ namespace {
 // the segment's own stream: creating and destroying segments while others run
 // (the serving path) then never synchronises the device, and freed blocks are
 // reused by the next segment instead of going back to the driver.
+// Once per device and process, whichever thread creates the first segment.
 void keep_pool_memory(int device)
 {
-    static bool done[64] = {false};
-    if (device < 0 || device >= 64 || done[device]) return;
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
-        uint64_t keep = ~0ull;
-        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-    }
-    done[device] = true;
+    static std::once_flag once[64];
+    if (device < 0 || device >= 64) return;
+    std::call_once(once[device], [device] {
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t keep = ~0ull;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+    });
 }
 
 template <class T>
@@ -964,8 +967,11 @@ int fbp_template_variances(int device, const double *templates, uint32_t n_templ
     return FBP_OK;
 }
 
-int fbp_unit_mean_clips(int device, const float *clips, uint64_t n, uint32_t width, const uint32_t *labels,
-                        uint32_t n_units, float *means, uint64_t *counts)
+}  // extern "C"
+
+template <class F>
+static int unit_mean_clips_impl(int device, const F *clips, uint64_t n, uint32_t width, const uint32_t *labels,
+                                uint32_t n_units, F *means, uint64_t *counts)
 {
     if ((n && (!clips || !labels)) || (n_units && width && !means)) return fail(FBP_EINVAL, "NULL argument");
     // rows of each unit in input order (counting sort)
@@ -989,19 +995,19 @@ int fbp_unit_mean_clips(int device, const float *clips, uint64_t n, uint32_t wid
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
     hipStream_t st = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
-    float *d_clips = nullptr, *d_means = nullptr;
+    F *d_clips = nullptr, *d_means = nullptr;
     uint32_t *d_rows = nullptr;
     uint64_t *d_off = nullptr;
     hipError_t e = dalloc(&d_clips, n * width, st);
     if (e == hipSuccess) e = dalloc(&d_rows, n, st);
     if (e == hipSuccess) e = dalloc(&d_off, (uint64_t)n_units + 1, st);
     if (e == hipSuccess) e = dalloc(&d_means, (uint64_t)n_units * width, st);
-    if (e == hipSuccess && n) e = hipMemcpyAsync(d_clips, clips, n * width * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && n) e = hipMemcpyAsync(d_clips, clips, n * width * sizeof(F), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && n) e = hipMemcpyAsync(d_rows, rows.data(), n * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), ((uint64_t)n_units + 1) * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_unit_means(d_clips, width, d_rows, d_off, n_units, d_means, st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(means, d_means, (uint64_t)n_units * width * 4, hipMemcpyDeviceToHost, st);
+        e = hipMemcpyAsync(means, d_means, (uint64_t)n_units * width * sizeof(F), hipMemcpyDeviceToHost, st);
     dfree(d_clips, st);
     dfree(d_rows, st);
     dfree(d_off, st);
@@ -1011,6 +1017,20 @@ int fbp_unit_mean_clips(int device, const float *clips, uint64_t n, uint32_t wid
     if (e == hipSuccess) e = es;
     HIP_TRY(e, "unit means");
     return FBP_OK;
+}
+
+extern "C" {
+
+int fbp_unit_mean_clips(int device, const float *clips, uint64_t n, uint32_t width, const uint32_t *labels,
+                        uint32_t n_units, float *means, uint64_t *counts)
+{
+    return unit_mean_clips_impl(device, clips, n, width, labels, n_units, means, counts);
+}
+
+int fbp_unit_mean_clips_f64(int device, const double *clips, uint64_t n, uint32_t width, const uint32_t *labels,
+                            uint32_t n_units, double *means, uint64_t *counts)
+{
+    return unit_mean_clips_impl(device, clips, n, width, labels, n_units, means, counts);
 }
 
 }  // extern "C"

@@ -20,62 +20,73 @@ from .segment import get_clips
 
 
 def assign_units_to_channels(neuron_labels, unit_means, n_channels, n_samples_per_chan, thresholds, sigma):
-    """Channel of every unit (full_binary_pursuit.py:367-383).
+    """Channel of every unit (the rule of full_binary_pursuit.py:367-383).
 
-    unit_means[u] is the mean clip of unit u over all channels; the unit goes
-    to the first channel with the largest (max - min) / (3 * thresholds[chan]
-    / sigma) of its mean clip on that channel.  Returns {channel: [units]}.
+    Each unit present in `neuron_labels` (ascending) is placed on the channel
+    where its mean clip (unit_means[unit], channel blocks of
+    n_samples_per_chan) has the largest peak-to-peak range relative to three
+    noise standard deviations, noise = thresholds[channel] / sigma; ties go to
+    the lowest channel, and ratios that are not above -1 (NaN) never win.
+    Returns {channel: [units in ascending order]} with every channel present.
     """
-    chans_to_template_labels = {chan: [] for chan in range(n_channels)}
+    noise = np.asarray([thresholds[c] / sigma for c in range(n_channels)], dtype=np.float64)
+    units_of = {c: [] for c in range(n_channels)}
     for unit in np.unique(neuron_labels):
-        curr_template = unit_means[unit]
-        unit_best_snr, unit_best_chan = -1.0, None
-        for chan in range(n_channels):
-            background_noise_std = thresholds[chan] / sigma
-            chan_template = curr_template[n_samples_per_chan * chan:n_samples_per_chan * (chan + 1)]
-            chan_snr = (np.amax(chan_template) - np.amin(chan_template)) / (3 * background_noise_std)
-            if chan_snr > unit_best_snr:
-                unit_best_snr, unit_best_chan = chan_snr, chan
-        chans_to_template_labels[unit_best_chan].append(unit)
-    return chans_to_template_labels
+        blocks = np.asarray(unit_means[unit])[:n_channels * n_samples_per_chan].reshape(n_channels,
+                                                                                      n_samples_per_chan)
+        spread = blocks.max(axis=1) - blocks.min(axis=1)   # in the clips' dtype, per channel
+        ratio = np.asarray([spread[c] / (3 * noise[c]) for c in range(n_channels)])
+        eligible = np.flatnonzero(ratio > -1.0)
+        if eligible.size == 0:
+            raise KeyError("unit %d has no channel with a finite range-to-noise ratio" % unit)
+        best = eligible[np.argmax(ratio[eligible])]   # first of the largest
+        units_of[int(best)].append(unit)
+    return units_of
+
+
+def _work_item_of(seg_w_items, channel):
+    for item in seg_w_items:
+        if item['channel'] == channel:
+            return item
+    raise RuntimeError("no work item of this segment covers channel %d" % channel)
+
+
+def _neighbourhood_clips(clips, rows, neighbors, starts, stops, width, v_dtype):
+    """Clips of the selected rows mapped onto a work item's neighbourhood:
+    block k of each row is columns starts[neighbors[k]]:stops[neighbors[k]]."""
+    out = np.zeros((rows.size, len(neighbors) * width), dtype=v_dtype)
+    for k, chan in enumerate(neighbors):
+        out[:, k * width:(k + 1) * width] = clips[rows, starts[chan]:stops[chan]]
+    return out
 
 
 def segment_output(crossings, neuron_labels, bp_bool, clips, chans_to_template_labels, seg_w_items, n_channels,
                    n_samples_per_chan, original_clip_starts, original_clip_stops, v_dtype, output_clips=False):
-    """Per-channel output (full_binary_pursuit.py:385-438): for each channel,
-    [events + work item onset, labels, clips mapped to the work item's
-    neighbourhood (or []), bp_bool, work item ID], units in the order they were
-    assigned; empty lists for a channel without units."""
-    seg_data = []
+    """Sorter output of one segment, one entry per channel (the layout of
+    full_binary_pursuit.py:385-438): [event indices shifted by the work
+    item's window onset, labels, clips (mapped to the work item's
+    neighbourhood with output_clips, else an empty column per unit), binary
+    pursuit flags, work item ID], the channel's units in assignment order, or
+    [[], [], [], [], ID] for a channel without units."""
+    result = []
     for chan in range(n_channels):
-        curr_item = next((w for w in seg_w_items if w['channel'] == chan), None)
-        if curr_item is None:
-            raise RuntimeError("Could not find a matching work item for unit")
-        if len(chans_to_template_labels[chan]) == 0:
-            seg_data.append([[], [], [], [], curr_item['ID']])
+        item = _work_item_of(seg_w_items, chan)
+        units = chans_to_template_labels[chan]
+        if not units:
+            result.append([[], [], [], [], item['ID']])
             continue
-        chan_events, chan_labels, chan_bp_bool, chan_clips = [], [], [], []
-        for unit in chans_to_template_labels[chan]:
-            select = neuron_labels == unit
-            chan_events.append(crossings[select])
-            chan_labels.append(neuron_labels[select])
-            chan_bp_bool.append(bp_bool[select])
-            if output_clips:
-                neighbors = curr_item['neighbors']
-                unit_clips = np.zeros((np.count_nonzero(select), neighbors.shape[0] * n_samples_per_chan),
-                                      dtype=v_dtype)
-                for neigh in range(neighbors.shape[0]):
-                    chan_ind = neighbors[neigh]
-                    unit_clips[:, neigh * n_samples_per_chan:(neigh + 1) * n_samples_per_chan] = \
-                        clips[select, original_clip_starts[chan_ind]:original_clip_stops[chan_ind]]
-                chan_clips.append(unit_clips)
-            else:
-                chan_clips.append([])
-        chan_events = np.hstack(chan_events)
-        chan_events += curr_item['index_window'][0]
-        seg_data.append([chan_events, np.hstack(chan_labels), np.vstack(chan_clips), np.hstack(chan_bp_bool),
-                         curr_item['ID']])
-    return seg_data
+        rows = [np.flatnonzero(neuron_labels == unit) for unit in units]
+        events = np.concatenate([crossings[r] for r in rows]) + item['index_window'][0]
+        labels = np.concatenate([neuron_labels[r] for r in rows])
+        flags = np.concatenate([bp_bool[r] for r in rows])
+        if output_clips:
+            clip_rows = np.vstack([_neighbourhood_clips(clips, r, item['neighbors'], original_clip_starts,
+                                                        original_clip_stops, n_samples_per_chan, v_dtype)
+                                   for r in rows])
+        else:
+            clip_rows = np.zeros((len(units), 0))
+        result.append([events, labels, clip_rows, flags, item['ID']])
+    return result
 
 
 def binary_pursuit_segment(voltage, v_dtype, sort_info, separability_metrics, seg_w_items,
@@ -101,8 +112,7 @@ def binary_pursuit_segment(voltage, v_dtype, sort_info, separability_metrics, se
         sort_info['separability_metrics'][seg_w_items[0]['seg_number']] = separability_metrics
     if len(crossings) > 0:
         n_units = int(np.amax(neuron_labels)) + 1
-        unit_means, _ = _lib.unit_mean_clips(np.asarray(clips, dtype=np.float32), neuron_labels, n_units,
-                                             device=device)
+        unit_means, _ = _lib.unit_mean_clips(clips, neuron_labels, n_units, device=device)
     else:
         unit_means = None
     chans_to_template_labels = assign_units_to_channels(

@@ -62,6 +62,22 @@ def test_unit_means_bitwise(oracle_mod):
     np.testing.assert_array_equal(counts, np.bincount(labels, minlength=units))
 
 
+@pytest.mark.parametrize("dtype", [np.int16, np.float64])
+def test_unit_means_float64_for_other_dtypes(dtype):
+    # np.mean of int16 or float64 clips accumulates in float64, row after row
+    rng = np.random.default_rng(6)
+    n, width, units = 5000, 3 * 41, 4
+    clips = (rng.standard_normal((n, width)) * 1000.0).astype(dtype)
+    labels = rng.integers(0, units, n)
+    print("case unit means", np.dtype(dtype).name, flush=True)
+    got, counts = _lib.unit_mean_clips(clips, labels, units)
+    assert got.dtype == np.float64
+    for u in range(units):
+        ref = np.mean(clips[labels == u, :], axis=0)
+        assert ref.dtype == np.float64
+        np.testing.assert_array_equal(got[u].view(np.uint64), ref.view(np.uint64))
+
+
 def test_resident_segment_clips(oracle_mod):
     rec = small_case(61, C=4, T=5, N=61, duration_s=0.3)
     V = rec['voltage']
