@@ -42,7 +42,8 @@ enum {
     FBP_ENODEV = -3,   /* no usable GPU (bpp.py:229-230 RuntimeError)          */
     FBP_EHIP = -4,     /* HIP runtime error                                    */
     FBP_EPASSES = -5,  /* pass cap reached before convergence                   */
-    FBP_ESTATE = -6    /* call out of order (e.g. results before run)           */
+    FBP_ESTATE = -6,   /* call out of order (e.g. results before run)           */
+    FBP_ETIMEDOUT = -7 /* fbp_server_wait: the job has not ended in time        */
 };
 
 /* Arithmetic of the likelihood correlation (binary_pursuit.cl:216-300). */
@@ -114,6 +115,46 @@ typedef struct fbp_stats {
 
 typedef struct fbp_segment fbp_segment;
 
+/* States of a job of the segment server (fbp_server_*). */
+enum {
+    FBP_JOB_QUEUED = 0,    /* submitted, waiting for a slot and HBM budget, or
+                              being created and uploaded                        */
+    FBP_JOB_RUNNING = 1,   /* its passes are being driven                       */
+    FBP_JOB_DONE = 2,      /* results readable through fbp_server_segment       */
+    FBP_JOB_FAILED = 3,
+    FBP_JOB_CANCELLED = 4
+};
+
+typedef struct fbp_server_stats {
+    uint64_t submitted, completed, failed, cancelled;
+    uint32_t queued;            /* waiting for admission                        */
+    uint32_t running;           /* being created or driven                      */
+    uint32_t peak_running;
+    uint32_t reserved;
+    uint64_t held_bytes;        /* device bytes the admitted jobs count for     */
+    uint64_t peak_held_bytes;
+    uint64_t budget_bytes;
+    uint64_t pass_steps;        /* passes consumed by the driver thread         */
+    double   driver_busy_s;     /* driver thread time spent reading counters and
+                                   enqueueing passes                            */
+    double   uptime_s;
+} fbp_server_stats;
+
+typedef struct fbp_job_info {
+    int32_t  state;             /* FBP_JOB_*                                    */
+    int32_t  rc;                /* run result: FBP_OK, FBP_EPASSES (capped,
+                                   results readable) or the failure's code      */
+    uint64_t reserve_bytes;     /* bytes reserved at admission (grows to what
+                                   the job held when that was more)             */
+    uint64_t bytes;             /* device bytes the job holds now               */
+    uint32_t passes;
+    uint32_t reserved;
+    double   t_submit, t_admit, t_start, t_finish;   /* seconds since the server
+                                   was created; -1 until reached                */
+} fbp_job_info;
+
+typedef struct fbp_server fbp_server;
+
 /* Library version (major*10000 + minor*100 + patch). No reference counterpart. */
 int fbp_version(void);
 
@@ -155,6 +196,11 @@ int fbp_segment_set_voltage_device(fbp_segment *seg, const float *d_voltage, uin
  * turned into the residual in place.  `stream` is a hipStream_t or NULL for
  * the segment's own stream.  Blocks until done. */
 int fbp_segment_run(fbp_segment *seg, void *stream);
+
+/* Device bytes the segment holds now: its buffers, the spike list (which
+ * grows while it runs) and clips once fetched.  Right after
+ * fbp_segment_create it equals fbp_segment_bytes.  No reference counterpart. */
+int fbp_segment_device_bytes(const fbp_segment *seg, uint64_t *bytes);
 
 /* Results of the last run. */
 int fbp_segment_num_spikes(const fbp_segment *seg, uint64_t *n);
@@ -274,6 +320,61 @@ int fbp_overlap_template_residuals(int device, const float *templates, uint32_t 
                                    uint32_t n_samples_per_chan, int32_t n_pre, int32_t n_post,
                                    const uint32_t *triples, uint64_t n_triples, double *best_residual,
                                    uint32_t *best_combo);
+
+/* Segment server: many segments sorted concurrently on one device.
+ * Replaces the serial per-segment loop of the reference's sorter
+ * (parallel/spikesorting_parallel.py:1312-1322 calls binary pursuit for one
+ * segment at a time, each a blocking call).  Jobs are admitted in submission
+ * order when fewer than max_running are staged or running and their
+ * reservation (fbp_segment_bytes of cfg plus extra_bytes, e.g. room for
+ * clips) fits in hbm_budget next to what the admitted jobs hold (each job
+ * counts for its reservation or its current bytes, whichever is larger; a
+ * finished job counts until it is released).  One driver thread advances
+ * every running job pass by pass on the job's own stream; no thread blocks on
+ * a pass.  Results are bitwise those of fbp_segment_run on the same input.
+ * hbm_budget 0: 85 % of the device's free memory at creation. */
+int fbp_server_create(int device, uint64_t hbm_budget, uint32_t max_running, fbp_server **out);
+/* Cancels queued and running jobs, frees every job and stops the threads. */
+void fbp_server_destroy(fbp_server *srv);
+
+/* Queue a segment.  templates/bias/thresholds/voltage as for
+ * fbp_segment_set_templates and fbp_segment_set_voltage; they are read while
+ * the job is staged, so they must stay valid and unchanged until its state is
+ * no longer FBP_JOB_QUEUED.  FBP_ENOMEM when the reservation exceeds the
+ * whole budget; FBP_EINVAL for a bad cfg. */
+int fbp_server_submit(fbp_server *srv, const fbp_config *cfg, const float *templates, const float *bias,
+                      const float *thresholds, const float *voltage, uint64_t row_stride, uint64_t extra_bytes,
+                      uint64_t *ticket);
+
+/* Current state of a job.  A failed job's message becomes this thread's
+ * fbp_last_error(). */
+int fbp_server_poll(fbp_server *srv, uint64_t ticket, fbp_job_info *info);
+
+/* Wait until the job has ended or timeout_us has passed (< 0: no limit);
+ * FBP_ETIMEDOUT when it has not ended.  info may be NULL. */
+int fbp_server_wait(fbp_server *srv, uint64_t ticket, int64_t timeout_us, fbp_job_info *info);
+
+/* Wait until some job has ended that this call has not reported yet, or
+ * timeout_us has passed (< 0: no limit): *ticket is that job (each ended job
+ * is reported once, in the order jobs ended).  FBP_ETIMEDOUT when none has.
+ * A finished job holds its device memory until it is released, so a consumer
+ * that takes results in completion order never stalls admission. */
+int fbp_server_wait_any(fbp_server *srv, int64_t timeout_us, uint64_t *ticket, fbp_job_info *info);
+
+/* Cancel a job: a queued one at once, a running one at its next pass
+ * boundary (fbp_server_wait then reports FBP_JOB_CANCELLED).  FBP_ESTATE when
+ * it has already ended. */
+int fbp_server_cancel(fbp_server *srv, uint64_t ticket);
+
+/* The finished job's segment, read with fbp_segment_num_spikes /
+ * get_spikes / get_clips / get_residual / get_stats from one thread at a
+ * time; valid until fbp_server_release.  FBP_ESTATE unless FBP_JOB_DONE. */
+int fbp_server_segment(fbp_server *srv, uint64_t ticket, fbp_segment **seg);
+
+/* Forget an ended job and free its device memory. */
+int fbp_server_release(fbp_server *srv, uint64_t ticket);
+
+int fbp_server_get_stats(fbp_server *srv, fbp_server_stats *stats);
 
 #ifdef __cplusplus
 }

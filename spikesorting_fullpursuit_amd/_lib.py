@@ -23,6 +23,9 @@ FBP_ENODEV = -3
 FBP_EHIP = -4
 FBP_EPASSES = -5
 FBP_ESTATE = -6
+FBP_ETIMEDOUT = -7
+
+JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "cancelled"}
 
 PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2, "fp8": 3}
 OVERLAP_MODES = {"direct": 0, "screened": 1}
@@ -37,6 +40,9 @@ EXPORTS = (
     "fbp_segment_voltage_device", "fbp_get_clips", "fbp_unit_mean_clips", "fbp_overlap_template_residuals",
     "fbp_clip_template_sse", "fbp_get_clips_host", "fbp_channel_covariance", "fbp_template_variances",
     "fbp_segment_get_window_list", "fbp_channel_covariance_noalloc", "fbp_unit_mean_clips_f64",
+    "fbp_segment_device_bytes", "fbp_server_create", "fbp_server_destroy", "fbp_server_submit", "fbp_server_poll",
+    "fbp_server_wait", "fbp_server_wait_any", "fbp_server_cancel", "fbp_server_segment", "fbp_server_release",
+    "fbp_server_get_stats",
 )
 
 
@@ -76,6 +82,48 @@ class FbpStats(ctypes.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class FbpServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("failed", ctypes.c_uint64),
+        ("cancelled", ctypes.c_uint64),
+        ("queued", ctypes.c_uint32),
+        ("running", ctypes.c_uint32),
+        ("peak_running", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("held_bytes", ctypes.c_uint64),
+        ("peak_held_bytes", ctypes.c_uint64),
+        ("budget_bytes", ctypes.c_uint64),
+        ("pass_steps", ctypes.c_uint64),
+        ("driver_busy_s", ctypes.c_double),
+        ("uptime_s", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class FbpJobInfo(ctypes.Structure):
+    _fields_ = [
+        ("state", ctypes.c_int32),
+        ("rc", ctypes.c_int32),
+        ("reserve_bytes", ctypes.c_uint64),
+        ("bytes", ctypes.c_uint64),
+        ("passes", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("t_submit", ctypes.c_double),
+        ("t_admit", ctypes.c_double),
+        ("t_start", ctypes.c_double),
+        ("t_finish", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        d["state"] = JOB_STATES.get(self.state, str(self.state))
+        return d
 
 
 class FbpError(RuntimeError):
@@ -138,8 +186,21 @@ def _declare(lib):
                                                    ctypes.c_uint32, vp, ctypes.c_uint64, vp, vp]
     lib.fbp_template_variances.argtypes = [ctypes.c_int, f64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                            f64p, f64p]
+    lib.fbp_segment_device_bytes.argtypes = [vp, u64p]
+    lib.fbp_server_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, P(vp)]
+    lib.fbp_server_destroy.argtypes = [vp]
+    lib.fbp_server_destroy.restype = None
+    lib.fbp_server_submit.argtypes = [vp, P(FbpConfig), f32p, f32p, f32p, f32p, ctypes.c_uint64, ctypes.c_uint64,
+                                      u64p]
+    lib.fbp_server_poll.argtypes = [vp, ctypes.c_uint64, P(FbpJobInfo)]
+    lib.fbp_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_int64, P(FbpJobInfo)]
+    lib.fbp_server_wait_any.argtypes = [vp, ctypes.c_int64, u64p, P(FbpJobInfo)]
+    lib.fbp_server_cancel.argtypes = [vp, ctypes.c_uint64]
+    lib.fbp_server_segment.argtypes = [vp, ctypes.c_uint64, P(vp)]
+    lib.fbp_server_release.argtypes = [vp, ctypes.c_uint64]
+    lib.fbp_server_get_stats.argtypes = [vp, P(FbpServerStats)]
     for name in EXPORTS:
-        if name not in ("fbp_last_error", "fbp_segment_destroy"):
+        if name not in ("fbp_last_error", "fbp_segment_destroy", "fbp_server_destroy"):
             getattr(lib, name).restype = ctypes.c_int
 
 
@@ -167,6 +228,8 @@ def check(rc, what=""):
         raise ValueError(msg or what)
     if rc == FBP_ENOMEM:
         raise MemoryError(msg or what)
+    if rc == FBP_ETIMEDOUT:
+        raise TimeoutError(msg or what)
     raise FbpError(rc, "%s failed (%d): %s" % (what, rc, msg))
 
 
@@ -225,10 +288,29 @@ class Segment:
         self.C, self.K, self.T, self.N = n_chans, n_samples, n_templates, template_len
         self.device = device
 
+    @classmethod
+    def _view(cls, handle, cfg, device):
+        """A non-owning Segment over a handle owned by someone else (a
+        server's finished job): close() does not destroy it."""
+        self = cls.__new__(cls)
+        self._lib = load()
+        self.cfg = cfg
+        self._h = handle
+        self._owned = False
+        self.C, self.K, self.T, self.N = cfg.n_chans, cfg.n_samples, cfg.n_templates, cfg.template_len
+        self.device = device
+        return self
+
     def close(self):
-        if self._h is not None and self._h.value:
+        if self._h is not None and self._h.value and getattr(self, "_owned", True):
             self._lib.fbp_segment_destroy(self._h)
         self._h = None
+
+    def device_bytes(self):
+        """Device bytes the segment holds now (buffers, spike list, clips)."""
+        b = ctypes.c_uint64(0)
+        check(self._lib.fbp_segment_device_bytes(self._h, ctypes.byref(b)), "fbp_segment_device_bytes")
+        return b.value
 
     def __del__(self):
         try:
@@ -462,3 +544,124 @@ def template_variances(templates, channel_covariance_mats, device=0):
     check(lib.fbp_template_variances(device, _f64p(tm), tm.shape[0], n_chans, N, _f64p(cov), _f64p(out)),
           "fbp_template_variances")
     return out
+
+
+class Server:
+    """The segment server of libfbp (fbp_server_*): jobs admitted by slot and
+    HBM budget, every running job advanced pass by pass by one native driver
+    thread.  submit() keeps the host arrays alive until the job is released."""
+
+    def __init__(self, device=0, hbm_budget=0, max_running=8):
+        self._lib = load()
+        self.device = device
+        self._h = ctypes.c_void_p(None)
+        check(self._lib.fbp_server_create(device, int(hbm_budget), int(max_running), ctypes.byref(self._h)),
+              "fbp_server_create")
+        self._inputs = {}
+        self._cfgs = {}
+        self._lock = threading.Lock()
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            self._lib.fbp_server_destroy(self._h)
+        self._h = None
+        with self._lock:
+            self._inputs.clear()
+            self._cfgs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, voltage, templates, bias, thresholds, template_len, max_shift, do_overlap=True, reduce_wg=256,
+               precision="bf16", overlap_mode="screened", max_passes=0, extra_bytes=0):
+        """Queue one segment; returns its ticket."""
+        v = voltage
+        if v.dtype != np.float32 or v.ndim != 2 or v.strides[1] != 4:
+            v = np.ascontiguousarray(v, dtype=np.float32)
+        C, K = v.shape
+        tm = np.ascontiguousarray(templates, dtype=np.float32)
+        b = np.ascontiguousarray(bias, dtype=np.float32)
+        th = np.ascontiguousarray(thresholds, dtype=np.float32)
+        T = b.size
+        if tm.size != T * C * template_len or th.size != T:
+            raise ValueError("templates, bias and thresholds do not match %d templates of %d x %d"
+                             % (T, C, template_len))
+        cfg = FbpConfig(C, K, T, template_len, max_shift, 1 if do_overlap else 0, reduce_wg, PRECISIONS[precision],
+                        OVERLAP_MODES[overlap_mode], max_passes, 0)
+        t = ctypes.c_uint64(0)
+        with self._lock:
+            check(self._lib.fbp_server_submit(self._h, ctypes.byref(cfg), _f32p(tm), _f32p(b), _f32p(th), _f32p(v),
+                                              v.strides[0] // 4, int(extra_bytes), ctypes.byref(t)),
+                  "fbp_server_submit")
+            self._inputs[t.value] = (v, tm, b, th)
+            self._cfgs[t.value] = cfg
+        return t.value
+
+    def _info(self, rc, info, what):
+        if rc == FBP_ETIMEDOUT:
+            return None
+        check(rc, what)
+        return info.as_dict()
+
+    def poll(self, ticket):
+        info = FbpJobInfo()
+        check(self._lib.fbp_server_poll(self._h, ticket, ctypes.byref(info)), "fbp_server_poll")
+        return info.as_dict()
+
+    def wait(self, ticket, timeout=None):
+        """Job info once it has ended, or None after timeout seconds."""
+        info = FbpJobInfo()
+        us = -1 if timeout is None else int(timeout * 1e6)
+        return self._info(self._lib.fbp_server_wait(self._h, ticket, us, ctypes.byref(info)), info,
+                          "fbp_server_wait")
+
+    def wait_any(self, timeout=None):
+        """(ticket, info) of the next job to end, or None after timeout seconds."""
+        info = FbpJobInfo()
+        t = ctypes.c_uint64(0)
+        us = -1 if timeout is None else int(timeout * 1e6)
+        d = self._info(self._lib.fbp_server_wait_any(self._h, us, ctypes.byref(t), ctypes.byref(info)), info,
+                       "fbp_server_wait_any")
+        return None if d is None else (t.value, d)
+
+    def cancel(self, ticket):
+        """True if the job was queued or running (it ends cancelled)."""
+        rc = self._lib.fbp_server_cancel(self._h, ticket)
+        if rc == FBP_ESTATE:
+            return False
+        check(rc, "fbp_server_cancel")
+        return True
+
+    def segment(self, ticket):
+        """Non-owning Segment of a finished job (valid until release)."""
+        h = ctypes.c_void_p(None)
+        check(self._lib.fbp_server_segment(self._h, ticket, ctypes.byref(h)), "fbp_server_segment")
+        with self._lock:
+            cfg = self._cfgs[ticket]
+        return Segment._view(h, cfg, self.device)
+
+    def error(self, ticket):
+        """Message of a failed job."""
+        info = FbpJobInfo()
+        check(self._lib.fbp_server_poll(self._h, ticket, ctypes.byref(info)), "fbp_server_poll")
+        return self._lib.fbp_last_error().decode(errors="replace") if info.state == 3 else ""
+
+    def release(self, ticket):
+        check(self._lib.fbp_server_release(self._h, ticket), "fbp_server_release")
+        with self._lock:
+            self._inputs.pop(ticket, None)
+            self._cfgs.pop(ticket, None)
+
+    def stats(self):
+        st = FbpServerStats()
+        check(self._lib.fbp_server_get_stats(self._h, ctypes.byref(st)), "fbp_server_get_stats")
+        return st.as_dict()

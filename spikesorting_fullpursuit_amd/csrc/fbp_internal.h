@@ -98,6 +98,24 @@ struct DeviceState {
 
 constexpr int kCompactItems = 1024;   // items per block of the compaction kernels
 
+}  // namespace fbp
+
+struct fbp_segment;
+
+namespace fbp {
+
+// Pass-by-pass driving of a segment (runtime.hip), for callers that advance
+// many segments from one thread.  segment_begin resets the segment and
+// enqueues its first pass on `st` (nullptr: the segment's own stream); once
+// segment_pass_event(g) has completed, segment_step reads the pass's counters
+// and either enqueues the next pass (*more = 1) or ends the run (*more = 0;
+// FBP_OK, FBP_EPASSES or an error, as fbp_segment_run returns).
+int segment_begin(fbp_segment *g, hipStream_t st);
+hipEvent_t segment_pass_event(const fbp_segment *g);
+int segment_step(fbp_segment *g, int *more);
+// Device bytes the segment holds now (buffers, spike list, clips).
+uint64_t segment_device_bytes(const fbp_segment *g);
+
 // Record the message of a failed C-ABI call for fbp_last_error() on this
 // thread (runtime.hip) and return `code`.
 int set_error(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -72,6 +73,13 @@ struct fbp_segment {
     fbp_stats stats{};
     uint32_t rq_cap = 0;         // capacity of the re-score queue
     float coef[4] = {0, 0, 0, 0};  // screen bound coefficients k1..k4
+    std::atomic<uint64_t> bytes{0};  // device bytes held (read by the server's admission thread)
+    // pass loop state between segment_begin and the last segment_step
+    hipStream_t run_stream = nullptr;
+    hipEvent_t ev_pass = nullptr;    // recorded after each pass's counter read-back
+    uint32_t nw = 0;                 // windows of the pass in flight
+    uint64_t total = 0;              // spikes before the pass in flight
+    uint32_t passes = 0;
 };
 
 namespace {
@@ -105,41 +113,125 @@ void dfree(void *p, hipStream_t st)
     if (p) (void)hipFreeAsync(p, st);
 }
 
+uint64_t round256(uint64_t b) { return (b + 255) / 256 * 256; }
+
+// Device bytes of one buffer of n elements as a segment accounts for them.
+template <class T>
+uint64_t buffer_bytes(uint64_t n) { return round256((n ? n : 1) * sizeof(T)); }
+
+// Allocation and release of a segment's buffers, tracked in g->bytes.
+template <class T>
+hipError_t salloc(fbp_segment *g, T **p, uint64_t n, hipStream_t st)
+{
+    const hipError_t e = dalloc(p, n, st);
+    if (e == hipSuccess) g->bytes += buffer_bytes<T>(n);
+    return e;
+}
+
+template <class T>
+void sfree(fbp_segment *g, T *p, uint64_t n, hipStream_t st)
+{
+    if (!p) return;
+    dfree(p, st);
+    g->bytes -= buffer_bytes<T>(n);
+}
+
+uint64_t rescore_queue_cap(const Dims &d)
+{
+    uint64_t cap = (uint64_t)d.T * d.K / 8;
+    if (cap < (1u << 20)) cap = 1u << 20;
+    if (cap > (512u << 20)) cap = 512u << 20;
+    return cap;
+}
+
+// Every buffer a segment allocates when it is created, in one list used both
+// to allocate them and to size a segment before creating it:
+// visit(pointer, element count) for each.
+template <class V>
+void for_each_buffer(const fbp_config *cfg, const Dims &d, DeviceState &s, V &&visit)
+{
+    const uint64_t W = d.W, CK = (uint64_t)d.C * d.K, T = d.T, CT16 = (T + 15) / 16;
+    const uint64_t nb = (W + kCompactItems - 1) / kCompactItems;
+    visit(s.V, CK);
+    visit(s.V0, CK);
+    visit(s.Tm, T * d.C * d.N);
+    visit(s.TmT, (uint64_t)d.C * d.N * d.Tp);
+    visit(s.bias, (uint64_t)d.Tp);
+    visit(s.thr, T);
+    visit(s.L, T * d.K);
+    visit(s.best_L, W);
+    visit(s.best_lab, W);
+    visit(s.best_idx, W);
+    visit(s.ov_idx, W);
+    visit(s.ov_lab, W);
+    visit(s.nc, W + 2);
+    visit(s.ov, W + 2);
+    visit(s.win, W);
+    visit(s.win_next, W);
+    visit(s.ov_list, W);
+    visit(s.sup_lo, T);
+    visit(s.sup_hi, T);
+    visit(s.tord, CT16 * 16);
+    visit(s.tile_lo, CT16);
+    visit(s.tile_hi, CT16);
+    visit(s.part_v, W);
+    visit(s.part_t, W);
+    visit(s.part_p, W);
+    visit(s.has, W);
+    visit(s.cand_idx, W);
+    visit(s.cand_lab, W);
+    visit(s.win_spike, W);
+    visit(s.blk, nb + 1);
+    visit(s.ctr, (uint64_t)1);
+    visit(s.tnorm, T);
+    visit(s.tscale, T);
+    visit(s.Tpad, T * d.C * (d.N + 2 * d.M));
+    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED) {
+        visit(s.tabAB, T * d.S * T * d.S);
+        visit(s.tabABmin, T * d.S * T);
+        visit(s.tabB2, T * d.S);
+        visit(s.TmQ, overlap_qtemplate_floats(d));
+        visit(s.ov_sorted, W);
+        visit(s.ovQ, (uint64_t)d.ov_batch * T * d.S);
+    }
+    if (cfg->precision != FBP_PREC_FP32) {
+        visit(s.rq, rescore_queue_cap(d));
+        visit(s.rq_shard, (uint64_t)kQueueShards * kShardStride);
+        visit(*(uint64_t **)&s.rdesc, (uint64_t)W * mfma_nchunk(d));
+        visit(*(uint64_t **)&s.TmB, (uint64_t)d.C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64 *
+                                        mfma_words(cfg->precision));
+    }
+}
+
+// Initial capacity of the spike list, in spikes.
+uint64_t initial_spike_cap(const Dims &d) { return 2ull * d.W; }
+
+// Bytes a segment allocates on the device when it is created (each block
+// rounded to 256 bytes), including the initial spike list.  The spike list
+// may grow while a segment runs and clips are allocated on demand: g->bytes
+// tracks what a segment holds.
+uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
+{
+    DeviceState dummy{};
+    uint64_t b = 0;
+    for_each_buffer(cfg, d, dummy, [&](auto *&p, uint64_t n) {
+        b += buffer_bytes<std::remove_reference_t<decltype(*p)>>(n);
+    });
+    return b + 2 * buffer_bytes<uint32_t>(initial_spike_cap(d));
+}
+
 void free_state(fbp_segment *g)
 {
     DeviceState &s = g->s;
-    void *ptrs[] = {s.V, s.V0, s.Tm, s.TmT, s.bias, s.thr, s.L, s.best_L, s.best_lab, s.best_idx, s.ov_idx,
-                    s.ov_lab, s.nc, s.ov, s.win, s.win_next, s.ov_list, s.has, s.cand_idx, s.cand_lab,
-                    s.win_spike, s.blk, s.ctr, s.sp_idx, s.sp_lab, s.clips, s.tabAB, s.tabABmin, s.tabB2,
-                    s.tnorm, s.TmB, s.rq, s.rq_shard, s.tscale, s.rdesc, s.TmQ, s.Tpad, s.ov_sorted, s.part_v, s.part_t,
-                    s.part_p, s.ovQ, s.sup_lo, s.sup_hi, s.tord, s.tile_lo, s.tile_hi};
-    for (void *p : ptrs) dfree(p, g->own_stream);
+    for_each_buffer(&g->cfg, g->d, s, [&](auto *&p, uint64_t) { dfree(p, g->own_stream); });
+    dfree(s.sp_idx, g->own_stream);
+    dfree(s.sp_lab, g->own_stream);
+    dfree(s.clips, g->own_stream);
     if (g->own_stream) (void)hipStreamSynchronize(g->own_stream);
     s = DeviceState{};
-}
-
-// Bytes a segment allocates on the device (each block rounded to 256 bytes),
-// including the initial spike list; clips are allocated on demand on top.
-uint64_t segment_bytes(const fbp_config *cfg, const Dims &d)
-{
-    auto r = [](uint64_t b) { return (b + 255) / 256 * 256; };
-    const uint64_t W = d.W, CK = (uint64_t)d.C * d.K, T = d.T;
-    const uint64_t nb = (W + kCompactItems - 1) / kCompactItems;
-    uint64_t b = 2 * r(CK * 4) + r(T * d.C * d.N * 4) + r((uint64_t)d.C * d.N * d.Tp * 4) + r(d.Tp * 4) +
-                 r(T * 4) + r(T * d.K * 4) + 5 * r(W * 4) + 2 * r(W + 2) + 3 * r(W * 4) + r(W) + 3 * r(W * 4) +
-                 r((nb + 1) * 4) + r(sizeof(Counters)) + r(T * 4) + 2 * r(W * 4) + r(T * 4) +
-                 r(T * d.C * (d.N + 2 * d.M) * 4) + 3 * r(W * 4) + 2 * r(T * 4) + 3 * r((T + 15) / 16 * 16 * 4);
-    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED)
-        b += r(T * d.S * T * d.S * 4) + r(T * d.S * T * 4) + r(T * d.S * 4) + r(overlap_qtemplate_floats(d) * 4) +
-             r(W * 4) + r((uint64_t)d.ov_batch * T * d.S * 4);
-    if (cfg->precision != FBP_PREC_FP32) {
-        uint64_t cap = T * d.K / 8;
-        if (cap < (1u << 20)) cap = 1u << 20;
-        if (cap > (512u << 20)) cap = 512u << 20;
-        b += r(cap * 8) + r((uint64_t)d.C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64 * 8 * mfma_words(cfg->precision)) +
-             r(W * mfma_nchunk(d) * 8);
-    }
-    return b;
+    g->bytes = 0;
+    g->cap = 0;
+    g->clips_cap = 0;
 }
 
 int set_device(const fbp_segment *g)
@@ -155,15 +247,15 @@ int ensure_spike_capacity(fbp_segment *g, uint64_t need, uint64_t keep, hipStrea
     uint64_t cap = g->cap * 2;
     if (cap < need) cap = need;
     uint32_t *ni = nullptr, *nl = nullptr;
-    HIP_TRY(dalloc(&ni, cap, st), "spike buffer");
-    hipError_t e = dalloc(&nl, cap, st);
-    if (e != hipSuccess) { dfree(ni, st); return hip_fail(e, "spike buffer"); }
+    HIP_TRY(salloc(g, &ni, cap, st), "spike buffer");
+    hipError_t e = salloc(g, &nl, cap, st);
+    if (e != hipSuccess) { sfree(g, ni, cap, st); return hip_fail(e, "spike buffer"); }
     if (keep > 0) {
         HIP_TRY(hipMemcpyAsync(ni, g->s.sp_idx, keep * 4, hipMemcpyDeviceToDevice, st), "grow spikes");
         HIP_TRY(hipMemcpyAsync(nl, g->s.sp_lab, keep * 4, hipMemcpyDeviceToDevice, st), "grow spikes");
     }
-    dfree(g->s.sp_idx, st);
-    dfree(g->s.sp_lab, st);
+    sfree(g, g->s.sp_idx, g->cap, st);
+    sfree(g, g->s.sp_lab, g->cap, st);
     g->s.sp_idx = ni;
     g->s.sp_lab = nl;
     g->cap = cap;
@@ -314,72 +406,18 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
     g->cfg = *cfg;
     g->d = dd;
     const Dims &d = g->d;
-    const uint32_t C = d.C, K = d.K, T = d.T, N = d.N;
+    const uint32_t C = d.C, N = d.N;
 
     int rc = FBP_OK;
     hipError_t e = hipSetDevice(device);
     keep_pool_memory(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->own_stream, hipStreamNonBlocking);
-    const uint64_t W = d.W, CK = (uint64_t)C * K;
     DeviceState &s = g->s;
-    const uint32_t nb = (uint32_t)((W + kCompactItems - 1) / kCompactItems);
-#define ALLOC(ptr, n)                                  \
-    if (e == hipSuccess) e = dalloc(&(ptr), (n), g->own_stream);
-    ALLOC(s.V, CK)
-    ALLOC(s.V0, CK)
-    ALLOC(s.Tm, (uint64_t)T * C * N)
-    ALLOC(s.TmT, (uint64_t)C * N * d.Tp)
-    ALLOC(s.bias, d.Tp)
-    ALLOC(s.thr, T)
-    ALLOC(s.L, (uint64_t)T * K)
-    ALLOC(s.best_L, W)
-    ALLOC(s.best_lab, W)
-    ALLOC(s.best_idx, W)
-    ALLOC(s.ov_idx, W)
-    ALLOC(s.ov_lab, W)
-    ALLOC(s.nc, W + 2)
-    ALLOC(s.ov, W + 2)
-    ALLOC(s.win, W)
-    ALLOC(s.win_next, W)
-    ALLOC(s.ov_list, W)
-    ALLOC(s.sup_lo, T)
-    ALLOC(s.sup_hi, T)
-    ALLOC(s.tord, (uint64_t)((T + 15) / 16) * 16)
-    ALLOC(s.tile_lo, (T + 15) / 16)
-    ALLOC(s.tile_hi, (T + 15) / 16)
-    ALLOC(s.part_v, W)
-    ALLOC(s.part_t, W)
-    ALLOC(s.part_p, W)
-    ALLOC(s.has, W)
-    ALLOC(s.cand_idx, W)
-    ALLOC(s.cand_lab, W)
-    ALLOC(s.win_spike, W)
-    ALLOC(s.blk, nb + 1)
-    ALLOC(s.ctr, 1)
-    ALLOC(s.tnorm, T)
-    ALLOC(s.tscale, T)
-    ALLOC(s.Tpad, (uint64_t)T * C * (N + 2 * d.M))
-    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED) {
-        ALLOC(s.tabAB, (uint64_t)T * d.S * T * d.S)
-        ALLOC(s.tabABmin, (uint64_t)T * d.S * T)
-        ALLOC(s.tabB2, (uint64_t)T * d.S)
-        ALLOC(s.TmQ, overlap_qtemplate_floats(d))
-        ALLOC(s.ov_sorted, W)
-        ALLOC(s.ovQ, (uint64_t)d.ov_batch * T * d.S)
-    }
+    for_each_buffer(cfg, d, s, [&](auto *&p, uint64_t n) {
+        if (e == hipSuccess) e = salloc(g, &p, n, g->own_stream);
+    });
     if (cfg->precision != FBP_PREC_FP32) {
-        uint64_t cap = (uint64_t)T * K / 8;
-        if (cap < (1u << 20)) cap = 1u << 20;
-        if (cap > (512u << 20)) cap = 512u << 20;
-        g->rq_cap = (uint32_t)cap;
-        ALLOC(s.rq, cap)
-        ALLOC(s.rq_shard, kQueueShards * kShardStride)
-        uint64_t *rd = nullptr;
-        ALLOC(rd, (uint64_t)W * mfma_nchunk(d))
-        s.rdesc = rd;
-        uint64_t *tmb = nullptr;
-        ALLOC(tmb, (uint64_t)C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64 * mfma_words(cfg->precision))
-        s.TmB = tmb;
+        g->rq_cap = (uint32_t)rescore_queue_cap(d);
         // Screen bound coefficients (see the correlation screen in kernels.hip):
         // unit roundoff u and flushed-subnormal floor delta of the format.
         const double u = cfg->precision == FBP_PREC_BF16 ? 0x1p-8 : cfg->precision == FBP_PREC_FP16 ? 0x1p-11 : 0x1p-4;
@@ -393,8 +431,8 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         g->coef[2] = (float)(cn * delta * delta * (1 + gp) * 1.01 + 1e-30);
         g->coef[3] = (float)(2 * gp * 1.01);
     }
-#undef ALLOC
     if (e == hipSuccess) e = hipEventCreate(&g->ev_start);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_pass, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&g->ev_stop);
     for (int q = 0; q < 4 && e == hipSuccess; q++) e = hipEventCreate(&g->ev_stage[q]);
     if (e == hipSuccess) e = hipHostMalloc((void **)&g->h_ctr, sizeof(Counters), hipHostMallocDefault);
@@ -403,7 +441,7 @@ int fbp_segment_create(int device, const fbp_config *cfg, fbp_segment **out)
         fbp_segment_destroy(g);
         return rc;
     }
-    rc = ensure_spike_capacity(g, 2 * W, 0, g->own_stream);
+    rc = ensure_spike_capacity(g, initial_spike_cap(d), 0, g->own_stream);
     if (rc == FBP_OK && hipStreamSynchronize(g->own_stream) != hipSuccess) rc = fail(FBP_EHIP, "allocation failed");
     if (rc) { fbp_segment_destroy(g); return rc; }
     *out = g;
@@ -418,6 +456,7 @@ void fbp_segment_destroy(fbp_segment *g)
     free_state(g);   // stream-ordered frees, then a wait on the segment's stream
     if (g->ev_start) (void)hipEventDestroy(g->ev_start);
     if (g->ev_stop) (void)hipEventDestroy(g->ev_stop);
+    if (g->ev_pass) (void)hipEventDestroy(g->ev_pass);
     for (int q = 0; q < 4; q++)
         if (g->ev_stage[q]) (void)hipEventDestroy(g->ev_stage[q]);
     if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
@@ -521,18 +560,88 @@ int fbp_segment_set_voltage_device(fbp_segment *g, const float *d_voltage, uint6
     return FBP_OK;
 }
 
-int fbp_segment_run(fbp_segment *g, void *stream)
+}  // extern "C"
+
+namespace {
+
+// Enqueue one pass over the g->nw windows of s.win on g->run_stream
+// (bpp.py:532-689), ending with the counter read-back and g->ev_pass.
+int issue_pass(fbp_segment *g)
+{
+    const Dims &d = g->d;
+    DeviceState &s = g->s;
+    hipStream_t st = g->run_stream;
+    const uint32_t nw = g->nw;
+    g->passes++;
+    if (int rc = ensure_spike_capacity(g, g->total + nw, g->total, st)) return rc;
+    HIP_TRY(hipEventRecord(g->ev_stage[0], st), "event");
+    if (g->cfg.precision == FBP_PREC_FP32)
+        HIP_TRY(launch_corr_fp32(d, s, nw, st), "correlation");               // bpp.py:538-549
+    else
+        HIP_TRY(launch_corr_screen(d, s, nw, g->cfg.precision, g->coef, g->rq_cap,
+                                   g->cfg.debug_flags & FBP_DEBUG_NO_RESCORE, st), "correlation screen");
+    HIP_TRY(hipEventRecord(g->ev_stage[1], st), "event");
+    HIP_TRY(launch_argmax(d, s, nw, st), "argmax");                            // bpp.py:551-562
+    if (g->cfg.do_overlap) {                                                   // bpp.py:565-661
+        HIP_TRY(launch_compact_ov(d, s, nw, st), "overlap list");
+        HIP_TRY(hipEventRecord(g->ev_stage[2], st), "event");
+        if (g->cfg.overlap_mode == FBP_OVERLAP_SCREENED) {
+            const uint32_t dbg = g->cfg.debug_flags & (FBP_DEBUG_OVERLAP_FULL | FBP_DEBUG_OVERLAP_GLOBAL |
+                                                       FBP_DEBUG_OVERLAP_EXACT | FBP_DEBUG_OVERLAP_ALL_ROWS);
+            HIP_TRY(launch_overlap_screened(d, s, nw, dbg, st), "overlap recheck (screened)");
+        } else {
+            HIP_TRY(launch_overlap_direct(d, s, nw, st), "overlap recheck");
+        }
+        HIP_TRY(launch_check_overlap(d, s, nw, st), "overlap reassign");
+        HIP_TRY(hipEventRecord(g->ev_stage[3], st), "event");
+    }
+    HIP_TRY(launch_pursuit_add(d, s, nw, st), "pursuit add");                  // bpp.py:663-671
+    HIP_TRY(launch_compact_spikes(d, s, nw, g->total, st), "spike list");
+    HIP_TRY(launch_pursuit_prune(d, s, nw, st), "pursuit prune");
+    HIP_TRY(launch_compact_next(d, s, st), "window list");                     // bpp.py:683-689
+    HIP_TRY(hipMemcpyAsync(g->h_ctr, s.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st), "counters");
+    HIP_TRY(hipEventRecord(g->ev_pass, st), "event");
+    return FBP_OK;
+}
+
+// The run is over (converged, capped or failed): final timings and stats.
+int finish_run(fbp_segment *g, int rc)
+{
+    if (rc == FBP_OK || rc == FBP_EPASSES) {
+        HIP_TRY(hipEventRecord(g->ev_stop, g->run_stream), "event");
+        HIP_TRY(hipEventSynchronize(g->ev_stop), "event");
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, g->ev_start, g->ev_stop);
+        g->stats.ms_total = ms;
+    }
+    g->n_spikes = g->total;
+    g->stats.n_passes = g->passes;
+    g->stats.n_spikes = g->total;
+    g->stats.overlap_combos = g->h_ctr->combos;
+    g->stats.screened_combos = g->h_ctr->screened;
+    // Results stay readable when the pass cap stopped the loop.
+    g->ran = (rc == FBP_OK || rc == FBP_EPASSES);
+    return rc;
+}
+
+}  // namespace
+
+int fbp::segment_begin(fbp_segment *g, hipStream_t stream)
 {
     if (!g) return fail(FBP_EINVAL, "NULL segment");
     if (!g->have_templates || !g->have_voltage) return fail(FBP_ESTATE, "templates and voltage must be set first");
     if (int rc = set_device(g)) return rc;
-    hipStream_t st = stream ? (hipStream_t)stream : g->own_stream;
+    hipStream_t st = stream ? stream : g->own_stream;
     const Dims &d = g->d;
     DeviceState &s = g->s;
+    g->run_stream = st;
     g->ran = false;
     g->n_spikes = 0;
     g->n_pending = 0;
     g->stats = fbp_stats{};
+    g->nw = d.W;
+    g->total = 0;
+    g->passes = 0;
     HIP_TRY(hipEventRecord(g->ev_start, st), "event");
     // Fresh state for the chunk (bpp.py:386-423); L needs no clearing because
     // pass 1 writes every position of it.
@@ -547,82 +656,73 @@ int fbp_segment_run(fbp_segment *g, void *stream)
     HIP_TRY(launch_fill_u8(s.ov, d.W + 2, 0, st), "init");
     HIP_TRY(launch_iota_u32(s.win, d.W, st), "init");
     HIP_TRY(launch_reset_counters(s.ctr, st), "init");
+    return issue_pass(g);
+}
 
-    uint32_t nw = d.W;
-    uint64_t total = 0;
-    uint32_t passes = 0;
-    int rc = FBP_OK;
-    for (;;) {
-        if (g->cfg.max_passes && passes >= g->cfg.max_passes) {
-            g->n_pending = nw;   // s.win holds the list the next pass would evaluate
-            rc = fail(FBP_EPASSES, "pass cap %u reached", passes);
-            break;
-        }
-        passes++;
-        if ((rc = ensure_spike_capacity(g, total + nw, total, st))) break;
-        HIP_TRY(hipEventRecord(g->ev_stage[0], st), "event");
-        if (g->cfg.precision == FBP_PREC_FP32)
-            HIP_TRY(launch_corr_fp32(d, s, nw, st), "correlation");               // bpp.py:538-549
-        else
-            HIP_TRY(launch_corr_screen(d, s, nw, g->cfg.precision, g->coef, g->rq_cap,
-                                       g->cfg.debug_flags & FBP_DEBUG_NO_RESCORE, st), "correlation screen");
-        HIP_TRY(hipEventRecord(g->ev_stage[1], st), "event");
-        HIP_TRY(launch_argmax(d, s, nw, st), "argmax");                            // bpp.py:551-562
-        if (g->cfg.do_overlap) {                                                   // bpp.py:565-661
-            HIP_TRY(launch_compact_ov(d, s, nw, st), "overlap list");
-            HIP_TRY(hipEventRecord(g->ev_stage[2], st), "event");
-            if (g->cfg.overlap_mode == FBP_OVERLAP_SCREENED) {
-                const uint32_t dbg = g->cfg.debug_flags & (FBP_DEBUG_OVERLAP_FULL | FBP_DEBUG_OVERLAP_GLOBAL |
-                                                           FBP_DEBUG_OVERLAP_EXACT | FBP_DEBUG_OVERLAP_ALL_ROWS);
-                HIP_TRY(launch_overlap_screened(d, s, nw, dbg, st), "overlap recheck (screened)");
-            } else {
-                HIP_TRY(launch_overlap_direct(d, s, nw, st), "overlap recheck");
-            }
-            HIP_TRY(launch_check_overlap(d, s, nw, st), "overlap reassign");
-            HIP_TRY(hipEventRecord(g->ev_stage[3], st), "event");
-        }
-        HIP_TRY(launch_pursuit_add(d, s, nw, st), "pursuit add");                  // bpp.py:663-671
-        HIP_TRY(launch_compact_spikes(d, s, nw, total, st), "spike list");
-        HIP_TRY(launch_pursuit_prune(d, s, nw, st), "pursuit prune");
-        HIP_TRY(launch_compact_next(d, s, st), "window list");                     // bpp.py:683-689
-        HIP_TRY(hipMemcpyAsync(g->h_ctr, s.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st), "counters");
-        HIP_TRY(hipStreamSynchronize(st), "pass");
-        const uint32_t n_new = g->h_ctr->n_new;
-        g->stats.windows_evaluated += nw;
-        g->stats.corr_launches++;
-        g->stats.corr_positions += (uint64_t)nw * d.N;
-        if (g->cfg.precision != FBP_PREC_FP32) g->stats.rescored_positions += g->h_ctr->n_rescore_pos;
-        float ms_c = 0.0f, ms_o = 0.0f;
-        (void)hipEventElapsedTime(&ms_c, g->ev_stage[0], g->ev_stage[1]);
-        g->stats.ms_correlation += ms_c;
-        if (g->cfg.do_overlap) {
-            g->stats.overlap_windows += g->h_ctr->n_ov;
-            g->stats.overlap_launches++;
-            (void)hipEventElapsedTime(&ms_o, g->ev_stage[2], g->ev_stage[3]);
-            g->stats.ms_overlap += ms_o;
-        }
-        if (n_new == 0) break;                                                     // bpp.py:675-676
-        HIP_TRY(launch_fill_u8(s.ov, d.W + 2, 0, st), "reset");                  // bpp.py:679-680
-        HIP_TRY(launch_residual(d, s, total, n_new, st), "residual");             // bpp.py:716-731
-        HIP_TRY(launch_clear_win_spike(d, s, nw, st), "reset");
-        HIP_TRY(launch_fill_u8(s.nc, d.W + 2, 0, st), "reset");                  // bpp.py:699-700
-        std::swap(s.win, s.win_next);
-        total += n_new;
-        nw = g->h_ctr->n_next;
+hipEvent_t fbp::segment_pass_event(const fbp_segment *g) { return g->ev_pass; }
+
+int fbp::segment_step(fbp_segment *g, int *more)
+{
+    *more = 0;
+    if (int rc = set_device(g)) return rc;
+    const Dims &d = g->d;
+    DeviceState &s = g->s;
+    hipStream_t st = g->run_stream;
+    const uint32_t nw = g->nw;
+    const uint32_t n_new = g->h_ctr->n_new;
+    g->stats.windows_evaluated += nw;
+    g->stats.corr_launches++;
+    g->stats.corr_positions += (uint64_t)nw * d.N;
+    if (g->cfg.precision != FBP_PREC_FP32) g->stats.rescored_positions += g->h_ctr->n_rescore_pos;
+    float ms_c = 0.0f, ms_o = 0.0f;
+    (void)hipEventElapsedTime(&ms_c, g->ev_stage[0], g->ev_stage[1]);
+    g->stats.ms_correlation += ms_c;
+    if (g->cfg.do_overlap) {
+        g->stats.overlap_windows += g->h_ctr->n_ov;
+        g->stats.overlap_launches++;
+        (void)hipEventElapsedTime(&ms_o, g->ev_stage[2], g->ev_stage[3]);
+        g->stats.ms_overlap += ms_o;
     }
-    HIP_TRY(hipEventRecord(g->ev_stop, st), "event");
-    HIP_TRY(hipEventSynchronize(g->ev_stop), "event");
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, g->ev_start, g->ev_stop);
-    g->n_spikes = total;
-    g->stats.n_passes = passes;
-    g->stats.n_spikes = total;
-    g->stats.ms_total = ms;
-    g->stats.overlap_combos = g->h_ctr->combos;
-    g->stats.screened_combos = g->h_ctr->screened;
-    // Results stay readable when the pass cap stopped the loop.
-    g->ran = (rc == FBP_OK || rc == FBP_EPASSES);
-    return rc;
+    if (n_new == 0) return finish_run(g, FBP_OK);                              // bpp.py:675-676
+    int rc = FBP_OK;
+    HIP_TRY(launch_fill_u8(s.ov, d.W + 2, 0, st), "reset");                    // bpp.py:679-680
+    HIP_TRY(launch_residual(d, s, g->total, n_new, st), "residual");           // bpp.py:716-731
+    HIP_TRY(launch_clear_win_spike(d, s, nw, st), "reset");
+    HIP_TRY(launch_fill_u8(s.nc, d.W + 2, 0, st), "reset");                    // bpp.py:699-700
+    std::swap(s.win, s.win_next);
+    g->total += n_new;
+    g->nw = g->h_ctr->n_next;
+    if (g->cfg.max_passes && g->passes >= g->cfg.max_passes) {
+        g->n_pending = g->nw;   // s.win holds the list the next pass would evaluate
+        fail(FBP_EPASSES, "pass cap %u reached", g->passes);
+        return finish_run(g, FBP_EPASSES);
+    }
+    if ((rc = issue_pass(g))) return finish_run(g, rc);
+    *more = 1;
+    return FBP_OK;
+}
+
+uint64_t fbp::segment_device_bytes(const fbp_segment *g) { return g->bytes.load(std::memory_order_relaxed); }
+
+extern "C" {
+
+int fbp_segment_run(fbp_segment *g, void *stream)
+{
+    int rc = segment_begin(g, (hipStream_t)stream);
+    if (rc) return rc;
+    for (;;) {
+        HIP_TRY(hipEventSynchronize(g->ev_pass), "pass");
+        int more = 0;
+        rc = segment_step(g, &more);
+        if (!more) return rc;
+    }
+}
+
+int fbp_segment_device_bytes(const fbp_segment *g, uint64_t *bytes)
+{
+    if (!g || !bytes) return fail(FBP_EINVAL, "NULL argument");
+    *bytes = segment_device_bytes(g);
+    return FBP_OK;
 }
 
 int fbp_segment_num_spikes(const fbp_segment *g, uint64_t *n)
@@ -652,10 +752,10 @@ int fbp_segment_get_clips(fbp_segment *g, float *clips)
     if (int rc = set_device(g)) return rc;
     const uint64_t n = g->n_spikes * g->d.C * g->d.N;
     if (n > g->clips_cap) {
-        dfree(g->s.clips, g->own_stream);
+        sfree(g, g->s.clips, g->clips_cap, g->own_stream);
         g->s.clips = nullptr;
         g->clips_cap = 0;
-        HIP_TRY(dalloc(&g->s.clips, n, g->own_stream), "clips");
+        HIP_TRY(salloc(g, &g->s.clips, n, g->own_stream), "clips");
         g->clips_cap = n;
     }
     HIP_TRY(launch_clips(g->d, g->s, g->n_spikes, g->own_stream), "clips");
