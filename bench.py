@@ -47,6 +47,10 @@ def main():
     ap.add_argument("--overlap-mode", default="screened", help="overlap arithmetic of the timed path")
     ap.add_argument("--plain-steps", type=int, default=1,
                     help="steps of the plain float32 path (fp32 correlation, direct overlap) timed for comparison")
+    ap.add_argument("--full-shift-steps", type=int, default=2,
+                    help="steps of the timed path at n_max_shift_inds = N - 1 (the full pipeline's setting)")
+    ap.add_argument("--large-probe-s", type=float, default=120.0,
+                    help="seconds of the 128-channel, 200-template recording timed as its own entry (0 = skip)")
     ap.add_argument("--no-overlap", action="store_true")
     ap.add_argument("--cpu-target-s", type=float, default=12.0, help="CPU baseline budget (s)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -112,20 +116,35 @@ def main():
 
     plain = None
     if args.plain_steps > 0 and (args.precision != "fp32" or args.overlap_mode != "direct"):
-        idx_fast, lab_fast = seg.spikes()
-        with _lib.Segment(C, K, T, N, M, do_overlap, 256, "fp32", "direct") as ps:
-            ps.set_templates(sm['templates'], bias, thr)
-            ps.set_voltage(V)
-            t = time.perf_counter()
-            for _ in range(args.plain_steps):
-                ps.run()
-            dt = (time.perf_counter() - t) / args.plain_steps
-            pi, pl = ps.spikes()
-        equal = bool(np.array_equal(pi, idx_fast) and np.array_equal(pl, lab_fast))
-        plain = {"path": "fp32 correlation + direct overlap (reference arithmetic)", "value": C * K / dt,
-                 "ms_per_step": 1000.0 * dt, "steps": args.plain_steps, "events_equal": equal,
-                 "speedup_of_timed_path": (C * K / (elapsed / args.steps)) / (C * K / dt)}
-        log("plain fp32 path: %.1f ms per step, spike lists identical: %s" % (1000 * dt, equal))
+        plain, _ = pursuit_entry(args, rec, "fp32", "direct", M, args.plain_steps, warmup=0,
+                                 ref_spikes=seg.spikes())
+        plain["path"] = "fp32 correlation + direct overlap (reference arithmetic)"
+        plain["speedup_of_timed_path"] = value / plain["value"]
+        log("plain fp32 path: %.1f ms per step, spike lists identical: %s" % (plain["ms_per_step"],
+                                                                           plain["events_equal"]))
+
+    m_full = None
+    if args.full_shift_steps > 0 and M != N - 1:
+        m_full, m_spk = pursuit_entry(args, rec, args.precision, args.overlap_mode, N - 1, args.full_shift_steps)
+        if args.plain_steps > 0:
+            ref, _ = pursuit_entry(args, rec, "fp32", "direct", N - 1, 1, warmup=0, ref_spikes=m_spk)
+            m_full["fp32_plain"] = {k: ref[k] for k in ("value", "ms_per_step", "n_spikes", "n_passes", "stages",
+                                                        "roofline")}
+            m_full["events_equal_fp32_plain"] = ref["events_equal"]
+        log("n_max_shift_inds = %d: %.1f ms per step (%.3g ch*samples/s), %d passes"
+            % (N - 1, m_full["ms_per_step"], m_full["value"], m_full["n_passes"]))
+
+    large = None
+    if args.large_probe_s > 0:
+        lrec = synthetic.config("128ch-600s", seed=args.seed, duration_s=args.large_probe_s)
+        LN = lrec['sort_info']['n_samples_per_chan']
+        large, _ = pursuit_entry(args, lrec, args.precision, args.overlap_mode, LN // 2, 1, warmup=1,
+                                 workload="128ch-600s")
+        large["workload"] = "128ch-600s"
+        large["duration_s"] = lrec['voltage'].shape[1] / lrec['sort_info']['sampling_rate']
+        log("large probe (128 ch, 200 templates, %.0f s): %.2f s per pursuit, %d passes, %.3g ch*samples/s"
+            % (large["duration_s"], large["ms_per_step"] / 1000, large["n_passes"], large["value"]))
+        del lrec
 
     prec = None
     if not args.no_precision_report:
@@ -167,6 +186,8 @@ def main():
                    "rescored_fraction": sums["rescored_positions"] / max(1, sums["corr_positions"] * T)},
         "stages": stages,
         "fp32_plain": plain,
+        "full_shift_range": m_full,
+        "large_probe": large,
         "serving": serve,
         "concurrent_segments": concurrent,
         "post_pursuit": post,
@@ -174,6 +195,46 @@ def main():
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def pursuit_entry(args, rec, precision, overlap_mode, M, steps, warmup=1, workload=None, ref_spikes=None):
+    """Time `steps` full pursuits of one recording on one segment (voltage
+    resident in HBM) with the given arithmetic and shift range; returns the
+    throughput, the stage split and the roofline entry of its dominant stage.
+    With ref_spikes = (onsets, labels), also whether the spike list is equal."""
+    from spikesorting_fullpursuit_amd import _lib
+    sm = rec['separability_metrics']
+    V = rec['voltage']
+    C, K = V.shape
+    T = sm['templates'].shape[0]
+    N = rec['sort_info']['n_samples_per_chan']
+    bias = np.float32(-0.5 * sm['template_SS'])
+    thr = np.float32(sm['neuron_lower_thresholds'])
+    sums = {"ms_correlation": 0.0, "ms_overlap": 0.0, "ms_total": 0.0, "corr_launches": 0,
+            "overlap_launches": 0, "corr_positions": 0, "overlap_combos": 0, "screened_combos": 0,
+            "overlap_windows": 0, "rescored_positions": 0}
+    with _lib.Segment(C, K, T, N, M, True, 256, precision, overlap_mode) as sg:
+        sg.set_templates(sm['templates'], bias, thr)
+        sg.set_voltage(V)
+        for _ in range(warmup):
+            sg.run()
+        t = time.perf_counter()
+        for _ in range(steps):
+            sg.run()
+            st = sg.stats()
+            for key in sums:
+                sums[key] += st[key]
+        dt = (time.perf_counter() - t) / steps
+        idx, lab = sg.spikes()
+    ns = argparse.Namespace(precision=precision, overlap_mode=overlap_mode, workload=workload or args.workload)
+    stages, roof = roofline_of(ns, sums, C, T, N, M, K, int(st["n_passes"]))
+    out = {"value": C * K / dt, "unit": "ch·samples/s", "ms_per_step": 1000.0 * dt, "steps": steps,
+           "precision": precision, "overlap_mode": overlap_mode, "n_max_shift_inds": M,
+           "n_spikes": int(idx.size), "n_passes": int(st["n_passes"]),
+           "device_ms_per_step": sums["ms_total"] / steps, "stages": stages, "roofline": roof}
+    if ref_spikes is not None:
+        out["events_equal"] = bool(np.array_equal(idx, ref_spikes[0]) and np.array_equal(lab, ref_spikes[1]))
+    return out, (idx, lab)
 
 
 def post_pursuit_run(seg, V, sort_info, T, repeats=3, max_clip_bytes=4 << 30):
@@ -422,10 +483,35 @@ def concurrent_segments_run(args, V, sm, bias, thr, N, M, do_overlap):
             "note": "equal time segments run concurrently, one HIP stream each; seams not merged"}
 
 
+def _serve(reqs, specs, args, paced, inflight):
+    """Run the requests through a PursuitServer: paced = submitted at their
+    Poisson arrival times, otherwise all at once (arrivals faster than the
+    server can serve them).  Returns (makespan, latency summary, results)."""
+    from spikesorting_fullpursuit_amd import serving
+    with serving.PursuitServer(max_inflight=inflight, precision=args.precision,
+                               overlap_mode=args.overlap_mode) as srv:
+        futs = []
+        t_start = time.perf_counter()
+        for sp, r in zip(specs, reqs):
+            if paced:
+                delay = sp['arrival_s'] - (time.perf_counter() - t_start)
+                if delay > 0:
+                    time.sleep(delay)
+            futs.append(srv.submit(r))
+        done = [f.result() for f in futs]
+        makespan = time.perf_counter() - t_start
+        summ = srv.latency_summary()
+    return makespan, summ, done
+
+
 def serving_mix_run(args):
     """The served mix: Poisson arrivals of segments of 5-120 s at 30/40 kHz,
     4-32 channels, 3-40 templates of 41-81 samples; admitted by HBM budget and
-    run concurrently.  Latency = arrival to result on the host."""
+    driven concurrently by the native segment server.  Latency = submission
+    to result on the host.  Three legs: every request alone, one after the
+    other (the reference's serial order); the mix at its arrival rate; and
+    the mix submitted at once, so the server is saturated and its throughput
+    is its capacity."""
     from spikesorting_fullpursuit_amd import serving, synthetic
     specs = synthetic.serving_mix(seed=args.seed + 100, n_requests=args.serve_requests, rate_hz=args.serve_rate)
     t0 = time.time()
@@ -437,29 +523,37 @@ def serving_mix_run(args):
                                            sm['neuron_lower_thresholds'], sp['n_samples_per_chan'], tag=len(reqs)))
     chs = sum(r.C * r.K for r in reqs)
     log("served mix: %d requests, %.2f G ch*samples, generated in %.1f s" % (len(reqs), chs / 1e9, time.time() - t0))
-    # warm the kernels of every shape class once, outside the timed mix
+    # warm the kernels of every shape class once, outside the timed legs
     serving.run_segment(reqs[0], args.precision, args.overlap_mode)
-    with serving.PursuitServer(max_inflight=args.serve_inflight, precision=args.precision,
-                               overlap_mode=args.overlap_mode) as srv:
-        futs = []
-        t_start = time.perf_counter()
-        for sp, r in zip(specs, reqs):
-            delay = sp['arrival_s'] - (time.perf_counter() - t_start)
-            if delay > 0:
-                time.sleep(delay)
-            futs.append(srv.submit(r))
-        done = [f.result() for f in futs]
-        makespan = time.perf_counter() - t_start
-        summ = srv.latency_summary()
+    t = time.perf_counter()
+    alone = [serving.run_segment(r, args.precision, args.overlap_mode)[0].size for r in reqs]
+    standalone_s = time.perf_counter() - t
+    log("served mix alone, one after the other: %.2f s" % standalone_s)
+    makespan, summ, done = _serve(reqs, specs, args, True, args.serve_inflight)
     spikes = sum(int(d.onsets.size) for d in done)
-    log("served mix: makespan %.2f s, p50 %.3f s, p99 %.3f s, peak %d in flight"
-        % (makespan, summ['p50_s'], summ['p99_s'], summ['peak_inflight']))
+    assert [d.onsets.size for d in done] == alone, "served spike counts differ from the standalone runs"
+    log("served mix at %.0f/s: makespan %.2f s, p50 %.3f s, p99 %.3f s, peak %d in flight"
+        % (args.serve_rate, makespan, summ['p50_s'], summ['p99_s'], summ['peak_inflight']))
+    saturated = []
+    for inflight in sorted({4, args.serve_inflight, 16}):
+        ms, su, dn = _serve(reqs, specs, args, False, inflight)
+        assert [d.onsets.size for d in dn] == alone, "saturated spike counts differ from the standalone runs"
+        saturated.append({"max_inflight": inflight, "makespan_s": ms, "throughput": chs / ms,
+                          "vs_standalone": standalone_s / ms, "latency_p50_s": su['p50_s'],
+                          "latency_p99_s": su['p99_s'], "peak_inflight": su['peak_inflight'],
+                          "peak_hbm_bytes": su['peak_bytes'], "pass_steps": su['pass_steps'],
+                          "driver_busy_fraction": su['driver_busy_s'] / max(ms, 1e-9)})
+        log("served mix saturated, %d in flight: makespan %.2f s (%.2fx the serial runs), p50 %.3f s, p99 %.3f s, "
+            "driver thread busy %.1f %%" % (inflight, ms, standalone_s / ms, su['p50_s'], su['p99_s'],
+                                            100 * su['driver_busy_s'] / ms))
     return {"requests": len(reqs), "arrival_rate_hz": args.serve_rate, "max_inflight": args.serve_inflight,
             "ch_samples": chs, "spikes": spikes, "makespan_s": makespan, "throughput": chs / makespan,
             "unit": "ch·samples/s", "latency_p50_s": summ['p50_s'], "latency_p99_s": summ['p99_s'],
             "latency_mean_s": summ['mean_s'], "queue_p99_s": summ['queue_p99_s'],
             "peak_inflight": summ['peak_inflight'], "peak_hbm_bytes": summ['peak_bytes'],
-            "arrivals_span_s": specs[-1]['arrival_s']}
+            "arrivals_span_s": specs[-1]['arrival_s'], "standalone_serial_s": standalone_s,
+            "standalone_throughput": chs / standalone_s, "saturated": saturated,
+            "events_equal_standalone": True}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kernels_16ch-300s_bf16-screened.json")
