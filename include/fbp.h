@@ -305,6 +305,22 @@ int fbp_channel_covariance_noalloc(int device, const float *d_clips, uint64_t n_
 int fbp_template_variances(int device, const double *templates, uint32_t n_templates, uint32_t n_chans, uint32_t N,
                            const double *cov, double *var);
 
+/* Template pairs at every lag of [lag_lo, lag_hi]: the alignment step of
+ * optimal_align_templates (consolidate.py:9-71) as check_noise_templates
+ * runs it for each (good, noisy) template pair (neuron_separability.py:
+ * 192-199).  templates: host float64 [n_templates][n_chans*N]; pairs: host
+ * uint32 [n_pairs][2] = (a, b) rows.  With L = n_chans*N:
+ *   xcorr[p][k]   = sum_i a[i + lag] b[i], 0 <= i, i + lag < L, lag = lag_lo + k
+ *                   (np.correlate(a, b, 'full')[lag + L - 1]);
+ *   aligned[p][k] = the same sum over the i where i and i + lag fall on one
+ *                   channel: np.dot of the two templates after each channel
+ *                   is shifted by lag and truncated (consolidate.py:34-54).
+ * float64, one fixed summation order per output.  aligned may be NULL.
+ * -(L-1) <= lag_lo <= lag_hi <= L-1. */
+int fbp_template_pair_lags(int device, const double *templates, uint32_t n_templates, uint32_t n_chans, uint32_t N,
+                           const uint32_t *pairs, uint64_t n_pairs, int32_t lag_lo, int32_t lag_hi, double *xcorr,
+                           double *aligned);
+
 /* Overlap-template search residuals (find_overlap_templates,
  * c_cython/sort_cython.pyx:333-456, before binary pursuit at
  * full_binary_pursuit.py:257-262).  templates: host float32 [n_templates]

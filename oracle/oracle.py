@@ -436,3 +436,230 @@ def separability_metrics(templates, channel_covariance_mats, sort_info):
                         for c in range(C)], dtype=np.float64)
     return {'template_SS': ss, 'neuron_variances': var, 'neuron_lower_thresholds': thr, 'neuron_lower_CI': ci,
             'peak_channel': peak, 'contamination': cont, 'channel_p_noise': p_noise}
+
+
+def optimal_align_templates(temp_1, temp_2, n_chans, max_shift=None, align_abs=False, zero_pad=False):
+    """consolidate.py:9-71 restated: the lag of the largest (absolute, with
+    align_abs) value of np.correlate(temp_1, temp_2, 'full'), within
+    +-max_shift of zero lag when given (first of equal maxima), then every
+    channel of both templates shifted by that lag and truncated (:34-54), or
+    zero padded back to full length (:56-69).  Returns (shifted 1, shifted 2,
+    lag, samples per channel)."""
+    L = temp_1.shape[0]
+    N = L // n_chans
+    if temp_1.shape[0] != temp_2.shape[0] or temp_1.ndim > 1 or N == 0:
+        raise ValueError("Input templates must be 1D vectors of the same size")
+    if L % n_chans != 0:
+        raise ValueError("Template shape[0] must be evenly divisible by n_chans")
+    xc = np.correlate(temp_1, temp_2, mode='full')
+    if align_abs:
+        xc = np.abs(xc)
+    centre = xc.shape[0] // 2
+    if max_shift is None:
+        lag = int(np.argmax(xc)) - centre
+    else:
+        lag = int(np.argmax(xc[centre - max_shift:centre + max_shift + 1])) - max_shift
+    if lag == 0:
+        return temp_1, temp_2, 0, N
+    a = [temp_1[c * N:(c + 1) * N] for c in range(n_chans)]
+    b = [temp_2[c * N:(c + 1) * N] for c in range(n_chans)]
+    if lag > 0:
+        s1 = np.hstack([x[lag:] for x in a])
+        s2 = np.hstack([x[:-lag] for x in b])
+    else:
+        s1 = np.hstack([x[:lag] for x in a])
+        s2 = np.hstack([x[-lag:] for x in b])
+    n_shift = s1.shape[0] // n_chans
+    if not zero_pad:
+        return s1, s2, lag, n_shift
+    p1, p2 = np.zeros(L), np.zeros(L)
+    for c in range(n_chans):
+        dst = slice(c * N + lag, (c + 1) * N) if lag >= 0 else slice(c * N, (c + 1) * N + lag)
+        p1[dst] = s1[c * n_shift:(c + 1) * n_shift]
+        p2[dst] = s2[c * n_shift:(c + 1) * n_shift]
+    return p1, p2, lag, N
+
+
+def find_noisy_templates(separability_metrics):
+    """neuron_separability.py:155-169: noisy where 2 * lower threshold
+    exceeds half the template's sum of squares."""
+    thr = separability_metrics['neuron_lower_thresholds']
+    ss = separability_metrics['template_SS']
+    return np.array([2 * thr[n] > 0.5 * ss[n] for n in range(ss.shape[0])], dtype=bool)
+
+
+def check_noise_templates(separability_metrics, sort_info, noisy_templates):
+    """neuron_separability.py:172-224 restated loop for loop: a noisy
+    template stays (is no longer marked) as soon as one template that is not
+    noisy, with nonzero likelihood variance, would take its spikes above
+    threshold with probability above norm.sf(sigma_bp_noise)."""
+    from scipy.stats import norm
+    C, N = sort_info['n_channels'], sort_info['n_samples_per_chan']
+    max_shift = N // 4 - 1
+    tm = separability_metrics['templates']
+    p_limit = norm.sf(sort_info['sigma_bp_noise'], 0, 1)
+    out = np.copy(noisy_templates)
+    for noise_n in range(tm.shape[0]):
+        if not noisy_templates[noise_n]:
+            continue
+        for n in range(tm.shape[0]):
+            if noisy_templates[n]:
+                continue
+            a, b, _, _ = optimal_align_templates(tm[n, :], tm[noise_n, :], C, max_shift=max_shift, align_abs=True)
+            expect = np.dot(a, b) - 0.5 * separability_metrics['template_SS'][n]
+            var = separability_metrics['neuron_variances'][n]
+            if var == 0.0:
+                continue
+            if norm.sf(separability_metrics['neuron_lower_thresholds'][n], expect, np.sqrt(var)) > p_limit:
+                out[noise_n] = False
+                break
+    return separability_metrics, out
+
+
+def delete_noise_units(separability_metrics, noisy_templates):
+    """neuron_separability.py:227-248: rows of the noisy templates removed
+    from every per-template entry, the per-channel entries kept."""
+    keep = ~noisy_templates
+    for key in list(separability_metrics.keys()):
+        if key in ('channel_covariance_mats', 'channel_p_noise', 'bp_n_samples_per_chan'):
+            continue
+        if key in ('templates', 'template_SS_by_chan'):
+            separability_metrics[key] = separability_metrics[key][keep, :]
+        elif key in ('template_SS', 'neuron_variances', 'neuron_lower_thresholds', 'neuron_lower_CI',
+                     'contamination', 'peak_channel'):
+            separability_metrics[key] = separability_metrics[key][keep]
+        elif key == 'template_covariance_mats':
+            for x in reversed(range(len(separability_metrics[key]))):
+                if noisy_templates[x]:
+                    del separability_metrics[key][x]
+    return separability_metrics
+
+
+def check_template_pair(template_1, template_2, chan_covariance_mats, sort_info):
+    """neuron_separability.py:56-85: probability that the likelihood of
+    template_2 exceeds that of template_1 when the voltage is template_1,
+    norm.cdf(0, E1 - E2, sqrt(sum_c d_c^T cov_c d_c)), d = template_1 -
+    template_2; 1 when that variance is not positive."""
+    from scipy.stats import norm
+    C, N = sort_info['n_channels'], sort_info['n_samples_per_chan']
+    e1 = 0.5 * np.dot(template_1, template_1)
+    e2 = np.dot(template_1, template_2) - 0.5 * np.dot(template_2, template_2)
+    var = 0
+    for c in range(C):
+        d = template_1[c * N:(c + 1) * N] - template_2[c * N:(c + 1) * N]
+        var += (d[None, :] @ chan_covariance_mats[c] @ d[:, None])[0, 0]
+    if var > 0:
+        return float(norm.cdf(0, e1 - e2, np.sqrt(var)))
+    return 1.
+
+
+def binary_pursuit_clip_width(events, voltage, sort_info):
+    """full_binary_pursuit.py:14-115 restated for the crossings `events` of a
+    segment: the mean clip over the widened clip width, trimmed at each end
+    while every channel's end sample is below 5 % of the median absolute
+    voltage (np.random.choice of 10 s of samples), never inside the original
+    width; returns ([-w, w], original column starts, stops)."""
+    C, N, fs = sort_info['n_channels'], sort_info['n_samples_per_chan'], sort_info['sampling_rate']
+    factor = sort_info['max_binary_pursuit_clip_width_factor']
+    plain = (np.arange(0, N * C, N, dtype=np.int64), np.arange(N, (N + 1) * C, N, dtype=np.int64))
+    w0 = max(np.abs(sort_info['clip_width']))
+    if factor <= 1.0 or len(events) == 0:
+        return [-1 * w0, w0], plain[0], plain[1]
+    events = np.sort(np.hstack(events))
+    bp_cw = [factor * v for v in sort_info['clip_width']]
+    pd = {'sampling_rate': fs, 'n_samples': voltage.shape[1], 'v_dtype': voltage.dtype}
+    clips, valid = get_clips(pd, voltage, np.arange(C), events, bp_cw)
+    if np.count_nonzero(valid) == 0:
+        return [-1 * w0, w0], plain[0], plain[1]
+    mean_clip = np.mean(clips, axis=0)
+    bp_n = clips.shape[1] // C
+    first = np.arange(0, bp_n * (C - 1) + 1, bp_n, dtype=np.int64)
+    last = np.arange(bp_n - 1, bp_n * C + 1, bp_n, dtype=np.int64)
+    inds = np.random.choice(voltage.shape[1], 10 * fs)
+    tol = 0.05 * np.median(np.median(np.abs(voltage[:, inds]), axis=1))
+    bp_win, _ = time_window_to_samples(bp_cw, fs)
+    win, _ = time_window_to_samples(sort_info['clip_width'], fs)
+    max_pre = -1 * bp_win[0] + win[0]
+    while np.all(np.abs(mean_clip[first]) < tol):
+        if first[0] >= max_pre:
+            break
+        first += 1
+    bp_cw[0] = -1 * (-1 * bp_win[0] - first[0]) / fs
+    min_post = (bp_n - bp_win[1]) + win[1] - 1
+    while np.all(np.abs(mean_clip[last]) < tol):
+        if last[0] <= min_post:
+            break
+        last -= 1
+    bp_cw[1] = (bp_win[1] - (bp_n - last[0])) / fs
+    start = (-1 * bp_win[0] + win[0]) - first[0]
+    stop = start + (win[1] - win[0])
+    n_new = last[0] - first[0]
+    m = max(np.abs(bp_cw))
+    return ([-1 * m, m], np.arange(start, n_new * C, n_new, dtype=np.int64),
+            np.arange(stop, (n_new + 1) * C, n_new, dtype=np.int64))
+
+
+def full_binary_pursuit_units(voltage, v_dtype, sort_info, unit_spikes, seg_w_items, chunk_len):
+    """full_binary_pursuit.py:117-438 restated for units given as spike index
+    arrays (one unit per array, the clustering output after consolidation),
+    each unit's template the mean of its clips: binary-pursuit clip width,
+    residual-clip noise covariance (np.random.randint rows per channel),
+    sum-of-two-templates removal (check_template_pair > norm.sf(sigma)),
+    separability metrics, noise-template removal, binary pursuit (one chunk
+    length), clips of the input voltage and the per-channel output lists.
+    Returns (output list, separability metrics, clip width used)."""
+    from scipy.stats import norm
+    C = sort_info['n_channels']
+    fs = sort_info['sampling_rate']
+    si = dict(sort_info)
+    original_n = si['n_samples_per_chan']
+    chan_win, _ = time_window_to_samples(si['clip_width'], fs)
+    width, starts, stops = binary_pursuit_clip_width([u for u in unit_spikes if len(u)], voltage, si)
+    si['clip_width'] = width
+    bp_win, _ = time_window_to_samples(width, fs)
+    N = bp_win[1] - bp_win[0]
+    si['n_samples_per_chan'] = N
+    pd = {'sampling_rate': fs, 'n_samples': voltage.shape[1], 'v_dtype': v_dtype}
+    templates, resid, counts = [], [], []
+    for u in unit_spikes:
+        clips, _ = get_clips(pd, voltage, np.arange(C), u, width)
+        t = np.mean(clips, axis=0)
+        templates.append(t)
+        resid.append(clips - t)
+        counts.append(u.shape[0])
+    templates = np.vstack(templates)
+    resid = np.vstack(resid)
+    n = resid.shape[0]
+    rows = None if si['n_cov_samples'] >= n else [np.random.randint(0, n, si['n_cov_samples']) for _ in range(C)]
+    covs = list(channel_covariance(resid, C, N, rows))
+    delete = np.zeros(templates.shape[0], dtype=bool)
+    sums, _ = find_overlap_templates(templates, N, C, np.abs(chan_win[0]) - 1, np.abs(chan_win[1]) - 1,
+                                     np.array(counts, dtype=np.int64))
+    for t_ind, tmpl, _ in sums:
+        if check_template_pair(templates[t_ind], tmpl, covs, si) > norm.sf(si['sigma']):
+            delete[t_ind] = True
+    templates = templates[~delete]
+    sm = separability_metrics(list(templates), covs, si)
+    sm['templates'] = templates
+    noisy = find_noisy_templates(sm)
+    sm['neuron_lower_thresholds'] = np.maximum(np.maximum(sm['neuron_lower_thresholds'], 0), sm['neuron_lower_CI'])
+    sm, noisy = check_noise_templates(sm, si, noisy)
+    keep = ~noisy
+    for key in ('templates', 'template_SS', 'neuron_variances', 'neuron_lower_thresholds', 'neuron_lower_CI',
+                'contamination', 'peak_channel'):
+        sm[key] = sm[key][keep]
+    ev, lab, bpb, _, _ = binary_pursuit(voltage, v_dtype, si, sm, original_n - 1, chunk_len)
+    ev, lab = np.asarray(ev, dtype=np.int64), np.asarray(lab, dtype=np.int64)
+    clips, _ = get_clips(pd, voltage, np.arange(C), ev, width)
+    out = []
+    assign = unit_channels(clips, lab, C, N, seg_w_items[0]['thresholds'], si['sigma']) if ev.size else \
+        {c: [] for c in range(C)}
+    for c in range(C):
+        item = [w for w in seg_w_items if w['channel'] == c][0]
+        units = assign[c]
+        if not units:
+            out.append([[], [], [], [], item['ID']])
+            continue
+        sel = np.concatenate([np.flatnonzero(lab == u) for u in units])
+        out.append([ev[sel] + item['index_window'][0], lab[sel], None, np.ones(sel.size, dtype=bool), item['ID']])
+    return out, sm, width

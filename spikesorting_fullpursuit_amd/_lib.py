@@ -42,7 +42,7 @@ EXPORTS = (
     "fbp_segment_get_window_list", "fbp_channel_covariance_noalloc", "fbp_unit_mean_clips_f64",
     "fbp_segment_device_bytes", "fbp_server_create", "fbp_server_destroy", "fbp_server_submit", "fbp_server_poll",
     "fbp_server_wait", "fbp_server_wait_any", "fbp_server_cancel", "fbp_server_segment", "fbp_server_release",
-    "fbp_server_get_stats",
+    "fbp_server_get_stats", "fbp_template_pair_lags",
 )
 
 
@@ -186,6 +186,8 @@ def _declare(lib):
                                                    ctypes.c_uint32, vp, ctypes.c_uint64, vp, vp]
     lib.fbp_template_variances.argtypes = [ctypes.c_int, f64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                            f64p, f64p]
+    lib.fbp_template_pair_lags.argtypes = [ctypes.c_int, f64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                           u32p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, f64p, f64p]
     lib.fbp_segment_device_bytes.argtypes = [vp, u64p]
     lib.fbp_server_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, P(vp)]
     lib.fbp_server_destroy.argtypes = [vp]
@@ -544,6 +546,27 @@ def template_variances(templates, channel_covariance_mats, device=0):
     check(lib.fbp_template_variances(device, _f64p(tm), tm.shape[0], n_chans, N, _f64p(cov), _f64p(out)),
           "fbp_template_variances")
     return out
+
+
+def template_pair_lags(templates, n_chans, pairs, lag_lo, lag_hi, aligned=True, device=0):
+    """fbp_template_pair_lags: for each row (a, b) of `pairs`, the flat
+    cross-correlation sum_i a[i + lag] b[i] and (with aligned) the
+    channel-aligned dot product at every lag of [lag_lo, lag_hi], float64
+    [n_pairs][n_lags] each (aligned is None without it)."""
+    lib = load()
+    tm = np.ascontiguousarray(templates, dtype=np.float64)
+    tm = tm.reshape(tm.shape[0] if tm.ndim > 1 else 1, -1)
+    if tm.shape[1] % n_chans:
+        raise ValueError("template length %d is not a multiple of n_chans = %d" % (tm.shape[1], n_chans))
+    N = tm.shape[1] // n_chans
+    pr = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+    n_lags = int(lag_hi) - int(lag_lo) + 1
+    xc = np.zeros((pr.shape[0], max(n_lags, 0)), dtype=np.float64)
+    al = np.zeros_like(xc) if aligned else None
+    check(lib.fbp_template_pair_lags(device, _f64p(tm), tm.shape[0], n_chans, N, _u32p(pr), pr.shape[0],
+                                     int(lag_lo), int(lag_hi), _f64p(xc), _f64p(al) if aligned else None),
+          "fbp_template_pair_lags")
+    return xc, al
 
 
 class Server:

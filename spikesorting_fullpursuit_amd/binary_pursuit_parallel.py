@@ -22,35 +22,35 @@ def plan_chunks(n_chans, n_samples, sampling_rate, n_templates, template_samples
     The device budget is max(0.5*total, total - 1 GiB), capped by
     max_gpu_memory.  One second of data is charged for the voltage, the
     likelihood buffer and per-window state including the overlap group buffers
-    (n_local_ID / 3 per window).  Onsets advance by chunk - 3N while
+    (a third of the per-window group count, twice).  Onsets advance by chunk - 3N while
     onset < n_samples - N.  Raises ValueError when a chunk would hold fewer
     than 4N samples (bpp.py:343-344).
     """
-    gpu_memory_size = max(gpu_total_bytes * 0.5, gpu_total_bytes - (1024 * 1024 * 1024))
+    budget = max(gpu_total_bytes * 0.5, gpu_total_bytes - (1024 * 1024 * 1024))
     if max_gpu_memory is not None:
-        gpu_memory_size = min(gpu_memory_size, max_gpu_memory)
-    assert gpu_memory_size > 0
+        budget = min(budget, max_gpu_memory)
+    assert budget > 0
     N = int(template_samples_per_chan)
-    items_per_index = np.int64((2 * n_max_shift_inds + 1) * (2 * n_max_shift_inds + 1) * n_templates)
-    n_local_ID = np.int64(((items_per_index - 1) / overlap_work_group) + 1)
-    constant_memory_usage = n_templates * n_chans * N * 4 + n_templates * 4
-    memory_usage_per_second = n_chans * sampling_rate * 4
-    memory_usage_per_second += n_templates * sampling_rate * 4
-    memory_usage_per_second += (sampling_rate / N) * (4 + 4 + 4 + 4 + 4 + (n_local_ID / 3) * 4 + (n_local_ID / 3) * 4)
-    num_seconds_per_chunk = (gpu_memory_size - constant_memory_usage) / memory_usage_per_second
+    combos_per_window = np.int64((2 * n_max_shift_inds + 1) * (2 * n_max_shift_inds + 1) * n_templates)
+    groups_per_window = np.int64(((combos_per_window - 1) / overlap_work_group) + 1)
+    fixed_bytes = n_templates * n_chans * N * 4 + n_templates * 4
+    bytes_per_second = n_chans * sampling_rate * 4
+    bytes_per_second += n_templates * sampling_rate * 4
+    bytes_per_second += (sampling_rate / N) * (4 + 4 + 4 + 4 + 4 + (groups_per_window / 3) * 4 + (groups_per_window / 3) * 4)
+    seconds_per_chunk = (budget - fixed_bytes) / bytes_per_second
     device_address_bits = 64
-    max_addressable_seconds = (((1 << device_address_bits) - 1) / (4 * sampling_rate * n_chans))
-    num_seconds_per_chunk = min(num_seconds_per_chunk, np.floor(max_addressable_seconds))
-    num_indices_per_chunk = int(np.floor(num_seconds_per_chunk * sampling_rate))
-    if num_indices_per_chunk < 4 * N:
+    addressable_seconds = (((1 << device_address_bits) - 1) / (4 * sampling_rate * n_chans))
+    seconds_per_chunk = min(seconds_per_chunk, np.floor(addressable_seconds))
+    samples_per_chunk = int(np.floor(seconds_per_chunk * sampling_rate))
+    if samples_per_chunk < 4 * N:
         raise ValueError("Cannot fit enough data on GPU to run binary pursuit. "
                          "Decrease template samples or get a new GPU.")
     onsets = []
     cur = 0
     while cur < n_samples - N:
         onsets.append(cur)
-        cur += num_indices_per_chunk - 3 * N
-    return num_indices_per_chunk, onsets
+        cur += samples_per_chunk - 3 * N
+    return samples_per_chunk, onsets
 
 
 def residual_span(chunk_onsets, k, template_samples_per_chan, n_samples):
@@ -71,8 +71,8 @@ def residual_span(chunk_onsets, k, template_samples_per_chan, n_samples):
 
 
 def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shift_inds=None,
-                   kernels_path=None, max_gpu_memory=None, *, return_residual=False, precision="fp32",
-                   overlap_mode="direct", reduce_work_group=256, device=0, max_passes=0, stats=None,
+                   kernels_path=None, max_gpu_memory=None, *, return_residual=False, precision="bf16",
+                   overlap_mode="screened", reduce_work_group=256, device=0, max_passes=0, stats=None,
                    verbose=False):
     """Full binary pursuit over `voltage` with the templates in
     `separability_metrics`; same arguments and returns as the reference's
@@ -92,8 +92,14 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
 
     Keyword-only extensions (no reference counterpart):
     return_residual: also return the residual voltage float32 [C][n_samples].
-    precision: correlation arithmetic, "fp32" (reference arithmetic).
-    overlap_mode: "direct" (reference arithmetic).
+    precision: correlation arithmetic: "bf16" (default), "fp16" or "fp8"
+        matrix-core screen whose candidates are re-scored in float32, or
+        "fp32" (the reference's float32 chain for every position).  Every
+        choice gives the same spikes, labels, clips and residual.
+    overlap_mode: overlap recheck arithmetic: "screened" (default; float64
+        bounds from matrix-core tables, exact float32 score of every
+        combination that can win) or "direct" (the reference's float32 score
+        of every combination).  Both give the same decisions.
     reduce_work_group: width of the overlap reduction whose tie-break the
         results follow (the reference uses its device's work-group size).
     stats: optional list; per-chunk engine statistics dicts are appended.
@@ -134,7 +140,7 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
     if verbose:
         print("Using", len(chunk_onsets), "chunks for binary pursuit.", flush=True)
 
-    secret_spike_indices, secret_spike_labels, secret_spike_bool, adjusted_spike_clips = [], [], [], []
+    chunk_onset_lists, chunk_label_lists, chunk_bool_lists, chunk_clip_lists = [], [], [], []
     residual = np.zeros((n_chans, n_samples), dtype=np.float32) if return_residual else None
     if return_residual and chunk_onsets:
         residual[:, :] = np.float32(voltage)
@@ -164,19 +170,19 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
                 print("Found", idx.size, "secret spikes in chunk", k + 1, flush=True)
             if idx.size > 0:
                 clips = seg.clips() if get_clips else []
-                secret_spike_indices.append(idx.astype(np.int64) + start_index)
-                secret_spike_labels.append(lab)
-                adjusted_spike_clips.append(clips)
-                secret_spike_bool.append(np.ones(idx.size, dtype=bool))
+                chunk_onset_lists.append(idx.astype(np.int64) + start_index)
+                chunk_label_lists.append(lab)
+                chunk_clip_lists.append(clips)
+                chunk_bool_lists.append(np.ones(idx.size, dtype=bool))
     finally:
         if seg is not None:
             seg.close()
 
-    if len(secret_spike_indices) > 0:
-        event_indices = np.int64(np.hstack(secret_spike_indices))
-        neuron_labels = np.int64(np.hstack(secret_spike_labels))
-        adjusted_clips = (np.vstack(adjusted_spike_clips)).astype(v_dtype)
-        binary_pursuit_spike_bool = np.hstack(secret_spike_bool)
+    if len(chunk_onset_lists) > 0:
+        event_indices = np.int64(np.hstack(chunk_onset_lists))
+        neuron_labels = np.int64(np.hstack(chunk_label_lists))
+        adjusted_clips = (np.vstack(chunk_clip_lists)).astype(v_dtype)
+        binary_pursuit_spike_bool = np.hstack(chunk_bool_lists)
         event_indices += clip_init_samples
     else:
         event_indices, neuron_labels, binary_pursuit_spike_bool, adjusted_clips = [], [], [], []

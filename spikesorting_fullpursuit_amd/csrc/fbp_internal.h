@@ -181,5 +181,10 @@ hipError_t launch_channel_covariance(const float *clips, uint64_t width, uint64_
                                      double *cov, hipStream_t st);
 hipError_t launch_template_variance(const double *Tm, uint32_t T, uint32_t C, uint32_t N, const double *cov,
                                     double *var, hipStream_t st);
+// flat cross-correlation and channel-aligned dot product of template pairs
+// at lags lag_lo .. lag_lo + n_lags - 1 (separability.hip)
+hipError_t launch_template_pair_lags(const double *Tm, uint32_t C, uint32_t N, const uint32_t *pairs,
+                                     uint64_t n_pairs, int32_t lag_lo, uint32_t n_lags, double *xcorr,
+                                     double *aligned, hipStream_t st);
 
 }  // namespace fbp

@@ -1067,6 +1067,47 @@ int fbp_template_variances(int device, const double *templates, uint32_t n_templ
     return FBP_OK;
 }
 
+int fbp_template_pair_lags(int device, const double *templates, uint32_t n_templates, uint32_t n_chans, uint32_t N,
+                           const uint32_t *pairs, uint64_t n_pairs, int32_t lag_lo, int32_t lag_hi, double *xcorr,
+                           double *aligned)
+{
+    if (n_pairs == 0) return FBP_OK;
+    if (!templates || !pairs || !xcorr) return fail(FBP_EINVAL, "NULL argument");
+    if (n_chans == 0 || N == 0) return fail(FBP_EINVAL, "empty templates");
+    const int64_t L = (int64_t)n_chans * N;
+    if (lag_hi < lag_lo || lag_lo <= -L || lag_hi >= L)
+        return fail(FBP_EINVAL, "lags %d..%d outside -(L-1)..L-1 for L = %lld", lag_lo, lag_hi, (long long)L);
+    if (n_pairs > 0x7fffffffull) return fail(FBP_EINVAL, "too many pairs");
+    for (uint64_t p = 0; p < 2 * n_pairs; p++)
+        if (pairs[p] >= n_templates) return fail(FBP_EINVAL, "pair entry %u out of range (%u templates)", pairs[p],
+                                                 n_templates);
+    const uint32_t n_lags = (uint32_t)(lag_hi - lag_lo + 1);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+    double *d_tm = nullptr, *d_x = nullptr, *d_al = nullptr;
+    uint32_t *d_pairs = nullptr;
+    const uint64_t ntm = (uint64_t)n_templates * L, nout = n_pairs * n_lags;
+    hipError_t e = dalloc(&d_tm, ntm, st);
+    if (e == hipSuccess) e = dalloc(&d_pairs, 2 * n_pairs, st);
+    if (e == hipSuccess) e = dalloc(&d_x, nout, st);
+    if (e == hipSuccess && aligned) e = dalloc(&d_al, nout, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_tm, templates, ntm * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_pairs, pairs, 2 * n_pairs * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_template_pair_lags(d_tm, n_chans, N, d_pairs, n_pairs, lag_lo, n_lags, d_x, d_al, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(xcorr, d_x, nout * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && aligned) e = hipMemcpyAsync(aligned, d_al, nout * 8, hipMemcpyDeviceToHost, st);
+    dfree(d_tm, st);
+    dfree(d_pairs, st);
+    dfree(d_x, st);
+    dfree(d_al, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    if (e == hipSuccess) e = es;
+    HIP_TRY(e, "template pair lags");
+    return FBP_OK;
+}
+
 }  // extern "C"
 
 template <class F>

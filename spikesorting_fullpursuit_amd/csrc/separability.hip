@@ -148,7 +148,68 @@ __global__ __launch_bounds__(kCovThreads) void k_template_variance(const double 
     if (threadIdx.x == 0) var[t] = total;
 }
 
+// Template pairs at a range of lags (consolidate.py:9-71 as called by
+// neuron_separability.py:192-196).  With a = templates[pairs[2p]] and
+// b = templates[pairs[2p+1]], flat vectors of L = C*N samples, and
+// lag = lag_lo + k:
+//   xcorr[p][k]   = sum_i a[i + lag] * b[i]  over 0 <= i, i + lag < L
+//                   (np.correlate(a, b, 'full')[lag + L - 1]),
+//   aligned[p][k] = the same sum over the i whose i and i + lag lie on the
+//                   same channel (the dot product of the two templates after
+//                   optimal_align_templates has shifted and truncated every
+//                   channel by `lag`).
+// One wave per (pair, lag): its lanes take i = i0 + lane + 64m in turn, and
+// the 64 partial sums are added by a fixed butterfly, so each output has one
+// summation order.  Templates are read from global memory (a pair's two
+// rows are reused by every lag of the block and stay in L2).
+constexpr uint32_t kLagWaves = 4;
+
+__global__ __launch_bounds__(kLagWaves * 64) void k_template_pair_lags(const double *__restrict__ Tm, uint32_t C,
+                                                                       uint32_t N, const uint32_t *__restrict__ pairs,
+                                                                       int32_t lag_lo, uint32_t n_lags,
+                                                                       double *__restrict__ xcorr,
+                                                                       double *__restrict__ aligned)
+{
+    const uint64_t p = blockIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const int64_t L = (int64_t)C * N;
+    const double *a = Tm + (uint64_t)pairs[2 * p] * L;
+    const double *b = Tm + (uint64_t)pairs[2 * p + 1] * L;
+    for (uint32_t k = blockIdx.y * kLagWaves + wave; k < n_lags; k += gridDim.y * kLagWaves) {
+        const int64_t lag = (int64_t)lag_lo + k;
+        const int64_t i0 = lag < 0 ? -lag : 0, i1 = lag < 0 ? L : L - lag;
+        double x = 0.0, al = 0.0;
+        for (int64_t i = i0 + lane; i < i1; i += 64) {
+            const double prod = a[i + lag] * b[i];
+            x += prod;
+            if ((i + lag) / N == i / N) al += prod;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            x += __shfl_xor(x, off);
+            al += __shfl_xor(al, off);
+        }
+        if (lane == 0) {
+            xcorr[p * n_lags + k] = x;
+            if (aligned) aligned[p * n_lags + k] = al;
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t launch_template_pair_lags(const double *Tm, uint32_t C, uint32_t N, const uint32_t *pairs,
+                                     uint64_t n_pairs, int32_t lag_lo, uint32_t n_lags, double *xcorr,
+                                     double *aligned, hipStream_t st)
+{
+    if (n_pairs == 0 || n_lags == 0) return hipSuccess;
+    if (n_pairs > 0x7fffffffull) return hipErrorInvalidValue;
+    // about 64 lag waves per block row, at most 1024 blocks along the lags
+    uint32_t gy = (n_lags + kLagWaves * 16 - 1) / (kLagWaves * 16);
+    if (gy > 1024) gy = 1024;
+    hipLaunchKernelGGL(k_template_pair_lags, dim3((uint32_t)n_pairs, gy), dim3(kLagWaves * 64), 0, st, Tm, C, N,
+                       pairs, lag_lo, n_lags, xcorr, aligned);
+    return hipGetLastError();
+}
 
 uint32_t channel_covariance_slices(uint64_t n_rows, uint64_t *slice)
 {
