@@ -70,9 +70,34 @@ def residual_span(chunk_onsets, k, template_samples_per_chan, n_samples):
     return lo, hi
 
 
+AUTO_ROUTES = (("bf16", "screened"), ("fp32", "screened"), ("bf16", "direct"), ("fp32", "direct"))
+
+
+def open_segment(n_chans, K, n_templates, N, M, do_overlap, reduce_work_group, precision, overlap_mode,
+                 max_passes, device, templates, bias, thresholds):
+    """A segment for one chunk with its templates set.  "auto" precision or
+    overlap mode takes the first route of AUTO_ROUTES that the library
+    accepts for these shapes (it refuses a shape with ValueError, before any
+    work); explicit choices are used as given."""
+    routes = [(p, o) for p, o in AUTO_ROUTES
+              if precision in ("auto", p) and overlap_mode in ("auto", o)] or [(precision, overlap_mode)]
+    for k, (prec, mode) in enumerate(routes):
+        seg = None
+        try:
+            seg = _lib.Segment(n_chans, K, n_templates, N, M, do_overlap, reduce_work_group, prec, mode,
+                               max_passes, device)
+            seg.set_templates(templates, bias, thresholds)
+            return seg
+        except ValueError:
+            if seg is not None:
+                seg.close()
+            if k + 1 == len(routes):
+                raise
+
+
 def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shift_inds=None,
-                   kernels_path=None, max_gpu_memory=None, *, return_residual=False, precision="bf16",
-                   overlap_mode="screened", reduce_work_group=256, device=0, max_passes=0, stats=None,
+                   kernels_path=None, max_gpu_memory=None, *, return_residual=False, precision="auto",
+                   overlap_mode="auto", reduce_work_group=256, device=0, max_passes=0, stats=None,
                    verbose=False):
     """Full binary pursuit over `voltage` with the templates in
     `separability_metrics`; same arguments and returns as the reference's
@@ -92,14 +117,19 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
 
     Keyword-only extensions (no reference counterpart):
     return_residual: also return the residual voltage float32 [C][n_samples].
-    precision: correlation arithmetic: "bf16" (default), "fp16" or "fp8"
-        matrix-core screen whose candidates are re-scored in float32, or
-        "fp32" (the reference's float32 chain for every position).  Every
-        choice gives the same spikes, labels, clips and residual.
-    overlap_mode: overlap recheck arithmetic: "screened" (default; float64
-        bounds from matrix-core tables, exact float32 score of every
-        combination that can win) or "direct" (the reference's float32 score
-        of every combination).  Both give the same decisions.
+    precision: correlation arithmetic: "bf16", "fp16" or "fp8" matrix-core
+        screen whose candidates are re-scored in float32, or "fp32" (the
+        reference's float32 chain for every position).  Every choice gives
+        the same spikes, labels, clips and residual.
+    overlap_mode: overlap recheck arithmetic: "screened" (float64 bounds from
+        matrix-core tables, exact float32 score of every combination that can
+        win) or "direct" (the reference's float32 score of every
+        combination).  Both give the same decisions.
+        With "auto" (the default for both) a chunk runs bf16 + screened when
+        its shapes fit those kernels (templates of at most 96 samples, the
+        screened overlap's LDS tables), else the first of fp32 + screened,
+        bf16 + direct, fp32 + direct that fits; all are GPU routes with the
+        same results.
     reduce_work_group: width of the overlap reduction whose tie-break the
         results follow (the reference uses its device's work-group size).
     stats: optional list; per-chunk engine statistics dicts are appended.
@@ -153,9 +183,10 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
             if seg is None or seg_len != K:
                 if seg is not None:
                     seg.close()
-                seg = _lib.Segment(n_chans, K, n_templates, N, n_max_shift_inds, do_overlap,
-                                   reduce_work_group, precision, overlap_mode, max_passes, device)
-                seg.set_templates(templates, template_sum_squared, neuron_lower_thresholds)
+                    seg = None
+                seg = open_segment(n_chans, K, n_templates, N, n_max_shift_inds, do_overlap, reduce_work_group,
+                                   precision, overlap_mode, max_passes, device, templates, template_sum_squared,
+                                   neuron_lower_thresholds)
                 seg_len = K
             chunk = voltage[:, start_index:stop_index]
             seg.set_voltage(chunk)
