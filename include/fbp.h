@@ -91,8 +91,14 @@ enum {
                                       parse                                      */
     FBP_DEBUG_OVERLAP_ALL_ROWS = 16, /* screened overlap: traverse every row, none
                                       skipped by its upper bound                 */
-    FBP_DEBUG_OVERLAP_SMALL_BATCH = 32 /* screened overlap: batches of 37 windows
+    FBP_DEBUG_OVERLAP_SMALL_BATCH = 32, /* screened overlap: batches of 37 windows
                                       (Q tables, then the screen, per batch)     */
+    FBP_DEBUG_OVERLAP_INBLOCK = 64, /* screened overlap: every exact score in the
+                                      screen's block (no deferred exact-score
+                                      kernel)                                    */
+    FBP_DEBUG_OVERLAP_FEW_SLOTS = 128 /* screened overlap: 64 deferred candidate
+                                      slots per batch, so most windows with
+                                      candidates score them in the screen       */
 };
 
 typedef struct fbp_stats {

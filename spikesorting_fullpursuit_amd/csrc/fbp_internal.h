@@ -23,6 +23,8 @@ struct Dims {
     uint64_t nl;       // ceil(items / reduce_wg) reduction groups per window
     uint32_t ov_batch; // screened overlap: windows per Q-table batch (rows of ovQ)
     uint32_t sup_max;  // widest channel support of a template (C until templates are set)
+    uint32_t xc_cap;   // screened overlap: candidate slots per batch of the deferred exact
+                       // scores (k_overlap_exact), 0 = every window scores in the screen
 };
 
 // The re-score queue is split into kQueueShards equal shards; a correlation
@@ -94,6 +96,9 @@ struct DeviceState {
     uint64_t *rq;        // re-score queue: (template << 32) | position
     uint32_t *rq_shard;  // per-shard reservation counters of the queue (entries, positions)
     void *rdesc;         // [W * chunks] (queue start, count) per (window, template chunk)
+    uint32_t *xc_cand;   // [xc_cap] deferred exact-score candidates of a batch (f << 16 | e)
+    uint32_t *xc_win;    // [ov_batch][4] deferred windows: (list position, first candidate, count, 0)
+    uint32_t *xc_ctr;    // [2] deferred windows and candidate slots taken in the batch
 };
 
 constexpr int kCompactItems = 1024;   // items per block of the compaction kernels
@@ -149,6 +154,9 @@ hipError_t launch_pad_templates(const Dims &d, const DeviceState &s, hipStream_t
 hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_t st);
 hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t max_ov, uint32_t debug,
                                   hipStream_t st);
+// candidate slots of the deferred exact scores for a batch (0: templates too
+// large for the exact-score kernel's LDS, every window scores in the screen)
+uint32_t overlap_exact_cap(const Dims &d);
 hipError_t launch_check_overlap(const Dims &d, const DeviceState &s, uint32_t max_ov, hipStream_t st);
 hipError_t launch_pursuit_add(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st);
 hipError_t launch_compact_spikes(const Dims &d, const DeviceState &s, uint32_t n_win,

@@ -1955,7 +1955,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov,
     const float *__restrict__ AB, const float *__restrict__ ABmin, const float *__restrict__ B2,
     const float *__restrict__ tnorm, const float *__restrict__ Tpad, const float *__restrict__ ovQ, uint32_t kbase,
-    const uint32_t *__restrict__ sup_lo, const uint32_t *__restrict__ sup_hi, uint32_t debug)
+    const uint32_t *__restrict__ sup_lo, const uint32_t *__restrict__ sup_hi, uint32_t debug,
+    uint32_t *__restrict__ xc_cand, uint32_t *__restrict__ xc_win, uint32_t *__restrict__ xc_ctr)
 {
     extern __shared__ float dyn[];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
@@ -1981,7 +1982,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ double r_ub[kOvWaves];
     for (uint32_t x = threadIdx.x; x < R; x += blockDim.x) szero[x] = 0.0f;
     for (uint32_t x = threadIdx.x; x < T; x += blockDim.x) stn[x] = tnorm[x];
-    __shared__ uint32_t n_cand, s_settled;
+    __shared__ uint32_t n_cand, s_settled, s_defer;
+    __shared__ uint32_t r_scan[kOvWaves];
     __shared__ float r_sc[4];
     __shared__ uint64_t r_key[4], r_p[4];
     __shared__ double r_d[4][2];
@@ -2048,6 +2050,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 s_maxcnt = 0;
                 n_cand = 0;
                 s_settled = 0;
+                s_defer = 0;
             }
             // staging (index quotients by FastDiv: every index is below the
             // LDS size); the likelihood signs at every in-bounds (t, s), where
@@ -2312,6 +2315,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             lo_max = fmax(fmax(r_d[0][0], r_d[1][0]), fmax(r_d[2][0], r_d[3][0]));
             hi_max = fmax(fmax(r_d[0][1], r_d[1][1]), fmax(r_d[2][1], r_d[3][1]));
             head_step(3);
+            // Reserve `total` slots of the batch's deferred candidates for this
+            // window (thread 0; s_defer = first slot + 1, or 0 when they do
+            // not fit and the window scores its candidates here).
+            auto reserve = [&](uint32_t total) {
+                if (threadIdx.x == 0) {
+                    const uint32_t off = atomicAdd(&xc_ctr[1], total);
+                    if ((uint64_t)off + total <= d.xc_cap) {
+                        const uint32_t w = atomicAdd(&xc_ctr[0], 1u);
+                        xc_win[4 * w] = k;
+                        xc_win[4 * w + 1] = off;
+                        xc_win[4 * w + 2] = total;
+                        xc_win[4 * w + 3] = 0;
+                        s_defer = off + 1;
+                    }
+                }
+                __syncthreads();
+                return s_defer != 0;
+            };
             // Nothing can beat best_L: parse rejects without any exact score.
             if (hi_max > (double)bl) {
                 uint32_t scored = 0;
@@ -2357,6 +2378,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     }
                     if (s_settled) {
                         // parsed from the interval
+                    } else if (d.xc_cap && reserve(nc)) {
+                        // exact scores in k_overlap_exact, which also parses
+                        for (uint32_t k2 = threadIdx.x; k2 < nc; k2 += blockDim.x)
+                            xc_cand[s_defer - 1 + k2] = cand[k2];
                     } else if (nc <= 16 && !(debug & 4u)) {
                         // one at a time, the partner template staged in LDS
                         for (uint32_t k2 = 0; k2 < nc; k2++) {
@@ -2377,24 +2402,247 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         }
                     }
                 } else {
-                    for_each_kept<1>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, sQ, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
-                        const double sh = screen(f, qe, ab);
-                        if (sh + E < lo_max) return;
-                        exact(f, t, s, Tpad + ((uint64_t)t * C + sup_lo[t]) * R);
-                    });
+                    // Some thread dropped a combination that reaches lo_max:
+                    // count every thread's qualifying combinations and place
+                    // them, in each thread's traversal order, after a block-wide
+                    // prefix sum of the counts.
+                    bool deferred = false;
+                    if (d.xc_cap) {
+                        // Candidates grouped by main shift f (a counting sort
+                        // over the S shifts in the candidate buffer, which this
+                        // path does not use otherwise), so that the exact-score
+                        // kernel's batches mostly share f; with more shifts than
+                        // the buffer holds, each thread's candidates follow a
+                        // block-wide prefix sum of the threads' counts.
+                        const bool by_f = S <= kOvCand;
+                        uint32_t *fpos = cand;   // [S] counts, then next slot per shift
+                        if (by_f) {
+                            for (uint32_t x = threadIdx.x; x < S; x += blockDim.x) fpos[x] = 0;
+                            __syncthreads();
+                        }
+                        uint32_t mine = 0;
+                        for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, sQ, sE, [&](uint32_t f, uint32_t, uint32_t, uint32_t, float ab, float qe, double E) {
+                            if (screen(f, qe, ab) + E >= lo_max) {
+                                if (by_f) atomicAdd(&fpos[f], 1u);
+                                else mine++;
+                            }
+                        });
+                        const uint32_t l = lane_of();
+                        uint32_t before = 0, total = 0, incl = mine;
+                        if (by_f) {
+                            __syncthreads();
+                            if (threadIdx.x < kWave) {   // exclusive prefix over the shifts, one wave
+                                uint32_t carry = 0;
+                                for (uint32_t x0 = 0; x0 < S; x0 += kWave) {
+                                    const uint32_t v = x0 + l < S ? fpos[x0 + l] : 0u;
+                                    uint32_t inc = v;
+                                    for (int off = 1; off < 64; off <<= 1) {
+                                        const uint32_t o = __shfl_up(inc, off);
+                                        if (l >= (uint32_t)off) inc += o;
+                                    }
+                                    if (x0 + l < S) fpos[x0 + l] = carry + inc - v;
+                                    carry += __shfl(inc, 63);
+                                }
+                                if (l == 0) r_scan[0] = carry;
+                            }
+                            __syncthreads();
+                            total = r_scan[0];
+                        } else {
+                            for (int off = 1; off < 64; off <<= 1) {
+                                const uint32_t o = __shfl_up(incl, off);
+                                if (l >= (uint32_t)off) incl += o;
+                            }
+                            if (l == 63) r_scan[threadIdx.x >> 6] = incl;
+                            __syncthreads();
+                            for (uint32_t w = 0; w < kOvWaves; w++) {
+                                if (w < (threadIdx.x >> 6)) before += r_scan[w];
+                                total += r_scan[w];
+                            }
+                        }
+                        if (reserve(total)) {
+                            const uint32_t first = s_defer - 1;
+                            uint32_t slot = first + before + incl - mine;
+                            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, sQ, sE, [&](uint32_t f, uint32_t, uint32_t, uint32_t e, float ab, float qe, double E) {
+                                if (screen(f, qe, ab) + E >= lo_max)
+                                    xc_cand[by_f ? first + atomicAdd(&fpos[f], 1u) : slot++] = (f << 16) | e;
+                            });
+                            deferred = true;
+                        }
+                    }
+                    if (!deferred)
+                        for_each_kept<1>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, sQ, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
+                            const double sh = screen(f, qe, ab);
+                            if (sh + E < lo_max) return;
+                            exact(f, t, s, Tpad + ((uint64_t)t * C + sup_lo[t]) * R);
+                        });
                 }
                 for (int off = 32; off > 0; off >>= 1) scored += __shfl_xor(scored, off);
                 if (lane_of() == 0) acc_scored += scored;
-                if (!s_settled) block_key_reduce(sc, key, bp, r_sc, r_key, r_p);
+                if (!s_settled && !s_defer) block_key_reduce(sc, key, bp, r_sc, r_key, r_p);
             }
         }
-        if (threadIdx.x == 0 && (wskip || !s_settled))
+        if (threadIdx.x == 0 && (wskip || !s_settled) && !(wskip ? 0u : s_defer))
             parse_overlap(d, L, thr, id, lab, bidx, bl, sc, bp, best_L, ov_idx, ov_lab, ov);
         head_step(3);
     }
     // statistics: one update per wave for all of its windows
     if (lane_of() == 0 && acc_screened) atomicAdd(&ctr_rw->screened, acc_screened);
     if (lane_of() == 0 && acc_scored) atomicAdd(&ctr_rw->combos, acc_scored);
+}
+
+// Deferred exact scores of the screened overlap (bp.cl:599-640 for the
+// candidates the screen could not rule out, then the parse of bp.cl:690-814).
+// One wave per deferred window, its lanes one candidate each, 64 at a time;
+// the templates sit in LDS ([T][C][N], staged once per block), the window's
+// voltage in the wave's lanes (one sample each, a channel at a time).
+// Each lane's chain is the reference's float32 order; the winner is reduced
+// in the reference's total order (score, then reduction group and lane).
+constexpr uint32_t kXcThreads = 1024;   // two blocks per CU (LDS), eight waves per SIMD
+constexpr uint64_t kXcMaxLds = 80 * 1024;
+
+// Zero floats before and after the exact-score kernel's LDS templates: a
+// partner read at any shift (row offset in [-M, N + M)) stays inside them.
+__host__ __device__ inline uint32_t overlap_exact_guard(const Dims &d) { return d.M + 8; }
+
+// The window's voltage of a channel is held one sample per lane (N <= 64)
+// and read by the wave at each step with v_readlane (a scalar operand), the
+// next channel's row loaded while the current one is summed.  UF: the lanes
+// share the main shift f, so the main template's shifted row is held the same
+// way; otherwise every lane reads its own from LDS.  The partner's samples
+// are read from LDS per lane; a read outside the row lands in the zero guard
+// around the templates or in a neighbouring row and is replaced by +0.0.  A
+// channel where neither of a lane's templates is nonzero leaves its chain
+// unchanged (every sum is +0.0, so ss stays 0 and acc, never -0, is kept).
+template <bool UF>
+__device__ __forceinline__ float exact_score_lds(const Dims &d, const float *ts, const float *__restrict__ V,
+                                                 uint32_t bidx, uint32_t lab, uint32_t lo_m, uint32_t hi_m, uint32_t t,
+                                                 uint32_t lo_t, uint32_t hi_t, uint32_t f, uint32_t s, uint32_t cw0,
+                                                 uint32_t cw1)
+{
+    const uint32_t N = d.N, C = d.C, lane = lane_of();
+    const int32_t da = (int32_t)d.M - (int32_t)f, db = (int32_t)d.M - (int32_t)s;
+    // valid samples of the shifted rows: main [ja0, ja0 + na), partner [jb0, jb0 + nb)
+    const uint32_t ja0 = da < 0 ? (uint32_t)(-da) : 0u, jb0 = db < 0 ? (uint32_t)(-db) : 0u;
+    const uint32_t ja1 = da > 0 ? (N > (uint32_t)da ? N - (uint32_t)da : 0u) : N;
+    const uint32_t jb1 = db > 0 ? (N > (uint32_t)db ? N - (uint32_t)db : 0u) : N;
+    const uint32_t na = ja1 > ja0 ? ja1 - ja0 : 0u, nb = jb1 > jb0 ? jb1 - jb0 : 0u;
+    auto vrow = [&](uint32_t c) {
+        return (c < cw1 && lane < N) ? V[(uint64_t)c * d.K + bidx + lane] : 0.0f;
+    };
+    float vreg = vrow(cw0);
+    float acc = 0.0f;
+    for (uint32_t c = cw0; c < cw1; c++) {
+        const float vnext = vrow(c + 1);
+        const bool ha = c >= lo_m && c < hi_m, hb = c >= lo_t && c < hi_t;
+        const float *ta = ts + ((uint64_t)lab * C + c) * N + da;
+        const float *tb = ts + ((uint64_t)t * C + c) * N + db;
+        const uint32_t nac = ha ? na : 0u, nbc = hb ? nb : 0u;
+        float areg = 0.0f;
+        if (UF) {
+            const float x = ta[lane < N ? lane : 0u];
+            areg = lane - ja0 < nac ? x : 0.0f;
+        }
+        // Steps past N (up to the next multiple of 8, at most 64) are no-ops:
+        // both operands are +0.0 there (masked, and lanes >= N of vreg and
+        // areg hold 0), so sum = +0.0 leaves ss and acc unchanged; the reads
+        // past a row stay inside the guard.
+        float ss = 0.0f;
+        for (uint32_t j = 0; j < N; j += 8) {
+            float av[8], bv[8];
+#pragma unroll
+            for (uint32_t q = 0; q < 8; q++) {
+                const float b = tb[j + q];
+                bv[q] = j + q - jb0 < nbc ? b : 0.0f;
+                if (!UF) {
+                    const float a = ta[j + q];
+                    av[q] = j + q - ja0 < nac ? a : 0.0f;
+                }
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < 8; q++) {
+                const float ax = UF ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(areg), j + q)) : av[q];
+                const float v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vreg), j + q));
+                const float sum = ax + bv[q];
+                ss = __builtin_fmaf(sum, sum, ss);      // bp.cl:621
+                acc = __builtin_fmaf(sum, v, acc);      // bp.cl:622
+            }
+        }
+        if (ss > 0.0f) acc = (float)((double)acc - 0.5 * (double)ss);   // bp.cl:627-630
+        vreg = vnext;
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(kXcThreads) void k_overlap_exact(
+    Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
+    const float *__restrict__ thr, const uint32_t *__restrict__ list, const uint32_t *__restrict__ xc_win,
+    const uint32_t *__restrict__ xc_ctr, const uint32_t *__restrict__ xc_cand, Counters *ctr, float *best_L,
+    const uint32_t *__restrict__ best_lab, const uint32_t *__restrict__ best_idx, uint32_t *ov_idx, uint32_t *ov_lab,
+    uint8_t *ov, const uint32_t *__restrict__ sup_lo, const uint32_t *__restrict__ sup_hi)
+{
+    extern __shared__ float xs[];   // zero guard, [T][C][N] templates, zero guard
+    const uint64_t TCN = (uint64_t)d.T * d.C * d.N;
+    const uint32_t G = overlap_exact_guard(d);
+    float *ts = xs + G;
+    for (uint64_t e = threadIdx.x; e < TCN + 2 * G; e += blockDim.x)
+        xs[e] = e >= G && e < G + TCN ? Tm[e - G] : 0.0f;
+    __syncthreads();
+    // every per-window value is wave-uniform (read through readfirstlane, so
+    // it lives in scalar registers and the window's loads are scalar)
+    auto uni = [](uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); };
+    const uint32_t n_win = uni(xc_ctr[0]);
+    const uint32_t lane = lane_of();
+    const uint32_t nwave = blockDim.x >> 6;
+    const uint32_t wg_bits = 31 - __builtin_clz(d.reduce_wg);
+    const uint64_t span = d.nl * d.reduce_wg;
+    const FastDiv divS(d.S);
+    unsigned long long scored = 0;
+    for (uint32_t w = uni(blockIdx.x * nwave + (threadIdx.x >> 6)); w < n_win; w = uni(w + gridDim.x * nwave)) {
+        const uint32_t k = uni(xc_win[4 * w]), first = uni(xc_win[4 * w + 1]), count = uni(xc_win[4 * w + 2]);
+        const uint32_t id = uni(list[k]), lab = uni(best_lab[id]), bidx = uni(best_idx[id]);
+        const float bl = __uint_as_float(uni(__float_as_uint(best_L[id])));
+        const uint32_t lo_m = uni(sup_lo[lab]), hi_m = uni(sup_hi[lab]);
+        float sc = -INFINITY;
+        uint64_t key = ~0ull, bp = 0;
+        for (uint32_t i0 = 0; i0 < count; i0 += 64) {
+            const bool act = i0 + lane < count;
+            const uint32_t cnd = act ? xc_cand[first + i0 + lane] : 0u;
+            const uint32_t f = cnd >> 16, e = cnd & 0xffffu;
+            const uint32_t t = divS.div(e), s = e - t * d.S;
+            const uint32_t lo_t = act ? sup_lo[t] : lo_m, hi_t = act ? sup_hi[t] : lo_m;
+            // channel range of the wave: the union of the lanes' supports
+            uint32_t c0 = min(lo_m, lo_t), c1 = max(hi_m, hi_t);
+            for (int off = 32; off > 0; off >>= 1) {
+                c0 = min(c0, (uint32_t)__shfl_xor(c0, off));
+                c1 = max(c1, (uint32_t)__shfl_xor(c1, off));
+            }
+            // batches that share the main shift (the screen groups a
+            // window's candidates by it) take the segmented chain
+            const uint32_t f0 = __builtin_amdgcn_readfirstlane(f);
+            const bool same_f = __ballot(act && f != f0) == 0;
+            const uint32_t cw0 = __builtin_amdgcn_readfirstlane(c0), cw1 = __builtin_amdgcn_readfirstlane(c1);
+            const float score = same_f
+                ? exact_score_lds<true>(d, ts, V, bidx, lab, lo_m, hi_m, t, act ? lo_t : 0u, act ? hi_t : 0u, f0, s,
+                                        cw0, cw1)
+                : exact_score_lds<false>(d, ts, V, bidx, lab, lo_m, hi_m, t, act ? lo_t : 0u, act ? hi_t : 0u, f, s,
+                                         cw0, cw1);
+            if (act) {
+                uint64_t p;
+                const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
+                if (key_better(score, kk, sc, key)) { sc = score; key = kk; bp = p; }
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const float o = __shfl_xor(sc, off);
+            const uint64_t ok = __shfl_xor(key, off), op = __shfl_xor(bp, off);
+            if (key_better(o, ok, sc, key)) { sc = o; key = ok; bp = op; }
+        }
+        if (lane == 0) {
+            parse_overlap(d, L, thr, id, lab, bidx, bl, sc, bp, best_L, ov_idx, ov_lab, ov);
+            scored += count;
+        }
+    }
+    if (lane == 0 && scored) atomicAdd(&ctr->combos, scored);
 }
 
 // check_overlap_reassignments (bp.cl:822-875).  ov bit 1 marks a window that
@@ -2868,6 +3116,16 @@ hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_
     return hipGetLastError();
 }
 
+uint32_t overlap_exact_cap(const Dims &d)
+{
+    // a channel's window samples are held one per lane
+    if (d.N > (uint32_t)kWave || ((uint64_t)d.T * d.C * d.N + 2 * overlap_exact_guard(d)) * 4 > kXcMaxLds) return 0;
+    uint64_t cap = (uint64_t)d.ov_batch * 256;
+    if (cap > (32ull << 20)) cap = 32ull << 20;
+    if (cap < 4096) cap = 4096;
+    return (uint32_t)cap;
+}
+
 hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t max_ov, uint32_t debug,
                                   hipStream_t st)
 {
@@ -2891,6 +3149,12 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
                                            (int)qlds);
         if (e != hipSuccess) return e;
     }
+    const size_t xlds = ((size_t)d.T * d.C * d.N + 2 * overlap_exact_guard(d)) * 4;
+    if (d.xc_cap && xlds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_overlap_exact, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)xlds);
+        if (e != hipSuccess) return e;
+    }
     // batches of ov_batch windows: Q tables (matrix cores), then the screen
     for (uint32_t kb = 0; kb < max_ov; kb += d.ov_batch) {
         const uint32_t nb = max_ov - kb < d.ov_batch ? max_ov - kb : d.ov_batch;
@@ -2902,13 +3166,28 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
             hipLaunchKernelGGL(k_overlap_q, dim3(grid), dim3(256), qlds, st, d, s.V, s.L, s.thr, list, s.ctr,
                                s.best_lab, s.best_idx, s.tabB2, (const uint4 *)s.TmQ, s.ovQ, kb, s.tord, s.tile_lo,
                                s.tile_hi);
+        }
+        if (d.xc_cap) {
+            hipError_t e = launch_fill_u32(s.xc_ctr, 2, 0u, st);
+            if (e != hipSuccess) return e;
+        }
+        if (mq) {
             hipLaunchKernelGGL(k_overlap_screened<true>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                                list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                               s.tabABmin, s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug);
+                               s.tabABmin, s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug,
+                               s.xc_cand, s.xc_win, s.xc_ctr);
         } else {
             hipLaunchKernelGGL(k_overlap_screened<false>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
                                list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                               s.tabABmin, s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug);
+                               s.tabABmin, s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug,
+                               s.xc_cand, s.xc_win, s.xc_ctr);
+        }
+        if (d.xc_cap) {
+            // every deferred window of the batch: one wave each, 16 per block
+            const uint32_t xgrid = (nb + 15) / 16 < 1024 ? (nb + 15) / 16 : 1024;
+            hipLaunchKernelGGL(k_overlap_exact, dim3(xgrid), dim3(kXcThreads), xlds, st, d, s.V, s.Tm, s.L, s.thr,
+                               list, s.xc_win, s.xc_ctr, s.xc_cand, s.ctr, s.best_L, s.best_lab, s.best_idx,
+                               s.ov_idx, s.ov_lab, s.ov, s.sup_lo, s.sup_hi);
         }
     }
     return hipGetLastError();

@@ -193,6 +193,11 @@ void for_each_buffer(const fbp_config *cfg, const Dims &d, DeviceState &s, V &&v
         visit(s.TmQ, overlap_qtemplate_floats(d));
         visit(s.ov_sorted, W);
         visit(s.ovQ, (uint64_t)d.ov_batch * T * d.S);
+        if (d.xc_cap) {
+            visit(s.xc_cand, (uint64_t)d.xc_cap);
+            visit(s.xc_win, (uint64_t)d.ov_batch * 4);
+            visit(s.xc_ctr, (uint64_t)2);
+        }
     }
     if (cfg->precision != FBP_PREC_FP32) {
         visit(s.rq, rescore_queue_cap(d));
@@ -389,6 +394,9 @@ static int make_dims(const fbp_config *cfg, Dims &d)
     if (cfg->overlap_mode == FBP_OVERLAP_SCREENED && overlap_screened_lds(d) > kMaxLds)
         return fail(FBP_EINVAL, "screened overlap needs %zu bytes of LDS (T*S too large)", overlap_screened_lds(d));
     d.sup_max = C;
+    d.xc_cap = (cfg->overlap_mode == FBP_OVERLAP_SCREENED && !(cfg->debug_flags & FBP_DEBUG_OVERLAP_INBLOCK))
+                   ? overlap_exact_cap(d) : 0;
+    if (d.xc_cap && (cfg->debug_flags & FBP_DEBUG_OVERLAP_FEW_SLOTS)) d.xc_cap = 64;
     return FBP_OK;
 }
 

@@ -76,15 +76,19 @@ def test_dense_16ch_spike_list_grows(oracle_mod, mode):
         assert g['stats']['screened_combos'] > 20 * g['stats']['overlap_combos']
 
 
-@pytest.mark.parametrize("debug", [0, 2, 4, 8, 12, 16, 18, 32, 50])
+@pytest.mark.parametrize("debug", [0, 2, 4, 8, 12, 16, 18, 32, 50, 64, 66, 68, 72, 76, 80, 128, 130, 136, 160])
 def test_screened_overlap_candidate_paths(oracle_mod, debug):
     # The screened overlap finds its exact-score candidates from per-thread
-    # lists (partner templates staged in LDS) and settles single candidates
-    # from their score interval, unless forced onto the second traversal (2),
-    # global partner reads (4) or exact scores for single candidates (8); rows
-    # of the traversal are skipped by their upper bound unless 16 is set; 32
-    # splits each pass's overlap windows into batches of 37 (the Q tables and
-    # the screen run per batch).  Every route gives the oracle's events.
+    # lists and settles single candidates from their score interval, unless
+    # forced onto the second traversal (2) or exact scores for single
+    # candidates (8); rows of the traversal are skipped by their upper bound
+    # unless 16 is set; 32 splits each pass's overlap windows into batches of
+    # 37 (the Q tables and the screen run per batch).  The candidates are
+    # scored by the deferred exact-score kernel (LDS templates, one wave per
+    # window), or in the screen's block with 64 (partner templates staged in
+    # LDS, or read from global memory with 4); 128 leaves 64 deferred slots
+    # per batch, so windows of both kinds meet in one batch.  Every route
+    # gives the oracle's events.
     from spikesorting_fullpursuit_amd import synthetic
     rec = synthetic.config("16ch-300s", seed=7, duration_s=0.5)
     sm = rec['separability_metrics']
