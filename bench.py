@@ -139,7 +139,7 @@ def main():
         lrec = synthetic.config("128ch-600s", seed=args.seed, duration_s=args.large_probe_s)
         LN = lrec['sort_info']['n_samples_per_chan']
         large, _ = pursuit_entry(args, lrec, args.precision, args.overlap_mode, LN // 2, 1, warmup=1,
-                                 workload="128ch-600s")
+                                 workload="128ch-600s", duration=args.large_probe_s)
         large["workload"] = "128ch-600s"
         large["duration_s"] = lrec['voltage'].shape[1] / lrec['sort_info']['sampling_rate']
         log("large probe (128 ch, 200 templates, %.0f s): %.2f s per pursuit, %d passes, %.3g ch*samples/s"
@@ -197,7 +197,8 @@ def main():
     return 0
 
 
-def pursuit_entry(args, rec, precision, overlap_mode, M, steps, warmup=1, workload=None, ref_spikes=None):
+def pursuit_entry(args, rec, precision, overlap_mode, M, steps, warmup=1, workload=None, ref_spikes=None,
+                  duration=None):
     """Time `steps` full pursuits of one recording on one segment (voltage
     resident in HBM) with the given arithmetic and shift range; returns the
     throughput, the stage split and the roofline entry of its dominant stage.
@@ -226,7 +227,8 @@ def pursuit_entry(args, rec, precision, overlap_mode, M, steps, warmup=1, worklo
                 sums[key] += st[key]
         dt = (time.perf_counter() - t) / steps
         idx, lab = sg.spikes()
-    ns = argparse.Namespace(precision=precision, overlap_mode=overlap_mode, workload=workload or args.workload)
+    ns = argparse.Namespace(precision=precision, overlap_mode=overlap_mode, workload=workload or args.workload,
+                            duration=duration if workload else args.duration)
     stages, roof = roofline_of(ns, sums, C, T, N, M, K, int(st["n_passes"]))
     out = {"value": C * K / dt, "unit": "ch·samples/s", "ms_per_step": 1000.0 * dt, "steps": steps,
            "precision": precision, "overlap_mode": overlap_mode, "n_max_shift_inds": M,
@@ -556,26 +558,55 @@ def serving_mix_run(args):
             "events_equal_standalone": True}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kernels_16ch-300s_bf16-screened.json")
+def profile_name(workload, duration_s, precision, overlap_mode, M):
+    """Base name of the committed rocprofv3 summary of one pursuit
+    configuration under profiles/ (kernel table, calibrated HBM traffic,
+    matrix-core counters)."""
+    if duration_s is not None:
+        workload = "%s-%ds" % (workload.split("-")[0], int(round(duration_s)))
+    return "kernels_%s_%s-%s_M%d" % (workload, precision, overlap_mode, M)
 
 
-def measured_traffic(kernels, passes, workload, precision, overlap_mode):
-    """HBM bytes per pass of a stage (FETCH_SIZE + WRITE_SIZE of its kernels,
-    calibrated against a copy of known size) from the committed rocprofv3
-    summary of this workload, or None."""
-    if workload != "16ch-300s" or precision != "bf16" or overlap_mode != "screened":
-        return None
+def load_profile(workload, duration_s, precision, overlap_mode, M):
     try:
-        tr = json.load(open(TRAFFIC_FILE))["traffic"]
-    except (OSError, ValueError, KeyError):
+        return json.load(open(os.path.join(ROOT, "profiles",
+                                           profile_name(workload, duration_s, precision, overlap_mode, M) + ".json")))
+    except (OSError, ValueError):
         return None
-    total = 0.0
-    for k in kernels:
-        t = tr.get(k)
-        if not t or t["fetch_bytes_per_dispatch"] is None or t["write_bytes_per_dispatch"] is None:
+
+
+# kernels of each stage, by name prefix (template arguments vary with the shapes)
+STAGE_KERNELS = {"correlation": ("k_corr_mfma", "k_rescore", "k_corr_fp32"),
+                 "overlap": ("k_overlap_q", "k_overlap_screened", "k_overlap_exact", "k_overlap_direct",
+                             "k_sort_ov_by_label", "k_check_overlap")}
+
+
+def measured_traffic(prof, stage, passes):
+    """HBM bytes per pass of a stage (FETCH_SIZE + WRITE_SIZE of its kernels,
+    calibrated against a copy of known size) from a committed rocprofv3
+    summary, or None.  The summary covers one warm-up and one timed run."""
+    if not prof:
+        return None
+    total, seen = 0.0, False
+    for k, t in prof.get("traffic", {}).items():
+        if not k.startswith(STAGE_KERNELS[stage]):
+            continue
+        if t["fetch_bytes_per_dispatch"] is None or t["write_bytes_per_dispatch"] is None:
             return None
         total += (t["fetch_bytes_per_dispatch"] + t["write_bytes_per_dispatch"]) * t["dispatches"]
-    return total / max(1, passes)
+        seen = True
+    runs = prof.get("runs", 2)
+    return total / runs / max(1, passes) if seen else None
+
+
+def measured_mfma(prof, stage):
+    """Matrix-core utilisation (percent of SIMD cycles) of a stage's matrix-core
+    kernels from a committed summary: {kernel: percent}."""
+    if not prof:
+        return None
+    out = {k: v["mfma_util_pct"] for k, v in prof.get("mfma", {}).items()
+           if k.startswith(STAGE_KERNELS[stage]) and v.get("mfma_util_pct") is not None}
+    return out or None
 
 
 def roofline_of(args, sums, C, T, N, M, K, passes_per_step):
@@ -619,24 +650,29 @@ def roofline_of(args, sums, C, T, N, M, K, passes_per_step):
         nbytes = 4.0 * sums["screened_combos"] + win * (2.0 * C * N * 4 + C * R * 4 + 3.0 * T * S * 4)
         valid = sum(min(N, N + s - M) - max(0, s - M) for s in range(S))
         qflop = 3.0 * 2.0 * C * T * valid * win
-        stages["overlap"] = {"kernel": "k_overlap_q + k_overlap_screened", "ms": sums["ms_overlap"],
+        stages["overlap"] = {"kernel": "k_overlap_q + k_overlap_screened + k_overlap_exact", "ms": sums["ms_overlap"],
                              "launches": sums["overlap_launches"], "bound": "hbm", "unit": "GB/s",
                              "achieved": nbytes / t_ov / 1e9 if t_ov else 0.0, "peak": PEAK_HBM_GBS,
                              "mfma_tflops": qflop / t_ov / 1e12 if t_ov else 0.0, "mfma_peak": PEAK_TFLOPS["bf16"],
                              "work": "4 B of the AB table per unskipped combination + per window voltage, "
                                      "main template, likelihood signs and the Q table (bytes); split-bf16 "
                                      "Q table (matrix-core flop)"}
+    prof = load_profile(args.workload, getattr(args, "duration", None), args.precision, args.overlap_mode, M)
+    for name, stg in stages.items():
+        util = measured_mfma(prof, name)
+        if util:
+            stg["mfma_util_pct"] = util
+        stg["traffic_per_pass"] = measured_traffic(prof, name, passes_per_step)
     dom = max(stages, key=lambda k: stages[k]["ms"])
     st = stages[dom]
-    kernels = {"correlation": ["k_corr_mfma<1, 4, 2>", "k_rescore_res"],
-               "overlap": ["k_overlap_q", "k_overlap_screened<true>", "k_sort_ov_by_label", "k_check_overlap"]}
-    traffic = measured_traffic(kernels.get(dom, []), passes_per_step, args.workload, args.precision,
-                               args.overlap_mode)
+    traffic = st["traffic_per_pass"]
     per_launch = st["achieved"] * (st["ms"] * 1e-3 / max(1, st["launches"])) * (1e9 if st["unit"] == "GB/s" else 1e12)
     roofline = {"bound": st["bound"], "kernel": st["kernel"], "achieved": st["achieved"], "peak": st["peak"],
                 "unit": st["unit"], "frac": st["achieved"] / st["peak"],
                 "algorithmic_per_launch": per_launch, "ms_per_launch": st["ms"] / max(1, st["launches"]),
                 "traffic": traffic, "traffic_unit": "bytes per pass (rocprofv3 FETCH_SIZE + WRITE_SIZE)",
+                "profile": ("profiles/" + profile_name(args.workload, getattr(args, "duration", None), args.precision,
+                                                       args.overlap_mode, M) + ".md") if prof else None,
                 "work": st["work"]}
     return stages, roofline
 
