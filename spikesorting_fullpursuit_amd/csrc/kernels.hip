@@ -1153,7 +1153,7 @@ __device__ __forceinline__ bool parse_choose(const Dims &d, const float *__restr
     return false;
 }
 
-__device__ void parse_overlap(const Dims &d, const float *__restrict__ L, const float *__restrict__ thr, uint32_t id,
+__device__ __forceinline__ void parse_overlap(const Dims &d, const float *__restrict__ L, const float *__restrict__ thr, uint32_t id,
                               uint32_t lab, uint32_t bidx, float bl, float sc, uint64_t bp, float *best_L,
                               uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov)
 {
@@ -1177,7 +1177,7 @@ __device__ void parse_overlap(const Dims &d, const float *__restrict__ L, const 
 // The same parse for a winner whose float32 score is only known to lie in
 // [lo, hi]: returns false, writing nothing, when a comparison of the score is
 // left open by the interval.
-__device__ bool parse_overlap_bounds(const Dims &d, const float *__restrict__ L, const float *__restrict__ thr,
+__device__ __forceinline__ bool parse_overlap_bounds(const Dims &d, const float *__restrict__ L, const float *__restrict__ thr,
                                      uint32_t id, uint32_t lab, uint32_t bidx, float bl, double lo, double hi,
                                      uint64_t bp, float *best_L, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov)
 {
@@ -1201,7 +1201,7 @@ __device__ bool parse_overlap_bounds(const Dims &d, const float *__restrict__ L,
 
 // Block-wide reduction of (score, key, work id) in winner order; the result is
 // valid in thread 0.
-__device__ void block_key_reduce(float &sc, uint64_t &key, uint64_t &bp, float *r_sc, uint64_t *r_key, uint64_t *r_p)
+__device__ __forceinline__ void block_key_reduce(float &sc, uint64_t &key, uint64_t &bp, float *r_sc, uint64_t *r_key, uint64_t *r_p)
 {
     for (int off = 32; off > 0; off >>= 1) {
         const float o = __shfl_xor(sc, off);
@@ -1259,14 +1259,14 @@ __device__ __forceinline__ float overlap_direct_score(const Dims &d, const float
 
 // The same score over the channels where either template is nonzero: the
 // main template's rows amain[c - lo_m] for c in [lo_m, hi_m), the partner's
-// bt[c - lo_t] for c in [lo_t, hi_t), a zero row elsewhere.  A channel where
+// bt[c - lo_t] for c in [lo_t, hi_t), +0.0 elsewhere.  A channel where
 // both are zero adds exactly 0 to the reference chain (every product is +-0,
 // the running sum is never -0 and ss = 0 skips the subtraction), so it is
 // skipped.
 __device__ __forceinline__ float overlap_support_score(const Dims &d, const float *__restrict__ sv,
                                                        const float *amain, uint32_t lo_m, uint32_t hi_m,
                                                        const float *bt, uint32_t lo_t, uint32_t hi_t,
-                                                       const float *zero, uint32_t f, uint32_t s)
+                                                       uint32_t f, uint32_t s)
 {
     const uint32_t R = d.N + 2 * d.M;
     const uint32_t c0 = min(lo_m, lo_t), c1 = max(hi_m, hi_t);
@@ -1274,17 +1274,18 @@ __device__ __forceinline__ float overlap_support_score(const Dims &d, const floa
     for (uint32_t c = c0; c < c1; c++) {
         const bool ha = c >= lo_m && c < hi_m, hb = c >= lo_t && c < hi_t;
         if (!ha && !hb) continue;
-        const float *a = (ha ? amain + (c - lo_m) * R : zero) + 2 * d.M - f;
-        const float *b = (hb ? bt + (uint64_t)(c - lo_t) * R : zero) + 2 * d.M - s;
+        const float *a = amain + (uint64_t)(ha ? c - lo_m : 0u) * R + 2 * d.M - f;
+        const float *b = bt + (uint64_t)(hb ? c - lo_t : 0u) * R + 2 * d.M - s;
         const float *vc = sv + c * d.N;
         float ss = 0.0f;
+        // operands are loaded eight samples at a time ahead of the chains
         uint32_t j = 0;
         for (; j + 8 <= d.N; j += 8) {
             float av[8], bv[8], vv[8];
 #pragma unroll
             for (int q = 0; q < 8; q++) {
-                av[q] = a[j + q];
-                bv[q] = b[j + q];
+                av[q] = ha ? a[j + q] : 0.0f;
+                bv[q] = hb ? b[j + q] : 0.0f;
                 vv[q] = vc[j + q];
             }
 #pragma unroll
@@ -1295,7 +1296,7 @@ __device__ __forceinline__ float overlap_support_score(const Dims &d, const floa
             }
         }
         for (; j < d.N; j++) {
-            const float sum = a[j] + b[j];
+            const float sum = (ha ? a[j] : 0.0f) + (hb ? b[j] : 0.0f);
             ss = __builtin_fmaf(sum, sum, ss);
             acc = __builtin_fmaf(sum, vc[j], acc);
         }
@@ -1518,9 +1519,6 @@ struct FastDiv {
     __device__ uint32_t div(uint32_t x) const { return (uint32_t)(((uint64_t)x * m) >> 32); }
 };
 
-// Screen candidates listed per window for the exact scores: four per thread.
-constexpr uint32_t kOvCand = 4 * 256;
-
 // Calls fn(f, t, s, e = t*S + s, AB[lab][f][t][s], sQ[e], sE[t]) for every
 // combination of one window that the reference does not skip (bp.cl:534-576)
 // and whose row (main shift f, template t) is listed in krow, in no
@@ -1620,7 +1618,7 @@ __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT
 // tile, up to four shift tiles (each A fragment feeds up to 12 MFMAs), and one
 // contiguous slice of the channels when there are too few tiles to occupy the
 // block's waves; slice partials are added in slice order afterwards.
-constexpr uint32_t kOvWaves = 4;   // waves of a k_overlap_screened block
+constexpr uint32_t kOvWaves = 4;   // waves of a k_overlap_q block
 
 // With at most four tiles, the templates past the last full 16-row tile
 // (TR = T % 16 of them, TR <= 8) share one tile as ND shift offsets: row g*TR + i holds template
@@ -1946,10 +1944,13 @@ __global__ void k_pack_qtemplates(Dims d, const float *__restrict__ Tm, const ui
 }
 
 // Register budget of four waves per SIMD (measured faster than three despite a
-// few spilled registers); LDS allows four blocks per CU at the 16-channel
-// shapes.
-template <bool MQ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_overlap_screened(
+// few spilled registers).  NW waves per block: 4 when the block also scores
+// its candidates (main and partner templates staged in LDS), 2 when the
+// exact scores are deferred to k_overlap_exact -- the lean block holds no
+// template rows, so twice as many windows are in flight per CU at the same
+// wave occupancy (the kernel is bound by the latency of its phases).
+template <bool MQ, int NW>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_overlap_screened(
     Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
     const float *__restrict__ thr, const uint32_t *__restrict__ ov_list, Counters *ctr_rw, float *best_L,
     const uint32_t *best_lab, const uint32_t *best_idx, uint32_t *ov_idx, uint32_t *ov_lab, uint8_t *ov,
@@ -1964,30 +1965,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const QPlan qp = q_plan(d);
     const FastDiv divN(N), divS(S), divT(T);
     double *sE = (double *)dyn;                         // [T] screen bound per template (this window)
+    constexpr bool kLean = NW == 2;
+    constexpr uint32_t kCand = 4 * NW * 64;             // candidate list: four per thread
     float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset
     float *smain = sv + C * N;                          // [SM][R] padded best-label template, its support rows
-    float *szero = smain + SM * R;                      // [R] zeros
-    float *sQ = szero + R;                              // [T][S] Q - B2/2
+    float *sQ = smain + (kLean ? 0u : SM * R);          // [T][S] Q - B2/2
     float *stn = sQ + TS;                               // [T] template norms
     uint32_t *flist = (uint32_t *)(stn + T);            // main shifts: positive first, then the rest
-    float *sb = (float *)(flist + S);                   // [SM][R] partner template of an exact score
-    uint32_t *cand = (uint32_t *)(sb + SM * R);         // [kOvCand] screen candidates (f << 16 | e)
-    uint16_t *krow = (uint16_t *)sb;                    // [T*S] rows kept by their bounds, until the candidates
-    uint16_t *plist = (uint16_t *)(sb + max(SM * R + kOvCand, (TS + 1) / 2));   // e = t*S + s with positive
+    float *sb = (float *)(flist + S);                   // [SM][R] partner template of an exact score (NW 4)
+    // NW 4: kept rows alias the partner template, read before the candidates
+    // are written; NW 2: kept rows, then the candidates
+    uint32_t *cand = kLean ? (uint32_t *)sb + (TS + 1) / 2 : (uint32_t *)(sb + SM * R);   // [kCand] (f << 16 | e)
+    uint16_t *krow = (uint16_t *)sb;                    // [T*S] rows kept by their bounds
+    uint16_t *plist = (uint16_t *)(kLean ? cand + kCand : (uint32_t *)sb + max(SM * R + kCand, (TS + 1) / 2));
                                                         // likelihood, grouped by t
     uint16_t *pofs = plist + TS, *pcnt = pofs + T;      // [T] start and length of each template's group
     uint16_t *tlist = pcnt + T;                         // [T] templates with a positive likelihood, ascending
     uint8_t *spos = (uint8_t *)(tlist + T);             // [T][S] bit1 in bounds, bit0 likelihood > 0
     float *sqmax = (float *)(((uintptr_t)(spos + TS) + 3) & ~(uintptr_t)3);   // [2][T] max over (positive) s of Q'
-    __shared__ double r_ub[kOvWaves];
-    for (uint32_t x = threadIdx.x; x < R; x += blockDim.x) szero[x] = 0.0f;
+    __shared__ double r_ub[NW];
     for (uint32_t x = threadIdx.x; x < T; x += blockDim.x) stn[x] = tnorm[x];
     __shared__ uint32_t n_cand, s_settled, s_defer;
-    __shared__ uint32_t r_scan[kOvWaves];
-    __shared__ float r_sc[4];
-    __shared__ uint64_t r_key[4], r_p[4];
-    __shared__ double r_d[4][2];
-    __shared__ double r_vv[4];
+    __shared__ uint32_t r_scan[NW];
+    __shared__ float r_sc[NW];
+    __shared__ uint64_t r_key[NW], r_p[NW];
+    __shared__ double r_d[NW][2];
+    __shared__ double r_vv[NW];
     __shared__ uint32_t n_ppos, n_fpos, n_fneg, n_kp, n_kn, s_maxcnt, n_tp;
     const Counters *ctr = ctr_rw;
     const uint32_t n_ov = ctr->n_ov;
@@ -2059,7 +2062,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             auto a_v = stage_arr<4>(C * N, [&](uint32_t e) { const uint32_t c = divN.div(e);
                                                               return V[(uint64_t)c * d.K + bidx + (e - c * N)]; },
                                     [&](uint32_t e, float x) { sv[e] = x; vv = __builtin_fma((double)x, (double)x, vv); });
-            auto a_main = stage_arr<8>((hi_m - lo_m) * R, [&](uint32_t e) { return Tpad[((uint64_t)lab * C + lo_m) * R + e]; },
+            auto a_main = stage_arr<8>(kLean ? 0u : (hi_m - lo_m) * R, [&](uint32_t e) { return Tpad[((uint64_t)lab * C + lo_m) * R + e]; },
                                        [&](uint32_t e, float x) { smain[e] = x; });
             auto a_q = stage_arr<10>(MQ ? TS : 0, [&](uint32_t e) { return ovQ[(uint64_t)(k - kbase) * TS + e]; },
                                      [&](uint32_t e, float q) { sQ[e] = q; });
@@ -2095,7 +2098,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             if (lane_of() == 0) r_vv[threadIdx.x >> 6] = vv;
             __syncthreads();
             head_step(2);
-            const double vn = sqrt(r_vv[0] + r_vv[1] + r_vv[2] + r_vv[3]) * (1.0 + 1e-12);
+            double vsum = 0.0;
+            for (int w = 0; w < NW; w++) vsum += r_vv[w];
+            const double vn = sqrt(vsum) * (1.0 + 1e-12);
             const double norm_lab = stn[lab];
             // fold the voltage-free half norms into Q (done by the matrix-core
             // kernel for MQ), and the bound per template
@@ -2114,7 +2119,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             {
                 const uint32_t l = lane_of();
                 const uint64_t below = (1ull << l) - 1;
-                for (uint32_t t = threadIdx.x >> 6; t < T; t += kOvWaves) {
+                for (uint32_t t = threadIdx.x >> 6; t < T; t += NW) {
                     float m = -INFINITY, mp = -INFINITY;
                     uint32_t cnt = 0, cneg = 0;
                     for (uint32_t s0 = 0; s0 < S; s0 += kWave) {
@@ -2259,7 +2264,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             for (int off = 32; off > 0; off >>= 1) lo_est = fmax(lo_est, __shfl_xor(lo_est, off));
             if (lane_of() == 0) r_ub[threadIdx.x >> 6] = lo_est;
             __syncthreads();
-            lo_est = fmax(fmax(r_ub[0], r_ub[1]), fmax(r_ub[2], r_ub[3]));
+            lo_est = r_ub[0];
+            for (int w = 1; w < NW; w++) lo_est = fmax(lo_est, r_ub[w]);
             if (debug & 16u) lo_est = -INFINITY;
             // kept rows: positive main shifts from the front, the others from the back
             auto keep_row = [&](uint32_t r, uint32_t a, uint32_t t, double u, int) {
@@ -2312,8 +2318,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             if (lane_of() == 0) { r_d[threadIdx.x >> 6][0] = lo_max; r_d[threadIdx.x >> 6][1] = hi_max; }
             if (lane_of() == 0) acc_screened += screened;
             __syncthreads();
-            lo_max = fmax(fmax(r_d[0][0], r_d[1][0]), fmax(r_d[2][0], r_d[3][0]));
-            hi_max = fmax(fmax(r_d[0][1], r_d[1][1]), fmax(r_d[2][1], r_d[3][1]));
+            lo_max = r_d[0][0];
+            hi_max = r_d[0][1];
+            for (int w = 1; w < NW; w++) {
+                lo_max = fmax(lo_max, r_d[w][0]);
+                hi_max = fmax(hi_max, r_d[w][1]);
+            }
             head_step(3);
             // Reserve `total` slots of the batch's deferred candidates for this
             // window (thread 0; s_defer = first slot + 1, or 0 when they do
@@ -2336,10 +2346,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             // Nothing can beat best_L: parse rejects without any exact score.
             if (hi_max > (double)bl) {
                 uint32_t scored = 0;
-                const float *tmain = smain;
+                // main template rows: staged in LDS (NW 4) or read from global memory
+                const float *tmain = kLean ? Tpad + ((uint64_t)lab * C + lo_m) * R : smain;
                 auto exact = [&](uint32_t f, uint32_t t, uint32_t s, const float *bt) {
                     const float score = overlap_support_score(d, sv, tmain, lo_m, hi_m, bt, sup_lo[t], sup_hi[t],
-                                                              szero, f, s);
+                                                              f, s);
                     scored++;
                     uint64_t p;
                     const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
@@ -2358,7 +2369,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     push(kv2, kc2);
                     push(kv3, kc3);
                     __syncthreads();
-                    const uint32_t nc = n_cand;   // <= 4 * blockDim.x = kOvCand
+                    const uint32_t nc = n_cand;   // <= 4 * blockDim.x = kCand
                     // A single candidate is the winner (its lower bound is lo_max,
                     // above every other upper bound); its float32 score is needed
                     // only when the interval [sh - E, sh + E] leaves a comparison
@@ -2382,7 +2393,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         // exact scores in k_overlap_exact, which also parses
                         for (uint32_t k2 = threadIdx.x; k2 < nc; k2 += blockDim.x)
                             xc_cand[s_defer - 1 + k2] = cand[k2];
-                    } else if (nc <= 16 && !(debug & 4u)) {
+                    } else if (!kLean && nc <= 16 && !(debug & 4u)) {
                         // one at a time, the partner template staged in LDS
                         for (uint32_t k2 = 0; k2 < nc; k2++) {
                             const uint32_t f = cand[k2] >> 16, e = cand[k2] & 0xffffu;
@@ -2414,7 +2425,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         // kernel's batches mostly share f; with more shifts than
                         // the buffer holds, each thread's candidates follow a
                         // block-wide prefix sum of the threads' counts.
-                        const bool by_f = S <= kOvCand;
+                        const bool by_f = S <= kCand;
                         uint32_t *fpos = cand;   // [S] counts, then next slot per shift
                         if (by_f) {
                             for (uint32_t x = threadIdx.x; x < S; x += blockDim.x) fpos[x] = 0;
@@ -2454,7 +2465,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                             }
                             if (l == 63) r_scan[threadIdx.x >> 6] = incl;
                             __syncthreads();
-                            for (uint32_t w = 0; w < kOvWaves; w++) {
+                            for (uint32_t w = 0; w < (uint32_t)NW; w++) {
                                 if (w < (threadIdx.x >> 6)) before += r_scan[w];
                                 total += r_scan[w];
                             }
@@ -3056,22 +3067,27 @@ hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t m
     return hipGetLastError();
 }
 
-static size_t overlap_screened_lds_mode(const Dims &d, bool)
+// Dynamic LDS of k_overlap_screened<MQ, nw> (the layout at the top of the kernel).
+static size_t overlap_screened_lds_nw(const Dims &d, int nw)
 {
     const uint64_t ts = (uint64_t)d.T * d.S, R = d.N + 2 * d.M;
-    const uint64_t sbc = (uint64_t)d.sup_max * R + kOvCand;   // partner template + candidates, or kept rows
-    const uint64_t krw = (ts + 1) / 2;
-    return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + ((uint64_t)d.sup_max + 1) * R * 4 + ts * 4 +
-           (uint64_t)d.T * 4 + (uint64_t)d.S * 4 + (sbc > krw ? sbc : krw) * 4 + ts * 2 + (uint64_t)d.T * 6 +
-           ts + 4 + (uint64_t)d.T * 8 + 16;
+    const uint64_t krw = (ts + 1) / 2, cand = 4ull * nw * 64;
+    const uint64_t main_rows = nw == 2 ? 0 : (uint64_t)d.sup_max * R;
+    const uint64_t region = nw == 2 ? krw + cand : (main_rows + cand > krw ? main_rows + cand : krw);
+    return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + main_rows * 4 + ts * 4 + (uint64_t)d.T * 4 +
+           (uint64_t)d.S * 4 + region * 4 + ts * 2 + (uint64_t)d.T * 6 + ts + 4 + (uint64_t)d.T * 8 + 16;
 }
+
+// Waves per screen block: 2 when the exact scores are deferred (lean block)
+static int overlap_screened_waves(const Dims &d) { return d.xc_cap ? 2 : 4; }
 
 // The matrix-core Q table is used whenever its kernel's LDS fits.
 static bool overlap_uses_mfma_q(const Dims &d) { return overlap_q_lds(d) <= 150 * 1024; }
 
 size_t overlap_screened_lds(const Dims &d)
 {
-    const size_t a = overlap_screened_lds_mode(d, overlap_uses_mfma_q(d));
+    // the four-wave layout holds the most; it must fit whichever block runs
+    const size_t a = overlap_screened_lds_nw(d, 4);
     const size_t b = overlap_uses_mfma_q(d) ? overlap_q_lds(d) : 0;
     return a > b ? a : b;
 }
@@ -3131,8 +3147,10 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
 {
     if (max_ov == 0) return hipSuccess;
     const bool mq = overlap_uses_mfma_q(d);
-    const size_t lds = overlap_screened_lds_mode(d, mq);
-    const void *fn = mq ? (const void *)k_overlap_screened<true> : (const void *)k_overlap_screened<false>;
+    const int nw = overlap_screened_waves(d);
+    const size_t lds = overlap_screened_lds_nw(d, nw);
+    const void *fn = nw == 2 ? (mq ? (const void *)k_overlap_screened<true, 2> : (const void *)k_overlap_screened<false, 2>)
+                             : (mq ? (const void *)k_overlap_screened<true, 4> : (const void *)k_overlap_screened<false, 4>);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -3160,8 +3178,10 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
         const uint32_t nb = max_ov - kb < d.ov_batch ? max_ov - kb : d.ov_batch;
         // up to 8192 blocks (eight per resident slot at four per CU): windows
         // differ in work, and smaller grids leave a longer tail (measured:
-        // 2048 and 4096 slower, 16384 no faster, one block per window slower)
-        const uint32_t grid = nb < 8192 ? nb : 8192;
+        // 2048 and 4096 slower, 16384 no faster, one block per window slower);
+        // the lean two-wave blocks are resident eight per CU, so twice as many
+        const uint32_t gmax = nw == 2 ? 16384 : 8192;
+        const uint32_t grid = nb < gmax ? nb : gmax;
         if (mq) {
             hipLaunchKernelGGL(k_overlap_q, dim3(grid), dim3(256), qlds, st, d, s.V, s.L, s.thr, list, s.ctr,
                                s.best_lab, s.best_idx, s.tabB2, (const uint4 *)s.TmQ, s.ovQ, kb, s.tord, s.tile_lo,
@@ -3171,16 +3191,19 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
             hipError_t e = launch_fill_u32(s.xc_ctr, 2, 0u, st);
             if (e != hipSuccess) return e;
         }
-        if (mq) {
-            hipLaunchKernelGGL(k_overlap_screened<true>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
-                               list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                               s.tabABmin, s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug,
-                               s.xc_cand, s.xc_win, s.xc_ctr);
-        } else {
-            hipLaunchKernelGGL(k_overlap_screened<false>, dim3(grid), dim3(256), lds, st, d, s.V, s.Tm, s.L, s.thr,
-                               list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.tabAB,
-                               s.tabABmin, s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug,
-                               s.xc_cand, s.xc_win, s.xc_ctr);
+        {
+#define FBP_SCREEN_ARGS d, s.V, s.Tm, s.L, s.thr, list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, \
+                        s.ov, s.tabAB, s.tabABmin, s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug, \
+                        s.xc_cand, s.xc_win, s.xc_ctr
+            if (nw == 2 && mq)
+                hipLaunchKernelGGL((k_overlap_screened<true, 2>), dim3(grid), dim3(128), lds, st, FBP_SCREEN_ARGS);
+            else if (nw == 2)
+                hipLaunchKernelGGL((k_overlap_screened<false, 2>), dim3(grid), dim3(128), lds, st, FBP_SCREEN_ARGS);
+            else if (mq)
+                hipLaunchKernelGGL((k_overlap_screened<true, 4>), dim3(grid), dim3(256), lds, st, FBP_SCREEN_ARGS);
+            else
+                hipLaunchKernelGGL((k_overlap_screened<false, 4>), dim3(grid), dim3(256), lds, st, FBP_SCREEN_ARGS);
+#undef FBP_SCREEN_ARGS
         }
         if (d.xc_cap) {
             // every deferred window of the batch: one wave each, 16 per block
