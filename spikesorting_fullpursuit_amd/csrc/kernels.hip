@@ -352,23 +352,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
     __shared__ int over;
     const int lane = threadIdx.x;
     const uint32_t CT = (d.T + 15) / 16;
-    const uint32_t w = win[blockIdx.x];
+    // one block per (listed window, chunk of CTB template tiles)
+    const uint32_t nchunk = (CT + CTB - 1) / CTB;
+    const uint32_t wi = blockIdx.x / nchunk, chunk = blockIdx.x - wi * nchunk;
+    const uint32_t w = win[wi];
     const uint64_t base = (uint64_t)w * d.N;
     const uint64_t wend = min(base + d.N, (uint64_t)d.K);
     const uint32_t np = (uint32_t)(wend - base);   // positions of this window
+    // channels on which the chunk's template tiles are nonzero (templates
+    // are ordered by channel support, tiles of 16); elsewhere every B
+    // fragment is zero and an MFMA would add exact zeros
+    const uint32_t ct0 = chunk * CTB;
+    const uint32_t nct = min((uint32_t)CTB, CT - ct0);
+    uint32_t tlo[CTB], thi[CTB], clo = d.C, chi = 0;
+#pragma unroll
+    for (int ct = 0; ct < CTB; ct++) {
+        tlo[ct] = ct < (int)nct ? tile_lo[ct0 + ct] : 0u;
+        thi[ct] = ct < (int)nct ? tile_hi[ct0 + ct] : 0u;
+        if (tlo[ct] < thi[ct]) {
+            clo = min(clo, tlo[ct]);
+            chi = max(chi, thi[ct]);
+        }
+    }
+    if (chi < clo) chi = clo;
 
-    // Per-position window norm bound |v_i| (float64 sums of the float32 samples):
-    // s2[x] = sum over channels of v[c][x]^2, accumulated while staging when all
-    // channels are staged at once, else in a pass of its own.
-    const bool staged_once = cb_max >= d.C;
+    // Per-position window norm bound |v_i| over the chunk's channels [clo, chi)
+    // (the only ones whose samples enter its products), float64 sums of the
+    // float32 samples: s2[x] = sum over those channels of v[c][x]^2,
+    // accumulated while staging when they are staged at once, else in a pass
+    // of its own.
+    const bool staged_once = cb_max >= chi - clo;
     const uint32_t nsq = RT * 16 + d.N;
     if (lane == 0) over = 0;
-    if (!staged_once) {
+    if (!staged_once || clo == chi) {
         for (uint32_t x = lane; x < nsq; x += 64) {
             double a = 0.0;
             const uint64_t g = base + x;
             if (g < d.K)
-                for (uint32_t c = 0; c < d.C; c++) {
+                for (uint32_t c = clo; c < chi; c++) {
                     const double v = V[(uint64_t)c * d.K + g];
                     a = __builtin_fma(v, v, a);
                 }
@@ -385,7 +406,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
             vnb[p] = up;
         }
     };
-    if (!staged_once) {
+    if (!staged_once || clo == chi) {
         __syncthreads();
         window_norms();
     }
@@ -437,29 +458,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
     };
 
     bool have = false;
-    for (uint32_t ct0 = 0; ct0 < CT; ct0 += CTB) {
-        const uint32_t nct = min((uint32_t)CTB, CT - ct0);
+    {
         float4v acc[RT][CTB];
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
             for (int ct = 0; ct < CTB; ct++) acc[rt][ct] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
-        // channels on which the chunk's template tiles are nonzero (templates
-        // are ordered by channel support, tiles of 16); elsewhere every B
-        // fragment is zero and an MFMA would add exact zeros
-        uint32_t tlo[CTB], thi[CTB], clo = d.C, chi = 0;
-#pragma unroll
-        for (int ct = 0; ct < CTB; ct++) {
-            tlo[ct] = ct < (int)nct ? tile_lo[ct0 + ct] : 0u;
-            thi[ct] = ct < (int)nct ? tile_hi[ct0 + ct] : 0u;
-            if (tlo[ct] < thi[ct]) {
-                clo = min(clo, tlo[ct]);
-                chi = max(chi, thi[ct]);
-            }
-        }
-        for (uint32_t c0 = 0; c0 < d.C; c0 += cb_max) {
-            const uint32_t cb = min(cb_max, d.C - c0);
-            if (!staged_once && (c0 + cb <= clo || c0 >= chi)) continue;
+        for (uint32_t c0 = clo; c0 < chi; c0 += cb_max) {
+            const uint32_t cb = min(cb_max, chi - c0);
             if (!(staged_once && have)) {
                 __syncthreads();
                 stage(c0, cb);
@@ -551,7 +557,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
                     const bool fl = force || !(lh <= -e);
                     acc[rt][ct][q] = fl ? 1.0f : 0.0f;   // keep the decision for the queue pass
                 }
-        if (no_rescore) continue;
+        if (no_rescore) return;
         // Queue entries are groups: a lane's four consecutive positions of one
         // template, with a mask of those that need float32.
         auto group_mask = [&](int rt, int ct) {
@@ -591,11 +597,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
         uint32_t pos = qbase + incl - nflag;
         // this (window, template chunk)'s slice of the queue, for the re-score kernels
         if (lane == 0) {
-            const uint32_t nchunk = (CT + CTB - 1) / CTB;
             const uint32_t kept = qbase >= cap ? 0u : min(tot, cap - qbase);
-            rdesc[(uint64_t)blockIdx.x * nchunk + ct0 / CTB] = make_uint2(sbase + qbase, kept);
+            rdesc[(uint64_t)wi * nchunk + chunk] = make_uint2(sbase + qbase, kept);
         }
-        if (!nflag) continue;
+        if (!nflag) return;
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
@@ -1231,6 +1236,55 @@ __device__ __forceinline__ float overlap_direct_score(const Dims &d, const float
         const float *vc = sv + c * d.N;
         float ss = 0.0f;
         // operands are loaded eight samples at a time ahead of the chains
+        uint32_t j = 0;
+        for (; j + 8 <= d.N; j += 8) {
+            float av[8], bv[8], vv[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                av[q] = a[j + q];
+                bv[q] = b[j + q];
+                vv[q] = vc[j + q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const float sum = av[q] + bv[q];
+                ss = __builtin_fmaf(sum, sum, ss);       // bp.cl:621
+                acc = __builtin_fmaf(sum, vv[q], acc);   // bp.cl:622
+            }
+        }
+        for (; j < d.N; j++) {
+            const float sum = a[j] + b[j];
+            ss = __builtin_fmaf(sum, sum, ss);
+            acc = __builtin_fmaf(sum, vc[j], acc);
+        }
+        if (ss > 0.0f) acc = (float)((double)acc - 0.5 * (double)ss);   // bp.cl:627-630
+    }
+    return acc;
+}
+
+// The same score over the channels where either template is nonzero, with
+// the rows outside a template's support read from a zero row in LDS (the
+// four-wave screen, whose template rows are in LDS): the
+// main template's rows amain[c - lo_m] for c in [lo_m, hi_m), the partner's
+// bt[c - lo_t] for c in [lo_t, hi_t), a zero row elsewhere.  A channel where
+// both are zero adds exactly 0 to the reference chain (every product is +-0,
+// the running sum is never -0 and ss = 0 skips the subtraction), so it is
+// skipped.
+__device__ __forceinline__ float overlap_support_score_z(const Dims &d, const float *__restrict__ sv,
+                                                       const float *amain, uint32_t lo_m, uint32_t hi_m,
+                                                       const float *bt, uint32_t lo_t, uint32_t hi_t,
+                                                       const float *zero, uint32_t f, uint32_t s)
+{
+    const uint32_t R = d.N + 2 * d.M;
+    const uint32_t c0 = min(lo_m, lo_t), c1 = max(hi_m, hi_t);
+    float acc = 0.0f;
+    for (uint32_t c = c0; c < c1; c++) {
+        const bool ha = c >= lo_m && c < hi_m, hb = c >= lo_t && c < hi_t;
+        if (!ha && !hb) continue;
+        const float *a = (ha ? amain + (c - lo_m) * R : zero) + 2 * d.M - f;
+        const float *b = (hb ? bt + (uint64_t)(c - lo_t) * R : zero) + 2 * d.M - s;
+        const float *vc = sv + c * d.N;
+        float ss = 0.0f;
         uint32_t j = 0;
         for (; j + 8 <= d.N; j += 8) {
             float av[8], bv[8], vv[8];
@@ -1969,7 +2023,8 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     constexpr uint32_t kCand = 4 * NW * 64;             // candidate list: four per thread
     float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset
     float *smain = sv + C * N;                          // [SM][R] padded best-label template, its support rows
-    float *sQ = smain + (kLean ? 0u : SM * R);          // [T][S] Q - B2/2
+    float *szero = smain + (kLean ? 0u : SM * R);       // [R] zeros (NW 4)
+    float *sQ = szero + (kLean ? 0u : R);               // [T][S] Q - B2/2
     float *stn = sQ + TS;                               // [T] template norms
     uint32_t *flist = (uint32_t *)(stn + T);            // main shifts: positive first, then the rest
     float *sb = (float *)(flist + S);                   // [SM][R] partner template of an exact score (NW 4)
@@ -1984,6 +2039,8 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     uint8_t *spos = (uint8_t *)(tlist + T);             // [T][S] bit1 in bounds, bit0 likelihood > 0
     float *sqmax = (float *)(((uintptr_t)(spos + TS) + 3) & ~(uintptr_t)3);   // [2][T] max over (positive) s of Q'
     __shared__ double r_ub[NW];
+    if (!kLean)
+        for (uint32_t x = threadIdx.x; x < R; x += blockDim.x) szero[x] = 0.0f;
     for (uint32_t x = threadIdx.x; x < T; x += blockDim.x) stn[x] = tnorm[x];
     __shared__ uint32_t n_cand, s_settled, s_defer;
     __shared__ uint32_t r_scan[NW];
@@ -2349,8 +2406,9 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                 // main template rows: staged in LDS (NW 4) or read from global memory
                 const float *tmain = kLean ? Tpad + ((uint64_t)lab * C + lo_m) * R : smain;
                 auto exact = [&](uint32_t f, uint32_t t, uint32_t s, const float *bt) {
-                    const float score = overlap_support_score(d, sv, tmain, lo_m, hi_m, bt, sup_lo[t], sup_hi[t],
-                                                              f, s);
+                    const float score =
+                        kLean ? overlap_support_score(d, sv, tmain, lo_m, hi_m, bt, sup_lo[t], sup_hi[t], f, s)
+                              : overlap_support_score_z(d, sv, tmain, lo_m, hi_m, bt, sup_lo[t], sup_hi[t], szero, f, s);
                     scored++;
                     uint64_t p;
                     const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
@@ -2938,7 +2996,7 @@ static hipError_t launch_corr_mfma_rt(const Dims &d, const DeviceState &s, uint3
     if (cb_max == 0) return hipErrorInvalidValue;
     const size_t lds = (size_t)cb_max * per_channel;
     ScreenCoef co{coef[0], coef[1], coef[2], coef[3]};
-    hipLaunchKernelGGL((k_corr_mfma<PREC, RT, CTB>), dim3(n_win), dim3(64), lds, st, d, s.V, s.Tm,
+    hipLaunchKernelGGL((k_corr_mfma<PREC, RT, CTB>), dim3(n_win * mfma_nchunk(d)), dim3(64), lds, st, d, s.V, s.Tm,
                        (const Frag<PREC> *)s.TmB, s.bias, s.tnorm, s.tscale, s.win, s.L, s.ctr, s.rq, rq_cap,
                        (uint2 *)s.rdesc, cb_max, co, no_rescore, s.tord, s.tile_lo, s.tile_hi, s.sup_lo, s.sup_hi,
                        s.rq_shard);
@@ -3072,8 +3130,9 @@ static size_t overlap_screened_lds_nw(const Dims &d, int nw)
 {
     const uint64_t ts = (uint64_t)d.T * d.S, R = d.N + 2 * d.M;
     const uint64_t krw = (ts + 1) / 2, cand = 4ull * nw * 64;
-    const uint64_t main_rows = nw == 2 ? 0 : (uint64_t)d.sup_max * R;
-    const uint64_t region = nw == 2 ? krw + cand : (main_rows + cand > krw ? main_rows + cand : krw);
+    const uint64_t partner = (uint64_t)d.sup_max * R;                           // NW 4: staged partner rows
+    const uint64_t main_rows = nw == 2 ? 0 : partner + R;                        // main rows + the zero row
+    const uint64_t region = nw == 2 ? krw + cand : (partner + cand > krw ? partner + cand : krw);
     return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + main_rows * 4 + ts * 4 + (uint64_t)d.T * 4 +
            (uint64_t)d.S * 4 + region * 4 + ts * 2 + (uint64_t)d.T * 6 + ts + 4 + (uint64_t)d.T * 8 + 16;
 }
