@@ -25,6 +25,8 @@ struct Dims {
     uint32_t sup_max;  // widest channel support of a template (C until templates are set)
     uint32_t xc_cap;   // screened overlap: candidate slots per batch of the deferred exact
                        // scores (k_overlap_exact), 0 = every window scores in the screen
+    uint32_t corr_ctb; // matrix-core correlation: template tiles per block (0 until templates are set)
+    uint32_t corr_span;// widest channel span of such a chunk of tiles (0 until templates are set)
 };
 
 // The re-score queue is split into kQueueShards equal shards; a correlation
@@ -141,6 +143,8 @@ uint32_t mfma_words(uint32_t precision);   // 8-byte words per template fragment
 uint32_t mfma_ks(const Dims &d, uint32_t precision);
 uint32_t mfma_ct(const Dims &d);
 uint32_t mfma_nchunk(const Dims &d);
+// tiles per correlation block for templates with these channel supports
+uint32_t mfma_ctb_for(uint32_t n_tiles, uint32_t n_chans, uint32_t widest_support);
 hipError_t launch_template_norms(const Dims &d, const DeviceState &s, uint32_t precision, hipStream_t st);
 hipError_t launch_pack_templates(const Dims &d, const DeviceState &s, uint32_t precision, hipStream_t st);
 hipError_t launch_corr_screen(const Dims &d, const DeviceState &s, uint32_t n_win, uint32_t precision,

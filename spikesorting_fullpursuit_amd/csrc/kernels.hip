@@ -2956,7 +2956,16 @@ uint32_t mfma_ks(const Dims &d, uint32_t precision)
     return (padded + mfma_kstep(precision) - 1) / mfma_kstep(precision);
 }
 uint32_t mfma_ct(const Dims &d) { return (d.T + 15) / 16; }
-uint32_t mfma_ctb(const Dims &d) { const uint32_t ct = mfma_ct(d); return ct >= 4 ? 4 : (ct == 3 ? 4 : ct); }
+// Templates that span every channel share one staging of the window: up to
+// four tiles per block.  Templates on narrow channel supports (at most half of
+// the channels) take two tiles per block: a chunk then spans few channels, so
+// the block stages little and more blocks (and waves) fit on a CU.
+uint32_t mfma_ctb_for(uint32_t ct, uint32_t n_chans, uint32_t widest_support)
+{
+    if (ct >= 2 && 2 * widest_support <= n_chans) return 2;
+    return ct >= 4 ? 4 : (ct == 3 ? 4 : ct);
+}
+uint32_t mfma_ctb(const Dims &d) { return d.corr_ctb ? d.corr_ctb : mfma_ctb_for(mfma_ct(d), d.C, d.C); }
 uint32_t mfma_nchunk(const Dims &d) { return (mfma_ct(d) + mfma_ctb(d) - 1) / mfma_ctb(d); }
 
 hipError_t launch_template_norms(const Dims &d, const DeviceState &s, uint32_t precision, hipStream_t st)
@@ -2992,7 +3001,11 @@ static hipError_t launch_corr_mfma_rt(const Dims &d, const DeviceState &s, uint3
     const uint32_t per_channel = Tr::kCopies * LA * Tr::kBytes;
     const uint32_t budget = 32 * 1024;
     uint32_t cb_max = budget / per_channel;
+    // no more channels than the widest chunk spans (every chunk then stages once)
+    const uint32_t span = d.corr_span ? d.corr_span : d.C;
+    if (cb_max > span) cb_max = span;
     if (cb_max > d.C) cb_max = d.C;
+    if (cb_max == 0 && span == 0) cb_max = 1;
     if (cb_max == 0) return hipErrorInvalidValue;
     const size_t lds = (size_t)cb_max * per_channel;
     ScreenCoef co{coef[0], coef[1], coef[2], coef[3]};

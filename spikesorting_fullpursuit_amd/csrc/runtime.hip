@@ -202,7 +202,8 @@ void for_each_buffer(const fbp_config *cfg, const Dims &d, DeviceState &s, V &&v
     if (cfg->precision != FBP_PREC_FP32) {
         visit(s.rq, rescore_queue_cap(d));
         visit(s.rq_shard, (uint64_t)kQueueShards * kShardStride);
-        visit(*(uint64_t **)&s.rdesc, (uint64_t)W * mfma_nchunk(d));
+        // the chunking is chosen when the templates are set: room for one per tile
+        visit(*(uint64_t **)&s.rdesc, (uint64_t)W * mfma_ct(d));
         visit(*(uint64_t **)&s.TmB, (uint64_t)d.C * mfma_ks(d, cfg->precision) * mfma_ct(d) * 64 *
                                         mfma_words(cfg->precision));
     }
@@ -515,6 +516,20 @@ int fbp_segment_set_templates(fbp_segment *g, const float *templates, const floa
         tlo[ct] = a;
         thi[ct] = b;
     }
+    // tiles per correlation block and the widest channel span of such a chunk
+    const uint32_t ctb = mfma_ctb_for(CT, g->d.C, sup_max);
+    uint32_t span = 0;
+    for (uint32_t c0 = 0; c0 < CT; c0 += ctb) {
+        uint32_t a = g->d.C, b = 0;
+        for (uint32_t ct = c0; ct < CT && ct < c0 + ctb; ct++)
+            if (tlo[ct] < thi[ct]) {
+                a = std::min(a, tlo[ct]);
+                b = std::max(b, thi[ct]);
+            }
+        if (a < b) span = std::max(span, b - a);
+    }
+    g->d.corr_ctb = ctb;
+    g->d.corr_span = span;
     if (g->cfg.overlap_mode == FBP_OVERLAP_SCREENED && overlap_screened_lds(g->d) > kMaxLds)
         return fail(FBP_EINVAL, "screened overlap needs %zu bytes of LDS (templates span %u channels)",
                     overlap_screened_lds(g->d), sup_max);
