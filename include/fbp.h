@@ -96,9 +96,13 @@ enum {
     FBP_DEBUG_OVERLAP_INBLOCK = 64, /* screened overlap: every exact score in the
                                       screen's block (no deferred exact-score
                                       kernel)                                    */
-    FBP_DEBUG_OVERLAP_FEW_SLOTS = 128 /* screened overlap: 64 deferred candidate
+    FBP_DEBUG_OVERLAP_FEW_SLOTS = 128, /* screened overlap: 64 deferred candidate
                                       slots per batch, so most windows with
                                       candidates score them in the screen       */
+    FBP_DEBUG_OVERLAP_WIDE = 256   /* screened overlap: the eight-wave screen block
+                                      and deferred exact scores that read the
+                                      templates from global memory (the route of
+                                      templates too large for LDS)              */
 };
 
 typedef struct fbp_stats {

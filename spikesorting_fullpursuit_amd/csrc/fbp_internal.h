@@ -27,6 +27,8 @@ struct Dims {
                        // scores (k_overlap_exact), 0 = every window scores in the screen
     uint32_t corr_ctb; // matrix-core correlation: template tiles per block (0 until templates are set)
     uint32_t corr_span;// widest channel span of such a chunk of tiles (0 until templates are set)
+    uint32_t xc_wide;  // screened overlap (test switch): eight-wave lean screen and exact scores
+                       // from the padded templates in global memory whatever their size
 };
 
 // The re-score queue is split into kQueueShards equal shards; a correlation

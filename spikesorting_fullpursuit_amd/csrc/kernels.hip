@@ -1999,10 +1999,12 @@ __global__ void k_pack_qtemplates(Dims d, const float *__restrict__ Tm, const ui
 
 // Register budget of four waves per SIMD (measured faster than three despite a
 // few spilled registers).  NW waves per block: 4 when the block also scores
-// its candidates (main and partner templates staged in LDS), 2 when the
+// its candidates (main and partner templates staged in LDS); 2 or 8 when the
 // exact scores are deferred to k_overlap_exact -- the lean block holds no
 // template rows, so twice as many windows are in flight per CU at the same
-// wave occupancy (the kernel is bound by the latency of its phases).
+// wave occupancy (the kernel is bound by the latency of its phases), and
+// where the per-window tables leave room for one block per CU (hundreds of
+// templates) eight waves share the window's phases.
 template <bool MQ, int NW>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_overlap_screened(
     Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
@@ -2019,17 +2021,17 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     const QPlan qp = q_plan(d);
     const FastDiv divN(N), divS(S), divT(T);
     double *sE = (double *)dyn;                         // [T] screen bound per template (this window)
-    constexpr bool kLean = NW == 2;
+    constexpr bool kLean = NW != 4;
     constexpr uint32_t kCand = 4 * NW * 64;             // candidate list: four per thread
     float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset
     float *smain = sv + C * N;                          // [SM][R] padded best-label template, its support rows
-    float *szero = smain + (kLean ? 0u : SM * R);       // [R] zeros (NW 4)
+    float *szero = smain + (kLean ? 0u : SM * R);       // [R] zeros (not lean)
     float *sQ = szero + (kLean ? 0u : R);               // [T][S] Q - B2/2
     float *stn = sQ + TS;                               // [T] template norms
     uint32_t *flist = (uint32_t *)(stn + T);            // main shifts: positive first, then the rest
-    float *sb = (float *)(flist + S);                   // [SM][R] partner template of an exact score (NW 4)
-    // NW 4: kept rows alias the partner template, read before the candidates
-    // are written; NW 2: kept rows, then the candidates
+    float *sb = (float *)(flist + S);                   // [SM][R] partner template of an exact score (not lean)
+    // not lean: kept rows alias the partner template, read before the
+    // candidates are written; lean: kept rows, then the candidates
     uint32_t *cand = kLean ? (uint32_t *)sb + (TS + 1) / 2 : (uint32_t *)(sb + SM * R);   // [kCand] (f << 16 | e)
     uint16_t *krow = (uint16_t *)sb;                    // [T*S] rows kept by their bounds
     uint16_t *plist = (uint16_t *)(kLean ? cand + kCand : (uint32_t *)sb + max(SM * R + kCand, (TS + 1) / 2));
@@ -2403,7 +2405,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             // Nothing can beat best_L: parse rejects without any exact score.
             if (hi_max > (double)bl) {
                 uint32_t scored = 0;
-                // main template rows: staged in LDS (NW 4) or read from global memory
+                // main template rows: staged in LDS (not lean) or read from global memory
                 const float *tmain = kLean ? Tpad + ((uint64_t)lab * C + lo_m) * R : smain;
                 auto exact = [&](uint32_t f, uint32_t t, uint32_t s, const float *bt) {
                     const float score =
@@ -2562,8 +2564,10 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
 // Deferred exact scores of the screened overlap (bp.cl:599-640 for the
 // candidates the screen could not rule out, then the parse of bp.cl:690-814).
 // One wave per deferred window, its lanes one candidate each, 64 at a time;
-// the templates sit in LDS ([T][C][N], staged once per block), the window's
-// voltage in the wave's lanes (one sample each, a channel at a time).
+// the templates sit in LDS ([T][C][N], staged once per block) when they fit
+// in kXcMaxLds, else they are read from their zero-padded copy in global
+// memory; the window's voltage is in the wave's lanes (one sample each, a
+// channel at a time).
 // Each lane's chain is the reference's float32 order; the winner is reduced
 // in the reference's total order (score, then reduction group and lane).
 constexpr uint32_t kXcThreads = 1024;   // two blocks per CU (LDS), eight waves per SIMD
@@ -2642,20 +2646,88 @@ __device__ __forceinline__ float exact_score_lds(const Dims &d, const float *ts,
     return acc;
 }
 
-__global__ __launch_bounds__(kXcThreads) void k_overlap_exact(
+// The same score with the templates read from the zero-padded rows in global
+// memory (templates too large for LDS; rows of R = N + 2M samples, window
+// sample j of a template at shift x is row entry 2M - x + j, +0.0 outside the
+// template).  The wave visits only the channels where the main template or
+// some lane's partner is nonzero (the others leave every chain unchanged),
+// the next one's voltage row loaded while the current one is summed.
+template <bool UF>
+__device__ __forceinline__ float exact_score_glb(const Dims &d, const float *__restrict__ Tpad,
+                                                 const float *__restrict__ V, uint32_t bidx, uint32_t lab,
+                                                 uint32_t lo_m, uint32_t hi_m, uint32_t t, uint32_t lo_t,
+                                                 uint32_t hi_t, uint32_t f, uint32_t s)
+{
+    const uint32_t N = d.N, C = d.C, R = N + 2 * d.M, lane = lane_of();
+    // smallest channel >= c in the main support or some lane's partner
+    // support (C if none), wave-uniform
+    auto next_ch = [&](uint32_t c) {
+        uint32_t x = c < hi_m ? max(c, lo_m) : C;
+        x = min(x, c < hi_t ? max(c, lo_t) : C);
+        for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor(x, off));
+        return (uint32_t)__builtin_amdgcn_readfirstlane(x);
+    };
+    auto vrow = [&](uint32_t c) { return (c < C && lane < N) ? V[(uint64_t)c * d.K + bidx + lane] : 0.0f; };
+    // UF: the main template's shifted row, one sample per lane
+    auto arow = [&](uint32_t c) {
+        return (UF && c >= lo_m && c < hi_m && lane < N) ? Tpad[((uint64_t)lab * C + c) * R + 2 * d.M - f + lane]
+                                                         : 0.0f;
+    };
+    uint32_t c = next_ch(0);
+    float vreg = vrow(c), areg = arow(c);
+    float acc = 0.0f;
+    while (c < C) {
+        const uint32_t cn = next_ch(c + 1);
+        const float vnext = vrow(cn), anext = arow(cn);
+        const bool ha = c >= lo_m && c < hi_m, hb = c >= lo_t && c < hi_t;
+        const float *ta = Tpad + ((uint64_t)lab * C + c) * R + 2 * d.M - f;
+        const float *tb = Tpad + ((uint64_t)t * C + c) * R + 2 * d.M - s;
+        // steps past N are no-ops: both operands +0.0 (unloaded), and lanes
+        // >= N of vreg and areg hold 0
+        float ss = 0.0f;
+        for (uint32_t j = 0; j < N; j += 8) {
+            float av[8], bv[8];
+#pragma unroll
+            for (uint32_t q = 0; q < 8; q++) {
+                bv[q] = (hb && j + q < N) ? tb[j + q] : 0.0f;
+                if (!UF) av[q] = (ha && j + q < N) ? ta[j + q] : 0.0f;
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < 8; q++) {
+                const float ax = UF ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(areg), j + q)) : av[q];
+                const float v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vreg), j + q));
+                const float sum = ax + bv[q];
+                ss = __builtin_fmaf(sum, sum, ss);      // bp.cl:621
+                acc = __builtin_fmaf(sum, v, acc);      // bp.cl:622
+            }
+        }
+        if (ss > 0.0f) acc = (float)((double)acc - 0.5 * (double)ss);   // bp.cl:627-630
+        vreg = vnext;
+        areg = anext;
+        c = cn;
+    }
+    return acc;
+}
+
+template <bool GT>
+__global__ __launch_bounds__(kXcThreads) __attribute__((amdgpu_waves_per_eu(GT ? 8 : 1))) void k_overlap_exact(
     Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
     const float *__restrict__ thr, const uint32_t *__restrict__ list, const uint32_t *__restrict__ xc_win,
     const uint32_t *__restrict__ xc_ctr, const uint32_t *__restrict__ xc_cand, Counters *ctr, float *best_L,
     const uint32_t *__restrict__ best_lab, const uint32_t *__restrict__ best_idx, uint32_t *ov_idx, uint32_t *ov_lab,
     uint8_t *ov, const uint32_t *__restrict__ sup_lo, const uint32_t *__restrict__ sup_hi)
 {
-    extern __shared__ float xs[];   // zero guard, [T][C][N] templates, zero guard
+    // GT: Tm is the zero-padded [T][C][N + 2M] copy in global memory;
+    // otherwise the [T][C][N] templates, staged here between zero guards
+    extern __shared__ float xs[];
     const uint64_t TCN = (uint64_t)d.T * d.C * d.N;
     const uint32_t G = overlap_exact_guard(d);
     float *ts = xs + G;
-    for (uint64_t e = threadIdx.x; e < TCN + 2 * G; e += blockDim.x)
-        xs[e] = e >= G && e < G + TCN ? Tm[e - G] : 0.0f;
-    __syncthreads();
+    if (!GT) {
+        for (uint64_t e = threadIdx.x; e < TCN + 2 * G; e += blockDim.x)
+            xs[e] = e >= G && e < G + TCN ? Tm[e - G] : 0.0f;
+        __syncthreads();
+    }
     // every per-window value is wave-uniform (read through readfirstlane, so
     // it lives in scalar registers and the window's loads are scalar)
     auto uni = [](uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); };
@@ -2690,11 +2762,14 @@ __global__ __launch_bounds__(kXcThreads) void k_overlap_exact(
             const uint32_t f0 = __builtin_amdgcn_readfirstlane(f);
             const bool same_f = __ballot(act && f != f0) == 0;
             const uint32_t cw0 = __builtin_amdgcn_readfirstlane(c0), cw1 = __builtin_amdgcn_readfirstlane(c1);
-            const float score = same_f
-                ? exact_score_lds<true>(d, ts, V, bidx, lab, lo_m, hi_m, t, act ? lo_t : 0u, act ? hi_t : 0u, f0, s,
-                                        cw0, cw1)
-                : exact_score_lds<false>(d, ts, V, bidx, lab, lo_m, hi_m, t, act ? lo_t : 0u, act ? hi_t : 0u, f, s,
-                                         cw0, cw1);
+            const uint32_t lt = act ? lo_t : 0u, ht = act ? hi_t : 0u;
+            float score;
+            if (GT)
+                score = same_f ? exact_score_glb<true>(d, Tm, V, bidx, lab, lo_m, hi_m, t, lt, ht, f0, s)
+                               : exact_score_glb<false>(d, Tm, V, bidx, lab, lo_m, hi_m, t, lt, ht, f, s);
+            else
+                score = same_f ? exact_score_lds<true>(d, ts, V, bidx, lab, lo_m, hi_m, t, lt, ht, f0, s, cw0, cw1)
+                               : exact_score_lds<false>(d, ts, V, bidx, lab, lo_m, hi_m, t, lt, ht, f, s, cw0, cw1);
             if (act) {
                 uint64_t p;
                 const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
@@ -3138,28 +3213,40 @@ hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t m
     return hipGetLastError();
 }
 
+constexpr uint64_t kLdsPerCu = 160 * 1024;
+
 // Dynamic LDS of k_overlap_screened<MQ, nw> (the layout at the top of the kernel).
 static size_t overlap_screened_lds_nw(const Dims &d, int nw)
 {
     const uint64_t ts = (uint64_t)d.T * d.S, R = d.N + 2 * d.M;
     const uint64_t krw = (ts + 1) / 2, cand = 4ull * nw * 64;
     const uint64_t partner = (uint64_t)d.sup_max * R;                           // NW 4: staged partner rows
-    const uint64_t main_rows = nw == 2 ? 0 : partner + R;                        // main rows + the zero row
-    const uint64_t region = nw == 2 ? krw + cand : (partner + cand > krw ? partner + cand : krw);
+    const uint64_t main_rows = nw != 4 ? 0 : partner + R;                        // main rows + the zero row
+    const uint64_t region = nw != 4 ? krw + cand : (partner + cand > krw ? partner + cand : krw);
     return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + main_rows * 4 + ts * 4 + (uint64_t)d.T * 4 +
            (uint64_t)d.S * 4 + region * 4 + ts * 2 + (uint64_t)d.T * 6 + ts + 4 + (uint64_t)d.T * 8 + 16;
 }
 
-// Waves per screen block: 2 when the exact scores are deferred (lean block)
-static int overlap_screened_waves(const Dims &d) { return d.xc_cap ? 2 : 4; }
+// Waves per screen block: 4 when the block scores its candidates; with the
+// exact scores deferred (lean block) 2, or 8 when fewer than four two-wave
+// blocks fit in a CU's LDS (4 if not even the eight-wave block fits beside
+// the static LDS, which then scores in place of the lean one).
+static int overlap_screened_waves(const Dims &d)
+{
+    if (!d.xc_cap) return 4;
+    if (!d.xc_wide && overlap_screened_lds_nw(d, 2) * 4 <= kLdsPerCu) return 2;
+    return overlap_screened_lds_nw(d, 8) + 1024 <= kLdsPerCu ? 8 : 4;
+}
 
 // The matrix-core Q table is used whenever its kernel's LDS fits.
 static bool overlap_uses_mfma_q(const Dims &d) { return overlap_q_lds(d) <= 150 * 1024; }
 
 size_t overlap_screened_lds(const Dims &d)
 {
-    // the four-wave layout holds the most; it must fit whichever block runs
-    const size_t a = overlap_screened_lds_nw(d, 4);
+    // the block that runs, and the four-wave layout (the largest but for
+    // the eight-wave candidate list)
+    const size_t a4 = overlap_screened_lds_nw(d, 4), an = overlap_screened_lds_nw(d, overlap_screened_waves(d));
+    const size_t a = a4 > an ? a4 : an;
     const size_t b = overlap_uses_mfma_q(d) ? overlap_q_lds(d) : 0;
     return a > b ? a : b;
 }
@@ -3207,7 +3294,7 @@ hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_
 uint32_t overlap_exact_cap(const Dims &d)
 {
     // a channel's window samples are held one per lane
-    if (d.N > (uint32_t)kWave || ((uint64_t)d.T * d.C * d.N + 2 * overlap_exact_guard(d)) * 4 > kXcMaxLds) return 0;
+    if (d.N > (uint32_t)kWave) return 0;
     uint64_t cap = (uint64_t)d.ov_batch * 256;
     if (cap > (32ull << 20)) cap = 32ull << 20;
     if (cap < 4096) cap = 4096;
@@ -3221,8 +3308,9 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
     const bool mq = overlap_uses_mfma_q(d);
     const int nw = overlap_screened_waves(d);
     const size_t lds = overlap_screened_lds_nw(d, nw);
-    const void *fn = nw == 2 ? (mq ? (const void *)k_overlap_screened<true, 2> : (const void *)k_overlap_screened<false, 2>)
-                             : (mq ? (const void *)k_overlap_screened<true, 4> : (const void *)k_overlap_screened<false, 4>);
+    const void *fn = nw == 2   ? (mq ? (const void *)k_overlap_screened<true, 2> : (const void *)k_overlap_screened<false, 2>)
+                     : nw == 8 ? (mq ? (const void *)k_overlap_screened<true, 8> : (const void *)k_overlap_screened<false, 8>)
+                               : (mq ? (const void *)k_overlap_screened<true, 4> : (const void *)k_overlap_screened<false, 4>);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -3239,10 +3327,13 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
                                            (int)qlds);
         if (e != hipSuccess) return e;
     }
-    const size_t xlds = ((size_t)d.T * d.C * d.N + 2 * overlap_exact_guard(d)) * 4;
+    // exact scores: templates staged in LDS when they fit, else read from Tpad
+    const size_t xlds_full = ((size_t)d.T * d.C * d.N + 2 * overlap_exact_guard(d)) * 4;
+    const bool xg = xlds_full > kXcMaxLds || d.xc_wide;
+    const size_t xlds = xg ? 0 : xlds_full;
     if (d.xc_cap && xlds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_overlap_exact, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)xlds);
+        hipError_t e = hipFuncSetAttribute((const void *)k_overlap_exact<false>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);
         if (e != hipSuccess) return e;
     }
     // batches of ov_batch windows: Q tables (matrix cores), then the screen
@@ -3271,6 +3362,10 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
                 hipLaunchKernelGGL((k_overlap_screened<true, 2>), dim3(grid), dim3(128), lds, st, FBP_SCREEN_ARGS);
             else if (nw == 2)
                 hipLaunchKernelGGL((k_overlap_screened<false, 2>), dim3(grid), dim3(128), lds, st, FBP_SCREEN_ARGS);
+            else if (nw == 8 && mq)
+                hipLaunchKernelGGL((k_overlap_screened<true, 8>), dim3(grid), dim3(512), lds, st, FBP_SCREEN_ARGS);
+            else if (nw == 8)
+                hipLaunchKernelGGL((k_overlap_screened<false, 8>), dim3(grid), dim3(512), lds, st, FBP_SCREEN_ARGS);
             else if (mq)
                 hipLaunchKernelGGL((k_overlap_screened<true, 4>), dim3(grid), dim3(256), lds, st, FBP_SCREEN_ARGS);
             else
@@ -3280,9 +3375,14 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
         if (d.xc_cap) {
             // every deferred window of the batch: one wave each, 16 per block
             const uint32_t xgrid = (nb + 15) / 16 < 1024 ? (nb + 15) / 16 : 1024;
-            hipLaunchKernelGGL(k_overlap_exact, dim3(xgrid), dim3(kXcThreads), xlds, st, d, s.V, s.Tm, s.L, s.thr,
-                               list, s.xc_win, s.xc_ctr, s.xc_cand, s.ctr, s.best_L, s.best_lab, s.best_idx,
-                               s.ov_idx, s.ov_lab, s.ov, s.sup_lo, s.sup_hi);
+            if (xg)
+                hipLaunchKernelGGL(k_overlap_exact<true>, dim3(xgrid), dim3(kXcThreads), 0, st, d, s.V, s.Tpad, s.L,
+                                   s.thr, list, s.xc_win, s.xc_ctr, s.xc_cand, s.ctr, s.best_L, s.best_lab,
+                                   s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.sup_lo, s.sup_hi);
+            else
+                hipLaunchKernelGGL(k_overlap_exact<false>, dim3(xgrid), dim3(kXcThreads), xlds, st, d, s.V, s.Tm,
+                                   s.L, s.thr, list, s.xc_win, s.xc_ctr, s.xc_cand, s.ctr, s.best_L, s.best_lab,
+                                   s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.sup_lo, s.sup_hi);
         }
     }
     return hipGetLastError();

@@ -398,6 +398,7 @@ static int make_dims(const fbp_config *cfg, Dims &d)
     d.xc_cap = (cfg->overlap_mode == FBP_OVERLAP_SCREENED && !(cfg->debug_flags & FBP_DEBUG_OVERLAP_INBLOCK))
                    ? overlap_exact_cap(d) : 0;
     if (d.xc_cap && (cfg->debug_flags & FBP_DEBUG_OVERLAP_FEW_SLOTS)) d.xc_cap = 64;
+    d.xc_wide = d.xc_cap && (cfg->debug_flags & FBP_DEBUG_OVERLAP_WIDE) ? 1u : 0u;
     return FBP_OK;
 }
 

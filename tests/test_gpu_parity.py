@@ -76,7 +76,8 @@ def test_dense_16ch_spike_list_grows(oracle_mod, mode):
         assert g['stats']['screened_combos'] > 20 * g['stats']['overlap_combos']
 
 
-@pytest.mark.parametrize("debug", [0, 2, 4, 8, 12, 16, 18, 32, 50, 64, 66, 68, 72, 76, 80, 128, 130, 136, 160])
+@pytest.mark.parametrize("debug", [0, 2, 4, 8, 12, 16, 18, 32, 50, 64, 66, 68, 72, 76, 80, 128, 130, 136, 160,
+                                   256, 258, 264, 272, 288, 384, 386])
 def test_screened_overlap_candidate_paths(oracle_mod, debug):
     # The screened overlap finds its exact-score candidates from per-thread
     # lists and settles single candidates from their score interval, unless
@@ -87,8 +88,11 @@ def test_screened_overlap_candidate_paths(oracle_mod, debug):
     # scored by the deferred exact-score kernel (LDS templates, one wave per
     # window), or in the screen's block with 64 (partner templates staged in
     # LDS, or read from global memory with 4); 128 leaves 64 deferred slots
-    # per batch, so windows of both kinds meet in one batch.  Every route
-    # gives the oracle's events.
+    # per batch, so windows of both kinds meet in one batch; 256 takes the
+    # route of templates too large for LDS (eight-wave screen blocks without
+    # template rows, exact scores read from the zero-padded templates in
+    # global memory, channels no lane's templates touch skipped).  Every
+    # route gives the oracle's events.
     from spikesorting_fullpursuit_amd import synthetic
     rec = synthetic.config("16ch-300s", seed=7, duration_s=0.5)
     sm = rec['separability_metrics']
