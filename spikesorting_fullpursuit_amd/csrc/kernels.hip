@@ -1725,10 +1725,17 @@ __device__ __forceinline__ uint64_t q_bfrag(const uint32_t *row, uint32_t y)
 
 // The tail tile's tasks (see QPlan): A fragments at
 // TmQ[(C*NKQ*TT) + (c*NKT + kq)][lane], one shift tile per task.
+// Store of a Q-table entry: with B2 (one channel slice), Q - B2/2 straight
+// into the window's row of ovQ; else the slice's partial sum.
+__device__ __forceinline__ void q_store(float *dst, const float *__restrict__ B2, uint32_t e, float q)
+{
+    dst[e] = B2 ? (float)((double)q - 0.5 * (double)B2[e]) : q;
+}
+
 __device__ __forceinline__ void q_tail_task(const Dims &d, const QPlan &qp, const uint4 *__restrict__ TmQ,
                                             const uint16_t *svh, const uint16_t *svl, float *dst,
-                                            const uint32_t *__restrict__ tord, uint32_t c0, uint32_t c1,
-                                            uint32_t tg)
+                                            const float *__restrict__ B2, const uint32_t *__restrict__ tord,
+                                            uint32_t c0, uint32_t c1, uint32_t tg)
 {
     const uint32_t S = d.S, T = d.T;
     const uint32_t lane = lane_of();
@@ -1775,14 +1782,17 @@ __device__ __forceinline__ void q_tail_task(const Dims &d, const QPlan &qp, cons
     for (int r = 0; r < 4; r++) {
         const uint32_t row = 4 * (lane >> 4) + r, g = row / qp.TR, i = row - g * qp.TR;
         const uint32_t ts = qp.TTF * 16 + i, s = (st0 + g) * 16 + (lane & 15);
-        if (g < nd && ts < T && s < S) dst[tord[ts] * S + s] = acch[r] + accl[r];
+        if (g < nd && ts < T && s < S) q_store(dst, B2, tord[ts] * S + s, acch[r] + accl[r]);
     }
 }
 
+// sQ: the [T][S] table of channel slice 0 (with B2 and a single slice, the
+// window's row of ovQ, Q - B2/2 stored directly); sQp: the other slices'.
 __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__restrict__ TmQ,
                               const uint16_t *svh, const uint16_t *svl, float *sQ, float *sQp,
-                              const uint32_t *__restrict__ tord, const uint32_t *__restrict__ tile_lo,
-                              const uint32_t *__restrict__ tile_hi, uint32_t rot)
+                              const float *__restrict__ B2, const uint32_t *__restrict__ tord,
+                              const uint32_t *__restrict__ tile_lo, const uint32_t *__restrict__ tile_hi,
+                              uint32_t rot)
 {
     const uint32_t S = d.S, T = d.T, TS = T * S;
     const uint32_t lane = lane_of();
@@ -1796,7 +1806,7 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
         if (rest >= qp.TTF * qp.NSG) {
             const uint32_t tl = qp.TT - 1;
             const uint32_t c0 = max(ks * d.C / qp.KS, tile_lo[tl]), c1 = max(c0, min((ks + 1) * d.C / qp.KS, tile_hi[tl]));
-            q_tail_task(d, qp, TmQ, svh, svl, ks == 0 ? sQ : sQp + (uint64_t)(ks - 1) * TS, tord, c0, c1,
+            q_tail_task(d, qp, TmQ, svh, svl, ks == 0 ? sQ : sQp + (uint64_t)(ks - 1) * TS, B2, tord, c0, c1,
                         rest - qp.TTF * qp.NSG);
             continue;
         }
@@ -1856,7 +1866,7 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const uint32_t ts = tt * 16 + 4 * (lane >> 4) + r;
-                if (ts < T && scol < S) dst[tord[ts] * S + scol] = acch[j][r] + accl[j][r];
+                if (ts < T && scol < S) q_store(dst, B2, tord[ts] * S + scol, acch[j][r] + accl[j][r]);
             }
         }
     }
@@ -1880,8 +1890,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const FastDiv divLPH(LPH);
     uint16_t *svh = (uint16_t *)dyn;                    // [C][LPH] zero-padded voltage, bf16 hi
     uint16_t *svl = svh + C * LPH;                      // [C][LPH] bf16 lo
+    // several channel slices: their partial tables, summed below; one slice:
+    // the table goes straight to ovQ (no LDS)
     float *sQp = (float *)(svl + C * LPH);              // [KS-1][T][S] channel-slice partials
-    float *sQ = sQp + (qp.KS - 1) * TS;                 // [T][S]
+    float *sQ = sQp + (qp.KS - 1) * TS;                 // [T][S] (KS > 1)
     const uint32_t kend = min(ctr->n_ov, kbase + d.ov_batch);
     // The zero padding of the staged rows is written once; a window stages
     // only its C*N samples.  The next window's header is loaded in three
@@ -1926,10 +1938,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                          put_v);
         head_step(2);
         __syncthreads();
-        q_table_split(d, qp, TmQ, svh, svl, sQ, sQp, tord, tile_lo, tile_hi, k & 3u);
+        float *dst = ovQ + (uint64_t)(k - kbase) * TS;
+        if (qp.KS == 1) {
+            q_table_split(d, qp, TmQ, svh, svl, dst, sQp, B2, tord, tile_lo, tile_hi, k & 3u);
+            head_step(3);
+            continue;
+        }
+        q_table_split(d, qp, TmQ, svh, svl, sQ, sQp, nullptr, tord, tile_lo, tile_hi, k & 3u);
         __syncthreads();
         head_step(3);
-        float *dst = ovQ + (uint64_t)(k - kbase) * TS;
         batched_stage<8>(TS, [&](uint32_t e) { return B2[e]; }, [&](uint32_t e, float b2) {
             float q = sQ[e];
             for (uint32_t j = 1; j < qp.KS; j++) q += sQp[(j - 1) * TS + e];
@@ -1941,7 +1958,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 size_t overlap_q_lds(const Dims &d)
 {
     const QPlan qp = q_plan(d);
-    return (uint64_t)d.C * qp.LPH * 4 + (uint64_t)qp.KS * d.T * d.S * 4;
+    return (uint64_t)d.C * qp.LPH * 4 + (qp.KS > 1 ? (uint64_t)qp.KS * d.T * d.S * 4 : 0);
 }
 
 // TmQ[(c*NKQ + kq)*TT + tt][lane] = {hi, lo} fragments (4 bf16 each, sample
