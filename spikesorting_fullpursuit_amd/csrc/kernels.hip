@@ -2188,31 +2188,36 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                 sE[t] = ecoef * (nsum * vn + nsum * nsum) + eabs;
             }
             if (!MQ) __syncthreads();   // Q - B2/2 complete
-            // One wave per template t, by ballot over s: the group of
-            // positive-likelihood (t, s) in plist, max over s of Q'(t, s) and
-            // over the positive s; for t = lab also the main shifts: the
-            // positive ones, then the other in-bounds ones, each ascending.
+            // Sixteen lanes per template t (four templates per wave), by
+            // ballot over s within the group: the group of positive-likelihood
+            // (t, s) in plist, ascending s; max over s of Q'(t, s) and over the
+            // positive s; for t = lab also the main shifts: the positive ones,
+            // then the other in-bounds ones, each ascending.
             {
-                const uint32_t l = lane_of();
-                const uint64_t below = (1ull << l) - 1;
-                for (uint32_t t = threadIdx.x >> 6; t < T; t += NW) {
+                const uint32_t l = lane_of(), gl = l & 15u;
+                const uint64_t gmask = 0xffffull << (l & 48u);
+                const uint64_t below = ((1ull << l) - 1) & gmask;
+                for (uint32_t t0 = (threadIdx.x >> 6) * 4; t0 < T; t0 += NW * 4) {
+                    const uint32_t t = t0 + (l >> 4);
+                    const bool act = t < T;
                     float m = -INFINITY, mp = -INFINITY;
                     uint32_t cnt = 0, cneg = 0;
-                    for (uint32_t s0 = 0; s0 < S; s0 += kWave) {
-                        const uint32_t s = s0 + l;
-                        const uint8_t sp = s < S ? spos[t * S + s] : (uint8_t)0;
-                        const float q = s < S ? sQ[t * S + s] : -INFINITY;
+                    for (uint32_t s0 = 0; s0 < S; s0 += 16) {
+                        const uint32_t s = s0 + gl;
+                        const bool in = act && s < S;
+                        const uint8_t sp = in ? spos[t * S + s] : (uint8_t)0;
+                        const float q = in ? sQ[t * S + s] : -INFINITY;
                         m = fmaxf(m, q);
                         if (sp & 1) mp = fmaxf(mp, q);
-                        cnt += __popcll(__ballot(sp & 1));
-                        cneg += __popcll(__ballot((sp & 3) == 2));
+                        cnt += __popcll(__ballot(sp & 1) & gmask);
+                        cneg += __popcll(__ballot((sp & 3) == 2) & gmask);
                     }
-                    for (int off = 32; off > 0; off >>= 1) {
+                    for (int off = 8; off > 0; off >>= 1) {   // within the group of sixteen lanes
                         m = fmaxf(m, __shfl_xor(m, off));
                         mp = fmaxf(mp, __shfl_xor(mp, off));
                     }
                     uint32_t base = 0;
-                    if (l == 0) {
+                    if (gl == 0 && act) {
                         base = atomicAdd(&n_ppos, cnt);
                         atomicMax(&s_maxcnt, cnt);
                         pofs[t] = (uint16_t)base;
@@ -2224,12 +2229,13 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                             n_fneg = cneg;
                         }
                     }
-                    base = __shfl(base, 0);
+                    base = __shfl(base, (int)(l & 48u));
                     uint32_t op = 0, on = 0;
-                    for (uint32_t s0 = 0; s0 < S; s0 += kWave) {
-                        const uint32_t s = s0 + l;
-                        const uint8_t sp = s < S ? spos[t * S + s] : (uint8_t)0;
-                        const uint64_t mpos = __ballot(sp & 1), mneg = __ballot((sp & 3) == 2);
+                    for (uint32_t s0 = 0; s0 < S; s0 += 16) {
+                        const uint32_t s = s0 + gl;
+                        const bool in = act && s < S;
+                        const uint8_t sp = in ? spos[t * S + s] : (uint8_t)0;
+                        const uint64_t mpos = __ballot(sp & 1) & gmask, mneg = __ballot((sp & 3) == 2) & gmask;
                         if (sp & 1) plist[base + op + __popcll(mpos & below)] = (uint16_t)(t * S + s);
                         if (t == lab) {
                             if (sp & 1) flist[op + __popcll(mpos & below)] = s;
