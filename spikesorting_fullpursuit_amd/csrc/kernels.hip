@@ -700,13 +700,17 @@ __device__ __forceinline__ float fma_chain_contig(float acc, const float *__rest
 }
 
 // Float32 re-score of one (window, template chunk)'s queued groups when the
-// templates do not fit in LDS: one wave; each batch of 64 groups is expanded
-// in LDS into its flagged positions, one chain per lane, over the channels
-// the lane's template is nonzero on, with template rows and voltage read from
-// global memory (L1/L2) eight samples ahead of the chain.  Each chain is the
-// reference's: bias, then c ascending, j = 0..N-1, fused multiply-adds; a
-// channel where the template is zero adds exactly 0 (the running sum starts at
-// the nonzero bias and is never -0), so it is skipped.
+// templates do not fit in LDS: one wave, one queued group per lane (four
+// consecutive positions of one template, a mask of those flagged).  The four
+// chains run together (fma_chain4: each template sample read once for the
+// four, the voltage sliding through registers) over the channels the
+// template is nonzero on, template rows and voltage read from global memory
+// (L1/L2) eight samples ahead; only the flagged positions are stored.  Each
+// chain is the reference's: bias, then c ascending, j = 0..N-1, fused
+// multiply-adds; a channel where the template is zero adds exactly 0 (the
+// running sum starts at the nonzero bias and is never -0), so it is skipped.
+// A group whose fourth chain would read past the recording's last sample
+// runs its flagged chains one at a time.
 __global__ __launch_bounds__(64) void k_rescore_sup(Dims d, const float *__restrict__ V,
                                                     const float *__restrict__ Tm, const float *__restrict__ bias,
                                                     const uint32_t *__restrict__ sup_lo,
@@ -714,35 +718,31 @@ __global__ __launch_bounds__(64) void k_rescore_sup(Dims d, const float *__restr
                                                     const uint64_t *__restrict__ rq, const uint2 *__restrict__ rdesc,
                                                     float *L)
 {
-    __shared__ uint64_t pos[4 * 64];   // (template << 32) | position of a batch's flagged positions
     const uint2 dsc = rdesc[blockIdx.x];
     const uint32_t cnt = dsc.y;
-    if (cnt == 0) return;
-    const uint32_t lane = threadIdx.x;
-    for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
-        const uint32_t e = e0 + lane;
-        const uint64_t ent = e < cnt ? rq[dsc.x + e] : 0;
-        const uint32_t m = (uint32_t)(ent >> 48) & 15u;
-        uint32_t incl = __popc(m);
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(incl, off);
-            if (lane >= (uint32_t)off) incl += y;
+    for (uint32_t e = threadIdx.x; e < cnt; e += 64) {
+        const uint64_t ent = rq[dsc.x + e];
+        const uint32_t m = (uint32_t)(ent >> 48) & 15u, t = (uint32_t)(ent >> 32) & 0xffffu;
+        const uint64_t i0 = (uint32_t)ent;
+        const float b = bias[t];
+        const uint32_t lo = sup_lo[t], hi = sup_hi[t];
+        const float *tt = Tm + (uint64_t)t * d.C * d.N;
+        float *lt = L + (uint64_t)t * d.K + i0;
+        if (i0 + d.N + 3 <= d.K) {
+            float acc[4] = {b, b, b, b};
+            for (uint32_t c = lo; c < hi; c++) fma_chain4(acc, tt + (uint64_t)c * d.N, 1, V + (uint64_t)c * d.K + i0, d.N);
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (m >> q & 1) lt[q] = acc[q];
+        } else {
+            for (int q = 0; q < 4; q++) {
+                if (!(m >> q & 1)) continue;
+                float acc = b;
+                for (uint32_t c = lo; c < hi; c++)
+                    acc = fma_chain_contig(acc, tt + (uint64_t)c * d.N, V + (uint64_t)c * d.K + i0 + q, d.N);
+                lt[q] = acc;
+            }
         }
-        const uint32_t total = __shfl(incl, 63);
-        uint32_t o = incl - __popc(m);
-        const uint64_t base = ent & 0x0000ffffffffffffull;   // template << 32 | first position
-        for (int q = 0; q < 4; q++)
-            if (m >> q & 1) pos[o++] = base + q;
-        __syncthreads();
-        for (uint32_t p = lane; p < total; p += 64) {
-            const uint32_t t = (uint32_t)(pos[p] >> 32);
-            const uint64_t i = pos[p] & 0xffffffffull;
-            float acc = bias[t];
-            for (uint32_t c = sup_lo[t]; c < sup_hi[t]; c++)
-                acc = fma_chain_contig(acc, Tm + ((uint64_t)t * d.C + c) * d.N, V + (uint64_t)c * d.K + i, d.N);
-            L[(uint64_t)t * d.K + i] = acc;
-        }
-        __syncthreads();
     }
 }
 
