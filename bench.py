@@ -125,7 +125,12 @@ def main():
 
     m_full = None
     if args.full_shift_steps > 0 and M != N - 1:
-        m_full, m_spk = pursuit_entry(args, rec, args.precision, args.overlap_mode, N - 1, args.full_shift_steps)
+        try:
+            m_full, m_spk = pursuit_entry(args, rec, args.precision, args.overlap_mode, N - 1, args.full_shift_steps)
+        except ValueError as exc:   # e.g. the screened overlap's per-window tables exceed LDS
+            m_full = {"skipped": str(exc)}
+            log("n_max_shift_inds = %d: skipped (%s)" % (N - 1, exc))
+    if m_full is not None and "skipped" not in m_full:
         if args.plain_steps > 0:
             ref, _ = pursuit_entry(args, rec, "fp32", "direct", N - 1, 1, warmup=0, ref_spikes=m_spk)
             m_full["fp32_plain"] = {k: ref[k] for k in ("value", "ms_per_step", "n_spikes", "n_passes", "stages",

@@ -99,10 +99,13 @@ enum {
     FBP_DEBUG_OVERLAP_FEW_SLOTS = 128, /* screened overlap: 64 deferred candidate
                                       slots per batch, so most windows with
                                       candidates score them in the screen       */
-    FBP_DEBUG_OVERLAP_WIDE = 256   /* screened overlap: the eight-wave screen block
+    FBP_DEBUG_OVERLAP_WIDE = 256,  /* screened overlap: the eight-wave screen block
                                       and deferred exact scores that read the
                                       templates from global memory (the route of
                                       templates too large for LDS)              */
+    FBP_DEBUG_FULL_CORRELATION = 512 /* recompute the likelihood of every listed
+                                      window and template, also where no voltage
+                                      under it changed since the last pass      */
 };
 
 typedef struct fbp_stats {

@@ -29,7 +29,18 @@ struct Dims {
     uint32_t corr_span;// widest channel span of such a chunk of tiles (0 until templates are set)
     uint32_t xc_wide;  // screened overlap (test switch): eight-wave lean screen and exact scores
                        // from the padded templates in global memory whatever their size
+    uint32_t dirty_g;  // channels per bit of a window's dirty mask (ceil(C / 64))
+    uint32_t dirty_off;// test switch: recompute every listed (window, template chunk)
 };
+
+// Bits of the channel groups (dirty_g channels each) that meet channels [lo, hi).
+__host__ __device__ inline uint64_t channel_mask(uint32_t lo, uint32_t hi, uint32_t g)
+{
+    if (lo >= hi) return 0;
+    const uint32_t b0 = lo / g, b1 = (hi - 1) / g;   // b1 <= 63
+    const uint64_t upto = b1 >= 63 ? ~0ull : (2ull << b1) - 1;
+    return upto & ~((1ull << b0) - 1);
+}
 
 // The re-score queue is split into kQueueShards equal shards; a correlation
 // block reserves entries in shard blockIdx % kQueueShards, whose counters
@@ -60,6 +71,8 @@ struct DeviceState {
     float *bias;         // [Tp]
     float *thr;          // [T]
     float *L;            // [T][K] likelihood
+    uint64_t *wdirty;    // [W] channel groups whose voltage changed under the window's positions
+                         // since its likelihood was last computed (all set at the start of a run)
     float *best_L;       // [W]
     uint32_t *best_lab;  // [W]
     uint32_t *best_idx;  // [W]

@@ -93,11 +93,14 @@ __global__ __launch_bounds__(64) void k_corr_fp32(Dims d, const float *__restric
                                                   const float *__restrict__ TmT,
                                                   const float *__restrict__ bias,
                                                   const uint32_t *__restrict__ win, float *__restrict__ L,
-                                                  uint32_t cb_max)
+                                                  uint32_t cb_max, const uint64_t *__restrict__ wdirty)
 {
     __shared__ float lds[kCorrLdsFloats];
     const int lane = threadIdx.x;
     const uint32_t w = win[blockIdx.x];
+    // no voltage under the window's positions changed since they were
+    // computed: L holds these values already
+    if (!d.dirty_off && wdirty[w] == 0) return;
     const uint64_t start = (uint64_t)w * d.N;
     const uint64_t stop = min(start + d.N, (uint64_t)d.K);
     const uint32_t span = kWave + d.N - 1;
@@ -340,7 +343,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
                                                   const uint32_t *__restrict__ tile_lo,
                                                   const uint32_t *__restrict__ tile_hi,
                                                   const uint32_t *__restrict__ sup_lo,
-                                                  const uint32_t *__restrict__ sup_hi, uint32_t *rq_shard)
+                                                  const uint32_t *__restrict__ sup_hi, uint32_t *rq_shard,
+                                                  const uint64_t *__restrict__ wdirty)
 {
     using Tr = NarrowTraits<PREC>;
     constexpr uint32_t KS = (RT * 16 + Tr::kK - 1) / Tr::kK;   // k-steps per channel (N padded to RT*16)
@@ -375,6 +379,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC != 3 &&
         }
     }
     if (chi < clo) chi = clo;
+    // no voltage on the chunk's channels changed under the window's positions
+    // since they were computed: L holds these values already
+    if (!d.dirty_off) {
+        const uint64_t cm = channel_mask(clo, chi, d.dirty_g);
+        if (cm && !(wdirty[w] & cm)) {
+            if (lane == 0) rdesc[(uint64_t)wi * nchunk + chunk] = make_uint2(0u, 0u);
+            return;
+        }
+    }
 
     // Per-position window norm bound |v_i| over the chunk's channels [clo, chi)
     // (the only ones whose samples enter its products), float64 sums of the
@@ -2903,7 +2916,9 @@ constexpr int kMaxCover = 32;
 __global__ __launch_bounds__(256) void k_residual(Dims d, float *V, const float *__restrict__ Tm,
                                                   const uint32_t *__restrict__ sp_idx,
                                                   const uint32_t *__restrict__ sp_lab, uint32_t n_new,
-                                                  const uint32_t *__restrict__ win_spike)
+                                                  const uint32_t *__restrict__ win_spike,
+                                                  const uint32_t *__restrict__ sup_lo,
+                                                  const uint32_t *__restrict__ sup_hi, uint64_t *wdirty)
 {
     // spikes of the pass overlapping this spike's span, in (onset, label, position) order
     __shared__ uint32_t ci[kMaxCover], cl[kMaxCover], cp[kMaxCover];
@@ -2913,6 +2928,15 @@ __global__ __launch_bounds__(256) void k_residual(Dims d, float *V, const float 
         if (lab >= d.T) continue;   // bp.cl:78-81 (block-uniform)
         __syncthreads();
         if (threadIdx.x == 0) {
+            // the likelihood positions whose samples [i, i + N) meet
+            // [idx, idx + N), on the channels where the template is nonzero
+            // (elsewhere the subtraction leaves every sample's value), must
+            // be computed again when listed
+            const uint64_t cm = channel_mask(sup_lo[lab], sup_hi[lab], d.dirty_g);
+            const uint64_t last = min((uint64_t)idx + d.N - 1, (uint64_t)d.K - 1);
+            const uint32_t w0 = idx + 1 > d.N ? (idx + 1 - d.N) / d.N : 0u;
+            const uint32_t w1 = min((uint32_t)(last / d.N), d.W - 1);
+            for (uint32_t w = w0; w <= w1; w++) atomicOr((unsigned long long *)&wdirty[w], (unsigned long long)cm);
             // windows whose spike can overlap [idx, idx + N): a spike added by
             // window w lies in [wN - M, (w+1)N + M)
             const int64_t lo = (int64_t)idx - 2 * (int64_t)d.N - (int64_t)d.M + 2;
@@ -3034,6 +3058,13 @@ hipError_t launch_transpose_templates(const Dims &d, const float *Tm, const floa
     return hipGetLastError();
 }
 
+// The listed windows' likelihood is current: their dirty masks are consumed.
+__global__ void k_clear_dirty(const uint32_t *__restrict__ win, uint32_t n_win, uint64_t *wdirty)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_win) wdirty[win[k]] = 0;
+}
+
 hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st)
 {
     if (n_win == 0) return hipSuccess;
@@ -3041,7 +3072,9 @@ hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win,
     uint32_t cb_max = kCorrLdsFloats / span;
     if (cb_max > d.C) cb_max = d.C;
     if (cb_max == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_corr_fp32, dim3(n_win), dim3(64), 0, st, d, s.V, s.TmT, s.bias, s.win, s.L, cb_max);
+    hipLaunchKernelGGL(k_corr_fp32, dim3(n_win), dim3(64), 0, st, d, s.V, s.TmT, s.bias, s.win, s.L, cb_max,
+                       s.wdirty);
+    hipLaunchKernelGGL(k_clear_dirty, dim3((n_win + 255) / 256), dim3(256), 0, st, s.win, n_win, s.wdirty);
     return hipGetLastError();
 }
 
@@ -3110,7 +3143,8 @@ static hipError_t launch_corr_mfma_rt(const Dims &d, const DeviceState &s, uint3
     hipLaunchKernelGGL((k_corr_mfma<PREC, RT, CTB>), dim3(n_win * mfma_nchunk(d)), dim3(64), lds, st, d, s.V, s.Tm,
                        (const Frag<PREC> *)s.TmB, s.bias, s.tnorm, s.tscale, s.win, s.L, s.ctr, s.rq, rq_cap,
                        (uint2 *)s.rdesc, cb_max, co, no_rescore, s.tord, s.tile_lo, s.tile_hi, s.sup_lo, s.sup_hi,
-                       s.rq_shard);
+                       s.rq_shard, s.wdirty);
+    hipLaunchKernelGGL(k_clear_dirty, dim3((n_win + 255) / 256), dim3(256), 0, st, s.win, n_win, s.wdirty);
     return hipGetLastError();
 }
 
@@ -3452,7 +3486,8 @@ hipError_t launch_residual(const Dims &d, const DeviceState &s, uint64_t spike_b
 {
     if (n_new == 0) return hipSuccess;
     hipLaunchKernelGGL(k_residual, dim3(grid_for(n_new, 1, 65536)), dim3(256), 0, st, d, s.V, s.Tm,
-                       s.sp_idx + spike_base, s.sp_lab + spike_base, n_new, s.win_spike);
+                       s.sp_idx + spike_base, s.sp_lab + spike_base, n_new, s.win_spike, s.sup_lo, s.sup_hi,
+                       s.wdirty);
     return hipGetLastError();
 }
 

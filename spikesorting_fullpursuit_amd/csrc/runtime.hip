@@ -159,6 +159,7 @@ void for_each_buffer(const fbp_config *cfg, const Dims &d, DeviceState &s, V &&v
     visit(s.bias, (uint64_t)d.Tp);
     visit(s.thr, T);
     visit(s.L, T * d.K);
+    visit(s.wdirty, W);
     visit(s.best_L, W);
     visit(s.best_lab, W);
     visit(s.best_idx, W);
@@ -399,6 +400,8 @@ static int make_dims(const fbp_config *cfg, Dims &d)
                    ? overlap_exact_cap(d) : 0;
     if (d.xc_cap && (cfg->debug_flags & FBP_DEBUG_OVERLAP_FEW_SLOTS)) d.xc_cap = 64;
     d.xc_wide = d.xc_cap && (cfg->debug_flags & FBP_DEBUG_OVERLAP_WIDE) ? 1u : 0u;
+    d.dirty_g = (C + 63) / 64;
+    d.dirty_off = (cfg->debug_flags & FBP_DEBUG_FULL_CORRELATION) ? 1u : 0u;
     return FBP_OK;
 }
 
@@ -676,6 +679,7 @@ int fbp::segment_begin(fbp_segment *g, hipStream_t stream)
     HIP_TRY(launch_fill_u32(s.ov_idx, d.W, 0u, st), "init");
     HIP_TRY(launch_fill_u32(s.ov_lab, d.W, 0u, st), "init");
     HIP_TRY(launch_fill_u32(s.win_spike, d.W, 0u, st), "init");
+    HIP_TRY(launch_fill_u32((uint32_t *)s.wdirty, 2ull * d.W, ~0u, st), "init");   // pass 1 computes everything
     HIP_TRY(launch_fill_u8(s.nc, d.W + 2, 0, st), "init");
     HIP_TRY(launch_fill_u8(s.ov, d.W + 2, 0, st), "init");
     HIP_TRY(launch_iota_u32(s.win, d.W, st), "init");

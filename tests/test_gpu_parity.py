@@ -77,7 +77,7 @@ def test_dense_16ch_spike_list_grows(oracle_mod, mode):
 
 
 @pytest.mark.parametrize("debug", [0, 2, 4, 8, 12, 16, 18, 32, 50, 64, 66, 68, 72, 76, 80, 128, 130, 136, 160,
-                                   256, 258, 264, 272, 288, 384, 386])
+                                   256, 258, 264, 272, 288, 384, 386, 512])
 def test_screened_overlap_candidate_paths(oracle_mod, debug):
     # The screened overlap finds its exact-score candidates from per-thread
     # lists and settles single candidates from their score interval, unless
@@ -91,8 +91,10 @@ def test_screened_overlap_candidate_paths(oracle_mod, debug):
     # per batch, so windows of both kinds meet in one batch; 256 takes the
     # route of templates too large for LDS (eight-wave screen blocks without
     # template rows, exact scores read from the zero-padded templates in
-    # global memory, channels no lane's templates touch skipped).  Every
-    # route gives the oracle's events.
+    # global memory, channels no lane's templates touch skipped); 512
+    # recomputes every listed window's likelihood, also the template chunks
+    # whose channels saw no voltage change since the last pass.  Every route
+    # gives the oracle's events.
     from spikesorting_fullpursuit_amd import synthetic
     rec = synthetic.config("16ch-300s", seed=7, duration_s=0.5)
     sm = rec['separability_metrics']
