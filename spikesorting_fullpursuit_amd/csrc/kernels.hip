@@ -1324,13 +1324,14 @@ __device__ __forceinline__ float overlap_support_score_z(const Dims &d, const fl
     return acc;
 }
 
-// The same score over the channels where either template is nonzero: the
+// The same score over the channels where either template is nonzero (the
+// window's voltage row c at sv + c * vstr): the
 // main template's rows amain[c - lo_m] for c in [lo_m, hi_m), the partner's
 // bt[c - lo_t] for c in [lo_t, hi_t), +0.0 elsewhere.  A channel where
 // both are zero adds exactly 0 to the reference chain (every product is +-0,
 // the running sum is never -0 and ss = 0 skips the subtraction), so it is
 // skipped.
-__device__ __forceinline__ float overlap_support_score(const Dims &d, const float *__restrict__ sv,
+__device__ __forceinline__ float overlap_support_score(const Dims &d, const float *__restrict__ sv, uint64_t vstr,
                                                        const float *amain, uint32_t lo_m, uint32_t hi_m,
                                                        const float *bt, uint32_t lo_t, uint32_t hi_t,
                                                        uint32_t f, uint32_t s)
@@ -1343,7 +1344,7 @@ __device__ __forceinline__ float overlap_support_score(const Dims &d, const floa
         if (!ha && !hb) continue;
         const float *a = amain + (uint64_t)(ha ? c - lo_m : 0u) * R + 2 * d.M - f;
         const float *b = bt + (uint64_t)(hb ? c - lo_t : 0u) * R + 2 * d.M - s;
-        const float *vc = sv + c * d.N;
+        const float *vc = sv + c * vstr;
         float ss = 0.0f;
         // operands are loaded eight samples at a time ahead of the chains
         uint32_t j = 0;
@@ -2053,11 +2054,16 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     double *sE = (double *)dyn;                         // [T] screen bound per template (this window)
     constexpr bool kLean = NW != 4;
     constexpr uint32_t kCand = 4 * NW * 64;             // candidate list: four per thread
-    float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset
-    float *smain = sv + C * N;                          // [SM][R] padded best-label template, its support rows
+    // kQG (the eight-wave lean block on matrix-core Q tables, for hundreds of
+    // templates): the window's Q table is read from ovQ and its voltage from
+    // V in global memory, so the LDS holds the per-template bookkeeping only
+    // and two blocks fit in a CU
+    constexpr bool kQG = MQ && NW == 8;
+    float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset (not kQG)
+    float *smain = sv + (kQG ? 0u : C * N);             // [SM][R] padded best-label template, its support rows
     float *szero = smain + (kLean ? 0u : SM * R);       // [R] zeros (not lean)
-    float *sQ = szero + (kLean ? 0u : R);               // [T][S] Q - B2/2
-    float *stn = sQ + TS;                               // [T] template norms
+    float *sQ = szero + (kLean ? 0u : R);               // [T][S] Q - B2/2 (not kQG)
+    float *stn = sQ + (kQG ? 0u : TS);                  // [T] template norms
     uint32_t *flist = (uint32_t *)(stn + T);            // main shifts: positive first, then the rest
     float *sb = (float *)(flist + S);                   // [SM][R] partner template of an exact score (not lean)
     // not lean: kept rows alias the partner template, read before the
@@ -2133,6 +2139,10 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
         const bool wskip = (Lb < thl) || ((uint64_t)bidx >= (uint64_t)(d.K - d.N));   // bp.cl:518-527
         float sc = -INFINITY;
         uint64_t key = ~0ull, bp = 0;
+        // the window's Q table and voltage rows (stride vstr)
+        const float *qT = kQG ? ovQ + (uint64_t)(k - kbase) * TS : sQ;
+        const float *vw = kQG ? V + bidx : sv;
+        const uint64_t vstr = kQG ? (uint64_t)d.K : (uint64_t)N;
         if (!wskip) {
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -2150,10 +2160,13 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             double vv = 0.0;
             auto a_v = stage_arr<4>(C * N, [&](uint32_t e) { const uint32_t c = divN.div(e);
                                                               return V[(uint64_t)c * d.K + bidx + (e - c * N)]; },
-                                    [&](uint32_t e, float x) { sv[e] = x; vv = __builtin_fma((double)x, (double)x, vv); });
+                                    [&](uint32_t e, float x) {
+                                        if (!kQG) sv[e] = x;
+                                        vv = __builtin_fma((double)x, (double)x, vv);
+                                    });
             auto a_main = stage_arr<8>(kLean ? 0u : (hi_m - lo_m) * R, [&](uint32_t e) { return Tpad[((uint64_t)lab * C + lo_m) * R + e]; },
                                        [&](uint32_t e, float x) { smain[e] = x; });
-            auto a_q = stage_arr<10>(MQ ? TS : 0, [&](uint32_t e) { return ovQ[(uint64_t)(k - kbase) * TS + e]; },
+            auto a_q = stage_arr<10>(MQ && !kQG ? TS : 0, [&](uint32_t e) { return ovQ[(uint64_t)(k - kbase) * TS + e]; },
                                      [&](uint32_t e, float q) { sQ[e] = q; });
             auto a_l = stage_arr<10>(TS, [&](uint32_t e) {
                 const uint32_t t = divS.div(e), s = e - t * S;
@@ -2219,7 +2232,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                         const uint32_t s = s0 + gl;
                         const bool in = act && s < S;
                         const uint8_t sp = in ? spos[t * S + s] : (uint8_t)0;
-                        const float q = in ? sQ[t * S + s] : -INFINITY;
+                        const float q = in ? qT[t * S + s] : -INFINITY;
                         m = fmaxf(m, q);
                         if (sp & 1) mp = fmaxf(mp, q);
                         cnt += __popcll(__ballot(sp & 1) & gmask);
@@ -2273,7 +2286,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             }
             __syncthreads();
             const float *ABl = AB + (uint64_t)lab * S * TS;
-            const float *sQl = sQ + lab * S;
+            const float *sQl = qT + lab * S;
             const uint32_t nfp = n_fpos, nfn = n_fneg, maxcnt = s_maxcnt;
             auto screen = [&](uint32_t f, float qe, float ab) {
                 return (double)sQl[f] + (double)qe - (double)ab;
@@ -2353,7 +2366,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                     const uint8_t sp = spos[t * S + s];
                     const bool allowed = (a < nfp ? (sp & 2) : (sp & 1)) && (f > s ? f - s : s - f) <= N;
                     if (allowed)
-                        lo_est = fmax(lo_est, screen(f, sQ[t * S + s], ABl[(uint64_t)f * TS + t * S + s]) - sE[t]);
+                        lo_est = fmax(lo_est, screen(f, qT[t * S + s], ABl[(uint64_t)f * TS + t * S + s]) - sE[t]);
                 }
             }
             for (int off = 32; off > 0; off >>= 1) lo_est = fmax(lo_est, __shfl_xor(lo_est, off));
@@ -2388,7 +2401,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             double kv0 = -INFINITY, kv1 = -INFINITY, kv2 = -INFINITY, kv3 = -INFINITY;
             uint32_t kc0 = 0, kc1 = 0, kc2 = 0, kc3 = 0;
             uint32_t screened = 0;
-            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, sQ, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
+            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
                 const double sh = screen(f, qe, ab);
                 const double hi = sh + E;
                 lo_max = fmax(lo_max, sh - E);
@@ -2445,7 +2458,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                 const float *tmain = kLean ? Tpad + ((uint64_t)lab * C + lo_m) * R : smain;
                 auto exact = [&](uint32_t f, uint32_t t, uint32_t s, const float *bt) {
                     const float score =
-                        kLean ? overlap_support_score(d, sv, tmain, lo_m, hi_m, bt, sup_lo[t], sup_hi[t], f, s)
+                        kLean ? overlap_support_score(d, vw, vstr, tmain, lo_m, hi_m, bt, sup_lo[t], sup_hi[t], f, s)
                               : overlap_support_score_z(d, sv, tmain, lo_m, hi_m, bt, sup_lo[t], sup_hi[t], szero, f, s);
                     scored++;
                     uint64_t p;
@@ -2474,7 +2487,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                         if (threadIdx.x == 0) {
                             const uint32_t f = cand[0] >> 16, e = cand[0] & 0xffffu;
                             const uint32_t t = e / S, s = e - t * S;
-                            const double sh = screen(f, sQ[e], ABl[(uint64_t)f * TS + e]);
+                            const double sh = screen(f, qT[e], ABl[(uint64_t)f * TS + e]);
                             const double E = sE[t];
                             uint64_t p;
                             (void)combo_key(d, wg_bits, span, t, f, s, p);
@@ -2528,7 +2541,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                             __syncthreads();
                         }
                         uint32_t mine = 0;
-                        for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, sQ, sE, [&](uint32_t f, uint32_t, uint32_t, uint32_t, float ab, float qe, double E) {
+                        for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t, uint32_t, uint32_t, float ab, float qe, double E) {
                             if (screen(f, qe, ab) + E >= lo_max) {
                                 if (by_f) atomicAdd(&fpos[f], 1u);
                                 else mine++;
@@ -2569,7 +2582,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                         if (reserve(total)) {
                             const uint32_t first = s_defer - 1;
                             uint32_t slot = first + before + incl - mine;
-                            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, sQ, sE, [&](uint32_t f, uint32_t, uint32_t, uint32_t e, float ab, float qe, double E) {
+                            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t, uint32_t, uint32_t e, float ab, float qe, double E) {
                                 if (screen(f, qe, ab) + E >= lo_max)
                                     xc_cand[by_f ? first + atomicAdd(&fpos[f], 1u) : slot++] = (f << 16) | e;
                             });
@@ -2577,7 +2590,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                         }
                     }
                     if (!deferred)
-                        for_each_kept<1>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, sQ, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
+                        for_each_kept<1>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
                             const double sh = screen(f, qe, ab);
                             if (sh + E < lo_max) return;
                             exact(f, t, s, Tpad + ((uint64_t)t * C + sup_lo[t]) * R);
@@ -3272,16 +3285,21 @@ hipError_t launch_overlap_direct(const Dims &d, const DeviceState &s, uint32_t m
 
 constexpr uint64_t kLdsPerCu = 160 * 1024;
 
+// The matrix-core Q table is used whenever its kernel's LDS fits.
+static bool overlap_uses_mfma_q(const Dims &d) { return overlap_q_lds(d) <= 150 * 1024; }
+
 // Dynamic LDS of k_overlap_screened<MQ, nw> (the layout at the top of the kernel).
 static size_t overlap_screened_lds_nw(const Dims &d, int nw)
 {
+    const bool qg = nw == 8 && overlap_uses_mfma_q(d);   // Q table and voltage read from global memory
     const uint64_t ts = (uint64_t)d.T * d.S, R = d.N + 2 * d.M;
     const uint64_t krw = (ts + 1) / 2, cand = 4ull * nw * 64;
     const uint64_t partner = (uint64_t)d.sup_max * R;                           // NW 4: staged partner rows
     const uint64_t main_rows = nw != 4 ? 0 : partner + R;                        // main rows + the zero row
     const uint64_t region = nw != 4 ? krw + cand : (partner + cand > krw ? partner + cand : krw);
-    return (uint64_t)d.T * 8 + (uint64_t)d.C * d.N * 4 + main_rows * 4 + ts * 4 + (uint64_t)d.T * 4 +
-           (uint64_t)d.S * 4 + region * 4 + ts * 2 + (uint64_t)d.T * 6 + ts + 4 + (uint64_t)d.T * 8 + 16;
+    return (uint64_t)d.T * 8 + (qg ? 0 : (uint64_t)d.C * d.N * 4) + main_rows * 4 + (qg ? 0 : ts * 4) +
+           (uint64_t)d.T * 4 + (uint64_t)d.S * 4 + region * 4 + ts * 2 + (uint64_t)d.T * 6 + ts + 4 +
+           (uint64_t)d.T * 8 + 16;
 }
 
 // Waves per screen block: 4 when the block scores its candidates; with the
@@ -3294,9 +3312,6 @@ static int overlap_screened_waves(const Dims &d)
     if (!d.xc_wide && overlap_screened_lds_nw(d, 2) * 4 <= kLdsPerCu) return 2;
     return overlap_screened_lds_nw(d, 8) + 1024 <= kLdsPerCu ? 8 : 4;
 }
-
-// The matrix-core Q table is used whenever its kernel's LDS fits.
-static bool overlap_uses_mfma_q(const Dims &d) { return overlap_q_lds(d) <= 150 * 1024; }
 
 size_t overlap_screened_lds(const Dims &d)
 {
