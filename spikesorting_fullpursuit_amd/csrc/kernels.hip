@@ -3304,21 +3304,17 @@ static size_t overlap_screened_lds_nw(const Dims &d, int nw)
 
 // Waves per screen block: 4 when the block scores its candidates; with the
 // exact scores deferred (lean block) 2, or 8 when fewer than four two-wave
-// blocks fit in a CU's LDS (4 if not even the eight-wave block fits beside
-// the static LDS, which then scores in place of the lean one).
+// blocks fit in a CU's LDS.
 static int overlap_screened_waves(const Dims &d)
 {
     if (!d.xc_cap) return 4;
-    if (!d.xc_wide && overlap_screened_lds_nw(d, 2) * 4 <= kLdsPerCu) return 2;
-    return overlap_screened_lds_nw(d, 8) + 1024 <= kLdsPerCu ? 8 : 4;
+    return !d.xc_wide && overlap_screened_lds_nw(d, 2) * 4 <= kLdsPerCu ? 2 : 8;
 }
 
 size_t overlap_screened_lds(const Dims &d)
 {
-    // the block that runs, and the four-wave layout (the largest but for
-    // the eight-wave candidate list)
-    const size_t a4 = overlap_screened_lds_nw(d, 4), an = overlap_screened_lds_nw(d, overlap_screened_waves(d));
-    const size_t a = a4 > an ? a4 : an;
+    // the block that runs
+    const size_t a = overlap_screened_lds_nw(d, overlap_screened_waves(d));
     const size_t b = overlap_uses_mfma_q(d) ? overlap_q_lds(d) : 0;
     return a > b ? a : b;
 }

@@ -390,16 +390,19 @@ static int make_dims(const fbp_config *cfg, Dims &d)
         if (nb > d.W) nb = d.W;
         d.ov_batch = (uint32_t)nb;
     }
-    // the screened overlap's LDS grows with the widest template support, known
-    // when the templates are set; here it must fit for the narrowest one
-    d.sup_max = 1;
-    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED && overlap_screened_lds(d) > kMaxLds)
-        return fail(FBP_EINVAL, "screened overlap needs %zu bytes of LDS (T*S too large)", overlap_screened_lds(d));
-    d.sup_max = C;
     d.xc_cap = (cfg->overlap_mode == FBP_OVERLAP_SCREENED && !(cfg->debug_flags & FBP_DEBUG_OVERLAP_INBLOCK))
                    ? overlap_exact_cap(d) : 0;
     if (d.xc_cap && (cfg->debug_flags & FBP_DEBUG_OVERLAP_FEW_SLOTS)) d.xc_cap = 64;
     d.xc_wide = d.xc_cap && (cfg->debug_flags & FBP_DEBUG_OVERLAP_WIDE) ? 1u : 0u;
+    // the screened overlap's LDS grows with the widest template support, known
+    // when the templates are set; here it must fit for the narrowest one
+    d.sup_max = 1;
+    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED && (uint64_t)d.T * d.S > 65535)   // 16-bit (t, s) entries
+        return fail(FBP_EINVAL, "screened overlap supports T*S up to 65535 (T*S = %llu)",
+                    (unsigned long long)d.T * d.S);
+    if (cfg->overlap_mode == FBP_OVERLAP_SCREENED && overlap_screened_lds(d) > kMaxLds)
+        return fail(FBP_EINVAL, "screened overlap needs %zu bytes of LDS (T*S too large)", overlap_screened_lds(d));
+    d.sup_max = C;
     d.dirty_g = (C + 63) / 64;
     d.dirty_off = (cfg->debug_flags & FBP_DEBUG_FULL_CORRELATION) ? 1u : 0u;
     return FBP_OK;
