@@ -1737,6 +1737,23 @@ __device__ __forceinline__ uint64_t q_bfrag(const uint32_t *row, uint32_t y)
     return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
+// The same four samples from a row copy in which they start at an even
+// sample (two aligned dwords, no shifts).  With QC = 2 the voltage is staged
+// twice, the second copy moved left by one sample; a lane's B fragments all
+// start at samples of its lane's parity, so it reads the copy of that parity.
+__device__ __forceinline__ uint64_t q_bfrag_even(const uint32_t *row, uint32_t y)
+{
+    const uint32_t w = y >> 1;
+    return (uint64_t)row[w] | ((uint64_t)row[w + 1] << 32);
+}
+
+template <int QC>
+__device__ __forceinline__ uint64_t q_frag(const uint16_t *svc, uint32_t rsel, uint32_t c, uint32_t LPH, uint32_t y)
+{
+    const uint32_t *row = (const uint32_t *)(svc + c * LPH);
+    return QC == 2 ? q_bfrag_even(row, y - rsel) : q_bfrag(row, y);
+}
+
 // The tail tile's tasks (see QPlan): A fragments at
 // TmQ[(C*NKQ*TT) + (c*NKT + kq)][lane], one shift tile per task.
 // Store of a Q-table entry: with B2 (one channel slice), Q - B2/2 straight
@@ -1746,6 +1763,7 @@ __device__ __forceinline__ void q_store(float *dst, const float *__restrict__ B2
     dst[e] = B2 ? (float)((double)q - 0.5 * (double)B2[e]) : q;
 }
 
+template <int QC>
 __device__ __forceinline__ void q_tail_task(const Dims &d, const QPlan &qp, const uint4 *__restrict__ TmQ,
                                             const uint16_t *svh, const uint16_t *svl, float *dst,
                                             const float *__restrict__ B2, const uint32_t *__restrict__ tord,
@@ -1753,6 +1771,8 @@ __device__ __forceinline__ void q_tail_task(const Dims &d, const QPlan &qp, cons
 {
     const uint32_t S = d.S, T = d.T;
     const uint32_t lane = lane_of();
+    const uint32_t rsel = QC == 2 ? (lane & 1u) : 0u, plane = d.C * qp.LPH;
+    const uint16_t *sh = svh + rsel * plane, *sl = svl + rsel * plane;
     const uint32_t st0 = tg * qp.ND;
     const uint32_t nd = qp.ST - st0 < qp.ND ? qp.ST - st0 : qp.ND;   // offsets inside the shift range
     const uint32_t nk = qp.NKQ + nd - 1;
@@ -1765,8 +1785,6 @@ __device__ __forceinline__ void q_tail_task(const Dims &d, const QPlan &qp, cons
     const float4v z = float4v{0.0f, 0.0f, 0.0f, 0.0f};
     float4v h0 = z, h1 = z, x0 = z, x1 = z, y0s = z, y1s = z;
     for (uint32_t c = c0; c < c1; c++) {
-        const uint32_t *rh = (const uint32_t *)(svh + c * qp.LPH);
-        const uint32_t *rl = (const uint32_t *)(svl + c * qp.LPH);
         uint4 an = ap[(uint64_t)c * qp.NKT * 64];
         for (uint32_t kq = 0; kq < nk; kq++) {
             const uint4 a = an;
@@ -1774,7 +1792,7 @@ __device__ __forceinline__ void q_tail_task(const Dims &d, const QPlan &qp, cons
             const uint64_t ah = (uint64_t)a.x | ((uint64_t)a.y << 32);
             const uint64_t al = (uint64_t)a.z | ((uint64_t)a.w << 32);
             const uint32_t y0 = kq * 16 + ybase;
-            const uint64_t bh = q_bfrag(rh, y0), bl = q_bfrag(rl, y0);
+            const uint64_t bh = q_frag<QC>(sh, rsel, c, qp.LPH, y0), bl = q_frag<QC>(sl, rsel, c, qp.LPH, y0);
             if (kq & 1) {
                 h1 = mfma_bf16_k16(ah, bh, h1);
                 x1 = mfma_bf16_k16(ah, bl, x1);
@@ -1802,6 +1820,7 @@ __device__ __forceinline__ void q_tail_task(const Dims &d, const QPlan &qp, cons
 
 // sQ: the [T][S] table of channel slice 0 (with B2 and a single slice, the
 // window's row of ovQ, Q - B2/2 stored directly); sQp: the other slices'.
+template <int QC>
 __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__restrict__ TmQ,
                               const uint16_t *svh, const uint16_t *svl, float *sQ, float *sQp,
                               const float *__restrict__ B2, const uint32_t *__restrict__ tord,
@@ -1810,6 +1829,8 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
 {
     const uint32_t S = d.S, T = d.T, TS = T * S;
     const uint32_t lane = lane_of();
+    const uint32_t rsel = QC == 2 ? (lane & 1u) : 0u, plane = d.C * qp.LPH;
+    const uint16_t *sh = svh + rsel * plane, *sl = svl + rsel * plane;
     const uint32_t wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const uint32_t ntask = (qp.TTF * qp.NSG + qp.NTG) * qp.KS;
     for (uint32_t it = wave; it < ntask; it += nwave) {
@@ -1820,7 +1841,7 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
         if (rest >= qp.TTF * qp.NSG) {
             const uint32_t tl = qp.TT - 1;
             const uint32_t c0 = max(ks * d.C / qp.KS, tile_lo[tl]), c1 = max(c0, min((ks + 1) * d.C / qp.KS, tile_hi[tl]));
-            q_tail_task(d, qp, TmQ, svh, svl, ks == 0 ? sQ : sQp + (uint64_t)(ks - 1) * TS, B2, tord, c0, c1,
+            q_tail_task<QC>(d, qp, TmQ, svh, svl, ks == 0 ? sQ : sQp + (uint64_t)(ks - 1) * TS, B2, tord, c0, c1,
                         rest - qp.TTF * qp.NSG);
             continue;
         }
@@ -1857,13 +1878,12 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
                 if (s0 + q >= send) break;
                 const uint64_t ah = (uint64_t)a[q].x | ((uint64_t)a[q].y << 32);
                 const uint64_t al = (uint64_t)a[q].z | ((uint64_t)a[q].w << 32);
-                const uint32_t *rh = (const uint32_t *)(svh + c * qp.LPH);
-                const uint32_t *rl = (const uint32_t *)(svl + c * qp.LPH);
                 const uint32_t y0 = kq * 16 + ybase;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     if ((uint32_t)j < nst) {
-                        const uint64_t bh = q_bfrag(rh, y0 + 16 * j), bl = q_bfrag(rl, y0 + 16 * j);
+                        const uint64_t bh = q_frag<QC>(sh, rsel, c, qp.LPH, y0 + 16 * j),
+                                       bl = q_frag<QC>(sl, rsel, c, qp.LPH, y0 + 16 * j);
                         acch[j] = mfma_bf16_k16(ah, bh, acch[j]);
                         accl[j] = mfma_bf16_k16(ah, bl, accl[j]);
                         accl[j] = mfma_bf16_k16(al, bh, accl[j]);
@@ -1889,6 +1909,7 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
 // Q - B2/2 of every unskipped overlap window of a batch, on the matrix cores
 // (q_table_split), into ovQ[k - kbase][T][S] for k_overlap_screened<true>.
 // The window skip rule is the screen kernel's; skipped windows are not written.
+template <int QC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_overlap_q(
     Dims d, const float *__restrict__ V, const float *__restrict__ L, const float *__restrict__ thr,
     const uint32_t *__restrict__ ov_list, const Counters *__restrict__ ctr, const uint32_t *__restrict__ best_lab,
@@ -1902,18 +1923,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const QPlan qp = q_plan(d);
     const uint32_t LPH = qp.LPH;
     const FastDiv divLPH(LPH);
-    uint16_t *svh = (uint16_t *)dyn;                    // [C][LPH] zero-padded voltage, bf16 hi
-    uint16_t *svl = svh + C * LPH;                      // [C][LPH] bf16 lo
+    // QC copies of the zero-padded voltage, copy r moved left by r samples
+    uint16_t *svh = (uint16_t *)dyn;                    // [QC][C][LPH] bf16 hi
+    uint16_t *svl = svh + QC * C * LPH;                 // [QC][C][LPH] bf16 lo
     // several channel slices: their partial tables, summed below; one slice:
     // the table goes straight to ovQ (no LDS)
-    float *sQp = (float *)(svl + C * LPH);              // [KS-1][T][S] channel-slice partials
+    float *sQp = (float *)(svl + QC * C * LPH);         // [KS-1][T][S] channel-slice partials
     float *sQ = sQp + (qp.KS - 1) * TS;                 // [T][S] (KS > 1)
     const uint32_t kend = min(ctr->n_ov, kbase + d.ov_batch);
     // The zero padding of the staged rows is written once; a window stages
     // only its C*N samples.  The next window's header is loaded in three
     // dependent steps during the current one (no block writes what another
     // block reads).
-    for (uint32_t e = threadIdx.x; e < C * LPH; e += blockDim.x) {
+    for (uint32_t e = threadIdx.x; e < QC * C * LPH; e += blockDim.x) {
         svh[e] = 0;
         svl[e] = 0;
     }
@@ -1930,10 +1952,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (nstep < 3 && upto >= 3) { nLb = L[(uint64_t)nlab * d.K + nbidx]; nthl = thr[nlab]; nstep = 3; }
     };
     auto put_v = [&](uint32_t e, float x) {
-        const uint32_t c = divN.div(e), y = c * LPH + M + (e - c * N);
+        const uint32_t c = divN.div(e), yr = M + (e - c * N), y = c * LPH + yr;
         const uint32_t h = to_bf16(x);
+        const uint16_t l = to_bf16(x - __uint_as_float(h << 16));
         svh[y] = (uint16_t)h;
-        svl[y] = to_bf16(x - __uint_as_float(h << 16));
+        svl[y] = l;
+        if (QC == 2 && yr > 0) {   // row sample yr at yr - 1 of copy 1
+            svh[C * LPH + y - 1] = (uint16_t)h;
+            svl[C * LPH + y - 1] = l;
+        }
     };
     head_step(3);
     for (uint32_t k = kbase + blockIdx.x; k < kend; k += gridDim.x) {
@@ -1954,11 +1981,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         __syncthreads();
         float *dst = ovQ + (uint64_t)(k - kbase) * TS;
         if (qp.KS == 1) {
-            q_table_split(d, qp, TmQ, svh, svl, dst, sQp, B2, tord, tile_lo, tile_hi, k & 3u);
+            q_table_split<QC>(d, qp, TmQ, svh, svl, dst, sQp, B2, tord, tile_lo, tile_hi, k & 3u);
             head_step(3);
             continue;
         }
-        q_table_split(d, qp, TmQ, svh, svl, sQ, sQp, nullptr, tord, tile_lo, tile_hi, k & 3u);
+        q_table_split<QC>(d, qp, TmQ, svh, svl, sQ, sQp, nullptr, tord, tile_lo, tile_hi, k & 3u);
         __syncthreads();
         head_step(3);
         batched_stage<8>(TS, [&](uint32_t e) { return B2[e]; }, [&](uint32_t e, float b2) {
@@ -1969,11 +1996,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
 }
 
-size_t overlap_q_lds(const Dims &d)
+static size_t overlap_q_lds_qc(const Dims &d, uint32_t qc)
 {
     const QPlan qp = q_plan(d);
-    return (uint64_t)d.C * qp.LPH * 4 + (qp.KS > 1 ? (uint64_t)qp.KS * d.T * d.S * 4 : 0);
+    return (uint64_t)qc * d.C * qp.LPH * 4 + (qp.KS > 1 ? (uint64_t)qp.KS * d.T * d.S * 4 : 0);
 }
+
+// Voltage copies of k_overlap_q: two (aligned fragment reads) where the block
+// still fits four times in a CU's LDS, else one.
+uint32_t overlap_q_copies(const Dims &d) { return overlap_q_lds_qc(d, 2) <= 40 * 1024 ? 2u : 1u; }
+
+size_t overlap_q_lds(const Dims &d) { return overlap_q_lds_qc(d, overlap_q_copies(d)); }
 
 // TmQ[(c*NKQ + kq)*TT + tt][lane] = {hi, lo} fragments (4 bf16 each, sample
 // kq*16 + 4*(lane/16) + i in bits 16i) of template t = tt*16 + lane%16 on
@@ -3391,8 +3424,8 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
     }
     const size_t qlds = overlap_q_lds(d);
     if (mq && qlds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_overlap_q, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)qlds);
+        hipError_t e = hipFuncSetAttribute(overlap_q_copies(d) == 2 ? (const void *)k_overlap_q<2> : (const void *)k_overlap_q<1>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)qlds);
         if (e != hipSuccess) return e;
     }
     // exact scores: templates staged in LDS when they fit, else read from Tpad
@@ -3414,9 +3447,14 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
         const uint32_t gmax = nw == 2 ? 16384 : 8192;
         const uint32_t grid = nb < gmax ? nb : gmax;
         if (mq) {
-            hipLaunchKernelGGL(k_overlap_q, dim3(grid), dim3(256), qlds, st, d, s.V, s.L, s.thr, list, s.ctr,
-                               s.best_lab, s.best_idx, s.tabB2, (const uint4 *)s.TmQ, s.ovQ, kb, s.tord, s.tile_lo,
-                               s.tile_hi);
+            if (overlap_q_copies(d) == 2)
+                hipLaunchKernelGGL(k_overlap_q<2>, dim3(grid), dim3(256), qlds, st, d, s.V, s.L, s.thr, list, s.ctr,
+                                   s.best_lab, s.best_idx, s.tabB2, (const uint4 *)s.TmQ, s.ovQ, kb, s.tord, s.tile_lo,
+                                   s.tile_hi);
+            else
+                hipLaunchKernelGGL(k_overlap_q<1>, dim3(grid), dim3(256), qlds, st, d, s.V, s.L, s.thr, list, s.ctr,
+                                   s.best_lab, s.best_idx, s.tabB2, (const uint4 *)s.TmQ, s.ovQ, kb, s.tord, s.tile_lo,
+                                   s.tile_hi);
         }
         if (d.xc_cap) {
             hipError_t e = launch_fill_u32(s.xc_ctr, 2, 0u, st);
