@@ -1831,7 +1831,10 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
     const uint32_t lane = lane_of();
     const uint32_t rsel = QC == 2 ? (lane & 1u) : 0u, plane = d.C * qp.LPH;
     const uint16_t *sh = svh + rsel * plane, *sl = svl + rsel * plane;
-    const uint32_t wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    // the wave index through readfirstlane: every task quantity below is then
+    // known to be wave-uniform (scalar registers, scalar branches, and load
+    // counters the compiler can track across the fragment prefetches)
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), nwave = blockDim.x >> 6;
     const uint32_t ntask = (qp.TTF * qp.NSG + qp.NTG) * qp.KS;
     for (uint32_t it = wave; it < ntask; it += nwave) {
         // with one task per wave, the waves take turns at the (lighter) tail
@@ -1879,14 +1882,30 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
                 const uint64_t ah = (uint64_t)a[q].x | ((uint64_t)a[q].y << 32);
                 const uint64_t al = (uint64_t)a[q].z | ((uint64_t)a[q].w << 32);
                 const uint32_t y0 = kq * 16 + ybase;
+                if (nst == 4) {   // every fragment read first, then the twelve products
+                    uint64_t bh[4], bl[4];
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    if ((uint32_t)j < nst) {
-                        const uint64_t bh = q_frag<QC>(sh, rsel, c, qp.LPH, y0 + 16 * j),
-                                       bl = q_frag<QC>(sl, rsel, c, qp.LPH, y0 + 16 * j);
-                        acch[j] = mfma_bf16_k16(ah, bh, acch[j]);
-                        accl[j] = mfma_bf16_k16(ah, bl, accl[j]);
-                        accl[j] = mfma_bf16_k16(al, bh, accl[j]);
+                    for (int j = 0; j < 4; j++) {
+                        bh[j] = q_frag<QC>(sh, rsel, c, qp.LPH, y0 + 16 * j);
+                        bl[j] = q_frag<QC>(sl, rsel, c, qp.LPH, y0 + 16 * j);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        acch[j] = mfma_bf16_k16(ah, bh[j], acch[j]);
+                        accl[j] = mfma_bf16_k16(ah, bl[j], accl[j]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; j++) accl[j] = mfma_bf16_k16(al, bh[j], accl[j]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        if ((uint32_t)j < nst) {
+                            const uint64_t bh = q_frag<QC>(sh, rsel, c, qp.LPH, y0 + 16 * j),
+                                           bl = q_frag<QC>(sl, rsel, c, qp.LPH, y0 + 16 * j);
+                            acch[j] = mfma_bf16_k16(ah, bh, acch[j]);
+                            accl[j] = mfma_bf16_k16(ah, bl, accl[j]);
+                            accl[j] = mfma_bf16_k16(al, bh, accl[j]);
+                        }
                     }
                 }
                 if (++kq == qp.NKQ) { kq = 0; c++; }
