@@ -789,7 +789,8 @@ __global__ __launch_bounds__(1024) void k_rescore_res(Dims d, const float *__res
     if (nwb == 0) return;   // the host launches this kernel only for span <= kRescoreStage
     for (uint32_t x = threadIdx.x; x < ntm; x += blockDim.x) lds[x] = TmT[x];
     __syncthreads();
-    const uint32_t lane = lane_of(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t lane = lane_of(), nw = blockDim.x >> 6;
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // scalar per-wave state
     float *vb = lds + ntm + wave * kRescoreStage;
     // the shards' kept entries, concatenated: shard j holds virtual entries
     // [spre[j], spre[j + 1]) at rq[j * cap + (v - spre[j])]
@@ -927,7 +928,7 @@ __global__ __launch_bounds__(256) void k_window_partials(Dims d, const float *__
                                                          const uint32_t *__restrict__ win, uint32_t n_win,
                                                          float *part_v, uint32_t *part_t, uint32_t *part_p)
 {
-    const uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t k = blockIdx.x * 4 + (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = lane_of();
     if (k >= n_win) return;
     const uint32_t w = win[k];
@@ -2275,7 +2276,8 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                 const uint32_t l = lane_of(), gl = l & 15u;
                 const uint64_t gmask = 0xffffull << (l & 48u);
                 const uint64_t below = ((1ull << l) - 1) & gmask;
-                for (uint32_t t0 = (threadIdx.x >> 6) * 4; t0 < T; t0 += NW * 4) {
+                for (uint32_t t0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * 4; t0 < T;
+                     t0 += NW * 4) {
                     const uint32_t t = t0 + (l >> 4);
                     const bool act = t < T;
                     float m = -INFINITY, mp = -INFINITY;
