@@ -88,6 +88,7 @@ struct DeviceState {
     uint32_t *part_t;    // [W] its template
     uint32_t *part_p;    // [W] its position
     float *ovQ;          // [ov_batch][T][S] screened overlap: Q - B2/2 of a batch of windows
+    double *ovVV;        // [ov_batch] squared norm of each such window's voltage (all channels)
     uint32_t *sup_lo;    // [T] first channel on which the template is nonzero
     uint32_t *sup_hi;    // [T] one past the last such channel (sup_lo == sup_hi: all zero)
     uint32_t *tord;      // [CT*16] matrix-core template order: by channel support, then index

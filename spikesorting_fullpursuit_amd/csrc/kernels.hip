@@ -1935,9 +1935,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const uint32_t *__restrict__ ov_list, const Counters *__restrict__ ctr, const uint32_t *__restrict__ best_lab,
     const uint32_t *__restrict__ best_idx, const float *__restrict__ B2, const uint4 *__restrict__ TmQ,
     float *__restrict__ ovQ, uint32_t kbase, const uint32_t *__restrict__ tord, const uint32_t *__restrict__ tile_lo,
-    const uint32_t *__restrict__ tile_hi)
+    const uint32_t *__restrict__ tile_hi, double *__restrict__ ovVV)
 {
     extern __shared__ float dyn[];
+    __shared__ double r_q[kOvWaves];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
     const uint32_t TS = T * S;
     const QPlan qp = q_plan(d);
@@ -1994,11 +1995,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             continue;
         }
         __syncthreads();   // the previous window's LDS reads are done
+        double vq = 0.0;   // the window's squared voltage norm, for the screen
         batched_stage<8>(CN, [&](uint32_t e) { const uint32_t c = divN.div(e);
                                                return V[(uint64_t)c * d.K + bidx + (e - c * N)]; },
-                         put_v);
+                         [&](uint32_t e, float x) {
+                             put_v(e, x);
+                             vq = __builtin_fma((double)x, (double)x, vq);
+                         });
+        for (int off = 32; off > 0; off >>= 1) vq += __shfl_xor(vq, off);
+        if (lane_of() == 0) r_q[threadIdx.x >> 6] = vq;
         head_step(2);
         __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (uint32_t w = 0; w < (blockDim.x >> 6); w++) t += r_q[w];
+            ovVV[k - kbase] = t;
+        }
         float *dst = ovQ + (uint64_t)(k - kbase) * TS;
         if (qp.KS == 1) {
             q_table_split<QC>(d, qp, TmQ, svh, svl, dst, sQp, B2, tord, tile_lo, tile_hi, k & 3u);
@@ -2097,7 +2109,8 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     const float *__restrict__ AB, const float *__restrict__ ABmin, const float *__restrict__ B2,
     const float *__restrict__ tnorm, const float *__restrict__ Tpad, const float *__restrict__ ovQ, uint32_t kbase,
     const uint32_t *__restrict__ sup_lo, const uint32_t *__restrict__ sup_hi, uint32_t debug,
-    uint32_t *__restrict__ xc_cand, uint32_t *__restrict__ xc_win, uint32_t *__restrict__ xc_ctr)
+    uint32_t *__restrict__ xc_cand, uint32_t *__restrict__ xc_win, uint32_t *__restrict__ xc_ctr,
+    const double *__restrict__ ovVV)
 {
     extern __shared__ float dyn[];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
@@ -2211,7 +2224,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             // LDS size); the likelihood signs at every in-bounds (t, s), where
             // the loaded value is -1 if (t, s) is out of bounds
             double vv = 0.0;
-            auto a_v = stage_arr<4>(C * N, [&](uint32_t e) { const uint32_t c = divN.div(e);
+            auto a_v = stage_arr<4>(kQG ? 0u : C * N, [&](uint32_t e) { const uint32_t c = divN.div(e);
                                                               return V[(uint64_t)c * d.K + bidx + (e - c * N)]; },
                                     [&](uint32_t e, float x) {
                                         if (!kQG) sv[e] = x;
@@ -2253,8 +2266,10 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             if (lane_of() == 0) r_vv[threadIdx.x >> 6] = vv;
             __syncthreads();
             head_step(2);
-            double vsum = 0.0;
-            for (int w = 0; w < NW; w++) vsum += r_vv[w];
+            double vsum = 0.0;   // kQG: from the Q-table kernel, which staged the same samples
+            if (kQG) vsum = ovVV[k - kbase];
+            else
+                for (int w = 0; w < NW; w++) vsum += r_vv[w];
             const double vn = sqrt(vsum) * (1.0 + 1e-12);
             const double norm_lab = stn[lab];
             // fold the voltage-free half norms into Q (done by the matrix-core
@@ -3471,11 +3486,11 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
             if (overlap_q_copies(d) == 2)
                 hipLaunchKernelGGL(k_overlap_q<2>, dim3(grid), dim3(256), qlds, st, d, s.V, s.L, s.thr, list, s.ctr,
                                    s.best_lab, s.best_idx, s.tabB2, (const uint4 *)s.TmQ, s.ovQ, kb, s.tord, s.tile_lo,
-                                   s.tile_hi);
+                                   s.tile_hi, s.ovVV);
             else
                 hipLaunchKernelGGL(k_overlap_q<1>, dim3(grid), dim3(256), qlds, st, d, s.V, s.L, s.thr, list, s.ctr,
                                    s.best_lab, s.best_idx, s.tabB2, (const uint4 *)s.TmQ, s.ovQ, kb, s.tord, s.tile_lo,
-                                   s.tile_hi);
+                                   s.tile_hi, s.ovVV);
         }
         if (d.xc_cap) {
             hipError_t e = launch_fill_u32(s.xc_ctr, 2, 0u, st);
@@ -3484,7 +3499,7 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
         {
 #define FBP_SCREEN_ARGS d, s.V, s.Tm, s.L, s.thr, list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, \
                         s.ov, s.tabAB, s.tabABmin, s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug, \
-                        s.xc_cand, s.xc_win, s.xc_ctr
+                        s.xc_cand, s.xc_win, s.xc_ctr, s.ovVV
             if (nw == 2 && mq)
                 hipLaunchKernelGGL((k_overlap_screened<true, 2>), dim3(grid), dim3(128), lds, st, FBP_SCREEN_ARGS);
             else if (nw == 2)

@@ -194,6 +194,7 @@ void for_each_buffer(const fbp_config *cfg, const Dims &d, DeviceState &s, V &&v
         visit(s.TmQ, overlap_qtemplate_floats(d));
         visit(s.ov_sorted, W);
         visit(s.ovQ, (uint64_t)d.ov_batch * T * d.S);
+        visit(s.ovVV, (uint64_t)d.ov_batch);
         if (d.xc_cap) {
             visit(s.xc_cand, (uint64_t)d.xc_cap);
             visit(s.xc_win, (uint64_t)d.ov_batch * 4);
