@@ -1598,13 +1598,14 @@ struct FastDiv {
 // plist[pofs[t], pofs[t] + pcnt[t]) (at most maxcnt).  Each thread loads the
 // operands of B combinations (global AB, LDS sQ, sE, spos) before it
 // evaluates them, so their latencies overlap.  Row and column indices advance
-// incrementally; no division per element.
-template <int B, class Fn>
+// incrementally; no division per element.  sE holds the bound per template
+// as float64 or, rounded up, as float32 (ET).
+template <int B, class ET, class Fn>
 __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT, const uint8_t *spos,
                                               const uint32_t *flist, const uint16_t *krow, uint32_t n_kp,
                                               uint32_t n_kn, const uint16_t *plist, const uint16_t *pofs,
                                               const uint16_t *pcnt, uint32_t maxcnt, const float *__restrict__ ABl,
-                                              const float *sQ, const double *sE, Fn fn)
+                                              const float *sQ, const ET *sE, Fn fn)
 {
     const uint32_t S = d.S, N = d.N, T = d.T, TS = d.T * d.S;
     const uint32_t step = blockDim.x;
@@ -1614,7 +1615,7 @@ __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT
         uint32_t k = threadIdx.x / S, s = threadIdx.x - k * S;
         for (uint32_t x0 = threadIdx.x; x0 < total; x0 += B * step) {
             float ab[B], qe[B];
-            double Et[B];
+            ET Et[B];
             uint32_t fq[B], tq[B], sq[B];
             bool okq[B];
 #pragma unroll
@@ -1625,7 +1626,7 @@ __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT
                 const bool ok = in && (spos[e] & 2) && (f > s ? f - s : s - f) <= N;
                 ab[q] = ok ? ABl[(uint64_t)f * TS + e] : 0.0f;
                 qe[q] = ok ? sQ[e] : 0.0f;
-                Et[q] = ok ? sE[t] : 0.0;
+                Et[q] = ok ? sE[t] : (ET)0;
                 fq[q] = f;
                 tq[q] = t;
                 sq[q] = s;
@@ -1645,7 +1646,7 @@ __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT
     uint32_t k = threadIdx.x / maxcnt, j = threadIdx.x - k * maxcnt;
     for (uint32_t x0 = threadIdx.x; x0 < total; x0 += B * step) {
         float ab[B], qe[B];
-        double Et[B];
+        ET Et[B];
         uint32_t fq[B], eq[B], tq[B];
         bool okq[B];
 #pragma unroll
@@ -1658,7 +1659,7 @@ __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT
             const bool ok = has && (f > s ? f - s : s - f) <= N;
             ab[q] = ok ? ABl[(uint64_t)f * TS + e] : 0.0f;
             qe[q] = ok ? sQ[e] : 0.0f;
-            Et[q] = ok ? sE[t] : 0.0;
+            Et[q] = ok ? sE[t] : (ET)0;
             fq[q] = f;
             eq[q] = e;
             tq[q] = t;
@@ -2142,6 +2143,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     uint16_t *tlist = pcnt + T;                         // [T] templates with a positive likelihood, ascending
     uint8_t *spos = (uint8_t *)(tlist + T);             // [T][S] bit1 in bounds, bit0 likelihood > 0
     float *sqmax = (float *)(((uintptr_t)(spos + TS) + 3) & ~(uintptr_t)3);   // [2][T] max over (positive) s of Q'
+    float *sEf = sqmax + 2 * T;                         // [T] sE (1 + 2^-20) rounded up to float32
     __shared__ double r_ub[NW];
     if (!kLean)
         for (uint32_t x = threadIdx.x; x < R; x += blockDim.x) szero[x] = 0.0f;
@@ -2279,7 +2281,12 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                                  [&](uint32_t e, float b2) { sQ[e] = (float)((double)sQ[e] - 0.5 * (double)b2); });
             for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
                 const double nsum = norm_lab + (double)stn[t];
-                sE[t] = ecoef * (nsum * vn + nsum * nsum) + eabs;
+                const double e = ecoef * (nsum * vn + nsum * nsum) + eabs;
+                sE[t] = e;
+                const double eu = e * (1.0 + 0x1p-20) + 0x1p-120;
+                float ef = (float)eu;
+                if ((double)ef < eu) ef = __uint_as_float(__float_as_uint(ef) + 1u);
+                sEf[t] = ef;
             }
             if (!MQ) __syncthreads();   // Q - B2/2 complete
             // Sixteen lanes per template t (four templates per wave), by
@@ -2466,33 +2473,41 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             // also keeps its four combinations with the largest upper bounds
             // (kv0 >= kv1 >= kv2 >= kv3, packed f << 16 | e in kc*) and the
             // largest upper bound it did not keep (kdrop).
-            double lo_max = -INFINITY, hi_max = -INFINITY, kdrop = -INFINITY;
-            double kv0 = -INFINITY, kv1 = -INFINITY, kv2 = -INFINITY, kv3 = -INFINITY;
+            // The bounds are evaluated in float32: sh = (x + q) - ab is within
+            // 2.001u·A of its exact value (u = 2^-24, A = |x| + |q| + |ab|),
+            // and the margin m = Ef + 2^-21·A (Ef >= E(1 + 2^-20), rounded up)
+            // covers that, E, and the rounding of A, m and sh ± m, so
+            // [sh - m, sh + m] contains [sh_exact - E, sh_exact + E].
+            float lo_mf = -INFINITY, hi_mf = -INFINITY, kdrop = -INFINITY;
+            float kv0 = -INFINITY, kv1 = -INFINITY, kv2 = -INFINITY, kv3 = -INFINITY;
             uint32_t kc0 = 0, kc1 = 0, kc2 = 0, kc3 = 0;
             uint32_t screened = 0;
-            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
-                const double sh = screen(f, qe, ab);
-                const double hi = sh + E;
-                lo_max = fmax(lo_max, sh - E);
-                hi_max = fmax(hi_max, hi);
+            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sEf, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, float Ef) {
+                const float x = sQl[f];
+                const float sh = (x + qe) - ab;
+                const float m = Ef + 0x1p-21f * ((fabsf(x) + fabsf(qe)) + fabsf(ab));
+                const float hi = sh + m;
+                lo_mf = fmaxf(lo_mf, sh - m);
+                hi_mf = fmaxf(hi_mf, hi);
                 screened++;
                 if (hi > kv3) {
-                    kdrop = fmax(kdrop, kv3);
+                    kdrop = fmaxf(kdrop, kv3);
                     kv3 = hi;
                     kc3 = (f << 16) | e;
-                    if (kv3 > kv2) { const double x = kv2; kv2 = kv3; kv3 = x; const uint32_t y = kc2; kc2 = kc3; kc3 = y; }
-                    if (kv2 > kv1) { const double x = kv1; kv1 = kv2; kv2 = x; const uint32_t y = kc1; kc1 = kc2; kc2 = y; }
-                    if (kv1 > kv0) { const double x = kv0; kv0 = kv1; kv1 = x; const uint32_t y = kc0; kc0 = kc1; kc1 = y; }
+                    if (kv3 > kv2) { const float x2 = kv2; kv2 = kv3; kv3 = x2; const uint32_t y = kc2; kc2 = kc3; kc3 = y; }
+                    if (kv2 > kv1) { const float x2 = kv1; kv1 = kv2; kv2 = x2; const uint32_t y = kc1; kc1 = kc2; kc2 = y; }
+                    if (kv1 > kv0) { const float x2 = kv0; kv0 = kv1; kv1 = x2; const uint32_t y = kc0; kc0 = kc1; kc1 = y; }
                 } else {
-                    kdrop = fmax(kdrop, hi);
+                    kdrop = fmaxf(kdrop, hi);
                 }
             });
             for (int off = 32; off > 0; off >>= 1) {
-                lo_max = fmax(lo_max, __shfl_xor(lo_max, off));
-                hi_max = fmax(hi_max, __shfl_xor(hi_max, off));
+                lo_mf = fmaxf(lo_mf, __shfl_xor(lo_mf, off));
+                hi_mf = fmaxf(hi_mf, __shfl_xor(hi_mf, off));
                 screened += __shfl_xor(screened, off);
             }
-            if (lane_of() == 0) { r_d[threadIdx.x >> 6][0] = lo_max; r_d[threadIdx.x >> 6][1] = hi_max; }
+            double lo_max, hi_max;
+            if (lane_of() == 0) { r_d[threadIdx.x >> 6][0] = lo_mf; r_d[threadIdx.x >> 6][1] = hi_mf; }
             if (lane_of() == 0) acc_screened += screened;
             __syncthreads();
             lo_max = r_d[0][0];
@@ -3368,7 +3383,7 @@ static size_t overlap_screened_lds_nw(const Dims &d, int nw)
     const uint64_t region = nw != 4 ? krw + cand : (partner + cand > krw ? partner + cand : krw);
     return (uint64_t)d.T * 8 + (qg ? 0 : (uint64_t)d.C * d.N * 4) + main_rows * 4 + (qg ? 0 : ts * 4) +
            (uint64_t)d.T * 4 + (uint64_t)d.S * 4 + region * 4 + ts * 2 + (uint64_t)d.T * 6 + ts + 4 +
-           (uint64_t)d.T * 8 + 16;
+           (uint64_t)d.T * 12 + 16;
 }
 
 // Waves per screen block: 4 when the block scores its candidates; with the
