@@ -2369,9 +2369,11 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             };
             // Row bounds.  For the main shift f of flist row a and template t,
             // every combination (f, t, s) of the row has sh + E <= U(a, t) =
-            // Q'(lab, f) + max_s Q'(t, s) - min_s AB(lab, f, t, s) + E_t, plus
-            // the float64 rounding of both sides (the max over s runs over the
-            // positive-likelihood s for rows of non-positive main shifts).  Each
+            // Q'(lab, f) + max_s Q'(t, s) - min_s AB(lab, f, t, s) + E_t
+            // (the max over s runs over the positive-likelihood s for rows of
+            // non-positive main shifts), evaluated in float32 with the margin
+            // of the traversal below (so it bounds the float32 upper bounds
+            // there too).  Each
             // wave evaluates the row with its largest U; the largest lower bound
             // sh - E found there, lo_est, is at most lo_max.  Rows with U below
             // lo_est hold no combination that reaches lo_max, so neither the
@@ -2409,10 +2411,10 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                     for (int q = 0; q < RB; q++) {
                         if (r0 + q * blockDim.x < nrows) {
                             const uint32_t f = flist[aq[q]];
-                            const double x = sQl[f], y = sqmax[(aq[q] < nfp ? 0 : T) + tq[q]], zz = z[q];
-                            const double E = sE[tq[q]];
-                            const double u = y == -INFINITY ? -(double)INFINITY   // no combination in the row
-                                             : x + y - zz + E + 0x1p-50 * (fabs(x) + fabs(y) + fabs(zz) + E);
+                            const float x = sQl[f], y = sqmax[(aq[q] < nfp ? 0 : T) + tq[q]], zz = z[q];
+                            const float m = sEf[tq[q]] + 0x1p-21f * ((fabsf(x) + fabsf(y)) + fabsf(zz));
+                            const float u = y == -INFINITY ? -INFINITY   // no combination in the row
+                                                           : ((x + y) - zz) + m;
                             use(aq[q] * T + tq[q], aq[q], tq[q], u, q);
                         }
                     }
@@ -2421,38 +2423,41 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             // With at most RB rows per thread, the second use of the bounds
             // reads them from registers.
             const bool one_batch = nrows <= RB * blockDim.x;
-            double uc[RB];
+            float uc[RB];
             uint32_t rc[RB];
-            double ub = -INFINITY;
+            float ub = -INFINITY;
             uint32_t ur = 0;
-            row_bounds([&](uint32_t r, uint32_t, uint32_t, double u, int q) {
+            row_bounds([&](uint32_t r, uint32_t, uint32_t, float u, int q) {
                 if (u > ub) { ub = u; ur = r; }
                 uc[q] = u;
                 rc[q] = r;
             });
             for (int off = 32; off > 0; off >>= 1) {
-                const double ou = __shfl_xor(ub, off);
+                const float ou = __shfl_xor(ub, off);
                 const uint32_t orr = __shfl_xor(ur, off);
                 if (ou > ub || (ou == ub && orr < ur)) { ub = ou; ur = orr; }
             }
-            double lo_est = -INFINITY;
-            if (ub > -INFINITY) {   // this wave's best row
+            float lo_est = -INFINITY;
+            if (ub > -INFINITY) {   // this wave's best row: float32 lower bounds as in the traversal
                 const uint32_t a = ur / T, t = ur - a * T, f = flist[a];
                 for (uint32_t s = lane_of(); s < S; s += kWave) {
                     const uint8_t sp = spos[t * S + s];
                     const bool allowed = (a < nfp ? (sp & 2) : (sp & 1)) && (f > s ? f - s : s - f) <= N;
-                    if (allowed)
-                        lo_est = fmax(lo_est, screen(f, qT[t * S + s], ABl[(uint64_t)f * TS + t * S + s]) - sE[t]);
+                    if (allowed) {
+                        const float x = sQl[f], qe = qT[t * S + s], ab = ABl[(uint64_t)f * TS + t * S + s];
+                        const float m = sEf[t] + 0x1p-21f * ((fabsf(x) + fabsf(qe)) + fabsf(ab));
+                        lo_est = fmaxf(lo_est, ((x + qe) - ab) - m);
+                    }
                 }
             }
-            for (int off = 32; off > 0; off >>= 1) lo_est = fmax(lo_est, __shfl_xor(lo_est, off));
+            for (int off = 32; off > 0; off >>= 1) lo_est = fmaxf(lo_est, __shfl_xor(lo_est, off));
             if (lane_of() == 0) r_ub[threadIdx.x >> 6] = lo_est;
             __syncthreads();
-            lo_est = r_ub[0];
-            for (int w = 1; w < NW; w++) lo_est = fmax(lo_est, r_ub[w]);
+            lo_est = (float)r_ub[0];
+            for (int w = 1; w < NW; w++) lo_est = fmaxf(lo_est, (float)r_ub[w]);
             if (debug & 16u) lo_est = -INFINITY;
             // kept rows: positive main shifts from the front, the others from the back
-            auto keep_row = [&](uint32_t r, uint32_t a, uint32_t t, double u, int) {
+            auto keep_row = [&](uint32_t r, uint32_t a, uint32_t t, float u, int) {
                 if (u < lo_est) return;
                 if (a < nfp) krow[atomicAdd(&n_kp, 1u)] = (uint16_t)r;
                 else if (pcnt[t]) krow[TS - 1 - atomicAdd(&n_kn, 1u)] = (uint16_t)r;
