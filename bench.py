@@ -98,6 +98,7 @@ def main():
                                                           seg.stats()['n_passes']))
     sums = {"ms_correlation": 0.0, "ms_overlap": 0.0, "ms_total": 0.0, "corr_launches": 0,
             "overlap_launches": 0, "corr_positions": 0, "overlap_combos": 0, "screened_combos": 0,
+            "corr_pairs": 0,
             "overlap_windows": 0, "rescored_positions": 0}
     t_start = time.perf_counter()
     for k in range(args.steps):
@@ -112,7 +113,8 @@ def main():
     ms_per_step = 1000.0 * elapsed / args.steps
     value = C * K * args.steps / elapsed
 
-    stages, roofline = roofline_of(args, sums, C, T, N, M, K, int(last["n_passes"]))
+    stages, roofline = roofline_of(args, sums, C, T, N, M, K, int(last["n_passes"]),
+                                   support_channels(sm['templates'], C, N))
 
     plain = None
     if args.plain_steps > 0 and (args.precision != "fp32" or args.overlap_mode != "direct"):
@@ -218,6 +220,7 @@ def pursuit_entry(args, rec, precision, overlap_mode, M, steps, warmup=1, worklo
     thr = np.float32(sm['neuron_lower_thresholds'])
     sums = {"ms_correlation": 0.0, "ms_overlap": 0.0, "ms_total": 0.0, "corr_launches": 0,
             "overlap_launches": 0, "corr_positions": 0, "overlap_combos": 0, "screened_combos": 0,
+            "corr_pairs": 0,
             "overlap_windows": 0, "rescored_positions": 0}
     with _lib.Segment(C, K, T, N, M, True, 256, precision, overlap_mode) as sg:
         sg.set_templates(sm['templates'], bias, thr)
@@ -234,7 +237,7 @@ def pursuit_entry(args, rec, precision, overlap_mode, M, steps, warmup=1, worklo
         idx, lab = sg.spikes()
     ns = argparse.Namespace(precision=precision, overlap_mode=overlap_mode, workload=workload or args.workload,
                             duration=duration if workload else args.duration)
-    stages, roof = roofline_of(ns, sums, C, T, N, M, K, int(st["n_passes"]))
+    stages, roof = roofline_of(ns, sums, C, T, N, M, K, int(st["n_passes"]), support_channels(sm['templates'], C, N))
     out = {"value": C * K / dt, "unit": "ch·samples/s", "ms_per_step": 1000.0 * dt, "steps": steps,
            "precision": precision, "overlap_mode": overlap_mode, "n_max_shift_inds": M,
            "n_spikes": int(idx.size), "n_passes": int(st["n_passes"]),
@@ -614,30 +617,52 @@ def measured_mfma(prof, stage):
     return out or None
 
 
-def roofline_of(args, sums, C, T, N, M, K, passes_per_step):
+def support_channels(templates, C, N):
+    """Channels per template between its first and last nonzero one, summed
+    over the templates (the channels the kernels compute on)."""
+    nz = np.abs(np.asarray(templates, dtype=np.float32).reshape(-1, C, N)).max(axis=2) > 0
+    total = 0
+    for row in nz:
+        idx = np.flatnonzero(row)
+        total += int(idx[-1] - idx[0] + 1) if idx.size else 0
+    return total
+
+
+def roofline_of(args, sums, C, T, N, M, K, passes_per_step, sup_total=None):
     """Algorithmic work of the two stages (HIP-event time summed over the timed
-    steps) and the roofline entry of the slower one."""
+    steps) and the roofline entry of the slower one.  The correlation's work
+    counts the (position, template) likelihoods it computed (listed blocks
+    whose voltage changed since their last pass), over each template's
+    channel support; the Q tables' over the supports too."""
     S = 2 * M + 1
     R = N + 2 * M
-    pos = sums["corr_positions"]
+    sup_total = C * T if sup_total is None else sup_total
+    sup_mean = sup_total / max(1, T)
+    pairs = sums["corr_pairs"]
     stages = {}
     t_corr = sums["ms_correlation"] * 1e-3
     if args.precision == "fp32":
-        flop = 2.0 * T * C * N * pos
+        flop = 2.0 * C * N * pairs
         stages["correlation"] = {"kernel": "k_corr_fp32", "ms": sums["ms_correlation"],
                                  "launches": sums["corr_launches"], "bound": "mfma", "unit": "TFLOP/s",
                                  "achieved": flop / t_corr / 1e12 if t_corr else 0.0, "peak": PEAK_TFLOPS["fp32"],
-                                 "work": "2*T*C*N flop per likelihood position (float32 FMA chain)"}
+                                 "work": "2*C*N flop per computed (position, template) likelihood (float32 "
+                                         "FMA chain over every channel)"}
     else:
-        nbytes = (4.0 * C + 4.0 * T) * pos
-        flop = 2.0 * T * C * N * pos
-        stages["correlation"] = {"kernel": "k_corr_mfma + k_rescore_res", "ms": sums["ms_correlation"],
+        nbytes = (4.0 * C / max(1, T) + 4.0) * pairs   # the position's voltage once over its T templates
+        flop = 2.0 * sup_mean * N * pairs
+        # float32 re-score with the templates resident in LDS when they fit (150 KB)
+        tp = -(-T // 8) * 8
+        rescore = "k_rescore_res" if (C * N * tp + 16 * 512) * 4 <= 150 * 1024 else "k_rescore_sup"
+        stages["correlation"] = {"kernel": "k_corr_mfma + " + rescore, "ms": sums["ms_correlation"],
                                  "launches": sums["corr_launches"], "bound": "hbm", "unit": "GB/s",
                                  "achieved": nbytes / t_corr / 1e9 if t_corr else 0.0, "peak": PEAK_HBM_GBS,
                                  "mfma_tflops": flop / t_corr / 1e12 if t_corr else 0.0,
                                  "mfma_peak": PEAK_TFLOPS[args.precision],
-                                 "work": "4*C bytes of voltage read + 4*T bytes of likelihood written per "
-                                         "position; 2*T*C*N matrix-core flop per position"}
+                                 "work": "per computed (position, template) likelihood: 4 bytes written "
+                                         "and 4*C/T bytes of voltage read (a position's voltage once over "
+                                         "its T templates); 2*N matrix-core flop per channel of the "
+                                         "template's support"}
     t_ov = sums["ms_overlap"] * 1e-3
     if args.overlap_mode == "direct":
         flop = (5.0 * C * N + C) * sums["overlap_combos"]
@@ -654,7 +679,7 @@ def roofline_of(args, sums, C, T, N, M, K, passes_per_step):
         win = sums.get("overlap_windows", 0)
         nbytes = 4.0 * sums["screened_combos"] + win * (2.0 * C * N * 4 + C * R * 4 + 3.0 * T * S * 4)
         valid = sum(min(N, N + s - M) - max(0, s - M) for s in range(S))
-        qflop = 3.0 * 2.0 * C * T * valid * win
+        qflop = 3.0 * 2.0 * sup_total * valid * win
         stages["overlap"] = {"kernel": "k_overlap_q + k_overlap_screened + k_overlap_exact", "ms": sums["ms_overlap"],
                              "launches": sums["overlap_launches"], "bound": "hbm", "unit": "GB/s",
                              "achieved": nbytes / t_ov / 1e9 if t_ov else 0.0, "peak": PEAK_HBM_GBS,

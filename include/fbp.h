@@ -124,6 +124,9 @@ typedef struct fbp_stats {
     uint64_t corr_positions;      /* likelihood positions computed, all passes  */
     uint64_t overlap_combos;      /* (t,f,s) combinations scored with the reference's
                                      float32 chain, all passes                    */
+    uint64_t corr_pairs;          /* (position, template) likelihoods computed, all
+                                     passes: corr_positions * T less the listed ones
+                                     whose template's channels saw no voltage change */
 } fbp_stats;
 
 typedef struct fbp_segment fbp_segment;

@@ -78,6 +78,7 @@ class FbpStats(ctypes.Structure):
         ("overlap_launches", ctypes.c_uint64),
         ("corr_positions", ctypes.c_uint64),
         ("overlap_combos", ctypes.c_uint64),
+        ("corr_pairs", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -60,6 +60,8 @@ struct Counters {
     uint32_t pad[2];
     unsigned long long combos;   // overlap combinations scored exactly since the run started
     unsigned long long screened; // combinations screened in float64 (screened overlap)
+    unsigned long long corr_pairs; // (position, template) likelihoods computed since the run
+                                   // started (listed blocks whose voltage changed)
 };
 
 // Device buffers of one chunk.

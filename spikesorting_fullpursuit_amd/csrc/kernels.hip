@@ -3160,11 +3160,42 @@ hipError_t launch_transpose_templates(const Dims &d, const float *Tm, const floa
     return hipGetLastError();
 }
 
-// The listed windows' likelihood is current: their dirty masks are consumed.
-__global__ void k_clear_dirty(const uint32_t *__restrict__ win, uint32_t n_win, uint64_t *wdirty)
+// The listed windows' likelihood is current: their dirty masks are consumed,
+// after counting the (position, template) likelihoods the correlation
+// computed (the blocks it did not skip): with ctb template tiles per chunk,
+// a chunk is computed when its channel groups meet the mask (or it has
+// none); with ctb = 0 (float32 path) a window is computed when its mask is
+// not empty.
+__global__ __launch_bounds__(256) void k_clear_dirty(Dims d, const uint32_t *__restrict__ win, uint32_t n_win,
+                                                     uint64_t *wdirty, const uint32_t *__restrict__ tile_lo,
+                                                     const uint32_t *__restrict__ tile_hi, uint32_t ctb,
+                                                     Counters *ctr)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n_win) wdirty[win[k]] = 0;
+    unsigned long long pairs = 0;
+    if (k < n_win) {
+        const uint32_t w = win[k];
+        const uint64_t m = wdirty[w];
+        const uint64_t np = min((uint64_t)d.N, (uint64_t)d.K - (uint64_t)w * d.N);
+        if (ctb == 0) {
+            if (m || d.dirty_off) pairs = np * d.T;
+        } else {
+            const uint32_t CT = (d.T + 15) / 16;
+            for (uint32_t c0 = 0; c0 < CT; c0 += ctb) {
+                uint32_t lo = d.C, hi = 0;
+                for (uint32_t ct = c0; ct < CT && ct < c0 + ctb; ct++)
+                    if (tile_lo[ct] < tile_hi[ct]) {
+                        lo = min(lo, tile_lo[ct]);
+                        hi = max(hi, tile_hi[ct]);
+                    }
+                const uint64_t cm = hi > lo ? channel_mask(lo, hi, d.dirty_g) : 0ull;
+                if (d.dirty_off || !cm || (m & cm)) pairs += np * (min(d.T, (c0 + ctb) * 16) - c0 * 16);
+            }
+        }
+        wdirty[w] = 0;
+    }
+    for (int off = 32; off > 0; off >>= 1) pairs += __shfl_xor(pairs, off);
+    if (lane_of() == 0 && pairs) atomicAdd(&ctr->corr_pairs, pairs);
 }
 
 hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win, hipStream_t st)
@@ -3176,7 +3207,8 @@ hipError_t launch_corr_fp32(const Dims &d, const DeviceState &s, uint32_t n_win,
     if (cb_max == 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_corr_fp32, dim3(n_win), dim3(64), 0, st, d, s.V, s.TmT, s.bias, s.win, s.L, cb_max,
                        s.wdirty);
-    hipLaunchKernelGGL(k_clear_dirty, dim3((n_win + 255) / 256), dim3(256), 0, st, s.win, n_win, s.wdirty);
+    hipLaunchKernelGGL(k_clear_dirty, dim3((n_win + 255) / 256), dim3(256), 0, st, d, s.win, n_win, s.wdirty,
+                       s.tile_lo, s.tile_hi, 0u, s.ctr);
     return hipGetLastError();
 }
 
@@ -3246,7 +3278,8 @@ static hipError_t launch_corr_mfma_rt(const Dims &d, const DeviceState &s, uint3
                        (const Frag<PREC> *)s.TmB, s.bias, s.tnorm, s.tscale, s.win, s.L, s.ctr, s.rq, rq_cap,
                        (uint2 *)s.rdesc, cb_max, co, no_rescore, s.tord, s.tile_lo, s.tile_hi, s.sup_lo, s.sup_hi,
                        s.rq_shard, s.wdirty);
-    hipLaunchKernelGGL(k_clear_dirty, dim3((n_win + 255) / 256), dim3(256), 0, st, s.win, n_win, s.wdirty);
+    hipLaunchKernelGGL(k_clear_dirty, dim3((n_win + 255) / 256), dim3(256), 0, st, d, s.win, n_win, s.wdirty,
+                       s.tile_lo, s.tile_hi, (uint32_t)CTB, s.ctr);
     return hipGetLastError();
 }
 

@@ -650,6 +650,7 @@ int finish_run(fbp_segment *g, int rc)
     g->stats.n_spikes = g->total;
     g->stats.overlap_combos = g->h_ctr->combos;
     g->stats.screened_combos = g->h_ctr->screened;
+    g->stats.corr_pairs = g->h_ctr->corr_pairs;
     // Results stay readable when the pass cap stopped the loop.
     g->ran = (rc == FBP_OK || rc == FBP_EPASSES);
     return rc;
