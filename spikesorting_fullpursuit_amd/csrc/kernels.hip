@@ -2126,8 +2126,12 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     // V in global memory, so the LDS holds the per-template bookkeeping only
     // and two blocks fit in a CU
     constexpr bool kQG = MQ && NW == 8;
-    float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset (not kQG)
-    float *smain = sv + (kQG ? 0u : C * N);             // [SM][R] padded best-label template, its support rows
+    // kVG (every lean block on matrix-core Q tables): the window's voltage is
+    // read from V when a candidate is scored in the block, and its norm comes
+    // from the Q-table kernel, so it is not staged
+    constexpr bool kVG = MQ && kLean;
+    float *sv = (float *)(sE + T);                      // [C][N] voltage at the best onset (not kVG)
+    float *smain = sv + (kVG ? 0u : C * N);             // [SM][R] padded best-label template, its support rows
     float *szero = smain + (kLean ? 0u : SM * R);       // [R] zeros (not lean)
     float *sQ = szero + (kLean ? 0u : R);               // [T][S] Q - B2/2 (not kQG)
     float *stn = sQ + (kQG ? 0u : TS);                  // [T] template norms
@@ -2209,8 +2213,8 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
         uint64_t key = ~0ull, bp = 0;
         // the window's Q table and voltage rows (stride vstr)
         const float *qT = kQG ? ovQ + (uint64_t)(k - kbase) * TS : sQ;
-        const float *vw = kQG ? V + bidx : sv;
-        const uint64_t vstr = kQG ? (uint64_t)d.K : (uint64_t)N;
+        const float *vw = kVG ? V + bidx : sv;
+        const uint64_t vstr = kVG ? (uint64_t)d.K : (uint64_t)N;
         if (!wskip) {
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -2226,10 +2230,10 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             // LDS size); the likelihood signs at every in-bounds (t, s), where
             // the loaded value is -1 if (t, s) is out of bounds
             double vv = 0.0;
-            auto a_v = stage_arr<4>(kQG ? 0u : C * N, [&](uint32_t e) { const uint32_t c = divN.div(e);
+            auto a_v = stage_arr<4>(kVG ? 0u : C * N, [&](uint32_t e) { const uint32_t c = divN.div(e);
                                                               return V[(uint64_t)c * d.K + bidx + (e - c * N)]; },
                                     [&](uint32_t e, float x) {
-                                        if (!kQG) sv[e] = x;
+                                        if (!kVG) sv[e] = x;
                                         vv = __builtin_fma((double)x, (double)x, vv);
                                     });
             auto a_main = stage_arr<8>(kLean ? 0u : (hi_m - lo_m) * R, [&](uint32_t e) { return Tpad[((uint64_t)lab * C + lo_m) * R + e]; },
@@ -2268,8 +2272,8 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             if (lane_of() == 0) r_vv[threadIdx.x >> 6] = vv;
             __syncthreads();
             head_step(2);
-            double vsum = 0.0;   // kQG: from the Q-table kernel, which staged the same samples
-            if (kQG) vsum = ovVV[k - kbase];
+            double vsum = 0.0;   // kVG: from the Q-table kernel, which staged the same samples
+            if (kVG) vsum = ovVV[k - kbase];
             else
                 for (int w = 0; w < NW; w++) vsum += r_vv[w];
             const double vn = sqrt(vsum) * (1.0 + 1e-12);
@@ -3413,13 +3417,15 @@ static bool overlap_uses_mfma_q(const Dims &d) { return overlap_q_lds(d) <= 150 
 // Dynamic LDS of k_overlap_screened<MQ, nw> (the layout at the top of the kernel).
 static size_t overlap_screened_lds_nw(const Dims &d, int nw)
 {
-    const bool qg = nw == 8 && overlap_uses_mfma_q(d);   // Q table and voltage read from global memory
+    const bool mq = overlap_uses_mfma_q(d);
+    const bool qg = nw == 8 && mq;                       // Q table read from global memory
+    const bool vg = nw != 4 && mq;                       // voltage read from global memory
     const uint64_t ts = (uint64_t)d.T * d.S, R = d.N + 2 * d.M;
     const uint64_t krw = (ts + 1) / 2, cand = 4ull * nw * 64;
     const uint64_t partner = (uint64_t)d.sup_max * R;                           // NW 4: staged partner rows
     const uint64_t main_rows = nw != 4 ? 0 : partner + R;                        // main rows + the zero row
     const uint64_t region = nw != 4 ? krw + cand : (partner + cand > krw ? partner + cand : krw);
-    return (uint64_t)d.T * 8 + (qg ? 0 : (uint64_t)d.C * d.N * 4) + main_rows * 4 + (qg ? 0 : ts * 4) +
+    return (uint64_t)d.T * 8 + (vg ? 0 : (uint64_t)d.C * d.N * 4) + main_rows * 4 + (qg ? 0 : ts * 4) +
            (uint64_t)d.T * 4 + (uint64_t)d.S * 4 + region * 4 + ts * 2 + (uint64_t)d.T * 6 + ts + 4 +
            (uint64_t)d.T * 12 + 16;
 }
