@@ -1862,24 +1862,27 @@ __device__ void q_table_split(const Dims &d, const QPlan &qp, const uint4 *__res
             accl[j] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
         }
         // steps (c, kq) in c-major order; the A fragment of step st is
-        // TmQ[(st*TT + tt)*64 + lane].  Fragments are fetched four steps ahead.
+        // TmQ[(st*TT + tt)*64 + lane].  Fragments are fetched QD steps ahead
+        // (two: with the eight B fragments of a step in flight, four would
+        // spill registers).
+        constexpr int QD = 2;
         const uint32_t sbeg = c0 * qp.NKQ, send = c1 * qp.NKQ;
         const uint4 *ap = TmQ + (uint64_t)tt * 64 + lane;
         const uint64_t astride = (uint64_t)qp.TT * 64;
         const uint32_t ybase = 4 * (lane >> 4) + st0 * 16 + (lane & 15);
-        uint4 an[4];
+        uint4 an[QD];
 #pragma unroll
-        for (int q = 0; q < 4; q++) an[q] = sbeg + q < send ? ap[(sbeg + q) * astride] : uint4{0u, 0u, 0u, 0u};
+        for (int q = 0; q < QD; q++) an[q] = sbeg + q < send ? ap[(sbeg + q) * astride] : uint4{0u, 0u, 0u, 0u};
         uint32_t c = c0, kq = 0;
-        for (uint32_t s0 = sbeg; s0 < send; s0 += 4) {
-            uint4 a[4];
+        for (uint32_t s0 = sbeg; s0 < send; s0 += QD) {
+            uint4 a[QD];
 #pragma unroll
-            for (int q = 0; q < 4; q++) a[q] = an[q];
+            for (int q = 0; q < QD; q++) a[q] = an[q];
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                an[q] = s0 + 4 + q < send ? ap[(s0 + 4 + q) * astride] : uint4{0u, 0u, 0u, 0u};
+            for (int q = 0; q < QD; q++)
+                an[q] = s0 + QD + q < send ? ap[(s0 + QD + q) * astride] : uint4{0u, 0u, 0u, 0u};
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < QD; q++) {
                 if (s0 + q >= send) break;
                 const uint64_t ah = (uint64_t)a[q].x | ((uint64_t)a[q].y << 32);
                 const uint64_t al = (uint64_t)a[q].z | ((uint64_t)a[q].w << 32);
