@@ -3577,8 +3577,19 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
 #undef FBP_SCREEN_ARGS
         }
         if (d.xc_cap) {
-            // every deferred window of the batch: one wave each, 16 per block
-            const uint32_t xgrid = (nb + 15) / 16 < 1024 ? (nb + 15) / 16 : 1024;
+            // every deferred window of the batch: one wave each, 16 per block,
+            // at most two blocks per CU (each stages the templates in LDS
+            // before its first window, so a grid sized for every overlap
+            // window would mostly stage for nothing: few windows defer)
+            uint32_t ncu = 256;
+            {
+                int dev = 0, n = 0;
+                if (hipGetDevice(&dev) == hipSuccess &&
+                    hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+                    ncu = (uint32_t)n;
+            }
+            const uint32_t xmax = 2 * ncu;
+            const uint32_t xgrid = (nb + 15) / 16 < xmax ? (nb + 15) / 16 : xmax;
             if (xg)
                 hipLaunchKernelGGL(k_overlap_exact<true>, dim3(xgrid), dim3(kXcThreads), 0, st, d, s.V, s.Tpad, s.L,
                                    s.thr, list, s.xc_win, s.xc_ctr, s.xc_cand, s.ctr, s.best_L, s.best_lab,
