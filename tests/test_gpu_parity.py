@@ -213,3 +213,31 @@ def test_large_template_set_rescore(oracle_mod):
     np.testing.assert_array_equal(g['residual'].view(np.uint32), o['residual'].view(np.uint32))
     pos = o['L'] > 0
     np.testing.assert_array_equal(g['L'][pos].view(np.uint32), o['L'][pos].view(np.uint32))
+
+
+@pytest.mark.parametrize("precision,mode,C,T,support", [("bf16", "screened", 96, 30, 8), ("fp16", "screened", 40, 16, 5),
+                                                        ("fp32", "direct", 32, 12, 6)])
+def test_incremental_correlation_equals_full(precision, mode, C, T, support):
+    # The correlation skips the (window, template chunk) blocks whose channels
+    # saw no voltage change since their last pass (with 96 channels a mask bit
+    # covers two).  What it leaves in place must be exactly what recomputing
+    # gives: the whole likelihood buffer (negative screened values included),
+    # the spike list and the residual equal a run that recomputes every
+    # listed block (FBP_DEBUG_FULL_CORRELATION), with fewer likelihoods
+    # computed.
+    from spikesorting_fullpursuit_amd import synthetic
+    rec = synthetic.make_recording(47 + C, C, 30000, 0.4, T, 41, (8.0, 20.0), (5.0, 60.0), gains="gauss",
+                                   spatial_sigma=2.0, support=support, overlap_frac=0.1)
+    sm = rec['separability_metrics']
+    rec['bias'] = np.float32(-0.5 * sm['template_SS'])
+    rec['thr'] = np.float32(sm['neuron_lower_thresholds'])
+    rec['N'] = 41
+    print("case incremental correlation", precision, mode, C, T, support, flush=True)
+    g = run_gpu(rec, 20, True, 256, precision=precision, overlap_mode=mode)
+    full = run_gpu(rec, 20, True, 256, precision=precision, overlap_mode=mode, debug_flags=512)
+    assert g['idx'].size > 0 and g['stats']['n_passes'] > 2
+    assert g['stats']['corr_pairs'] < full['stats']['corr_pairs']
+    np.testing.assert_array_equal(g['idx'], full['idx'])
+    np.testing.assert_array_equal(g['lab'], full['lab'])
+    np.testing.assert_array_equal(g['residual'].view(np.uint32), full['residual'].view(np.uint32))
+    np.testing.assert_array_equal(g['L'].view(np.uint32), full['L'].view(np.uint32))
