@@ -1458,13 +1458,17 @@ __global__ __launch_bounds__(256) void k_overlap_direct(
 // combination the reference does not skip, and bounds |screen - float32 score|
 // by E = ecoef * ((|t_lab|+|t_t|)*|v| + (|t_lab|+|t_t|)^2), which covers the
 // float32 rounding of the reference's fused chain (gamma_{CN+2C+8} with a
-// doubled unit roundoff) and the float32 storage of Q and the tables.  Every
+// doubled unit roundoff) and the float32 storage of Q and the tables.  |t_t|
+// may be the norm of the partner shifted by s and truncated to the window
+// (every term of the chain, of Q(t,s), B2(t,s) and AB that holds the partner
+// holds only that part), which is what the screen uses where it has room.  Every
 // combination whose upper bound reaches the largest lower bound is then scored
 // exactly with the reference's float32 chain (overlap_direct_score) and reduced
 // in the reference's winner order, so the result equals the direct form's.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_overlap_tables(Dims d, const float *__restrict__ Tm, float *__restrict__ AB,
-                                                       float *__restrict__ B2, float *__restrict__ tnorm,
+                                                       float *__restrict__ B2, float *__restrict__ Ns,
+                                                       float *__restrict__ tnorm,
                                                        const uint32_t *__restrict__ sup_lo,
                                                        const uint32_t *__restrict__ sup_hi)
 {
@@ -1499,6 +1503,11 @@ __global__ __launch_bounds__(256) void k_overlap_tables(Dims d, const float *__r
                 for (int32_t j = lo; j < hi; j++) acc = __builtin_fma((double)b[j + sa], (double)b[j + sa], acc);
             }
             B2[q] = (float)acc;
+            // upper bound of the shifted, truncated template's norm
+            const double nrm = sqrt(acc) * (1.0 + 1e-12);
+            float up = (float)nrm;
+            if ((double)up < nrm) up = __uint_as_float(__float_as_uint(up) + 1u);
+            Ns[q] = up;
         } else {
             const uint32_t t = (uint32_t)(e - nab - (uint64_t)T * S);
             double acc = 0.0;
@@ -1599,8 +1608,9 @@ struct FastDiv {
 // operands of B combinations (global AB, LDS sQ, sE, spos) before it
 // evaluates them, so their latencies overlap.  Row and column indices advance
 // incrementally; no division per element.  sE holds the bound per template
-// as float64 or, rounded up, as float32 (ET).
-template <int B, class ET, class Fn>
+// as float64 or, rounded up, as float32 (ET); with PE, per partner (t, s)
+// instead ([T][S], float32).
+template <int B, bool PE, class ET, class Fn>
 __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT, const uint8_t *spos,
                                               const uint32_t *flist, const uint16_t *krow, uint32_t n_kp,
                                               uint32_t n_kn, const uint16_t *plist, const uint16_t *pofs,
@@ -1626,7 +1636,7 @@ __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT
                 const bool ok = in && (spos[e] & 2) && (f > s ? f - s : s - f) <= N;
                 ab[q] = ok ? ABl[(uint64_t)f * TS + e] : 0.0f;
                 qe[q] = ok ? sQ[e] : 0.0f;
-                Et[q] = ok ? sE[t] : (ET)0;
+                Et[q] = ok ? sE[PE ? e : t] : (ET)0;
                 fq[q] = f;
                 tq[q] = t;
                 sq[q] = s;
@@ -1659,7 +1669,7 @@ __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT
             const bool ok = has && (f > s ? f - s : s - f) <= N;
             ab[q] = ok ? ABl[(uint64_t)f * TS + e] : 0.0f;
             qe[q] = ok ? sQ[e] : 0.0f;
-            Et[q] = ok ? sE[t] : (ET)0;
+            Et[q] = ok ? sE[PE ? e : t] : (ET)0;
             fq[q] = f;
             eq[q] = e;
             tq[q] = t;
@@ -2114,7 +2124,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     const float *__restrict__ tnorm, const float *__restrict__ Tpad, const float *__restrict__ ovQ, uint32_t kbase,
     const uint32_t *__restrict__ sup_lo, const uint32_t *__restrict__ sup_hi, uint32_t debug,
     uint32_t *__restrict__ xc_cand, uint32_t *__restrict__ xc_win, uint32_t *__restrict__ xc_ctr,
-    const double *__restrict__ ovVV)
+    const double *__restrict__ ovVV, const float *__restrict__ tabNs)
 {
     extern __shared__ float dyn[];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
@@ -2151,6 +2161,10 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     uint8_t *spos = (uint8_t *)(tlist + T);             // [T][S] bit1 in bounds, bit0 likelihood > 0
     float *sqmax = (float *)(((uintptr_t)(spos + TS) + 3) & ~(uintptr_t)3);   // [2][T] max over (positive) s of Q'
     float *sEf = sqmax + 2 * T;                         // [T] sE (1 + 2^-20) rounded up to float32
+    // kEs (not kQG): the bound per partner (t, s), from the norm of the
+    // partner's shifted, truncated template instead of its whole norm
+    constexpr bool kEs = !kQG;
+    float *sEs = sEf + T;                               // [T][S] float32, rounded up (kEs)
     __shared__ double r_ub[NW];
     if (!kLean)
         for (uint32_t x = threadIdx.x; x < R; x += blockDim.x) szero[x] = 0.0f;
@@ -2295,6 +2309,14 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                 if ((double)ef < eu) ef = __uint_as_float(__float_as_uint(ef) + 1u);
                 sEf[t] = ef;
             }
+            if constexpr (kEs)
+                batched_stage<8>(TS, [&](uint32_t e) { return tabNs[e]; }, [&](uint32_t e, float nb) {
+                    const double nsum = norm_lab + (double)nb;
+                    const double eu = (ecoef * (nsum * vn + nsum * nsum) + eabs) * (1.0 + 0x1p-20) + 0x1p-120;
+                    float ef = (float)eu;
+                    if ((double)ef < eu) ef = __uint_as_float(__float_as_uint(ef) + 1u);
+                    sEs[e] = ef;
+                });
             if (!MQ) __syncthreads();   // Q - B2/2 complete
             // Sixteen lanes per template t (four templates per wave), by
             // ballot over s within the group: the group of positive-likelihood
@@ -2494,7 +2516,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             float kv0 = -INFINITY, kv1 = -INFINITY, kv2 = -INFINITY, kv3 = -INFINITY;
             uint32_t kc0 = 0, kc1 = 0, kc2 = 0, kc3 = 0;
             uint32_t screened = 0;
-            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sEf, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, float Ef) {
+            for_each_kept<4, kEs>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, kEs ? sEs : sEf, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, float Ef) {
                 const float x = sQl[f];
                 const float sh = (x + qe) - ab;
                 const float m = Ef + 0x1p-21f * ((fabsf(x) + fabsf(qe)) + fabsf(ab));
@@ -2561,6 +2583,20 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                     const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
                     if (key_better(score, kk, sc, key)) { sc = score; key = kk; bp = p; }
                 };
+                // kept combinations with their bound: per partner (t, s) as
+                // float32 (kEs) or per template as float64
+                auto kept4 = [&](auto fn) {
+                    if constexpr (kEs)
+                        for_each_kept<4, true>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sEs, fn);
+                    else
+                        for_each_kept<4, false>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, fn);
+                };
+                auto kept1 = [&](auto fn) {
+                    if constexpr (kEs)
+                        for_each_kept<1, true>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sEs, fn);
+                    else
+                        for_each_kept<1, false>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, fn);
+                };
                 // Exact scores of every combination with sh + E >= lo_max
                 // (bp.cl:599-640).  They are all among the kept ones unless a
                 // thread dropped one that reaches lo_max; then the screen is
@@ -2584,7 +2620,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                             const uint32_t f = cand[0] >> 16, e = cand[0] & 0xffffu;
                             const uint32_t t = e / S, s = e - t * S;
                             const double sh = screen(f, qT[e], ABl[(uint64_t)f * TS + e]);
-                            const double E = sE[t];
+                            const double E = kEs ? (double)sEs[e] : sE[t];
                             uint64_t p;
                             (void)combo_key(d, wg_bits, span, t, f, s, p);
                             s_settled = parse_overlap_bounds(d, L, thr, id, lab, bidx, bl, sh - E, sh + E, p, best_L,
@@ -2637,7 +2673,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                             __syncthreads();
                         }
                         uint32_t mine = 0;
-                        for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t, uint32_t, uint32_t, float ab, float qe, double E) {
+                        kept4([&](uint32_t f, uint32_t, uint32_t, uint32_t, float ab, float qe, double E) {
                             if (screen(f, qe, ab) + E >= lo_max) {
                                 if (by_f) atomicAdd(&fpos[f], 1u);
                                 else mine++;
@@ -2678,7 +2714,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                         if (reserve(total)) {
                             const uint32_t first = s_defer - 1;
                             uint32_t slot = first + before + incl - mine;
-                            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t, uint32_t, uint32_t e, float ab, float qe, double E) {
+                            kept4([&](uint32_t f, uint32_t, uint32_t, uint32_t e, float ab, float qe, double E) {
                                 if (screen(f, qe, ab) + E >= lo_max)
                                     xc_cand[by_f ? first + atomicAdd(&fpos[f], 1u) : slot++] = (f << 16) | e;
                             });
@@ -2686,7 +2722,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                         }
                     }
                     if (!deferred)
-                        for_each_kept<1>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
+                        kept1([&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
                             const double sh = screen(f, qe, ab);
                             if (sh + E < lo_max) return;
                             exact(f, t, s, Tpad + ((uint64_t)t * C + sup_lo[t]) * R);
@@ -2708,85 +2744,65 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
 
 // Deferred exact scores of the screened overlap (bp.cl:599-640 for the
 // candidates the screen could not rule out, then the parse of bp.cl:690-814).
-// One wave per deferred window, its lanes one candidate each, 64 at a time;
-// the templates sit in LDS ([T][C][N], staged once per block) when they fit
-// in kXcMaxLds, else they are read from their zero-padded copy in global
-// memory; the window's voltage is in the wave's lanes (one sample each, a
-// channel at a time).
+// One wave per deferred window, its lanes one candidate each, 64 at a time.
+// The templates sit in LDS as rows of N samples, each followed by M zeros
+// (staged once per block) when that fits in kXcMaxLds, else they are read
+// from their zero-padded copy in global memory.
 // Each lane's chain is the reference's float32 order; the winner is reduced
 // in the reference's total order (score, then reduction group and lane).
-constexpr uint32_t kXcThreads = 1024;   // two blocks per CU (LDS), eight waves per SIMD
-constexpr uint64_t kXcMaxLds = 80 * 1024;
+constexpr uint32_t kXcThreads = 1024;   // sixteen waves per block
+constexpr uint64_t kXcMaxLds = 156 * 1024;
 
-// Zero floats before and after the exact-score kernel's LDS templates: a
-// partner read at any shift (row offset in [-M, N + M)) stays inside them.
-__host__ __device__ inline uint32_t overlap_exact_guard(const Dims &d) { return d.M + 8; }
-
-// The window's voltage of a channel is held one sample per lane (N <= 64)
-// and read by the wave at each step with v_readlane (a scalar operand), the
-// next channel's row loaded while the current one is summed.  UF: the lanes
-// share the main shift f, so the main template's shifted row is held the same
-// way; otherwise every lane reads its own from LDS.  The partner's samples
-// are read from LDS per lane; a read outside the row lands in the zero guard
-// around the templates or in a neighbouring row and is replaced by +0.0.  A
-// channel where neither of a lane's templates is nonzero leaves its chain
-// unchanged (every sum is +0.0, so ss stays 0 and acc, never -0, is kept).
-template <bool UF>
-__device__ __forceinline__ float exact_score_lds(const Dims &d, const float *ts, const float *__restrict__ V,
-                                                 uint32_t bidx, uint32_t lab, uint32_t lo_m, uint32_t hi_m, uint32_t t,
-                                                 uint32_t lo_t, uint32_t hi_t, uint32_t f, uint32_t s, uint32_t cw0,
-                                                 uint32_t cw1)
+// LDS floats of the exact-score kernel's padded template rows: M zeros, then
+// T*C rows of N samples and M zeros each.  Row entry x of template row r is
+// at M + r*(N + M) + x; every x in [-M, N + M) reads the row or the zeros
+// around it.
+__host__ __device__ inline uint64_t overlap_exact_lds_floats(const Dims &d)
 {
-    const uint32_t N = d.N, C = d.C, lane = lane_of();
-    const int32_t da = (int32_t)d.M - (int32_t)f, db = (int32_t)d.M - (int32_t)s;
-    // valid samples of the shifted rows: main [ja0, ja0 + na), partner [jb0, jb0 + nb)
-    const uint32_t ja0 = da < 0 ? (uint32_t)(-da) : 0u, jb0 = db < 0 ? (uint32_t)(-db) : 0u;
-    const uint32_t ja1 = da > 0 ? (N > (uint32_t)da ? N - (uint32_t)da : 0u) : N;
-    const uint32_t jb1 = db > 0 ? (N > (uint32_t)db ? N - (uint32_t)db : 0u) : N;
-    const uint32_t na = ja1 > ja0 ? ja1 - ja0 : 0u, nb = jb1 > jb0 ? jb1 - jb0 : 0u;
-    auto vrow = [&](uint32_t c) {
-        return (c < cw1 && lane < N) ? V[(uint64_t)c * d.K + bidx + lane] : 0.0f;
-    };
-    float vreg = vrow(cw0);
+    return (uint64_t)d.M + (uint64_t)d.T * d.C * (d.N + d.M);
+}
+
+// One lane's float32 chain over the channels [cw0, cw1) with the templates in
+// padded LDS rows.  Window sample j of a template shifted by x is row entry
+// M - x + j, a zero outside the template, so no sample is masked; a channel
+// where both templates are zero adds +0.0 sums, which leave ss at 0 and acc,
+// never -0, unchanged.  The voltage index is the same in every lane, so its
+// samples are scalar loads; where the lanes share the main shift f, so is the
+// main template's row, and its samples are broadcast reads.
+__device__ __forceinline__ float exact_score_pad(const Dims &d, const float *tp, const float *__restrict__ V,
+                                                 uint32_t bidx, uint32_t lab, uint32_t t, uint32_t f, uint32_t s,
+                                                 uint32_t cw0, uint32_t cw1)
+{
+    const uint32_t N = d.N, C = d.C, P = d.N + d.M;
+    const uint32_t n8 = N & ~7u;
     float acc = 0.0f;
     for (uint32_t c = cw0; c < cw1; c++) {
-        const float vnext = vrow(c + 1);
-        const bool ha = c >= lo_m && c < hi_m, hb = c >= lo_t && c < hi_t;
-        const float *ta = ts + ((uint64_t)lab * C + c) * N + da;
-        const float *tb = ts + ((uint64_t)t * C + c) * N + db;
-        const uint32_t nac = ha ? na : 0u, nbc = hb ? nb : 0u;
-        float areg = 0.0f;
-        if (UF) {
-            const float x = ta[lane < N ? lane : 0u];
-            areg = lane - ja0 < nac ? x : 0.0f;
-        }
-        // Steps past N (up to the next multiple of 8, at most 64) are no-ops:
-        // both operands are +0.0 there (masked, and lanes >= N of vreg and
-        // areg hold 0), so sum = +0.0 leaves ss and acc unchanged; the reads
-        // past a row stay inside the guard.
+        // row r's entry x is at M + r*P + x
+        const float *ta = tp + ((uint64_t)lab * C + c) * P + 2 * d.M - f;
+        const float *tb = tp + ((uint64_t)t * C + c) * P + 2 * d.M - s;
+        const float *vc = V + (uint64_t)c * d.K + bidx;
         float ss = 0.0f;
-        for (uint32_t j = 0; j < N; j += 8) {
-            float av[8], bv[8];
+        for (uint32_t j = 0; j < n8; j += 8) {
+            float av[8], bv[8], vv[8];
 #pragma unroll
             for (uint32_t q = 0; q < 8; q++) {
-                const float b = tb[j + q];
-                bv[q] = j + q - jb0 < nbc ? b : 0.0f;
-                if (!UF) {
-                    const float a = ta[j + q];
-                    av[q] = j + q - ja0 < nac ? a : 0.0f;
-                }
+                vv[q] = vc[j + q];
+                av[q] = ta[j + q];
+                bv[q] = tb[j + q];
             }
 #pragma unroll
             for (uint32_t q = 0; q < 8; q++) {
-                const float ax = UF ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(areg), j + q)) : av[q];
-                const float v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vreg), j + q));
-                const float sum = ax + bv[q];
+                const float sum = av[q] + bv[q];
                 ss = __builtin_fmaf(sum, sum, ss);      // bp.cl:621
-                acc = __builtin_fmaf(sum, v, acc);      // bp.cl:622
+                acc = __builtin_fmaf(sum, vv[q], acc);  // bp.cl:622
             }
+        }
+        for (uint32_t j = n8; j < N; j++) {
+            const float sum = ta[j] + tb[j];
+            ss = __builtin_fmaf(sum, sum, ss);
+            acc = __builtin_fmaf(sum, vc[j], acc);
         }
         if (ss > 0.0f) acc = (float)((double)acc - 0.5 * (double)ss);   // bp.cl:627-630
-        vreg = vnext;
     }
     return acc;
 }
@@ -2863,20 +2879,22 @@ __global__ __launch_bounds__(kXcThreads) __attribute__((amdgpu_waves_per_eu(GT ?
     uint8_t *ov, const uint32_t *__restrict__ sup_lo, const uint32_t *__restrict__ sup_hi)
 {
     // GT: Tm is the zero-padded [T][C][N + 2M] copy in global memory;
-    // otherwise the [T][C][N] templates, staged here between zero guards
+    // otherwise the [T][C][N] templates, staged here as padded rows
     extern __shared__ float xs[];
-    const uint64_t TCN = (uint64_t)d.T * d.C * d.N;
-    const uint32_t G = overlap_exact_guard(d);
-    float *ts = xs + G;
-    if (!GT) {
-        for (uint64_t e = threadIdx.x; e < TCN + 2 * G; e += blockDim.x)
-            xs[e] = e >= G && e < G + TCN ? Tm[e - G] : 0.0f;
-        __syncthreads();
-    }
     // every per-window value is wave-uniform (read through readfirstlane, so
     // it lives in scalar registers and the window's loads are scalar)
     auto uni = [](uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); };
     const uint32_t n_win = uni(xc_ctr[0]);
+    // blocks with no deferred window leave before staging
+    if (blockIdx.x * (blockDim.x >> 6) >= n_win) return;
+    if (!GT) {   // a wave per row: N samples, then M zeros
+        const uint32_t N = d.N, P = d.N + d.M, rows = d.T * d.C;
+        for (uint32_t x = threadIdx.x; x < d.M; x += blockDim.x) xs[x] = 0.0f;
+        for (uint32_t r = threadIdx.x >> 6; r < rows; r += blockDim.x >> 6)
+            for (uint32_t x = lane_of(); x < P; x += 64)
+                xs[d.M + r * P + x] = x < N ? Tm[(uint64_t)r * N + x] : 0.0f;
+        __syncthreads();
+    }
     const uint32_t lane = lane_of();
     const uint32_t nwave = blockDim.x >> 6;
     const uint32_t wg_bits = 31 - __builtin_clz(d.reduce_wg);
@@ -2913,8 +2931,8 @@ __global__ __launch_bounds__(kXcThreads) __attribute__((amdgpu_waves_per_eu(GT ?
                 score = same_f ? exact_score_glb<true>(d, Tm, V, bidx, lab, lo_m, hi_m, t, lt, ht, f0, s)
                                : exact_score_glb<false>(d, Tm, V, bidx, lab, lo_m, hi_m, t, lt, ht, f, s);
             else
-                score = same_f ? exact_score_lds<true>(d, ts, V, bidx, lab, lo_m, hi_m, t, lt, ht, f0, s, cw0, cw1)
-                               : exact_score_lds<false>(d, ts, V, bidx, lab, lo_m, hi_m, t, lt, ht, f, s, cw0, cw1);
+                score = same_f ? exact_score_pad(d, xs, V, bidx, lab, t, f0, s, cw0, cw1)
+                               : exact_score_pad(d, xs, V, bidx, lab, t, f, s, cw0, cw1);
             if (act) {
                 uint64_t p;
                 const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
@@ -3430,7 +3448,7 @@ static size_t overlap_screened_lds_nw(const Dims &d, int nw)
     const uint64_t region = nw != 4 ? krw + cand : (partner + cand > krw ? partner + cand : krw);
     return (uint64_t)d.T * 8 + (vg ? 0 : (uint64_t)d.C * d.N * 4) + main_rows * 4 + (qg ? 0 : ts * 4) +
            (uint64_t)d.T * 4 + (uint64_t)d.S * 4 + region * 4 + ts * 2 + (uint64_t)d.T * 6 + ts + 4 +
-           (uint64_t)d.T * 12 + 16;
+           (uint64_t)d.T * 12 + (qg ? 0 : ts * 4) + 16;
 }
 
 // Waves per screen block: 4 when the block scores its candidates; with the
@@ -3481,7 +3499,7 @@ hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_
 {
     const uint64_t total = (uint64_t)d.T * d.S * d.T * d.S + (uint64_t)d.T * d.S + d.T;
     hipLaunchKernelGGL(k_overlap_tables, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st, d, s.Tm, s.tabAB,
-                       s.tabB2, s.tnorm, s.sup_lo, s.sup_hi);
+                       s.tabB2, s.tabNs, s.tnorm, s.sup_lo, s.sup_hi);
     const uint64_t nmin = (uint64_t)d.T * d.S * d.T;
     hipLaunchKernelGGL(k_overlap_abmin, dim3(grid_for(nmin, 256, 65536)), dim3(256), 0, st, d, s.tabAB, s.tabABmin);
     const uint64_t nq = overlap_qtemplate_floats(d);
@@ -3531,7 +3549,7 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
         if (e != hipSuccess) return e;
     }
     // exact scores: templates staged in LDS when they fit, else read from Tpad
-    const size_t xlds_full = ((size_t)d.T * d.C * d.N + 2 * overlap_exact_guard(d)) * 4;
+    const size_t xlds_full = (size_t)overlap_exact_lds_floats(d) * 4;
     const bool xg = xlds_full > kXcMaxLds || d.xc_wide;
     const size_t xlds = xg ? 0 : xlds_full;
     if (d.xc_cap && xlds > 64 * 1024) {
@@ -3565,7 +3583,7 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
         {
 #define FBP_SCREEN_ARGS d, s.V, s.Tm, s.L, s.thr, list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, \
                         s.ov, s.tabAB, s.tabABmin, s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug, \
-                        s.xc_cand, s.xc_win, s.xc_ctr, s.ovVV
+                        s.xc_cand, s.xc_win, s.xc_ctr, s.ovVV, s.tabNs
             if (nw == 2 && mq)
                 hipLaunchKernelGGL((k_overlap_screened<true, 2>), dim3(grid), dim3(128), lds, st, FBP_SCREEN_ARGS);
             else if (nw == 2)
