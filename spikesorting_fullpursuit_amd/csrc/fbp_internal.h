@@ -108,7 +108,6 @@ struct DeviceState {
     float *tabAB;        // [T][S][T][S] screened overlap: shifted-template cross products
     float *tabABmin;     // [T][S][T]    min over the last shift of tabAB (row bounds of the screen)
     float *tabB2;        // [T][S]       squared norms of shifted, truncated templates
-    float *tabNs;        // [T][S]       their norms (upper bounds)
     float *tnorm;        // [T]          template norms (upper bounds)
     float *TmQ;          // screened overlap: split-bf16 A fragments of the matrix-core Q table (uint4 each)
     float *Tpad;         // [T][C][N + 2M] templates at [M, M + N), +0.0 around (overlap scores)

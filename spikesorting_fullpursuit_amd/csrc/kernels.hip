@@ -1458,17 +1458,13 @@ __global__ __launch_bounds__(256) void k_overlap_direct(
 // combination the reference does not skip, and bounds |screen - float32 score|
 // by E = ecoef * ((|t_lab|+|t_t|)*|v| + (|t_lab|+|t_t|)^2), which covers the
 // float32 rounding of the reference's fused chain (gamma_{CN+2C+8} with a
-// doubled unit roundoff) and the float32 storage of Q and the tables.  |t_t|
-// may be the norm of the partner shifted by s and truncated to the window
-// (every term of the chain, of Q(t,s), B2(t,s) and AB that holds the partner
-// holds only that part), which is what the screen uses where it has room.  Every
+// doubled unit roundoff) and the float32 storage of Q and the tables.  Every
 // combination whose upper bound reaches the largest lower bound is then scored
 // exactly with the reference's float32 chain (overlap_direct_score) and reduced
 // in the reference's winner order, so the result equals the direct form's.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_overlap_tables(Dims d, const float *__restrict__ Tm, float *__restrict__ AB,
-                                                       float *__restrict__ B2, float *__restrict__ Ns,
-                                                       float *__restrict__ tnorm,
+                                                       float *__restrict__ B2, float *__restrict__ tnorm,
                                                        const uint32_t *__restrict__ sup_lo,
                                                        const uint32_t *__restrict__ sup_hi)
 {
@@ -1503,11 +1499,6 @@ __global__ __launch_bounds__(256) void k_overlap_tables(Dims d, const float *__r
                 for (int32_t j = lo; j < hi; j++) acc = __builtin_fma((double)b[j + sa], (double)b[j + sa], acc);
             }
             B2[q] = (float)acc;
-            // upper bound of the shifted, truncated template's norm
-            const double nrm = sqrt(acc) * (1.0 + 1e-12);
-            float up = (float)nrm;
-            if ((double)up < nrm) up = __uint_as_float(__float_as_uint(up) + 1u);
-            Ns[q] = up;
         } else {
             const uint32_t t = (uint32_t)(e - nab - (uint64_t)T * S);
             double acc = 0.0;
@@ -1608,9 +1599,8 @@ struct FastDiv {
 // operands of B combinations (global AB, LDS sQ, sE, spos) before it
 // evaluates them, so their latencies overlap.  Row and column indices advance
 // incrementally; no division per element.  sE holds the bound per template
-// as float64 or, rounded up, as float32 (ET); with PE, per partner (t, s)
-// instead ([T][S], float32).
-template <int B, bool PE, class ET, class Fn>
+// as float64 or, rounded up, as float32 (ET).
+template <int B, class ET, class Fn>
 __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT, const uint8_t *spos,
                                               const uint32_t *flist, const uint16_t *krow, uint32_t n_kp,
                                               uint32_t n_kn, const uint16_t *plist, const uint16_t *pofs,
@@ -1636,7 +1626,7 @@ __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT
                 const bool ok = in && (spos[e] & 2) && (f > s ? f - s : s - f) <= N;
                 ab[q] = ok ? ABl[(uint64_t)f * TS + e] : 0.0f;
                 qe[q] = ok ? sQ[e] : 0.0f;
-                Et[q] = ok ? sE[PE ? e : t] : (ET)0;
+                Et[q] = ok ? sE[t] : (ET)0;
                 fq[q] = f;
                 tq[q] = t;
                 sq[q] = s;
@@ -1669,7 +1659,7 @@ __device__ __forceinline__ void for_each_kept(const Dims &d, const FastDiv &divT
             const bool ok = has && (f > s ? f - s : s - f) <= N;
             ab[q] = ok ? ABl[(uint64_t)f * TS + e] : 0.0f;
             qe[q] = ok ? sQ[e] : 0.0f;
-            Et[q] = ok ? sE[PE ? e : t] : (ET)0;
+            Et[q] = ok ? sE[t] : (ET)0;
             fq[q] = f;
             eq[q] = e;
             tq[q] = t;
@@ -2124,7 +2114,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     const float *__restrict__ tnorm, const float *__restrict__ Tpad, const float *__restrict__ ovQ, uint32_t kbase,
     const uint32_t *__restrict__ sup_lo, const uint32_t *__restrict__ sup_hi, uint32_t debug,
     uint32_t *__restrict__ xc_cand, uint32_t *__restrict__ xc_win, uint32_t *__restrict__ xc_ctr,
-    const double *__restrict__ ovVV, const float *__restrict__ tabNs)
+    const double *__restrict__ ovVV)
 {
     extern __shared__ float dyn[];
     const uint32_t S = d.S, T = d.T, N = d.N, M = d.M, C = d.C;
@@ -2161,10 +2151,6 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     uint8_t *spos = (uint8_t *)(tlist + T);             // [T][S] bit1 in bounds, bit0 likelihood > 0
     float *sqmax = (float *)(((uintptr_t)(spos + TS) + 3) & ~(uintptr_t)3);   // [2][T] max over (positive) s of Q'
     float *sEf = sqmax + 2 * T;                         // [T] sE (1 + 2^-20) rounded up to float32
-    // kEs (not kQG): the bound per partner (t, s), from the norm of the
-    // partner's shifted, truncated template instead of its whole norm
-    constexpr bool kEs = !kQG;
-    float *sEs = sEf + T;                               // [T][S] float32, rounded up (kEs)
     __shared__ double r_ub[NW];
     if (!kLean)
         for (uint32_t x = threadIdx.x; x < R; x += blockDim.x) szero[x] = 0.0f;
@@ -2309,14 +2295,6 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                 if ((double)ef < eu) ef = __uint_as_float(__float_as_uint(ef) + 1u);
                 sEf[t] = ef;
             }
-            if constexpr (kEs)
-                batched_stage<8>(TS, [&](uint32_t e) { return tabNs[e]; }, [&](uint32_t e, float nb) {
-                    const double nsum = norm_lab + (double)nb;
-                    const double eu = (ecoef * (nsum * vn + nsum * nsum) + eabs) * (1.0 + 0x1p-20) + 0x1p-120;
-                    float ef = (float)eu;
-                    if ((double)ef < eu) ef = __uint_as_float(__float_as_uint(ef) + 1u);
-                    sEs[e] = ef;
-                });
             if (!MQ) __syncthreads();   // Q - B2/2 complete
             // Sixteen lanes per template t (four templates per wave), by
             // ballot over s within the group: the group of positive-likelihood
@@ -2516,7 +2494,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             float kv0 = -INFINITY, kv1 = -INFINITY, kv2 = -INFINITY, kv3 = -INFINITY;
             uint32_t kc0 = 0, kc1 = 0, kc2 = 0, kc3 = 0;
             uint32_t screened = 0;
-            for_each_kept<4, kEs>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, kEs ? sEs : sEf, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, float Ef) {
+            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sEf, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, float Ef) {
                 const float x = sQl[f];
                 const float sh = (x + qe) - ab;
                 const float m = Ef + 0x1p-21f * ((fabsf(x) + fabsf(qe)) + fabsf(ab));
@@ -2583,20 +2561,6 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                     const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
                     if (key_better(score, kk, sc, key)) { sc = score; key = kk; bp = p; }
                 };
-                // kept combinations with their bound: per partner (t, s) as
-                // float32 (kEs) or per template as float64
-                auto kept4 = [&](auto fn) {
-                    if constexpr (kEs)
-                        for_each_kept<4, true>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sEs, fn);
-                    else
-                        for_each_kept<4, false>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, fn);
-                };
-                auto kept1 = [&](auto fn) {
-                    if constexpr (kEs)
-                        for_each_kept<1, true>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sEs, fn);
-                    else
-                        for_each_kept<1, false>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, fn);
-                };
                 // Exact scores of every combination with sh + E >= lo_max
                 // (bp.cl:599-640).  They are all among the kept ones unless a
                 // thread dropped one that reaches lo_max; then the screen is
@@ -2620,7 +2584,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                             const uint32_t f = cand[0] >> 16, e = cand[0] & 0xffffu;
                             const uint32_t t = e / S, s = e - t * S;
                             const double sh = screen(f, qT[e], ABl[(uint64_t)f * TS + e]);
-                            const double E = kEs ? (double)sEs[e] : sE[t];
+                            const double E = sE[t];
                             uint64_t p;
                             (void)combo_key(d, wg_bits, span, t, f, s, p);
                             s_settled = parse_overlap_bounds(d, L, thr, id, lab, bidx, bl, sh - E, sh + E, p, best_L,
@@ -2673,7 +2637,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                             __syncthreads();
                         }
                         uint32_t mine = 0;
-                        kept4([&](uint32_t f, uint32_t, uint32_t, uint32_t, float ab, float qe, double E) {
+                        for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t, uint32_t, uint32_t, float ab, float qe, double E) {
                             if (screen(f, qe, ab) + E >= lo_max) {
                                 if (by_f) atomicAdd(&fpos[f], 1u);
                                 else mine++;
@@ -2714,7 +2678,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                         if (reserve(total)) {
                             const uint32_t first = s_defer - 1;
                             uint32_t slot = first + before + incl - mine;
-                            kept4([&](uint32_t f, uint32_t, uint32_t, uint32_t e, float ab, float qe, double E) {
+                            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t, uint32_t, uint32_t e, float ab, float qe, double E) {
                                 if (screen(f, qe, ab) + E >= lo_max)
                                     xc_cand[by_f ? first + atomicAdd(&fpos[f], 1u) : slot++] = (f << 16) | e;
                             });
@@ -2722,7 +2686,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                         }
                     }
                     if (!deferred)
-                        kept1([&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
+                        for_each_kept<1>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t t, uint32_t s, uint32_t e, float ab, float qe, double E) {
                             const double sh = screen(f, qe, ab);
                             if (sh + E < lo_max) return;
                             exact(f, t, s, Tpad + ((uint64_t)t * C + sup_lo[t]) * R);
@@ -2767,18 +2731,22 @@ __host__ __device__ inline uint64_t overlap_exact_lds_floats(const Dims &d)
 // M - x + j, a zero outside the template, so no sample is masked; a channel
 // where both templates are zero adds +0.0 sums, which leave ss at 0 and acc,
 // never -0, unchanged.  The voltage index is the same in every lane, so its
-// samples are scalar loads; where the lanes share the main shift f, so is the
-// main template's row, and its samples are broadcast reads.
+// samples are scalar loads; UF: the lanes share the main shift f, and the
+// main template's samples are scalar loads too, from its zero-padded copy in
+// global memory (Tg: rows of N + 2M, window sample j at 2M - f + j).
+template <bool UF>
 __device__ __forceinline__ float exact_score_pad(const Dims &d, const float *tp, const float *__restrict__ V,
-                                                 uint32_t bidx, uint32_t lab, uint32_t t, uint32_t f, uint32_t s,
-                                                 uint32_t cw0, uint32_t cw1)
+                                                 const float *__restrict__ Tg, uint32_t bidx, uint32_t lab,
+                                                 uint32_t t, uint32_t f, uint32_t s, uint32_t cw0, uint32_t cw1)
 {
+    const uint32_t R = d.N + 2 * d.M;
     const uint32_t N = d.N, C = d.C, P = d.N + d.M;
     const uint32_t n8 = N & ~7u;
     float acc = 0.0f;
     for (uint32_t c = cw0; c < cw1; c++) {
         // row r's entry x is at M + r*P + x
-        const float *ta = tp + ((uint64_t)lab * C + c) * P + 2 * d.M - f;
+        const float *ta = UF ? Tg + ((uint64_t)lab * C + c) * R + 2 * d.M - f
+                             : tp + ((uint64_t)lab * C + c) * P + 2 * d.M - f;
         const float *tb = tp + ((uint64_t)t * C + c) * P + 2 * d.M - s;
         const float *vc = V + (uint64_t)c * d.K + bidx;
         float ss = 0.0f;
@@ -2876,10 +2844,12 @@ __global__ __launch_bounds__(kXcThreads) __attribute__((amdgpu_waves_per_eu(GT ?
     const float *__restrict__ thr, const uint32_t *__restrict__ list, const uint32_t *__restrict__ xc_win,
     const uint32_t *__restrict__ xc_ctr, const uint32_t *__restrict__ xc_cand, Counters *ctr, float *best_L,
     const uint32_t *__restrict__ best_lab, const uint32_t *__restrict__ best_idx, uint32_t *ov_idx, uint32_t *ov_lab,
-    uint8_t *ov, const uint32_t *__restrict__ sup_lo, const uint32_t *__restrict__ sup_hi)
+    uint8_t *ov, const uint32_t *__restrict__ sup_lo, const uint32_t *__restrict__ sup_hi,
+    const float *__restrict__ Tg)
 {
     // GT: Tm is the zero-padded [T][C][N + 2M] copy in global memory;
-    // otherwise the [T][C][N] templates, staged here as padded rows
+    // otherwise the [T][C][N] templates, staged here as padded rows (Tg:
+    // the zero-padded copy, for the main rows shared by a batch)
     extern __shared__ float xs[];
     // every per-window value is wave-uniform (read through readfirstlane, so
     // it lives in scalar registers and the window's loads are scalar)
@@ -2931,8 +2901,8 @@ __global__ __launch_bounds__(kXcThreads) __attribute__((amdgpu_waves_per_eu(GT ?
                 score = same_f ? exact_score_glb<true>(d, Tm, V, bidx, lab, lo_m, hi_m, t, lt, ht, f0, s)
                                : exact_score_glb<false>(d, Tm, V, bidx, lab, lo_m, hi_m, t, lt, ht, f, s);
             else
-                score = same_f ? exact_score_pad(d, xs, V, bidx, lab, t, f0, s, cw0, cw1)
-                               : exact_score_pad(d, xs, V, bidx, lab, t, f, s, cw0, cw1);
+                score = same_f ? exact_score_pad<true>(d, xs, V, Tg, bidx, lab, t, f0, s, cw0, cw1)
+                               : exact_score_pad<false>(d, xs, V, Tg, bidx, lab, t, f, s, cw0, cw1);
             if (act) {
                 uint64_t p;
                 const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
@@ -3448,7 +3418,7 @@ static size_t overlap_screened_lds_nw(const Dims &d, int nw)
     const uint64_t region = nw != 4 ? krw + cand : (partner + cand > krw ? partner + cand : krw);
     return (uint64_t)d.T * 8 + (vg ? 0 : (uint64_t)d.C * d.N * 4) + main_rows * 4 + (qg ? 0 : ts * 4) +
            (uint64_t)d.T * 4 + (uint64_t)d.S * 4 + region * 4 + ts * 2 + (uint64_t)d.T * 6 + ts + 4 +
-           (uint64_t)d.T * 12 + (qg ? 0 : ts * 4) + 16;
+           (uint64_t)d.T * 12 + 16;
 }
 
 // Waves per screen block: 4 when the block scores its candidates; with the
@@ -3499,7 +3469,7 @@ hipError_t launch_overlap_tables(const Dims &d, const DeviceState &s, hipStream_
 {
     const uint64_t total = (uint64_t)d.T * d.S * d.T * d.S + (uint64_t)d.T * d.S + d.T;
     hipLaunchKernelGGL(k_overlap_tables, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st, d, s.Tm, s.tabAB,
-                       s.tabB2, s.tabNs, s.tnorm, s.sup_lo, s.sup_hi);
+                       s.tabB2, s.tnorm, s.sup_lo, s.sup_hi);
     const uint64_t nmin = (uint64_t)d.T * d.S * d.T;
     hipLaunchKernelGGL(k_overlap_abmin, dim3(grid_for(nmin, 256, 65536)), dim3(256), 0, st, d, s.tabAB, s.tabABmin);
     const uint64_t nq = overlap_qtemplate_floats(d);
@@ -3583,7 +3553,7 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
         {
 #define FBP_SCREEN_ARGS d, s.V, s.Tm, s.L, s.thr, list, s.ctr, s.best_L, s.best_lab, s.best_idx, s.ov_idx, s.ov_lab, \
                         s.ov, s.tabAB, s.tabABmin, s.tabB2, s.tnorm, s.Tpad, s.ovQ, kb, s.sup_lo, s.sup_hi, debug, \
-                        s.xc_cand, s.xc_win, s.xc_ctr, s.ovVV, s.tabNs
+                        s.xc_cand, s.xc_win, s.xc_ctr, s.ovVV
             if (nw == 2 && mq)
                 hipLaunchKernelGGL((k_overlap_screened<true, 2>), dim3(grid), dim3(128), lds, st, FBP_SCREEN_ARGS);
             else if (nw == 2)
@@ -3615,11 +3585,11 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
             if (xg)
                 hipLaunchKernelGGL(k_overlap_exact<true>, dim3(xgrid), dim3(kXcThreads), 0, st, d, s.V, s.Tpad, s.L,
                                    s.thr, list, s.xc_win, s.xc_ctr, s.xc_cand, s.ctr, s.best_L, s.best_lab,
-                                   s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.sup_lo, s.sup_hi);
+                                   s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.sup_lo, s.sup_hi, s.Tpad);
             else
                 hipLaunchKernelGGL(k_overlap_exact<false>, dim3(xgrid), dim3(kXcThreads), xlds, st, d, s.V, s.Tm,
                                    s.L, s.thr, list, s.xc_win, s.xc_ctr, s.xc_cand, s.ctr, s.best_L, s.best_lab,
-                                   s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.sup_lo, s.sup_hi);
+                                   s.best_idx, s.ov_idx, s.ov_lab, s.ov, s.sup_lo, s.sup_hi, s.Tpad);
         }
     }
     return hipGetLastError();

@@ -191,7 +191,6 @@ void for_each_buffer(const fbp_config *cfg, const Dims &d, DeviceState &s, V &&v
         visit(s.tabAB, T * d.S * T * d.S);
         visit(s.tabABmin, T * d.S * T);
         visit(s.tabB2, T * d.S);
-        visit(s.tabNs, T * d.S);
         visit(s.TmQ, overlap_qtemplate_floats(d));
         visit(s.ov_sorted, W);
         visit(s.ovQ, (uint64_t)d.ov_batch * T * d.S);
