@@ -2726,53 +2726,73 @@ __host__ __device__ inline uint64_t overlap_exact_lds_floats(const Dims &d)
     return (uint64_t)d.M + (uint64_t)d.T * d.C * (d.N + d.M);
 }
 
-// One lane's float32 chain over the channels [cw0, cw1) with the templates in
-// padded LDS rows.  Window sample j of a template shifted by x is row entry
-// M - x + j, a zero outside the template, so no sample is masked; a channel
-// where both templates are zero adds +0.0 sums, which leave ss at 0 and acc,
-// never -0, unchanged.  The voltage index is the same in every lane, so its
-// samples are scalar loads; UF: the lanes share the main shift f, and the
-// main template's samples are scalar loads too, from its zero-padded copy in
-// global memory (Tg: rows of N + 2M, window sample j at 2M - f + j).
-template <bool UF>
-__device__ __forceinline__ float exact_score_pad(const Dims &d, const float *tp, const float *__restrict__ V,
-                                                 const float *__restrict__ Tg, uint32_t bidx, uint32_t lab,
-                                                 uint32_t t, uint32_t f, uint32_t s, uint32_t cw0, uint32_t cw1)
+// A lane's float32 chains of NC candidates over the channels [cw0, cw1) with
+// the templates in padded LDS rows.  Window sample j of a template shifted by
+// x is row entry M - x + j, a zero outside the template, so no sample is
+// masked; a channel where both templates are zero adds +0.0 sums, which leave
+// ss at 0 and acc, never -0, unchanged.  The voltage index is the same in
+// every lane, so its samples are scalar loads, shared by the NC chains; UF:
+// the lanes share the main shift f, and the main template's samples are
+// scalar loads too, from its zero-padded copy in global memory (Tg: rows of
+// N + 2M, window sample j at 2M - f + j).
+template <bool UF, int NC>
+__device__ __forceinline__ void exact_score_pad(const Dims &d, const float *tp, const float *__restrict__ V,
+                                                const float *__restrict__ Tg, uint32_t bidx, uint32_t lab,
+                                                const uint32_t (&t)[NC], const uint32_t (&f)[NC],
+                                                const uint32_t (&s)[NC], uint32_t cw0, uint32_t cw1,
+                                                float (&acc)[NC])
 {
-    const uint32_t R = d.N + 2 * d.M;
-    const uint32_t N = d.N, C = d.C, P = d.N + d.M;
+    const uint32_t N = d.N, C = d.C, P = d.N + d.M, R = d.N + 2 * d.M;
     const uint32_t n8 = N & ~7u;
-    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NC; k++) acc[k] = 0.0f;
     for (uint32_t c = cw0; c < cw1; c++) {
         // row r's entry x is at M + r*P + x
-        const float *ta = UF ? Tg + ((uint64_t)lab * C + c) * R + 2 * d.M - f
-                             : tp + ((uint64_t)lab * C + c) * P + 2 * d.M - f;
-        const float *tb = tp + ((uint64_t)t * C + c) * P + 2 * d.M - s;
+        const float *ta[NC], *tb[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            ta[k] = UF ? Tg + ((uint64_t)lab * C + c) * R + 2 * d.M - f[k]
+                       : tp + ((uint64_t)lab * C + c) * P + 2 * d.M - f[k];
+            tb[k] = tp + ((uint64_t)t[k] * C + c) * P + 2 * d.M - s[k];
+        }
         const float *vc = V + (uint64_t)c * d.K + bidx;
-        float ss = 0.0f;
+        float ss[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) ss[k] = 0.0f;
         for (uint32_t j = 0; j < n8; j += 8) {
-            float av[8], bv[8], vv[8];
+            float vv[8], av[NC][8], bv[NC][8];
 #pragma unroll
             for (uint32_t q = 0; q < 8; q++) {
                 vv[q] = vc[j + q];
-                av[q] = ta[j + q];
-                bv[q] = tb[j + q];
+#pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    av[k][q] = (UF && k > 0) ? av[0][q] : ta[k][j + q];
+                    bv[k][q] = tb[k][j + q];
+                }
             }
 #pragma unroll
             for (uint32_t q = 0; q < 8; q++) {
-                const float sum = av[q] + bv[q];
-                ss = __builtin_fmaf(sum, sum, ss);      // bp.cl:621
-                acc = __builtin_fmaf(sum, vv[q], acc);  // bp.cl:622
+#pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    const float sum = av[k][q] + bv[k][q];
+                    ss[k] = __builtin_fmaf(sum, sum, ss[k]);        // bp.cl:621
+                    acc[k] = __builtin_fmaf(sum, vv[q], acc[k]);    // bp.cl:622
+                }
             }
         }
         for (uint32_t j = n8; j < N; j++) {
-            const float sum = ta[j] + tb[j];
-            ss = __builtin_fmaf(sum, sum, ss);
-            acc = __builtin_fmaf(sum, vc[j], acc);
+            const float v = vc[j];
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                const float sum = ta[k][j] + tb[k][j];
+                ss[k] = __builtin_fmaf(sum, sum, ss[k]);
+                acc[k] = __builtin_fmaf(sum, v, acc[k]);
+            }
         }
-        if (ss > 0.0f) acc = (float)((double)acc - 0.5 * (double)ss);   // bp.cl:627-630
+#pragma unroll
+        for (int k = 0; k < NC; k++)
+            if (ss[k] > 0.0f) acc[k] = (float)((double)acc[k] - 0.5 * (double)ss[k]);   // bp.cl:627-630
     }
-    return acc;
 }
 
 // The same score with the templates read from the zero-padded rows in global
@@ -2878,35 +2898,79 @@ __global__ __launch_bounds__(kXcThreads) __attribute__((amdgpu_waves_per_eu(GT ?
         const uint32_t lo_m = uni(sup_lo[lab]), hi_m = uni(sup_hi[lab]);
         float sc = -INFINITY;
         uint64_t key = ~0ull, bp = 0;
-        for (uint32_t i0 = 0; i0 < count; i0 += 64) {
-            const bool act = i0 + lane < count;
-            const uint32_t cnd = act ? xc_cand[first + i0 + lane] : 0u;
-            const uint32_t f = cnd >> 16, e = cnd & 0xffffu;
-            const uint32_t t = divS.div(e), s = e - t * d.S;
-            const uint32_t lo_t = act ? sup_lo[t] : lo_m, hi_t = act ? sup_hi[t] : lo_m;
-            // channel range of the wave: the union of the lanes' supports
-            uint32_t c0 = min(lo_m, lo_t), c1 = max(hi_m, hi_t);
-            for (int off = 32; off > 0; off >>= 1) {
-                c0 = min(c0, (uint32_t)__shfl_xor(c0, off));
-                c1 = max(c1, (uint32_t)__shfl_xor(c1, off));
+        if (GT) {
+            for (uint32_t i0 = 0; i0 < count; i0 += 64) {
+                const bool act = i0 + lane < count;
+                const uint32_t cnd = act ? xc_cand[first + i0 + lane] : 0u;
+                const uint32_t f = cnd >> 16, e = cnd & 0xffffu;
+                const uint32_t t = divS.div(e), s = e - t * d.S;
+                const uint32_t lo_t = act ? sup_lo[t] : lo_m, hi_t = act ? sup_hi[t] : lo_m;
+                // batches that share the main shift (the screen groups a
+                // window's candidates by it) take the segmented chain
+                const uint32_t f0 = __builtin_amdgcn_readfirstlane(f);
+                const bool same_f = __ballot(act && f != f0) == 0;
+                const uint32_t lt = act ? lo_t : 0u, ht = act ? hi_t : 0u;
+                const float score = same_f ? exact_score_glb<true>(d, Tm, V, bidx, lab, lo_m, hi_m, t, lt, ht, f0, s)
+                                           : exact_score_glb<false>(d, Tm, V, bidx, lab, lo_m, hi_m, t, lt, ht, f, s);
+                if (act) {
+                    uint64_t p;
+                    const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
+                    if (key_better(score, kk, sc, key)) { sc = score; key = kk; bp = p; }
+                }
             }
-            // batches that share the main shift (the screen groups a
-            // window's candidates by it) take the segmented chain
-            const uint32_t f0 = __builtin_amdgcn_readfirstlane(f);
-            const bool same_f = __ballot(act && f != f0) == 0;
-            const uint32_t cw0 = __builtin_amdgcn_readfirstlane(c0), cw1 = __builtin_amdgcn_readfirstlane(c1);
-            const uint32_t lt = act ? lo_t : 0u, ht = act ? hi_t : 0u;
-            float score;
-            if (GT)
-                score = same_f ? exact_score_glb<true>(d, Tm, V, bidx, lab, lo_m, hi_m, t, lt, ht, f0, s)
-                               : exact_score_glb<false>(d, Tm, V, bidx, lab, lo_m, hi_m, t, lt, ht, f, s);
-            else
-                score = same_f ? exact_score_pad<true>(d, xs, V, Tg, bidx, lab, t, f0, s, cw0, cw1)
-                               : exact_score_pad<false>(d, xs, V, Tg, bidx, lab, t, f, s, cw0, cw1);
-            if (act) {
-                uint64_t p;
-                const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
-                if (key_better(score, kk, sc, key)) { sc = score; key = kk; bp = p; }
+        } else {
+            // two candidates per lane (128 per step, two independent chains
+            // sharing the voltage and main-row loads); a step whose second
+            // half is empty runs one chain
+            for (uint32_t i0 = 0; i0 < count; i0 += 128) {
+                bool act[2];
+                uint32_t f[2], t[2], s[2];
+                uint32_t c0 = lo_m, c1 = hi_m;
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    act[k] = i0 + 64 * k + lane < count;
+                    const uint32_t cnd = act[k] ? xc_cand[first + i0 + 64 * k + lane] : 0u;
+                    f[k] = cnd >> 16;
+                    const uint32_t e = cnd & 0xffffu;
+                    t[k] = divS.div(e);
+                    s[k] = e - t[k] * d.S;
+                    if (act[k]) {   // channel range: the union of the lanes' supports
+                        c0 = min(c0, sup_lo[t[k]]);
+                        c1 = max(c1, sup_hi[t[k]]);
+                    }
+                }
+                if (!act[1]) { f[1] = f[0]; t[1] = t[0]; s[1] = s[0]; }
+                for (int off = 32; off > 0; off >>= 1) {
+                    c0 = min(c0, (uint32_t)__shfl_xor(c0, off));
+                    c1 = max(c1, (uint32_t)__shfl_xor(c1, off));
+                }
+                const uint32_t cw0 = __builtin_amdgcn_readfirstlane(c0), cw1 = __builtin_amdgcn_readfirstlane(c1);
+                const uint32_t f0 = __builtin_amdgcn_readfirstlane(f[0]);
+                const bool same_f = __ballot((act[0] && f[0] != f0) || (act[1] && f[1] != f0)) == 0;
+                const bool two = __ballot(act[1]) != 0;
+                float score[2] = {0.0f, 0.0f};
+                if (two) {
+                    if (same_f) {
+                        const uint32_t fu[2] = {f0, f0};
+                        exact_score_pad<true, 2>(d, xs, V, Tg, bidx, lab, t, fu, s, cw0, cw1, score);
+                    } else {
+                        exact_score_pad<false, 2>(d, xs, V, Tg, bidx, lab, t, f, s, cw0, cw1, score);
+                    }
+                } else {
+                    const uint32_t t1[1] = {t[0]}, f1[1] = {same_f ? f0 : f[0]}, s1[1] = {s[0]};
+                    float sc1[1];
+                    if (same_f) exact_score_pad<true, 1>(d, xs, V, Tg, bidx, lab, t1, f1, s1, cw0, cw1, sc1);
+                    else exact_score_pad<false, 1>(d, xs, V, Tg, bidx, lab, t1, f1, s1, cw0, cw1, sc1);
+                    score[0] = sc1[0];
+                }
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    if (act[k]) {
+                        uint64_t p;
+                        const uint64_t kk = combo_key(d, wg_bits, span, t[k], f[k], s[k], p);
+                        if (key_better(score[k], kk, sc, key)) { sc = score[k]; key = kk; bp = p; }
+                    }
+                }
             }
         }
         for (int off = 32; off > 0; off >>= 1) {
