@@ -2877,12 +2877,14 @@ __global__ __launch_bounds__(kXcThreads) __attribute__((amdgpu_waves_per_eu(GT ?
     const uint32_t n_win = uni(xc_ctr[0]);
     // blocks with no deferred window leave before staging
     if (blockIdx.x * (blockDim.x >> 6) >= n_win) return;
-    if (!GT) {   // a wave per row: N samples, then M zeros
-        const uint32_t N = d.N, P = d.N + d.M, rows = d.T * d.C;
+    if (!GT) {   // rows of N samples, then M zeros; eight loads in flight per thread
+        const uint32_t N = d.N, P = d.N + d.M;
+        const FastDiv divP(P);
         for (uint32_t x = threadIdx.x; x < d.M; x += blockDim.x) xs[x] = 0.0f;
-        for (uint32_t r = threadIdx.x >> 6; r < rows; r += blockDim.x >> 6)
-            for (uint32_t x = lane_of(); x < P; x += 64)
-                xs[d.M + r * P + x] = x < N ? Tm[(uint64_t)r * N + x] : 0.0f;
+        batched_stage<8>(d.T * d.C * P, [&](uint32_t x) {
+            const uint32_t r = divP.div(x), j = x - r * P;
+            return j < N ? Tm[(uint64_t)r * N + j] : 0.0f;
+        }, [&](uint32_t x, float v) { xs[d.M + x] = v; });
         __syncthreads();
     }
     const uint32_t lane = lane_of();
