@@ -787,7 +787,14 @@ __global__ __launch_bounds__(1024) void k_rescore_res(Dims d, const float *__res
     const uint32_t span = ((N + 15) / 16) * 16 + N;   // window positions (padded) + template length
     const uint32_t nwb = kRescoreStage / span < 4 ? kRescoreStage / span : 4u;   // windows per batch
     if (nwb == 0) return;   // the host launches this kernel only for span <= kRescoreStage
-    for (uint32_t x = threadIdx.x; x < ntm; x += blockDim.x) lds[x] = TmT[x];
+    for (uint32_t b = threadIdx.x; b < ntm; b += 8 * blockDim.x) {   // eight loads in flight per thread
+        float v[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++) v[q] = b + q * blockDim.x < ntm ? TmT[b + q * blockDim.x] : 0.0f;
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++)
+            if (b + q * blockDim.x < ntm) lds[b + q * blockDim.x] = v[q];
+    }
     __syncthreads();
     const uint32_t lane = lane_of(), nw = blockDim.x >> 6;
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // scalar per-wave state
