@@ -2802,65 +2802,56 @@ __device__ __forceinline__ void exact_score_pad(const Dims &d, const float *tp, 
     }
 }
 
-// The same score with the templates read from the zero-padded rows in global
-// memory (templates too large for LDS; rows of R = N + 2M samples, window
-// sample j of a template at shift x is row entry 2M - x + j, +0.0 outside the
-// template).  The wave visits only the channels where the main template or
-// some lane's partner is nonzero (the others leave every chain unchanged),
-// the next one's voltage row loaded while the current one is summed.
-template <bool UF>
+// The same score with the templates read from their zero-padded rows in
+// global memory (templates too large for LDS; rows of R = N + 2M samples,
+// window sample j of a template at shift x is row entry 2M - x + j, +0.0
+// outside the template).  Each lane walks its own channels: those of the
+// main template's support [lo_m, hi_m) and its partner's [lo_t, hi_t), in
+// ascending order (a channel where both are zero leaves the chain unchanged),
+// with its own voltage, main and partner rows; an inactive lane walks none.
 __device__ __forceinline__ float exact_score_glb(const Dims &d, const float *__restrict__ Tpad,
                                                  const float *__restrict__ V, uint32_t bidx, uint32_t lab,
                                                  uint32_t lo_m, uint32_t hi_m, uint32_t t, uint32_t lo_t,
-                                                 uint32_t hi_t, uint32_t f, uint32_t s)
+                                                 uint32_t hi_t, uint32_t f, uint32_t s, bool act)
 {
-    const uint32_t N = d.N, C = d.C, R = N + 2 * d.M, lane = lane_of();
-    // smallest channel >= c in the main support or some lane's partner
-    // support (C if none), wave-uniform
-    auto next_ch = [&](uint32_t c) {
-        uint32_t x = c < hi_m ? max(c, lo_m) : C;
-        x = min(x, c < hi_t ? max(c, lo_t) : C);
-        for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor(x, off));
-        return (uint32_t)__builtin_amdgcn_readfirstlane(x);
-    };
-    auto vrow = [&](uint32_t c) { return (c < C && lane < N) ? V[(uint64_t)c * d.K + bidx + lane] : 0.0f; };
-    // UF: the main template's shifted row, one sample per lane
-    auto arow = [&](uint32_t c) {
-        return (UF && c >= lo_m && c < hi_m && lane < N) ? Tpad[((uint64_t)lab * C + c) * R + 2 * d.M - f + lane]
-                                                         : 0.0f;
-    };
-    uint32_t c = next_ch(0);
-    float vreg = vrow(c), areg = arow(c);
+    const uint32_t N = d.N, C = d.C, R = N + 2 * d.M, n8 = N & ~7u;
+    uint32_t c = min(lo_m, lo_t);
+    const uint32_t cend = act ? max(hi_m, hi_t) : 0u;
     float acc = 0.0f;
-    while (c < C) {
-        const uint32_t cn = next_ch(c + 1);
-        const float vnext = vrow(cn), anext = arow(cn);
-        const bool ha = c >= lo_m && c < hi_m, hb = c >= lo_t && c < hi_t;
+    while (c < cend) {
         const float *ta = Tpad + ((uint64_t)lab * C + c) * R + 2 * d.M - f;
         const float *tb = Tpad + ((uint64_t)t * C + c) * R + 2 * d.M - s;
-        // steps past N are no-ops: both operands +0.0 (unloaded), and lanes
-        // >= N of vreg and areg hold 0
+        const float *vc = V + (uint64_t)c * d.K + bidx;
         float ss = 0.0f;
-        for (uint32_t j = 0; j < N; j += 8) {
-            float av[8], bv[8];
+        for (uint32_t j = 0; j < n8; j += 8) {
+            float av[8], bv[8], vv[8];
 #pragma unroll
             for (uint32_t q = 0; q < 8; q++) {
-                bv[q] = (hb && j + q < N) ? tb[j + q] : 0.0f;
-                if (!UF) av[q] = (ha && j + q < N) ? ta[j + q] : 0.0f;
+                av[q] = ta[j + q];
+                bv[q] = tb[j + q];
+                vv[q] = vc[j + q];
             }
 #pragma unroll
             for (uint32_t q = 0; q < 8; q++) {
-                const float ax = UF ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(areg), j + q)) : av[q];
-                const float v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vreg), j + q));
-                const float sum = ax + bv[q];
+                const float sum = av[q] + bv[q];
                 ss = __builtin_fmaf(sum, sum, ss);      // bp.cl:621
-                acc = __builtin_fmaf(sum, v, acc);      // bp.cl:622
+                acc = __builtin_fmaf(sum, vv[q], acc);  // bp.cl:622
             }
         }
+        for (uint32_t j = n8; j < N; j++) {
+            const float sum = ta[j] + tb[j];
+            ss = __builtin_fmaf(sum, sum, ss);
+            acc = __builtin_fmaf(sum, vc[j], acc);
+        }
         if (ss > 0.0f) acc = (float)((double)acc - 0.5 * (double)ss);   // bp.cl:627-630
-        vreg = vnext;
-        areg = anext;
-        c = cn;
+        // next channel of either support
+        c++;
+        if (!(c >= lo_m && c < hi_m) && !(c >= lo_t && c < hi_t)) {
+            uint32_t nx = cend;
+            if (lo_m > c) nx = min(nx, lo_m);
+            if (lo_t > c) nx = min(nx, lo_t);
+            c = nx;
+        }
     }
     return acc;
 }
@@ -2914,13 +2905,7 @@ __global__ __launch_bounds__(kXcThreads) __attribute__((amdgpu_waves_per_eu(GT ?
                 const uint32_t f = cnd >> 16, e = cnd & 0xffffu;
                 const uint32_t t = divS.div(e), s = e - t * d.S;
                 const uint32_t lo_t = act ? sup_lo[t] : lo_m, hi_t = act ? sup_hi[t] : lo_m;
-                // batches that share the main shift (the screen groups a
-                // window's candidates by it) take the segmented chain
-                const uint32_t f0 = __builtin_amdgcn_readfirstlane(f);
-                const bool same_f = __ballot(act && f != f0) == 0;
-                const uint32_t lt = act ? lo_t : 0u, ht = act ? hi_t : 0u;
-                const float score = same_f ? exact_score_glb<true>(d, Tm, V, bidx, lab, lo_m, hi_m, t, lt, ht, f0, s)
-                                           : exact_score_glb<false>(d, Tm, V, bidx, lab, lo_m, hi_m, t, lt, ht, f, s);
+                const float score = exact_score_glb(d, Tm, V, bidx, lab, lo_m, hi_m, t, lo_t, hi_t, f, s, act);
                 if (act) {
                     uint64_t p;
                     const uint64_t kk = combo_key(d, wg_bits, span, t, f, s, p);
