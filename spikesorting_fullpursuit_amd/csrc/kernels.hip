@@ -2112,6 +2112,24 @@ __global__ void k_pack_qtemplates(Dims d, const float *__restrict__ Tm, const ui
 // wave occupancy (the kernel is bound by the latency of its phases), and
 // where the per-window tables leave room for one block per CU (hundreds of
 // templates) eight waves share the window's phases.
+constexpr uint64_t kXcMaxLds = 156 * 1024;
+
+// LDS floats of the exact-score kernel's padded template rows: M zeros, then
+// T*C rows of N samples and M zeros each.  Row entry x of template row r is
+// at M + r*(N + M) + x; every x in [-M, N + M) reads the row or the zeros
+// around it.
+__host__ __device__ inline uint64_t overlap_exact_lds_floats(const Dims &d)
+{
+    return (uint64_t)d.M + (uint64_t)d.T * d.C * (d.N + d.M);
+}
+
+// The exact-score kernel reads the templates from global memory (they do not
+// fit in its LDS, or the test switch asks for it).
+__host__ __device__ inline bool overlap_exact_global(const Dims &d)
+{
+    return overlap_exact_lds_floats(d) * 4 > kXcMaxLds || d.xc_wide;
+}
+
 template <bool MQ, int NW>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_overlap_screened(
     Dims d, const float *__restrict__ V, const float *__restrict__ Tm, const float *__restrict__ L,
@@ -2637,16 +2655,21 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                         // kernel's batches mostly share f; with more shifts than
                         // the buffer holds, each thread's candidates follow a
                         // block-wide prefix sum of the threads' counts.
-                        const bool by_f = S <= kCand;
-                        uint32_t *fpos = cand;   // [S] counts, then next slot per shift
+                        // Templates in global memory: grouped by partner template
+                        // instead, so a batch's lanes read few partner rows (each
+                        // lane walks its own channels there).
+                        const bool by_t = overlap_exact_global(d);
+                        const uint32_t KD = by_t ? T : S;   // group keys
+                        const bool by_f = KD <= kCand;
+                        uint32_t *fpos = cand;   // [KD] counts, then next slot per key
                         if (by_f) {
-                            for (uint32_t x = threadIdx.x; x < S; x += blockDim.x) fpos[x] = 0;
+                            for (uint32_t x = threadIdx.x; x < KD; x += blockDim.x) fpos[x] = 0;
                             __syncthreads();
                         }
                         uint32_t mine = 0;
-                        for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t, uint32_t, uint32_t, float ab, float qe, double E) {
+                        for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t t, uint32_t, uint32_t, float ab, float qe, double E) {
                             if (screen(f, qe, ab) + E >= lo_max) {
-                                if (by_f) atomicAdd(&fpos[f], 1u);
+                                if (by_f) atomicAdd(&fpos[by_t ? t : f], 1u);
                                 else mine++;
                             }
                         });
@@ -2654,16 +2677,16 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                         uint32_t before = 0, total = 0, incl = mine;
                         if (by_f) {
                             __syncthreads();
-                            if (threadIdx.x < kWave) {   // exclusive prefix over the shifts, one wave
+                            if (threadIdx.x < kWave) {   // exclusive prefix over the keys, one wave
                                 uint32_t carry = 0;
-                                for (uint32_t x0 = 0; x0 < S; x0 += kWave) {
-                                    const uint32_t v = x0 + l < S ? fpos[x0 + l] : 0u;
+                                for (uint32_t x0 = 0; x0 < KD; x0 += kWave) {
+                                    const uint32_t v = x0 + l < KD ? fpos[x0 + l] : 0u;
                                     uint32_t inc = v;
                                     for (int off = 1; off < 64; off <<= 1) {
                                         const uint32_t o = __shfl_up(inc, off);
                                         if (l >= (uint32_t)off) inc += o;
                                     }
-                                    if (x0 + l < S) fpos[x0 + l] = carry + inc - v;
+                                    if (x0 + l < KD) fpos[x0 + l] = carry + inc - v;
                                     carry += __shfl(inc, 63);
                                 }
                                 if (l == 0) r_scan[0] = carry;
@@ -2685,9 +2708,9 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                         if (reserve(total)) {
                             const uint32_t first = s_defer - 1;
                             uint32_t slot = first + before + incl - mine;
-                            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t, uint32_t, uint32_t e, float ab, float qe, double E) {
+                            for_each_kept<4>(d, divT, spos, flist, krow, nkp, nkn, plist, pofs, pcnt, maxcnt, ABl, qT, sE, [&](uint32_t f, uint32_t t, uint32_t, uint32_t e, float ab, float qe, double E) {
                                 if (screen(f, qe, ab) + E >= lo_max)
-                                    xc_cand[by_f ? first + atomicAdd(&fpos[f], 1u) : slot++] = (f << 16) | e;
+                                    xc_cand[by_f ? first + atomicAdd(&fpos[by_t ? t : f], 1u) : slot++] = (f << 16) | e;
                             });
                             deferred = true;
                         }
@@ -2722,16 +2745,6 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
 // Each lane's chain is the reference's float32 order; the winner is reduced
 // in the reference's total order (score, then reduction group and lane).
 constexpr uint32_t kXcThreads = 1024;   // sixteen waves per block
-constexpr uint64_t kXcMaxLds = 156 * 1024;
-
-// LDS floats of the exact-score kernel's padded template rows: M zeros, then
-// T*C rows of N samples and M zeros each.  Row entry x of template row r is
-// at M + r*(N + M) + x; every x in [-M, N + M) reads the row or the zeros
-// around it.
-__host__ __device__ inline uint64_t overlap_exact_lds_floats(const Dims &d)
-{
-    return (uint64_t)d.M + (uint64_t)d.T * d.C * (d.N + d.M);
-}
 
 // A lane's float32 chains of NC candidates over the channels [cw0, cw1) with
 // the templates in padded LDS rows.  Window sample j of a template shifted by
@@ -3578,7 +3591,7 @@ hipError_t launch_overlap_screened(const Dims &d, const DeviceState &s, uint32_t
     }
     // exact scores: templates staged in LDS when they fit, else read from Tpad
     const size_t xlds_full = (size_t)overlap_exact_lds_floats(d) * 4;
-    const bool xg = xlds_full > kXcMaxLds || d.xc_wide;
+    const bool xg = overlap_exact_global(d);
     const size_t xlds = xg ? 0 : xlds_full;
     if (d.xc_cap && xlds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_overlap_exact<false>,
