@@ -2176,6 +2176,12 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     uint8_t *spos = (uint8_t *)(tlist + T);             // [T][S] bit1 in bounds, bit0 likelihood > 0
     float *sqmax = (float *)(((uintptr_t)(spos + TS) + 3) & ~(uintptr_t)3);   // [2][T] max over (positive) s of Q'
     float *sEf = sqmax + 2 * T;                         // [T] sE (1 + 2^-20) rounded up to float32
+    float *svcs = sEf + T;                              // [C] squared voltage norm per channel (kVG)
+    // Sparse channel supports (kVG): the bound of a template pair takes the
+    // voltage norm and the operation counts over the channels of their two
+    // supports only (every term on another channel is an exact zero, which
+    // neither the reference's chain nor the matrix-core sums round).
+    const bool sparse = kVG && 2 * SM <= C;
     __shared__ double r_ub[NW];
     if (!kLean)
         for (uint32_t x = threadIdx.x; x < R; x += blockDim.x) szero[x] = 0.0f;
@@ -2298,6 +2304,27 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             }
             for (int off = 32; off > 0; off >>= 1) vv += __shfl_xor(vv, off);
             if (lane_of() == 0) r_vv[threadIdx.x >> 6] = vv;
+            if (sparse) {   // a wave per channel row, four rows in flight; float64 squares (exact), rounded up
+                const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), l = lane_of();
+                for (uint32_t c0 = wv * 4; c0 < C; c0 += NW * 4) {
+                    double a[4] = {0.0, 0.0, 0.0, 0.0};
+                    for (uint32_t j = l; j < N; j += kWave) {
+                        float x[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) x[q] = c0 + q < C ? V[(uint64_t)(c0 + q) * d.K + bidx + j] : 0.0f;
+#pragma unroll
+                        for (int q = 0; q < 4; q++) a[q] = __builtin_fma((double)x[q], (double)x[q], a[q]);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        for (int off = 32; off > 0; off >>= 1) a[q] += __shfl_xor(a[q], off);
+                        const double sc = a[q] * (1.0 + 0x1p-40);
+                        float up = (float)sc;
+                        if ((double)up < sc) up = __uint_as_float(__float_as_uint(up) + 1u);
+                        if (l == 0 && c0 + q < C) svcs[c0 + q] = up;
+                    }
+                }
+            }
             __syncthreads();
             head_step(2);
             double vsum = 0.0;   // kVG: from the Q-table kernel, which staged the same samples
@@ -2313,7 +2340,28 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
                                  [&](uint32_t e, float b2) { sQ[e] = (float)((double)sQ[e] - 0.5 * (double)b2); });
             for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
                 const double nsum = norm_lab + (double)stn[t];
-                const double e = ecoef * (nsum * vn + nsum * nsum) + eabs;
+                double e;
+                if (sparse) {   // over the union of the two supports
+                    const uint32_t lt = sup_lo[t], ht = sup_hi[t];
+                    double vs = 0.0;
+                    uint32_t u = 0;
+                    for (uint32_t c = min(lo_m, lt); c < max(hi_m, ht); c++)
+                        if ((c >= lo_m && c < hi_m) || (c >= lt && c < ht)) {
+                            vs += (double)svcs[c];
+                            u++;
+                        }
+                    const uint32_t um = max(hi_m - lo_m, ht - lt);
+                    const double vnu = sqrt(vs) * (1.0 + 1e-12);
+                    const double no = (double)u * N + 2.0 * u + 8.0;
+                    const double gu = no * u2 / (1.0 - no * u2);
+                    const double nqu = (double)um * qp.NKQ * 16 + qp.KS + 2.0;
+                    const double q1 = nqu * u2 / (1.0 - nqu * u2), q2 = 2.0 * nqu * u2 / (1.0 - 2.0 * nqu * u2);
+                    const double gqu = MQ ? q1 * 1.004 + q2 * 0.0079 + 3.1 * 0x1p-16 : 0.0;
+                    const double eau = MQ ? 2.0 * 3.0 * nqu * 0x1p-126 : 0.0;
+                    e = (gu + 2.0 * gqu + 4.0 * u2) * 1.01 * (nsum * vnu + nsum * nsum) + eau;
+                } else {
+                    e = ecoef * (nsum * vn + nsum * nsum) + eabs;
+                }
                 sE[t] = e;
                 const double eu = e * (1.0 + 0x1p-20) + 0x1p-120;
                 float ef = (float)eu;
@@ -3489,7 +3537,7 @@ static size_t overlap_screened_lds_nw(const Dims &d, int nw)
     const uint64_t region = nw != 4 ? krw + cand : (partner + cand > krw ? partner + cand : krw);
     return (uint64_t)d.T * 8 + (vg ? 0 : (uint64_t)d.C * d.N * 4) + main_rows * 4 + (qg ? 0 : ts * 4) +
            (uint64_t)d.T * 4 + (uint64_t)d.S * 4 + region * 4 + ts * 2 + (uint64_t)d.T * 6 + ts + 4 +
-           (uint64_t)d.T * 12 + 16;
+           (uint64_t)d.T * 12 + (vg ? (uint64_t)d.C * 4 : 0) + 16;
 }
 
 // Waves per screen block: 4 when the block scores its candidates; with the
