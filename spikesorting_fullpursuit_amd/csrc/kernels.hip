@@ -2180,8 +2180,11 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     // Sparse channel supports (kVG): the bound of a template pair takes the
     // voltage norm and the operation counts over the channels of their two
     // supports only (every term on another channel is an exact zero, which
-    // neither the reference's chain nor the matrix-core sums round).
-    const bool sparse = kVG && 2 * SM <= C;
+    // neither the reference's chain nor the matrix-core sums round).  Used
+    // for shift ranges beyond N/2, where partners shifted nearly out of the
+    // window leave many combinations within the bound (at N/2 the per-channel
+    // norms cost the screen more than the fewer exact scores save).
+    const bool sparse = kVG && 2 * SM <= C && 2 * M > N;
     __shared__ double r_ub[NW];
     if (!kLean)
         for (uint32_t x = threadIdx.x; x < R; x += blockDim.x) szero[x] = 0.0f;

@@ -172,10 +172,13 @@ def test_low_precision_dense_16ch(oracle_mod, precision):
     print("re-scored fraction of (t, i):", frac, flush=True)
 
 
-@pytest.mark.parametrize("precision,mode", [("fp32", "screened"), ("bf16", "screened"), ("fp32", "direct")])
-def test_sparse_channel_support(oracle_mod, precision, mode):
+@pytest.mark.parametrize("precision,mode,M", [("fp32", "screened", 20), ("bf16", "screened", 20), ("fp32", "direct", 20),
+                                              ("bf16", "screened", 40)])
+def test_sparse_channel_support(oracle_mod, precision, mode, M):
     # Templates nonzero on 6 of 32 channels (a dense probe): the screened
     # overlap stores and scores only the channels a template is nonzero on.
+    # At n_max_shift_inds = N - 1 the screen's bound of a template pair is
+    # taken over the channels of their two supports.
     from spikesorting_fullpursuit_amd import synthetic
     rec = synthetic.make_recording(41, 32, 30000, 0.4, 12, 41, (8.0, 20.0), (5.0, 60.0), gains="gauss",
                                    spatial_sigma=2.0, support=6, overlap_frac=0.1)
@@ -183,9 +186,9 @@ def test_sparse_channel_support(oracle_mod, precision, mode):
     rec['bias'] = np.float32(-0.5 * sm['template_SS'])
     rec['thr'] = np.float32(sm['neuron_lower_thresholds'])
     rec['N'] = 41
-    print("case sparse support", precision, mode, flush=True)
-    g = run_gpu(rec, 20, True, 256, precision=precision, overlap_mode=mode)
-    o = run_oracle(oracle_mod, rec, 20, True, 256)
+    print("case sparse support", precision, mode, M, flush=True)
+    g = run_gpu(rec, M, True, 256, precision=precision, overlap_mode=mode)
+    o = run_oracle(oracle_mod, rec, M, True, 256)
     assert o['idx'].size > 0
     np.testing.assert_array_equal(g['idx'], o['idx'])
     np.testing.assert_array_equal(g['lab'], o['lab'])
