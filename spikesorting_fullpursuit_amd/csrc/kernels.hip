@@ -767,7 +767,8 @@ __global__ __launch_bounds__(64) void k_rescore_sup(Dims d, const float *__restr
 // (wave-level ordering only; the next channel's samples are fetched while the
 // current one runs).  Used when the templates fit in LDS.
 constexpr uint32_t kRescoreResWaves = 16;
-constexpr uint32_t kRescoreStage = 512;   // voltage floats a wave stages per channel
+constexpr uint32_t kRescoreStage = 768;    // voltage floats a wave stages per channel
+constexpr uint32_t kRescoreWin = 6;        // windows a wave batch stages together
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -785,7 +786,7 @@ __global__ __launch_bounds__(1024) void k_rescore_res(Dims d, const float *__res
     const uint32_t N = d.N, Tp = d.Tp, C = d.C;
     const uint32_t ntm = C * N * Tp;
     const uint32_t span = ((N + 15) / 16) * 16 + N;   // window positions (padded) + template length
-    const uint32_t nwb = kRescoreStage / span < 4 ? kRescoreStage / span : 4u;   // windows per batch
+    const uint32_t nwb = kRescoreStage / span < kRescoreWin ? kRescoreStage / span : kRescoreWin;   // windows per batch
     if (nwb == 0) return;   // the host launches this kernel only for span <= kRescoreStage
     for (uint32_t b = threadIdx.x; b < ntm; b += 8 * blockDim.x) {   // eight loads in flight per thread
         float v[8];
@@ -834,12 +835,12 @@ __global__ __launch_bounds__(1024) void k_rescore_res(Dims d, const float *__res
         const uint32_t rank = __popcll(fm & ((2ull << lane) - 1)) - 1;   // in: lane 0 is a first
         const bool ok = in && rank < nwb;
         const uint64_t okm = __ballot(ok);
-        uint32_t wb[4];
+        uint32_t wb[kRescoreWin];
         uint32_t nwin = 0;
         {
             uint64_t f = fm;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
+            for (int r = 0; r < (int)kRescoreWin; r++) {
                 const int l = f ? __builtin_ctzll(f) : 0;
                 wb[r] = __shfl(wid, l) * N;
                 if (f && (uint32_t)r < nwb) nwin = r + 1;
@@ -856,7 +857,7 @@ __global__ __launch_bounds__(1024) void k_rescore_res(Dims d, const float *__res
 #pragma unroll
         for (uint32_t q = 0; q < kQ; q++) {
             const uint32_t x = lane + 64u * q, r = x / span, o = x - r * span;
-            const uint32_t rr = r < 4 ? r : 3u;
+            const uint32_t rr = r < kRescoreWin ? r : kRescoreWin - 1;
             has[q] = r < nwin && (uint64_t)wb[rr] + o < d.K;
             src[q] = has[q] ? wb[rr] + o : 0u;
         }
